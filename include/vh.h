@@ -1,0 +1,214 @@
+/* vh.h - C ABI of libvh, an MI355X (gfx950) voxel-hashing TSDF engine.
+ *
+ * The reference exports one C++ class, ark::GpuTsdfGenerator (include/tsdf.cuh:548-683),
+ * from libTSDF.so (CMakeLists.txt:15-16). libvh exports the same three operations -
+ * construct, integrate a depth frame at a pose, extract the mesh - as plain extern "C"
+ * functions over pointers and sizes, plus the extensions the reference lacks (trilinear
+ * SDF query, its backward pass, Gauss-Newton pose refinement, FP16 voxel storage).
+ * A header-only C++ class with the reference's constructor and method names sits on top
+ * of this ABI (voxel-hashing-sdf_amd/host/GpuTsdfGenerator.hpp).
+ *
+ * Conventions (kept from the reference):
+ *   - depth is H x W row-major, metres (float) or uint16 scaled by depth_scale
+ *     (SaveFrame.cpp:179-180 uses 0.001); 0 marks an invalid pixel; a pixel is used iff
+ *     0 < d <= max_depth (src/tsdf.cu:715);
+ *   - c2w is a row-major 4x4 camera-to-world matrix (PointCloudGenerator.cpp:118-125);
+ *   - voxel (i,j,k) sits at (i,j,k)*vox_size, blocks are 8^3 voxels, in-block order
+ *     (x*8+y)*8+z (src/tsdf.cu:24-64);
+ *   - the TSDF is normalised: min(1, (d - z)/trunc), voxels with d - z <= -trunc are not
+ *     updated (src/tsdf.cu:718-738).
+ *
+ * Every entry point returns a vh_status. No entry point exits, throws or traps; the text of
+ * the last failure on the calling thread is returned by vh_last_error().
+ * A volume is externally synchronised: calls on one volume must not overlap.
+ */
+#ifndef VH_H
+#define VH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_ABI_VERSION 1
+
+typedef enum vh_status {
+  VH_OK = 0,
+  VH_ERR_INVALID_ARG = 1,
+  VH_ERR_OOM = 2,
+  VH_ERR_POOL_FULL = 3,  /* a block could not get pool memory; it was not integrated */
+  VH_ERR_HASH_FULL = 4,  /* a key found no free slot within the probe limit */
+  VH_ERR_DEVICE = 5,     /* a HIP runtime call failed */
+  VH_ERR_RANGE = 6,      /* a truncation band left the representable block range */
+  VH_ERR_TRUNCATED = 7,  /* an output buffer was too small; the required size is reported */
+  VH_ERR_IO = 8,
+  VH_ERR_NO_DEVICE = 9
+} vh_status;
+
+typedef enum vh_storage {
+  VH_STORAGE_F32 = 0, /* float tsdf + float weight, 8 B per voxel */
+  VH_STORAGE_F16 = 1  /* binary16 tsdf + uint16 weight, 4 B per voxel; updates in FP32 */
+} vh_storage;
+
+typedef enum vh_depth_format { VH_DEPTH_U16 = 0, VH_DEPTH_F32 = 1 } vh_depth_format;
+
+typedef struct vh_volume vh_volume;
+
+/* Engine parameters. Replaces the 15-argument constructor
+ * GpuTsdfGenerator(width, height, fx, fy, cx, cy, max_depth, origin_x/y/z, vox_size,
+ * trunc_m, vox_dim_x/y/z) (include/tsdf.cuh:605-607, src/tsdf.cu:1248-1372) and the hash
+ * sizes the reference hard-codes (src/tsdf.cu:1488,1492). origin and vox_dim are accepted
+ * for compatibility and ignored, as the reference's hash path ignores them. */
+typedef struct vh_config {
+  int32_t width, height;
+  float fx, fy, cx, cy;
+  float max_depth;       /* metres */
+  float min_depth;       /* metres: near end of a truncation band (reference: 0.1) */
+  float vox_size;        /* metres */
+  float trunc;           /* metres (mu) */
+  int32_t log2_buckets;  /* 2^log2_buckets hash buckets of 4 slots each (1..30) */
+  int32_t pool_blocks;   /* voxel-block pool capacity */
+  int32_t storage;       /* vh_storage */
+  int32_t weight_cap;    /* 0 = uncapped; VH_STORAGE_F16 requires 1..65535 */
+  int32_t device;        /* HIP device ordinal */
+  float origin[3];       /* ignored */
+  int32_t vox_dim[3];    /* ignored */
+  int32_t reserved[8];
+} vh_config;
+
+/* One input frame in host memory (replaces processFrame's float* depth, uchar* rgb,
+ * float* c2w, include/tsdf.cuh:609-610). */
+typedef struct vh_frame {
+  const void* depth;     /* height rows of pitch bytes */
+  int32_t depth_format;  /* vh_depth_format */
+  float depth_scale;     /* metres per unit for VH_DEPTH_U16; ignored for VH_DEPTH_F32 */
+  int64_t pitch;         /* bytes per depth row; 0 = tightly packed */
+  const uint8_t* rgb;    /* may be NULL; colour is not integrated by this version */
+  float c2w[16];         /* row-major camera-to-world */
+} vh_frame;
+
+typedef struct vh_stats {
+  int64_t frames;             /* frames integrated since create/reset */
+  int64_t blocks_allocated;   /* pool blocks in use */
+  int64_t pool_capacity;
+  int64_t slots_used;         /* hash slots holding a key (>= blocks_allocated) */
+  int64_t touched_last;       /* distinct blocks crossed by truncation bands, last frame */
+  int64_t touched_total;      /* sum of touched_last over all frames */
+  int64_t new_total;          /* blocks allocated since create/reset */
+  int32_t storage;
+  int32_t flags;              /* bit 0 pool full, bit 1 hash full, bit 2 range */
+  double integrate_ms;        /* profiling mode: summed integration-kernel time */
+  double alloc_ms;            /* profiling mode: summed allocation-kernel time */
+  int64_t profiled_frames;
+} vh_stats;
+
+/* Triangle soup in host memory, owned by the library (freed by vh_mesh_free). */
+typedef struct vh_mesh {
+  int64_t n_tris;
+  float* xyz;       /* n_tris * 9 floats: three vertices in metres */
+  int32_t* cubes;   /* n_tris * 4: cube's lattice x,y,z and (cube_case << 8 | tri_in_cube) */
+} vh_mesh;
+
+typedef struct vh_gn_stats {
+  int32_t iters;
+  int32_t n_valid[64];  /* points with a valid trilinear sample, per iteration */
+  double rms[64];       /* residual RMS (normalised TSDF units) before each step */
+  double step_norm[64]; /* |delta xi| of each step */
+} vh_gn_stats;
+
+/* ------------------------------------------------------------------ lifetime */
+void vh_config_default(vh_config* cfg);
+/* GpuTsdfGenerator::GpuTsdfGenerator (src/tsdf.cu:1248-1372). */
+int vh_create(const vh_config* cfg, vh_volume** out);
+/* GpuTsdfGenerator::Shutdown (src/tsdf.cu:1620-1638), which the reference never calls. */
+int vh_destroy(vh_volume* vol);
+/* Empties the volume (all blocks freed). No reference counterpart: the reference rebuilds
+ * its per-frame tables in processFrame (src/tsdf.cu:1488-1492). */
+int vh_reset(vh_volume* vol);
+const char* vh_last_error(void);
+int vh_abi_version(void);
+
+/* ------------------------------------------------------------------ integration */
+/* GpuTsdfGenerator::processFrame(depth, rgb, c2w) (src/tsdf.cu:1484-1598): allocates
+ * every block crossed by a valid pixel's truncation band and integrates those blocks.
+ * Host buffers are copied before return; the call is synchronous. */
+int vh_integrate(vh_volume* vol, const vh_frame* frame);
+
+/* Device-resident input: binds n_frames depth images (tightly packed, device memory,
+ * frame f at d_depth + f*width*height*elem) and n_frames row-major 4x4 poses
+ * (device memory). Later vh_integrate_next calls consume them in order. */
+int vh_bind_sequence(vh_volume* vol, const void* d_depth, int32_t depth_format,
+                     float depth_scale, const float* d_c2w, int32_t n_frames);
+/* Enqueues the next n frames of the bound sequence on the volume's stream (one captured
+ * HIP graph launch per frame unless profiling is on). Asynchronous; see vh_sync. */
+int vh_integrate_next(vh_volume* vol, int32_t n);
+/* Waits for queued work and reports any pool/hash/range overflow raised by it. */
+int vh_sync(vh_volume* vol);
+/* Profiling mode: frames launch eagerly with HIP events around each kernel; the summed
+ * kernel times appear in vh_stats. */
+int vh_set_profiling(vh_volume* vol, int32_t on);
+int vh_get_stats(vh_volume* vol, vh_stats* out);
+/* The volume's HIP stream (hipStream_t), for callers that order their own work on it. */
+void* vh_stream(vh_volume* vol);
+
+/* ------------------------------------------------------------------ mesh */
+/* GpuTsdfGenerator::SavePLY -> tsdf2mesh (src/tsdf.cu:1696-1708,1760-1888), but computed
+ * by marching cubes over every allocated block on the GPU rather than gathered from
+ * per-frame caches. Cubes need all 8 corners allocated and observed (weight > 0). */
+int vh_extract_mesh(vh_volume* vol, vh_mesh* out);
+int vh_mesh_free(vh_mesh* mesh);
+/* Device variant: writes up to capacity triangles (9 floats each) to d_xyz and, if
+ * d_cubes is not NULL, 4 ints each to d_cubes; *n_tris receives the full count
+ * (VH_ERR_TRUNCATED if it exceeds capacity). Synchronous. */
+int vh_extract_mesh_device(vh_volume* vol, float* d_xyz, int32_t* d_cubes, int64_t capacity,
+                           int64_t* n_tris);
+/* Writes the mesh as binary little-endian PLY (vertices of each triangle listed in order). */
+int vh_save_ply(vh_volume* vol, const char* path);
+
+/* ------------------------------------------------------------------ block export */
+/* Copies out every allocated block: keys (3 ints each), tsdf and weight (512 floats each, in
+ * (x*8+y)*8+z order, binary16 values widened exactly). *n receives the block count
+ * (VH_ERR_TRUNCATED if it exceeds capacity; nothing is written then). Any pointer may be
+ * NULL to skip that array. */
+int vh_export_blocks(vh_volume* vol, int32_t* keys, float* tsdf, float* weight,
+                     int64_t capacity, int64_t* n);
+
+/* ------------------------------------------------------------------ query (extension) */
+/* Trilinear SDF at n world points (host arrays, 3 floats each). valid[i] = 1 iff all 8
+ * lattice corners are allocated and observed. No reference counterpart. */
+int vh_query_sdf(vh_volume* vol, const float* pts, int64_t n, float* sdf, uint8_t* valid);
+/* Device variant; asynchronous on the volume's stream. */
+int vh_query_sdf_device(vh_volume* vol, const float* d_pts, int64_t n, float* d_sdf,
+                        uint8_t* d_valid);
+/* Backward of r_i = sdf(R p_i + t) with respect to a left se(3) perturbation
+ * xi = (rho, phi) of the pose (T <- exp(xi^) T) and to the voxel tsdf values.
+ * d_pts are camera-frame points, c2w the pose (host, row-major 4x4), d_grad_r the upstream
+ * gradient dL/dr_i. Outputs (device, each may be NULL):
+ *   d_r           n residuals (0 where invalid), d_valid n flags,
+ *   d_J           n x 6 per-point dr/dxi,
+ *   d_pose_grad   6 doubles: sum_i dL/dr_i * dr_i/dxi,
+ *   d_voxel_grad  pool_blocks*512 floats, accumulated (+=) dL/dtsdf per pool voxel.
+ * Asynchronous on the volume's stream. */
+int vh_query_sdf_backward_device(vh_volume* vol, const float* d_pts, int64_t n,
+                                 const float* c2w, const float* d_grad_r, float* d_r,
+                                 uint8_t* d_valid, float* d_J, double* d_pose_grad,
+                                 float* d_voxel_grad);
+/* Gauss-Newton refinement of c2w (host, row-major 4x4, updated in place, FP64) so that the
+ * camera-frame points p_i (host, n x 3) land on the zero level set: minimises
+ * sum_i sdf(R p_i + t)^2. J^T J and J^T r are reduced in FP64 on the GPU and the 6x6
+ * system is solved there (Cholesky). stats may be NULL. */
+int vh_gn_refine(vh_volume* vol, const float* pts, int64_t n, double c2w[16], int32_t iters,
+                 vh_gn_stats* stats);
+
+/* Pool layout for callers that wrap the voxel memory (e.g. as a tensor): device pointer of
+ * the voxel pool (pool_blocks*512 voxels; FP32: float2 {tsdf, weight}; FP16: uint32
+ * {binary16 tsdf | uint16 weight << 16}) and of the block keys (pool_blocks packed keys). */
+int vh_pool_view(vh_volume* vol, void** d_voxels, void** d_block_keys, int64_t* n_blocks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VH_H */

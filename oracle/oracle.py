@@ -1,0 +1,169 @@
+"""ctypes loader of the CPU oracle (oracle/libvh_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
+CPU-baseline leg of bench.py, always as the checker of libvh, never as the thing measured
+or shipped. See vh_oracle.cpp for the reference lines each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvh_oracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.vho_create.restype = P
+    lib.vho_create.argtypes = [C.c_int, C.c_int] + [C.c_float] * 8 + [C.c_int, C.c_int]
+    lib.vho_destroy.argtypes = [P]
+    lib.vho_set_threads.argtypes = [C.c_int]
+    lib.vho_max_threads.restype = C.c_int
+    lib.vho_integrate.restype = C.c_int64
+    lib.vho_integrate.argtypes = [P, P, C.c_float, P]
+    lib.vho_num_blocks.restype = C.c_int64
+    lib.vho_num_blocks.argtypes = [P]
+    lib.vho_touched.restype = C.c_int64
+    lib.vho_touched.argtypes = [P, P, C.c_int64]
+    lib.vho_export.restype = C.c_int64
+    lib.vho_export.argtypes = [P, P, P, P, P, P, C.c_int64]
+    lib.vho_mc.restype = C.c_int64
+    lib.vho_mc.argtypes = [P, P, P, C.c_int64]
+    lib.vho_mc_cube.restype = C.c_int
+    lib.vho_mc_cube.argtypes = [P, P]
+    lib.vho_tri_case.restype = C.c_char_p
+    lib.vho_tri_case.argtypes = [C.c_int]
+    lib.vho_query.argtypes = [P, P, C.c_int64, P, P, P, P]
+    lib.vho_query64.argtypes = [P, P, C.c_int64, P, P, P]
+    lib.vho_band.restype = C.c_int
+    lib.vho_band.argtypes = [P, P, C.c_int, C.c_int, C.c_float, P, C.c_int]
+    lib.vho_f32_to_f16.restype = C.c_uint16
+    lib.vho_f32_to_f16.argtypes = [C.c_float]
+    lib.vho_f16_to_f32.restype = C.c_float
+    lib.vho_f16_to_f32.argtypes = [C.c_uint16]
+    lib.vho_vertex_interp.argtypes = [P, P, C.c_float, C.c_float, P]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def sort_blocks(keys, *arrays):
+    """Sorts blocks by (x, y, z); returns the sorted keys and arrays."""
+    order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+    return (keys[order],) + tuple(a[order] for a in arrays)
+
+
+class Oracle:
+    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0):
+        lib = load()
+        if threads:
+            lib.vho_set_threads(threads)
+        st = w["storage"] if storage is None else storage
+        cap = w["weight_cap"] if weight_cap is None else weight_cap
+        self.w = dict(w)
+        self._h = lib.vho_create(w["width"], w["height"], w["fx"], w["fy"], w["cx"], w["cy"], w["max_depth"],
+                                 w["min_depth"], w["vox_size"], w["trunc"], st, cap)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.vho_destroy(self._h)
+            self._h = None
+
+    def integrate(self, depth_u16: np.ndarray, c2w, scale=0.001) -> int:
+        d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
+        T = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        return int(_lib.vho_integrate(self._h, _p(d), scale, _p(T)))
+
+    def num_blocks(self) -> int:
+        return int(_lib.vho_num_blocks(self._h))
+
+    def touched(self) -> np.ndarray:
+        n = _lib.vho_touched(self._h, None, 0)
+        out = np.zeros((n, 3), np.int32)
+        _lib.vho_touched(self._h, _p(out), n)
+        return out
+
+    def export(self):
+        """Sorted (keys, tsdf, w, tsdf64, w64)."""
+        n = self.num_blocks()
+        keys = np.zeros((n, 3), np.int32)
+        s = np.zeros((n, 512), np.float32)
+        w = np.zeros((n, 512), np.float32)
+        s64 = np.zeros((n, 512), np.float64)
+        w64 = np.zeros((n, 512), np.float64)
+        _lib.vho_export(self._h, _p(keys), _p(s), _p(w), _p(s64), _p(w64), n)
+        return sort_blocks(keys, s, w, s64, w64)
+
+    def mc(self):
+        n = _lib.vho_mc(self._h, None, None, 0)
+        xyz = np.zeros((n, 9), np.float32)
+        cubes = np.zeros((n, 4), np.int32)
+        _lib.vho_mc(self._h, _p(xyz), _p(cubes), n)
+        return xyz, cubes
+
+    def query(self, pts):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        n = pts.shape[0]
+        s32 = np.zeros(n, np.float32)
+        s64 = np.zeros(n, np.float64)
+        g64 = np.zeros((n, 3), np.float64)
+        valid = np.zeros(n, np.uint8)
+        _lib.vho_query(self._h, _p(pts), n, _p(s32), _p(s64), _p(g64), _p(valid))
+        return s32, s64, g64, valid.astype(bool)
+
+    def query64(self, pts):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        s = np.zeros(n, np.float64)
+        g = np.zeros((n, 3), np.float64)
+        valid = np.zeros(n, np.uint8)
+        _lib.vho_query64(self._h, _p(pts), n, _p(s), _p(g), _p(valid))
+        return s, g, valid.astype(bool)
+
+    def band(self, c2w, u, v, d):
+        T = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        out = np.zeros((64, 3), np.int32)
+        n = _lib.vho_band(self._h, _p(T), u, v, d, _p(out), 64)
+        return out[: min(n, 64)]
+
+
+def mc_cube(vals):
+    load()
+    v = np.ascontiguousarray(vals, np.float32)
+    out = np.zeros((5, 9), np.float32)
+    n = _lib.vho_mc_cube(_p(v), _p(out))
+    return out[:n]
+
+
+def tri_case(ci: int) -> str:
+    return load().vho_tri_case(ci).decode()
+
+
+def f32_to_f16_bits(x: float) -> int:
+    return int(load().vho_f32_to_f16(x))
+
+
+def f16_bits_to_f32(h: int) -> float:
+    return float(load().vho_f16_to_f32(h))
+
+
+def vertex_interp(a, b, va, vb):
+    load()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.zeros(3, np.float32)
+    _lib.vho_vertex_interp(_p(a), _p(b), va, vb, _p(out))
+    return out

@@ -1,0 +1,551 @@
+// vh_oracle.cpp - CPU restatement of the reference's voxel-hashing TSDF path.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker for libvh: it is loaded by the
+// tests, by the smoke check and by the CPU-baseline leg of the benchmark, and by nothing
+// that ships. It shares no source with libvh; it restates the algorithm from the
+// reference (citations below) with the documented corrections of the reference's defects,
+// evaluating every FP32 operation in the same order as the GPU kernels so that an FP32
+// (or FP16-storage) twin reproduces the GPU bit-for-bit, and carrying an FP64 shadow of
+// the voxel values for accuracy measurements.
+//
+// Reference behaviour restated here:
+//   lattice / block / in-block order          src/tsdf.cu:24-64
+//   world<->camera transforms, projection      src/tsdf.cu:66-116 (base2cam, cam2base, cam2frame)
+//   integration skip rules, truncation         src/tsdf.cu:599-751 (IntegrateHashKernel)
+//   running average with the old weight        src/tsdf.cu:853-858 (dense Integrate: the intent)
+//   block selection along the pixel ray        src/tsdf.cu:2088-2238 (re-specified: truncation band)
+//   marching cubes corner order, sign rule     include/tsdf.cuh:234-241, src/tsdf.cu:978-1061
+//   vertex interpolation with 1e-5 snaps       src/tsdf.cu:1640-1660
+//   Bourke triangle table                      include/tsdf.cuh:278-534
+// Functions without a reference counterpart (trilinear query, its gradient, binary16
+// rounding) are restated from their textbook definitions.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ------------------------------------------------------------------ Bourke triTable
+// One string per cube case: the triangle list as hex edge indices (3 per triangle).
+const char* const kTri[256] = {
+#include "mc_tri_strings.inc"
+};
+// corner offsets in the reference's order (include/tsdf.cuh:234-241)
+const int kCorner[8][3] = {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0},
+                           {0, 0, 1}, {0, 1, 1}, {1, 1, 1}, {1, 0, 1}};
+// edge -> corner pair, as in the VertexInterp calls of src/tsdf.cu:1001-1037
+const int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                          {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+int hexval(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
+
+// ------------------------------------------------------------------ binary16 (RNE)
+uint16_t to_half(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t sign = (u >> 31) << 15;
+  const int exp = (int)((u >> 23) & 0xff);
+  uint32_t man = u & 0x7fffff;
+  if (exp == 0xff) return (uint16_t)(sign | 0x7c00 | (man ? 0x200 : 0));
+  // value = 1.man * 2^(exp-127); half keeps 10 fraction bits at exponent >= -14
+  int e = exp - 127;
+  if (exp == 0) return (uint16_t)sign;  // float subnormals are far below half's range
+  uint64_t sig = (uint64_t)(man | 0x800000);  // 24-bit significand, value = sig * 2^(e-23)
+  int shift;                                  // bits to drop to reach half's last place
+  uint32_t he;
+  if (e >= -14) {
+    shift = 13;
+    he = (uint32_t)(e + 15);
+  } else {
+    shift = 13 + (-14 - e);
+    he = 0;
+  }
+  if (shift > 40) return (uint16_t)sign;
+  uint64_t q = sig >> shift;
+  const uint64_t rem = sig & ((1ull << shift) - 1);
+  const uint64_t halfway = 1ull << (shift - 1);
+  if (rem > halfway || (rem == halfway && (q & 1))) ++q;
+  uint32_t bits;
+  if (he == 0) {
+    bits = (uint32_t)q;  // subnormal; a carry into 0x400 is the smallest normal, correctly
+  } else {
+    // q has 11 significant bits (1.xxx); a carry to 0x800 bumps the exponent
+    if (q == 0x800) {
+      q = 0x400;
+      ++he;
+    }
+    if (he >= 31) return (uint16_t)(sign | 0x7c00);
+    bits = (he << 10) | (uint32_t)(q & 0x3ff);
+  }
+  return (uint16_t)(sign | bits);
+}
+
+float from_half(uint16_t h) {
+  const int sign = (h >> 15) & 1, e = (h >> 10) & 31, m = h & 0x3ff;
+  double v;
+  if (e == 0) v = std::ldexp((double)m, -24);
+  else if (e == 31) v = m ? NAN : INFINITY;
+  else v = std::ldexp((double)(m | 0x400), e - 25);
+  return (float)(sign ? -v : v);
+}
+
+// ------------------------------------------------------------------ keys
+inline int64_t key_of(int x, int y, int z) {
+  return ((int64_t)(x + (1 << 20)) << 42) | ((int64_t)(y + (1 << 20)) << 21) | (int64_t)(z + (1 << 20));
+}
+inline void key_xyz(int64_t k, int& x, int& y, int& z) {
+  x = (int)((k >> 42) & 0x1fffff) - (1 << 20);
+  y = (int)((k >> 21) & 0x1fffff) - (1 << 20);
+  z = (int)(k & 0x1fffff) - (1 << 20);
+}
+inline int floor_div8(int v) { return v >= 0 ? v / 8 : -((-v + 7) / 8); }
+
+struct Oracle {
+  int W, H;
+  float fx, fy, cx, cy, max_depth, min_depth, vox, mu;
+  int storage;  // 0: FP32 twin, 1: FP16 twin
+  int cap;      // 0 = uncapped
+  std::unordered_map<int64_t, int64_t> index;  // key -> block
+  std::vector<int64_t> keys;                   // block -> key
+  std::vector<float> s32, w32;                 // FP32 twin
+  std::vector<uint16_t> s16, w16;              // FP16 twin
+  std::vector<double> s64, w64;                // FP64 shadow
+  std::vector<int64_t> touched;                // last frame, sorted keys
+  int64_t frames = 0;
+
+  int64_t add_block(int64_t k) {
+    const int64_t b = (int64_t)keys.size();
+    keys.push_back(k);
+    index.emplace(k, b);
+    s32.resize(s32.size() + 512, 0.0f);
+    w32.resize(w32.size() + 512, 0.0f);
+    s16.resize(s16.size() + 512, 0);
+    w16.resize(w16.size() + 512, 0);
+    s64.resize(s64.size() + 512, 0.0);
+    w64.resize(w64.size() + 512, 0.0);
+    return b;
+  }
+  int64_t find(int64_t k) const {
+    auto it = index.find(k);
+    return it == index.end() ? -1 : it->second;
+  }
+  // twin voxel value (FP32 or widened FP16) and weight
+  void twin(int64_t b, int i, float& s, float& w) const {
+    const size_t v = (size_t)b * 512 + i;
+    if (storage == 1) {
+      s = from_half(s16[v]);
+      w = (float)w16[v];
+    } else {
+      s = s32[v];
+      w = w32[v];
+    }
+  }
+};
+
+// ------------------------------------------------------------------ band traversal
+// Blocks crossed by the segment of pixel (u,v) between camera depths max(d-mu, min_depth)
+// and d+mu. Camera point: ((u-cx)*t/fx, (v-cy)*t/fy, t) as frame2cam (src/tsdf.cu:67-73);
+// world point R p + t as cam2base (src/tsdf.cu:95-105); block coordinates world / (8 vox).
+// Walk: from floor(start) to floor(end), stepping the axis whose next face is crossed
+// first among the axes that still have to move (lowest axis on ties).
+template <class Emit>
+void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit emit) {
+  float t0 = d - o.mu;
+  if (t0 < o.min_depth) t0 = o.min_depth;
+  const float t1 = d + o.mu;
+  const float bsz = o.vox * 8.0f;
+  float cam[2][3];
+  const float ts[2] = {t0, t1};
+  for (int e = 0; e < 2; ++e) {
+    cam[e][0] = ((float)u - o.cx) * ts[e] / o.fx;
+    cam[e][1] = ((float)v - o.cy) * ts[e] / o.fy;
+    cam[e][2] = ts[e];
+  }
+  float p[2][3];
+  for (int e = 0; e < 2; ++e)
+    for (int r = 0; r < 3; ++r)
+      p[e][r] = (cam[e][0] * T[4 * r] + cam[e][1] * T[4 * r + 1] + cam[e][2] * T[4 * r + 2] + T[4 * r + 3]) / bsz;
+  for (int e = 0; e < 2; ++e)
+    for (int r = 0; r < 3; ++r)
+      if (!(std::fabs(p[e][r]) < 1048000.0f)) return;
+  int cur[3], end[3];
+  float dir[3];
+  for (int r = 0; r < 3; ++r) {
+    cur[r] = (int)std::floor(p[0][r]);
+    end[r] = (int)std::floor(p[1][r]);
+    dir[r] = p[1][r] - p[0][r];
+  }
+  emit(cur[0], cur[1], cur[2]);
+  while (cur[0] != end[0] || cur[1] != end[1] || cur[2] != end[2]) {
+    int best = -1;
+    float bt = 0.0f;
+    for (int r = 0; r < 3; ++r) {
+      if (cur[r] == end[r]) continue;
+      const float face = (float)(end[r] > cur[r] ? cur[r] + 1 : cur[r]);
+      const float t = (face - p[0][r]) / dir[r];
+      if (best < 0 || t < bt) {
+        best = r;
+        bt = t;
+      }
+    }
+    cur[best] += end[best] > cur[best] ? 1 : -1;
+    emit(cur[0], cur[1], cur[2]);
+  }
+}
+
+// ------------------------------------------------------------------ integration
+void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
+  const int W = o.W, H = o.H;
+  std::vector<float> d((size_t)W * H);
+  for (size_t i = 0; i < d.size(); ++i) d[i] = (float)dep[i] * scale;
+
+  // 1. touched set
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  std::vector<std::vector<int64_t>> local((size_t)nth);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int v = 0; v < H; ++v) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    auto& out = local[(size_t)tid];
+    for (int u = 0; u < W; ++u) {
+      const float dv = d[(size_t)v * W + u];
+      if (!(dv > 0.0f && dv <= o.max_depth)) continue;
+      band_blocks(o, T, u, v, dv, [&](int x, int y, int z) { out.push_back(key_of(x, y, z)); });
+    }
+  }
+  std::vector<int64_t> t;
+  for (auto& l : local) t.insert(t.end(), l.begin(), l.end());
+  std::sort(t.begin(), t.end());
+  t.erase(std::unique(t.begin(), t.end()), t.end());
+  o.touched = t;
+  std::vector<int64_t> blk(t.size());
+  for (size_t k = 0; k < t.size(); ++k) {
+    int64_t b = o.find(t[k]);
+    blk[k] = b >= 0 ? b : o.add_block(t[k]);
+  }
+
+  // 2. voxels of the touched blocks
+  const double T64[12] = {T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11]};
+  const double vox64 = o.vox, mu64 = o.mu, scale64 = scale;
+  const float capf = (float)o.cap;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long k = 0; k < (long)t.size(); ++k) {
+    int bx, by, bz;
+    key_xyz(t[(size_t)k], bx, by, bz);
+    const size_t base = (size_t)blk[(size_t)k] * 512;
+    for (int i = 0; i < 512; ++i) {
+      const int gx = bx * 8 + (i >> 6), gy = by * 8 + ((i >> 3) & 7), gz = bz * 8 + (i & 7);
+      // FP32 geometry: decides observation, pixel and the FP32 distance
+      const float px = (float)gx * o.vox, py = (float)gy * o.vox, pz = (float)gz * o.vox;
+      const float q0 = px - T[3], q1 = py - T[7], q2 = pz - T[11];
+      const float xc = T[0] * q0 + T[4] * q1 + T[8] * q2;
+      const float yc = T[1] * q0 + T[5] * q1 + T[9] * q2;
+      const float zc = T[2] * q0 + T[6] * q1 + T[10] * q2;
+      if (!(zc > 0.0f)) continue;
+      const float uf = std::round(o.fx * (xc / zc) + o.cx);
+      const float vf = std::round(o.fy * (yc / zc) + o.cy);
+      if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
+      const size_t pix = (size_t)vf * W + (size_t)uf;
+      const float dv = d[pix];
+      if (!(dv > 0.0f && dv <= o.max_depth)) continue;
+      const float diff = dv - zc;
+      if (diff <= -o.mu) continue;
+      const float dist = std::fmin(1.0f, diff / o.mu);
+      // twin update: mean with the old weight, then the cap
+      if (o.storage == 1) {
+        float s = from_half(o.s16[base + i]), w = (float)o.w16[base + i];
+        const float wn = w + 1.0f;
+        s = (s * w + dist) / wn;
+        w = (capf > 0.0f && wn > capf) ? capf : wn;
+        o.s16[base + i] = to_half(s);
+        o.w16[base + i] = (uint16_t)w;
+      } else {
+        float s = o.s32[base + i], w = o.w32[base + i];
+        const float wn = w + 1.0f;
+        s = (s * w + dist) / wn;
+        w = (capf > 0.0f && wn > capf) ? capf : wn;
+        o.s32[base + i] = s;
+        o.w32[base + i] = w;
+      }
+      // FP64 shadow: same observation, distance and mean in double
+      const double X = gx * vox64 - T64[3], Y = gy * vox64 - T64[7], Z = gz * vox64 - T64[11];
+      const double z64 = T64[2] * X + T64[6] * Y + T64[10] * Z;
+      const double d64 = (double)dep[pix] * scale64;
+      const double dist64 = std::min(1.0, (d64 - z64) / mu64);
+      double s = o.s64[base + i], w = o.w64[base + i];
+      const double wn = w + 1.0;
+      s = (s * w + dist64) / wn;
+      w = (o.cap > 0 && wn > o.cap) ? (double)o.cap : wn;
+      o.s64[base + i] = s;
+      o.w64[base + i] = w;
+    }
+  }
+  ++o.frames;
+}
+
+// ------------------------------------------------------------------ marching cubes
+struct Tri {
+  int32_t c[4];
+  float p[9];
+};
+
+// VertexInterp(iso = 0) in voxel units (src/tsdf.cu:1640-1660); |x| < 1e-5 compares the
+// float magnitude against the double constant, as the reference does.
+void interp(const float a[3], const float b[3], float va, float vb, float out[3]) {
+  if ((double)std::fabs(va) < 0.00001) { std::memcpy(out, a, 12); return; }
+  if ((double)std::fabs(vb) < 0.00001) { std::memcpy(out, b, 12); return; }
+  if ((double)std::fabs(va - vb) < 0.00001) { std::memcpy(out, a, 12); return; }
+  const float m = (0.0f - va) / (vb - va);
+  for (int k = 0; k < 3; ++k) out[k] = a[k] + m * (b[k] - a[k]);
+}
+
+void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& out) {
+  float val[8];
+  for (int c = 0; c < 8; ++c) {
+    const int x = gx + kCorner[c][0], y = gy + kCorner[c][1], z = gz + kCorner[c][2];
+    const int64_t b = o.find(key_of(floor_div8(x), floor_div8(y), floor_div8(z)));
+    if (b < 0) return;
+    const int i = ((x - floor_div8(x) * 8) * 8 + (y - floor_div8(y) * 8)) * 8 + (z - floor_div8(z) * 8);
+    float s, w;
+    o.twin(b, i, s, w);
+    if (!(w > 0.0f)) return;  // unobserved corner: no surface (corrects the reference, which has no test)
+    val[c] = s;
+  }
+  int ci = 0;
+  for (int c = 0; c < 8; ++c)
+    if (val[c] < 0.0f) ci |= 1 << c;
+  const char* tl = kTri[ci];
+  const int n = (int)std::strlen(tl) / 3;
+  for (int t = 0; t < n; ++t) {
+    Tri tr;
+    for (int j = 0; j < 3; ++j) {
+      const int e = hexval(tl[3 * t + j]);
+      const int ca = kEdge[e][0], cb = kEdge[e][1];
+      const float pa[3] = {(float)(gx + kCorner[ca][0]), (float)(gy + kCorner[ca][1]), (float)(gz + kCorner[ca][2])};
+      const float pb[3] = {(float)(gx + kCorner[cb][0]), (float)(gy + kCorner[cb][1]), (float)(gz + kCorner[cb][2])};
+      interp(pa, pb, val[ca], val[cb], tr.p + 3 * j);
+    }
+    auto same = [&](int a, int b) {
+      return tr.p[3 * a] == tr.p[3 * b] && tr.p[3 * a + 1] == tr.p[3 * b + 1] && tr.p[3 * a + 2] == tr.p[3 * b + 2];
+    };
+    if (same(0, 1) || same(1, 2) || same(2, 0)) continue;  // degenerate (typo-free version of :1055-1058)
+    for (int k = 0; k < 9; ++k) tr.p[k] *= o.vox;
+    tr.c[0] = gx; tr.c[1] = gy; tr.c[2] = gz; tr.c[3] = (ci << 8) | t;
+    out.push_back(tr);
+  }
+}
+
+// ------------------------------------------------------------------ trilinear query
+template <class R>
+bool sample(const Oracle& o, R qx, R qy, R qz, R& sdf, R g[3]) {
+  const R vox = (R)o.vox;
+  const R gq[3] = {qx / vox, qy / vox, qz / vox};
+  int ix[3];
+  R f[3];
+  for (int k = 0; k < 3; ++k) {
+    if (!(std::fabs(gq[k]) < (R)8.0e6)) return false;
+    const R fl = std::floor(gq[k]);
+    ix[k] = (int)fl;
+    f[k] = gq[k] - fl;
+  }
+  R s[8];
+  for (int k = 0; k < 8; ++k) {
+    const int x = ix[0] + (k >> 2), y = ix[1] + ((k >> 1) & 1), z = ix[2] + (k & 1);
+    const int64_t b = o.find(key_of(floor_div8(x), floor_div8(y), floor_div8(z)));
+    if (b < 0) return false;
+    const int i = ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
+    float sv, wv;
+    o.twin(b, i, sv, wv);
+    if (!(wv > 0.0f)) return false;
+    s[k] = (R)sv;
+  }
+  const R ax[2] = {(R)1 - f[0], f[0]}, ay[2] = {(R)1 - f[1], f[1]}, az[2] = {(R)1 - f[2], f[2]};
+  R acc = 0, gx = 0, gy = 0, gz = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+    const R w = (ax[dx] * ay[dy]) * az[dz];
+    acc = acc + w * s[k];
+  }
+  for (int k = 0; k < 8; ++k) {
+    const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+    const R sx = dx ? (R)1 : (R)-1, sy = dy ? (R)1 : (R)-1, sz = dz ? (R)1 : (R)-1;
+    gx = gx + ((sx * ay[dy]) * az[dz]) * s[k];
+    gy = gy + ((ax[dx] * sy) * az[dz]) * s[k];
+    gz = gz + ((ax[dx] * ay[dy]) * sz) * s[k];
+  }
+  sdf = acc;
+  g[0] = gx / vox;
+  g[1] = gy / vox;
+  g[2] = gz / vox;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* vho_create(int W, int H, float fx, float fy, float cx, float cy, float max_depth, float min_depth, float vox,
+                 float mu, int storage, int cap) {
+  Oracle* o = new Oracle();
+  o->W = W; o->H = H; o->fx = fx; o->fy = fy; o->cx = cx; o->cy = cy;
+  o->max_depth = max_depth; o->min_depth = min_depth; o->vox = vox; o->mu = mu;
+  o->storage = storage; o->cap = cap;
+  return o;
+}
+void vho_destroy(void* p) { delete static_cast<Oracle*>(p); }
+void vho_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+int vho_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+int64_t vho_integrate(void* p, const uint16_t* depth, float scale, const float* c2w) {
+  Oracle* o = static_cast<Oracle*>(p);
+  integrate(*o, depth, scale, c2w);
+  return (int64_t)o->touched.size();
+}
+int64_t vho_num_blocks(void* p) { return (int64_t)static_cast<Oracle*>(p)->keys.size(); }
+
+// Last frame's touched blocks, sorted by packed key; returns the count.
+int64_t vho_touched(void* p, int32_t* xyz, int64_t cap) {
+  Oracle* o = static_cast<Oracle*>(p);
+  const int64_t n = (int64_t)o->touched.size();
+  if (xyz && cap >= n)
+    for (int64_t k = 0; k < n; ++k) key_xyz(o->touched[(size_t)k], xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]);
+  return n;
+}
+
+// All blocks in insertion order: keys, twin values (FP16 widened) and the FP64 shadow.
+int64_t vho_export(void* p, int32_t* xyz, float* s, float* w, double* s64, double* w64, int64_t cap) {
+  Oracle* o = static_cast<Oracle*>(p);
+  const int64_t n = (int64_t)o->keys.size();
+  if (cap < n) return n;
+  for (int64_t b = 0; b < n; ++b) {
+    if (xyz) key_xyz(o->keys[(size_t)b], xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]);
+    for (int i = 0; i < 512; ++i) {
+      const size_t v = (size_t)b * 512 + i;
+      float sv, wv;
+      o->twin(b, i, sv, wv);
+      if (s) s[v] = sv;
+      if (w) w[v] = wv;
+      if (s64) s64[v] = o->s64[v];
+      if (w64) w64[v] = o->w64[v];
+    }
+  }
+  return n;
+}
+
+// Marching cubes over every block; triangles ordered by cube then by table position.
+int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) {
+  Oracle* o = static_cast<Oracle*>(p);
+  std::vector<int64_t> order(o->keys.begin(), o->keys.end());
+  std::sort(order.begin(), order.end());
+  std::vector<std::vector<Tri>> per((size_t)order.size());
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long k = 0; k < (long)order.size(); ++k) {
+    int bx, by, bz;
+    key_xyz(order[(size_t)k], bx, by, bz);
+    for (int i = 0; i < 512; ++i)
+      cube_triangles(*o, bx * 8 + (i >> 6), by * 8 + ((i >> 3) & 7), bz * 8 + (i & 7), per[(size_t)k]);
+  }
+  int64_t n = 0;
+  for (auto& v : per) n += (int64_t)v.size();
+  if (cap < n) return n;
+  int64_t m = 0;
+  for (auto& v : per)
+    for (const Tri& t : v) {
+      if (xyz) std::memcpy(xyz + 9 * m, t.p, 36);
+      if (cubes) std::memcpy(cubes + 4 * m, t.c, 16);
+      ++m;
+    }
+  return n;
+}
+
+// Triangles of one cube from 8 given corner values (all observed), lattice origin (0,0,0),
+// vox = 1: for exhaustive checks of the 256 cases.
+int vho_mc_cube(const float* val, float* xyz) {
+  int ci = 0;
+  for (int c = 0; c < 8; ++c)
+    if (val[c] < 0.0f) ci |= 1 << c;
+  const char* tl = kTri[ci];
+  const int n = (int)std::strlen(tl) / 3;
+  for (int t = 0; t < n; ++t)
+    for (int j = 0; j < 3; ++j) {
+      const int e = hexval(tl[3 * t + j]);
+      const int ca = kEdge[e][0], cb = kEdge[e][1];
+      const float pa[3] = {(float)kCorner[ca][0], (float)kCorner[ca][1], (float)kCorner[ca][2]};
+      const float pb[3] = {(float)kCorner[cb][0], (float)kCorner[cb][1], (float)kCorner[cb][2]};
+      interp(pa, pb, val[ca], val[cb], xyz + 9 * t + 3 * j);
+    }
+  return n;
+}
+// The triangle table in this file's form (edge digits), for cross-checks.
+const char* vho_tri_case(int ci) { return (ci >= 0 && ci < 256) ? kTri[ci] : ""; }
+
+// Trilinear samples: FP32 twin and FP64 evaluation on the same stored values.
+void vho_query(void* p, const float* pts, int64_t n, float* sdf32, double* sdf64, double* grad64, uint8_t* valid) {
+  Oracle* o = static_cast<Oracle*>(p);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    float s = 0.0f, g[3];
+    double s6 = 0.0, g6[3] = {0, 0, 0};
+    const bool ok = sample<float>(*o, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], s, g);
+    if (ok) sample<double>(*o, (double)pts[3 * i], (double)pts[3 * i + 1], (double)pts[3 * i + 2], s6, g6);
+    if (sdf32) sdf32[i] = ok ? s : 0.0f;
+    if (sdf64) sdf64[i] = ok ? s6 : 0.0;
+    if (grad64)
+      for (int k = 0; k < 3; ++k) grad64[3 * i + k] = ok ? g6[k] : 0.0;
+    if (valid) valid[i] = ok ? 1 : 0;
+  }
+}
+// FP64 trilinear at double-precision world points (for finite differences).
+void vho_query64(void* p, const double* pts, int64_t n, double* sdf, double* grad, uint8_t* valid) {
+  Oracle* o = static_cast<Oracle*>(p);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double s = 0.0, g[3] = {0, 0, 0};
+    const bool ok = sample<double>(*o, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], s, g);
+    if (sdf) sdf[i] = ok ? s : 0.0;
+    if (grad)
+      for (int k = 0; k < 3; ++k) grad[3 * i + k] = ok ? g[k] : 0.0;
+    if (valid) valid[i] = ok ? 1 : 0;
+  }
+}
+
+// Blocks crossed by one pixel's truncation band, in walk order; returns the count.
+int vho_band(void* p, const float* c2w, int u, int v, float d, int32_t* xyz, int cap) {
+  Oracle* o = static_cast<Oracle*>(p);
+  int n = 0;
+  band_blocks(*o, c2w, u, v, d, [&](int x, int y, int z) {
+    if (n < cap) { xyz[3 * n] = x; xyz[3 * n + 1] = y; xyz[3 * n + 2] = z; }
+    ++n;
+  });
+  return n;
+}
+
+uint16_t vho_f32_to_f16(float f) { return to_half(f); }
+float vho_f16_to_f32(uint16_t h) { return from_half(h); }
+void vho_vertex_interp(const float* a, const float* b, float va, float vb, float* out) { interp(a, b, va, vb, out); }
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+"""GPU parity of allocation, integration and marching cubes against the CPU oracle.
+
+Bar: the allocated block set, per-frame touched counts, FP32 (and FP16-storage) voxel
+values and the marching-cubes triangles are bit-identical to the oracle's twin; the FP32
+values stay within 5e-5 of the oracle's FP64 shadow (normalised TSDF units).
+"""
+import numpy as np
+import pytest
+
+from conftest import look_down_z, plane_depth
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(vh, oracle, w, depth, poses, **over):
+    cfg = vh.config_from_workload(w, **over)
+    vol = vh.Volume(cfg)
+    ref = oracle.Oracle(w, storage=cfg.storage, weight_cap=cfg.weight_cap)
+    touched = []
+    for k in range(len(depth)):
+        vol.integrate(depth[k], poses[k])
+        n_ref = ref.integrate(depth[k], poses[k])
+        touched.append((vol.stats()["touched_last"], n_ref))
+    return vol, ref, touched
+
+
+def gpu_sorted(oracle, vol):
+    keys, s, wt = vol.export_blocks()
+    return oracle.sort_blocks(keys, s, wt)
+
+
+def assert_same_volume(oracle, vol, ref):
+    keys, s, wt = gpu_sorted(oracle, vol)
+    rk, rs, rw, r64, rw64 = ref.export()
+    assert keys.shape == rk.shape and np.array_equal(keys, rk), "allocated block sets differ"
+    assert np.array_equal(wt, rw), "weights differ"
+    bad = s.view(np.uint32) != rs.view(np.uint32)
+    assert not bad.any(), f"{bad.sum()} tsdf values differ, max {np.abs(s - rs).max()}"
+    return keys, s, wt, r64
+
+
+def assert_same_mesh(vol, ref):
+    xyz, cubes = vol.extract_mesh()
+    rxyz, rcubes = ref.mc()
+    assert len(xyz) == len(rxyz)
+    o = np.lexsort(cubes.T[::-1])
+    ro = np.lexsort(rcubes.T[::-1])
+    assert np.array_equal(cubes[o], rcubes[ro]), "triangle topology differs"
+    assert np.array_equal(xyz[o].view(np.uint32), rxyz[ro].view(np.uint32)), "vertex positions differ"
+    return len(xyz)
+
+
+def test_room_small_fp32_bit_exact(vh, oracle, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    print(f"room_small_1cm: {len(depth)} frames FP32", flush=True)
+    vol, ref, touched = run_pair(vh, oracle, w, depth, poses)
+    assert all(a == b for a, b in touched), touched
+    keys, s, wt, r64 = assert_same_volume(oracle, vol, ref)
+    m = wt > 0
+    err = np.abs(s[m] - r64[m])
+    print(f"blocks {len(keys)}, FP32 vs FP64 max {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}")
+    assert err.max() < 5e-5
+    n = assert_same_mesh(vol, ref)
+    assert n > 1000
+    st = vol.stats()
+    assert st["frames"] == len(depth) and st["blocks_allocated"] == len(keys)
+    assert st["touched_total"] == sum(a for a, _ in touched)
+    vol.close()
+
+
+def test_room_small_fp16_bit_exact(vh, oracle, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    print("room_small_1cm: 40 frames FP16 storage, weight cap 255", flush=True)
+    vol, ref, touched = run_pair(vh, oracle, w, depth, poses, storage=1, weight_cap=255)
+    assert all(a == b for a, b in touched)
+    keys, s, wt, r64 = assert_same_volume(oracle, vol, ref)
+    m = wt > 0
+    err = np.abs(s[m] - r64[m])
+    print(f"FP16 vs FP64 max {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}")
+    assert err.max() < 2e-3
+    assert_same_mesh(vol, ref)
+    vol.close()
+
+
+def test_device_sequence_graph_matches_host_path(vh, oracle, wl, small_frames):
+    import torch
+
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    n = 12
+    print("room_small_1cm: 12 frames through bind_sequence + graph launches", flush=True)
+    d_depth = torch.from_numpy(depth[:n].view(np.int16).copy()).cuda()
+    d_pose = torch.from_numpy(poses[:n].copy()).cuda()
+    torch.cuda.synchronize()
+    vol = vh.Volume(vh.config_from_workload(w))
+    vol.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, n)
+    vol.integrate_next(n)
+    vol.sync()
+    ref = oracle.Oracle(w)
+    for k in range(n):
+        ref.integrate(depth[k], poses[k])
+    assert vol.stats()["frames"] == n
+    assert_same_volume(oracle, vol, ref)
+    # profiling mode (eager launches with events) continues the same sequence identically
+    vol2 = vh.Volume(vh.config_from_workload(w))
+    vol2.set_profiling(True)
+    vol2.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, n)
+    vol2.integrate_next(n)
+    vol2.sync()
+    st = vol2.stats()
+    assert st["profiled_frames"] == n and st["integrate_ms"] > 0 and st["alloc_ms"] > 0
+    assert_same_volume(oracle, vol2, ref)
+    vol.close()
+    vol2.close()
+
+
+def test_float_depth_input_matches_u16(vh, oracle, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    vol = vh.Volume(vh.config_from_workload(w))
+    for k in range(4):
+        vol.integrate(depth[k].astype(np.float32) * np.float32(0.001), poses[k])
+    ref = oracle.Oracle(w)
+    for k in range(4):
+        ref.integrate(depth[k], poses[k])
+    assert_same_volume(oracle, vol, ref)
+    vol.close()
+
+
+def test_pitched_depth_rows(vh, oracle, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    padded = np.zeros((480, 700), np.uint16)
+    padded[:, :640] = depth[0]
+    vol = vh.Volume(vh.config_from_workload(w))
+    vol.integrate(padded[:, :640], poses[0])  # non-contiguous view: pitch 1400 bytes
+    ref = oracle.Oracle(w)
+    ref.integrate(depth[0], poses[0])
+    assert_same_volume(oracle, vol, ref)
+    vol.close()
+
+
+def test_plane_closed_form_on_gpu(vh, oracle, wl):
+    w = wl.WORKLOADS["room_small_1cm"]
+    vol = vh.Volume(vh.config_from_workload(w))
+    ref = oracle.Oracle(w)
+    T = look_down_z(0.013, -0.021, 0.0)
+    for D in (1500, 1500, 1512):
+        vol.integrate(plane_depth(w, D), T)
+        ref.integrate(plane_depth(w, D), T)
+    assert_same_volume(oracle, vol, ref)
+    assert_same_mesh(vol, ref)
+    vol.close()
+
+
+def test_empty_and_out_of_range_frames(vh, wl):
+    w = wl.WORKLOADS["room_small_1cm"]
+    vol = vh.Volume(vh.config_from_workload(w))
+    vol.integrate(plane_depth(w, 0), look_down_z())
+    vol.integrate(plane_depth(w, 6000), look_down_z())  # beyond max_depth
+    st = vol.stats()
+    assert st["blocks_allocated"] == 0 and st["touched_last"] == 0 and st["frames"] == 2
+    xyz, cubes = vol.extract_mesh()
+    assert len(xyz) == 0
+    with pytest.raises(vh.VhError) as e:
+        vol.integrate(plane_depth(w, 1500), np.full((4, 4), np.nan, np.float32))
+    assert e.value.code == vh.VH_ERR_INVALID_ARG
+    far = look_down_z(3.0e5, 0.0, 0.0)  # 3e5 m / 8 cm blocks: beyond the 21-bit key range
+    with pytest.raises(vh.VhError) as e:
+        vol.integrate(plane_depth(w, 1500), far)
+    assert e.value.code == vh.VH_ERR_RANGE
+    vol.close()
+
+
+def test_pool_full_is_reported(vh, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    vol = vh.Volume(vh.config_from_workload(w, pool_blocks=64))
+    with pytest.raises(vh.VhError) as e:
+        vol.integrate(depth[0], poses[0])
+    assert e.value.code == vh.VH_ERR_POOL_FULL
+    st = vol.stats()
+    assert st["blocks_allocated"] == 64
+    vol.close()
+
+
+def test_hash_full_is_reported(vh, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    vol = vh.Volume(vh.config_from_workload(w, log2_buckets=4))  # 64 slots in total
+    with pytest.raises(vh.VhError) as e:
+        vol.integrate(depth[0], poses[0])
+    assert e.value.code == vh.VH_ERR_HASH_FULL
+    vol.close()
+
+
+def test_reset_empties_the_volume(vh, oracle, wl, small_frames):
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    vol = vh.Volume(vh.config_from_workload(w))
+    for k in range(3):
+        vol.integrate(depth[k], poses[k])
+    vol.reset()
+    assert vol.stats()["blocks_allocated"] == 0
+    vol.integrate(depth[5], poses[5])
+    ref = oracle.Oracle(w)
+    ref.integrate(depth[5], poses[5])
+    assert_same_volume(oracle, vol, ref)
+    vol.close()

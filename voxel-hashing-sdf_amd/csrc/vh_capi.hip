@@ -1,0 +1,777 @@
+// vh_capi.hip - extern "C" entry points of libvh (see include/vh.h).
+//
+// One volume owns one HIP stream (default priority) and all of its device memory. The
+// per-frame pipeline (k_alloc -> k_integrate) is captured once into a HIP graph per input
+// kind (host-staged frames, bound device sequence); its kernels read the frame index and
+// counters from device memory, so a launch needs no host-side parameter updates.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vh.h"
+#include "vh_internal.h"
+#include "vh_math.h"
+
+using namespace vh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define VH_HIP(call)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(VH_ERR_DEVICE, std::string(#call " failed: ") + hipGetErrorString(e_));      \
+  } while (0)
+
+constexpr int kIntegrateGrid = 1024;  // persistent integration workgroups (4 per CU)
+constexpr int kMcGrid = 2048;
+constexpr int kGnGrid = 512;
+
+}  // namespace
+
+struct vh_volume {
+  vh_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t n_slots = 0;
+  // device tables
+  unsigned long long* keys = nullptr;
+  unsigned long long* vals = nullptr;
+  uint32_t* stamps = nullptr;
+  unsigned long long* block_keys = nullptr;
+  void* voxels = nullptr;
+  uint32_t* touched = nullptr;
+  FrameCtl* ctl = nullptr;
+  // staged host input (one frame, both formats) and its pose
+  void* stage_depth = nullptr;
+  float* stage_pose = nullptr;
+  // bound device sequence
+  const void* seq_depth = nullptr;
+  const float* seq_pose = nullptr;
+  int32_t seq_n = 0, seq_format = VH_DEPTH_U16;
+  float seq_scale = 0.001f;
+  // graphs: [0] staged u16, [1] staged f32, [2] bound sequence
+  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<hipEvent_t> ev_pending;  // groups of 3: start, after alloc, after integrate
+  double alloc_ms = 0.0, integrate_ms = 0.0;
+  int64_t profiled_frames = 0;
+  // marching-cubes output buffer (grown on demand)
+  float* mc_tris = nullptr;
+  int32_t* mc_cubes = nullptr;
+  int64_t mc_cap = 0;
+  // scratch for queries / Gauss-Newton
+  double* gn_partials = nullptr;
+  double* gn_pose = nullptr;
+  double* gn_stats = nullptr;
+  float* bw_pose = nullptr;
+  float graph_scale = -1.0f;  // depth scale baked into the staged uint16 graph
+  std::mutex mu;
+};
+
+namespace {
+
+int set_device(vh_volume* v) {
+  VH_HIP(hipSetDevice(v->device));
+  return VH_OK;
+}
+
+size_t voxel_bytes(int storage) { return storage == VH_STORAGE_F16 ? 4 : 8; }
+
+FrameArgs frame_args(vh_volume* v, int which) {
+  FrameArgs a{};
+  const vh_config& c = v->cfg;
+  if (which == 2) {
+    a.depth = v->seq_depth;
+    a.poses = v->seq_pose;
+    a.n_seq = (uint32_t)v->seq_n;
+    a.cursor_id = 1;
+    a.depth_scale = v->seq_format == VH_DEPTH_U16 ? v->seq_scale : 1.0f;
+  } else {
+    a.depth = v->stage_depth;
+    a.poses = v->stage_pose;
+    a.n_seq = 1;
+    a.cursor_id = 0;
+    a.depth_scale = 0.0f;  // set per frame through stage_pose[16] (see below)
+  }
+  a.W = c.width;
+  a.H = c.height;
+  a.K = Intr{c.fx, c.fy, c.cx, c.cy};
+  a.max_depth = c.max_depth;
+  a.min_depth = c.min_depth;
+  a.vox = c.vox_size;
+  a.bsz = c.vox_size * (float)kB;
+  a.mu = c.trunc;
+  a.cap = (float)c.weight_cap;
+  a.hash = HashView{v->keys, v->vals, c.log2_buckets, ((uint64_t)1 << c.log2_buckets) - 1};
+  a.stamps = v->stamps;
+  a.block_keys = v->block_keys;
+  a.pool_cap = (uint32_t)c.pool_blocks;
+  a.touched = v->touched;
+  a.voxels = v->voxels;
+  a.ctl = v->ctl;
+  return a;
+}
+
+int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale) {
+  FrameArgs a = frame_args(v, which);
+  int u16;
+  if (which == 2) {
+    u16 = v->seq_format == VH_DEPTH_U16;
+  } else {
+    u16 = which == 0;
+    a.depth_scale = u16 ? staged_scale : 1.0f;
+  }
+  VH_HIP(launch_alloc(a, u16, v->stream));
+  VH_HIP(launch_integrate(a, v->cfg.storage, u16, kIntegrateGrid, v->stream));
+  return VH_OK;
+}
+
+hipEvent_t take_event(vh_volume* v) {
+  if (!v->ev_pool.empty()) {
+    hipEvent_t e = v->ev_pool.back();
+    v->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int launch_frame(vh_volume* v, int which, float staged_scale) {
+  if (v->profiling) {
+    hipEvent_t e0 = take_event(v), e1 = take_event(v), e2 = take_event(v);
+    if (!e0 || !e1 || !e2) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
+    FrameArgs a = frame_args(v, which);
+    int u16 = which == 2 ? v->seq_format == VH_DEPTH_U16 : which == 0;
+    if (which != 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+    VH_HIP(hipEventRecord(e0, v->stream));
+    VH_HIP(launch_alloc(a, u16, v->stream));
+    VH_HIP(hipEventRecord(e1, v->stream));
+    VH_HIP(launch_integrate(a, v->cfg.storage, u16, kIntegrateGrid, v->stream));
+    VH_HIP(hipEventRecord(e2, v->stream));
+    v->ev_pending.push_back(e0);
+    v->ev_pending.push_back(e1);
+    v->ev_pending.push_back(e2);
+    return VH_OK;
+  }
+  // The staged graphs bake the depth scale into their kernel arguments; a different scale
+  // re-captures them.
+  static_assert(sizeof(float) == 4, "");
+  hipGraphExec_t& g = v->graph[which];
+  if (!g) {
+    hipGraph_t graph = nullptr;
+    VH_HIP(hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_frame_kernels(v, which, staged_scale);
+    hipError_t e = hipStreamEndCapture(v->stream, &graph);
+    if (rc != VH_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    VH_HIP(e);
+    e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    VH_HIP(e);
+  }
+  VH_HIP(hipGraphLaunch(g, v->stream));
+  return VH_OK;
+}
+
+void drop_graphs(vh_volume* v, int which_mask) {
+  for (int k = 0; k < 3; ++k)
+    if ((which_mask >> k) & 1 && v->graph[k]) {
+      (void)hipGraphExecDestroy(v->graph[k]);
+      v->graph[k] = nullptr;
+    }
+}
+
+int collect_profile(vh_volume* v) {
+  for (size_t k = 0; k + 2 < v->ev_pending.size(); k += 3) {
+    float a = 0.0f, b = 0.0f;
+    VH_HIP(hipEventElapsedTime(&a, v->ev_pending[k], v->ev_pending[k + 1]));
+    VH_HIP(hipEventElapsedTime(&b, v->ev_pending[k + 1], v->ev_pending[k + 2]));
+    v->alloc_ms += a;
+    v->integrate_ms += b;
+    v->profiled_frames += 1;
+  }
+  for (hipEvent_t e : v->ev_pending) v->ev_pool.push_back(e);
+  v->ev_pending.clear();
+  return VH_OK;
+}
+
+int read_ctl(vh_volume* v, FrameCtl* out) {
+  VH_HIP(hipMemcpyAsync(out, v->ctl, sizeof(FrameCtl), hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return VH_OK;
+}
+
+// Waits for the stream, folds profiling events, reports and clears overflow flags.
+int sync_and_check(vh_volume* v) {
+  VH_HIP(hipStreamSynchronize(v->stream));
+  int rc = collect_profile(v);
+  if (rc) return rc;
+  uint32_t flags = 0;
+  VH_HIP(hipMemcpy(&flags, &v->ctl->flags, sizeof flags, hipMemcpyDeviceToHost));
+  if (!flags) return VH_OK;
+  const uint32_t zero = 0;
+  VH_HIP(hipMemcpy(&v->ctl->flags, &zero, sizeof zero, hipMemcpyHostToDevice));
+  if (flags & kFlagPoolFull)
+    return fail(VH_ERR_POOL_FULL, "voxel-block pool is full: blocks were not integrated");
+  if (flags & kFlagHashFull)
+    return fail(VH_ERR_HASH_FULL, "hash table probe limit reached: blocks were not allocated");
+  return fail(VH_ERR_RANGE, "a truncation band left the representable block range");
+}
+
+int reset_tables(vh_volume* v) {
+  VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
+  FrameCtl c{};
+  VH_HIP(hipMemcpyAsync(&c, v->ctl, sizeof c, hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  const uint32_t frame_id = c.frame_id ? c.frame_id : 1u;
+  std::memset(&c, 0, sizeof c);
+  c.frame_id = frame_id;  // stamps and birth frames stay comparable across resets
+  VH_HIP(hipMemcpyAsync(v->ctl, &c, sizeof c, hipMemcpyHostToDevice, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return VH_OK;
+}
+
+int validate(const vh_config* c) {
+  if (c->width <= 0 || c->height <= 0 || c->width > 16384 || c->height > 16384)
+    return fail(VH_ERR_INVALID_ARG, "width/height out of range");
+  if (!(c->fx > 0.0f) || !(c->fy > 0.0f)) return fail(VH_ERR_INVALID_ARG, "fx, fy must be > 0");
+  if (!(c->vox_size > 0.0f) || !(c->trunc > 0.0f)) return fail(VH_ERR_INVALID_ARG, "vox_size, trunc must be > 0");
+  if (!(c->max_depth > 0.0f) || !(c->min_depth >= 0.0f)) return fail(VH_ERR_INVALID_ARG, "bad depth range");
+  if (c->log2_buckets < 1 || c->log2_buckets > 28) return fail(VH_ERR_INVALID_ARG, "log2_buckets must be in 1..28");
+  if (c->pool_blocks <= 0 || c->pool_blocks > (1 << 27)) return fail(VH_ERR_INVALID_ARG, "pool_blocks must be in 1..2^27");
+  if (c->storage != VH_STORAGE_F32 && c->storage != VH_STORAGE_F16) return fail(VH_ERR_INVALID_ARG, "unknown storage");
+  if (c->weight_cap < 0) return fail(VH_ERR_INVALID_ARG, "weight_cap must be >= 0");
+  if (c->storage == VH_STORAGE_F16 && (c->weight_cap < 1 || c->weight_cap > 65535))
+    return fail(VH_ERR_INVALID_ARG, "FP16 storage needs weight_cap in 1..65535");
+  return VH_OK;
+}
+
+template <class T>
+int dev_alloc(T** p, size_t bytes) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(VH_ERR_OOM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  *p = static_cast<T*>(q);
+  return VH_OK;
+}
+
+void free_volume(vh_volume* v) {
+  if (!v) return;
+  (void)hipSetDevice(v->device);
+  if (v->stream) (void)hipStreamSynchronize(v->stream);
+  drop_graphs(v, 7);
+  for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
+  void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
+                  v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->gn_partials, v->gn_pose, v->gn_stats,
+                  v->bw_pose};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (v->stream) (void)hipStreamDestroy(v->stream);
+  delete v;
+}
+
+QueryArgs query_args(vh_volume* v) {
+  QueryArgs q{};
+  q.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
+  q.voxels = v->voxels;
+  q.pool_cap = (uint32_t)v->cfg.pool_blocks;
+  q.vox = v->cfg.vox_size;
+  return q;
+}
+
+int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
+  McArgs a{};
+  a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
+  a.block_keys = v->block_keys;
+  a.voxels = v->voxels;
+  a.pool_cap = (uint32_t)v->cfg.pool_blocks;
+  a.ctl = v->ctl;
+  a.vox = v->cfg.vox_size;
+  a.tris = d_xyz;
+  a.cubes = d_cubes;
+  a.capacity = (uint32_t)std::min<int64_t>(capacity, 0xffffffffll);
+  VH_HIP(hipMemsetAsync(&v->ctl->mc_tri_count, 0, sizeof(uint32_t), v->stream));
+  VH_HIP(launch_mc(a, v->cfg.storage, kMcGrid, v->stream));
+  uint32_t cnt = 0;
+  VH_HIP(hipMemcpyAsync(&cnt, &v->ctl->mc_tri_count, sizeof cnt, hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  *n_tris = cnt;
+  if ((int64_t)cnt > capacity) return fail(VH_ERR_TRUNCATED, "mesh larger than the output buffer");
+  return VH_OK;
+}
+
+int ensure_mc_buffer(vh_volume* v, int64_t cap) {
+  if (cap <= v->mc_cap) return VH_OK;
+  if (v->mc_tris) (void)hipFree(v->mc_tris);
+  if (v->mc_cubes) (void)hipFree(v->mc_cubes);
+  v->mc_tris = nullptr;
+  v->mc_cubes = nullptr;
+  v->mc_cap = 0;
+  int rc = dev_alloc(&v->mc_tris, (size_t)cap * 9 * sizeof(float));
+  if (rc) return rc;
+  rc = dev_alloc(&v->mc_cubes, (size_t)cap * 4 * sizeof(int32_t));
+  if (rc) return rc;
+  v->mc_cap = cap;
+  return VH_OK;
+}
+
+int ensure_gn_scratch(vh_volume* v) {
+  if (v->gn_partials) return VH_OK;
+  int rc = dev_alloc(&v->gn_partials, (size_t)kGnGrid * kGnAcc * sizeof(double));
+  if (rc) return rc;
+  rc = dev_alloc(&v->gn_pose, 16 * sizeof(double));
+  if (rc) return rc;
+  rc = dev_alloc(&v->bw_pose, 16 * sizeof(float));
+  if (rc) return rc;
+  return dev_alloc(&v->gn_stats, 3 * 64 * sizeof(double));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vh_last_error(void) { return g_last_error.c_str(); }
+int vh_abi_version(void) { return VH_ABI_VERSION; }
+
+void vh_config_default(vh_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof *c);
+  c->width = 640;
+  c->height = 480;
+  c->fx = 525.0f;
+  c->fy = 525.0f;
+  c->cx = 319.5f;
+  c->cy = 239.5f;
+  c->max_depth = 5.0f;
+  c->min_depth = 0.1f;
+  c->vox_size = 0.01f;
+  c->trunc = 0.04f;
+  c->log2_buckets = 18;
+  c->pool_blocks = 1 << 18;
+  c->storage = VH_STORAGE_F32;
+  c->weight_cap = 0;
+  c->device = 0;
+}
+
+int vh_create(const vh_config* cfg, vh_volume** out) {
+  if (!cfg || !out) return fail(VH_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return fail(VH_ERR_NO_DEVICE, "no HIP device available");
+  }
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(VH_ERR_INVALID_ARG, "device ordinal out of range");
+  vh_volume* v = new (std::nothrow) vh_volume();
+  if (!v) return fail(VH_ERR_OOM, "host allocation failed");
+  v->cfg = *cfg;
+  v->device = cfg->device;
+  if ((rc = set_device(v))) { delete v; return rc; }
+  if (hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete v;
+    return fail(VH_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  v->n_slots = (uint64_t)kSlotsPerBucket << cfg->log2_buckets;
+  const size_t pool = (size_t)cfg->pool_blocks;
+  const size_t frame_px = (size_t)cfg->width * cfg->height;
+  if ((rc = dev_alloc(&v->keys, v->n_slots * 8)) || (rc = dev_alloc(&v->vals, v->n_slots * 8)) ||
+      (rc = dev_alloc(&v->stamps, v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
+      (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||
+      (rc = dev_alloc(&v->touched, pool * 4)) || (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
+      (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float)))) {
+    std::string msg = g_last_error;
+    free_volume(v);
+    return fail(rc, msg);
+  }
+  hipError_t e = hipMemsetAsync(v->stamps, 0, v->n_slots * 4, v->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(v->ctl, 0, sizeof(FrameCtl), v->stream);
+  if (e != hipSuccess) {
+    free_volume(v);
+    return fail(VH_ERR_DEVICE, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
+  }
+  if ((rc = reset_tables(v))) {
+    std::string msg = g_last_error;
+    free_volume(v);
+    return fail(rc, msg);
+  }
+  *out = v;
+  return VH_OK;
+}
+
+int vh_destroy(vh_volume* v) {
+  if (!v) return fail(VH_ERR_INVALID_ARG, "null volume");
+  free_volume(v);
+  return VH_OK;
+}
+
+int vh_reset(vh_volume* v) {
+  if (!v) return fail(VH_ERR_INVALID_ARG, "null volume");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return reset_tables(v);
+}
+
+int vh_integrate(vh_volume* v, const vh_frame* f) {
+  if (!v || !f || !f->depth) return fail(VH_ERR_INVALID_ARG, "null argument");
+  if (f->depth_format != VH_DEPTH_U16 && f->depth_format != VH_DEPTH_F32)
+    return fail(VH_ERR_INVALID_ARG, "unknown depth format");
+  if (f->depth_format == VH_DEPTH_U16 && !(f->depth_scale > 0.0f))
+    return fail(VH_ERR_INVALID_ARG, "depth_scale must be > 0 for uint16 depth");
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(f->c2w[k])) return fail(VH_ERR_INVALID_ARG, "pose has a non-finite entry");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  const size_t elem = f->depth_format == VH_DEPTH_U16 ? 2 : 4;
+  const size_t row = (size_t)v->cfg.width * elem;
+  const size_t pitch = f->pitch ? (size_t)f->pitch : row;
+  if (pitch < row) return fail(VH_ERR_INVALID_ARG, "pitch smaller than a row");
+  VH_HIP(hipMemcpy2DAsync(v->stage_depth, row, f->depth, pitch, row, (size_t)v->cfg.height, hipMemcpyHostToDevice,
+                          v->stream));
+  VH_HIP(hipMemcpyAsync(v->stage_pose, f->c2w, 16 * sizeof(float), hipMemcpyHostToDevice, v->stream));
+  const int which = f->depth_format == VH_DEPTH_U16 ? 0 : 1;
+  // the staged uint16 graph bakes the depth scale; re-capture when it changes
+  if (which == 0 && f->depth_scale != v->graph_scale) {
+    drop_graphs(v, 1);
+    v->graph_scale = f->depth_scale;
+  }
+  rc = launch_frame(v, which, f->depth_scale);
+  if (rc) return rc;
+  return sync_and_check(v);
+}
+
+int vh_bind_sequence(vh_volume* v, const void* d_depth, int32_t fmt, float scale, const float* d_c2w, int32_t n) {
+  if (!v || !d_depth || !d_c2w || n <= 0) return fail(VH_ERR_INVALID_ARG, "bad sequence");
+  if (fmt != VH_DEPTH_U16 && fmt != VH_DEPTH_F32) return fail(VH_ERR_INVALID_ARG, "unknown depth format");
+  if (fmt == VH_DEPTH_U16 && !(scale > 0.0f)) return fail(VH_ERR_INVALID_ARG, "depth_scale must be > 0");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  VH_HIP(hipStreamSynchronize(v->stream));
+  const bool same = v->seq_depth == d_depth && v->seq_pose == d_c2w && v->seq_n == n && v->seq_format == fmt &&
+                    v->seq_scale == scale;
+  v->seq_depth = d_depth;
+  v->seq_pose = d_c2w;
+  v->seq_n = n;
+  v->seq_format = fmt;
+  v->seq_scale = scale;
+  if (!same) drop_graphs(v, 4);  // the captured graph holds the sequence pointers
+  const uint32_t zero = 0;
+  VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return VH_OK;
+}
+
+int vh_integrate_next(vh_volume* v, int32_t n) {
+  if (!v || n < 0) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (!v->seq_depth) return fail(VH_ERR_INVALID_ARG, "no sequence bound");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) {
+    rc = launch_frame(v, 2, v->seq_scale);
+    if (rc) return rc;
+  }
+  return VH_OK;
+}
+
+int vh_sync(vh_volume* v) {
+  if (!v) return fail(VH_ERR_INVALID_ARG, "null volume");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  return sync_and_check(v);
+}
+
+int vh_set_profiling(vh_volume* v, int32_t on) {
+  if (!v) return fail(VH_ERR_INVALID_ARG, "null volume");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  VH_HIP(hipStreamSynchronize(v->stream));
+  rc = collect_profile(v);
+  if (rc) return rc;
+  v->profiling = on != 0;
+  v->alloc_ms = v->integrate_ms = 0.0;
+  v->profiled_frames = 0;
+  return VH_OK;
+}
+
+int vh_get_stats(vh_volume* v, vh_stats* s) {
+  if (!v || !s) return fail(VH_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  VH_HIP(hipStreamSynchronize(v->stream));
+  if ((rc = collect_profile(v))) return rc;
+  FrameCtl c{};
+  if ((rc = read_ctl(v, &c))) return rc;
+  std::memset(s, 0, sizeof *s);
+  s->frames = (int64_t)c.frames;
+  s->blocks_allocated = std::min<int64_t>(c.pool_count, v->cfg.pool_blocks);
+  s->pool_capacity = v->cfg.pool_blocks;
+  s->slots_used = c.pool_count;
+  s->touched_last = c.touched_last;
+  s->touched_total = (int64_t)c.touched_total;
+  s->new_total = (int64_t)c.new_total;
+  s->storage = v->cfg.storage;
+  s->flags = (int32_t)c.flags;
+  s->integrate_ms = v->integrate_ms;
+  s->alloc_ms = v->alloc_ms;
+  s->profiled_frames = v->profiled_frames;
+  return VH_OK;
+}
+
+void* vh_stream(vh_volume* v) { return v ? (void*)v->stream : nullptr; }
+
+int vh_extract_mesh_device(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
+  if (!v || !n_tris || capacity < 0 || (capacity > 0 && !d_xyz)) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  return run_mc(v, d_xyz, d_cubes, capacity, n_tris);
+}
+
+int vh_extract_mesh(vh_volume* v, vh_mesh* out) {
+  if (!v || !out) return fail(VH_ERR_INVALID_ARG, "null argument");
+  std::memset(out, 0, sizeof *out);
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  int64_t n = 0;
+  if ((rc = ensure_mc_buffer(v, std::max<int64_t>(v->mc_cap, 1 << 20)))) return rc;
+  rc = run_mc(v, v->mc_tris, v->mc_cubes, v->mc_cap, &n);
+  if (rc == VH_ERR_TRUNCATED) {
+    if ((rc = ensure_mc_buffer(v, n + (n >> 3) + 1024))) return rc;
+    rc = run_mc(v, v->mc_tris, v->mc_cubes, v->mc_cap, &n);
+  }
+  if (rc) return rc;
+  out->n_tris = n;
+  out->xyz = static_cast<float*>(std::malloc((size_t)std::max<int64_t>(n, 1) * 9 * sizeof(float)));
+  out->cubes = static_cast<int32_t*>(std::malloc((size_t)std::max<int64_t>(n, 1) * 4 * sizeof(int32_t)));
+  if (!out->xyz || !out->cubes) {
+    std::free(out->xyz);
+    std::free(out->cubes);
+    std::memset(out, 0, sizeof *out);
+    return fail(VH_ERR_OOM, "host allocation failed");
+  }
+  if (n) {
+    VH_HIP(hipMemcpy(out->xyz, v->mc_tris, (size_t)n * 9 * sizeof(float), hipMemcpyDeviceToHost));
+    VH_HIP(hipMemcpy(out->cubes, v->mc_cubes, (size_t)n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+  return VH_OK;
+}
+
+int vh_mesh_free(vh_mesh* m) {
+  if (!m) return fail(VH_ERR_INVALID_ARG, "null mesh");
+  std::free(m->xyz);
+  std::free(m->cubes);
+  std::memset(m, 0, sizeof *m);
+  return VH_OK;
+}
+
+int vh_save_ply(vh_volume* v, const char* path) {
+  if (!v || !path) return fail(VH_ERR_INVALID_ARG, "null argument");
+  vh_mesh m{};
+  int rc = vh_extract_mesh(v, &m);
+  if (rc) return rc;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    vh_mesh_free(&m);
+    return fail(VH_ERR_IO, std::string("cannot open ") + path);
+  }
+  std::fprintf(f,
+               "ply\nformat binary_little_endian 1.0\nelement vertex %lld\nproperty float x\nproperty float y\n"
+               "property float z\nelement face %lld\nproperty list uchar int vertex_indices\nend_header\n",
+               (long long)(m.n_tris * 3), (long long)m.n_tris);
+  bool ok = std::fwrite(m.xyz, sizeof(float), (size_t)m.n_tris * 9, f) == (size_t)m.n_tris * 9;
+  for (int64_t t = 0; ok && t < m.n_tris; ++t) {
+    unsigned char c = 3;
+    int32_t idx[3] = {(int32_t)(3 * t), (int32_t)(3 * t + 1), (int32_t)(3 * t + 2)};
+    ok = std::fwrite(&c, 1, 1, f) == 1 && std::fwrite(idx, 4, 3, f) == 3;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  vh_mesh_free(&m);
+  return ok ? VH_OK : fail(VH_ERR_IO, "write failed");
+}
+
+int vh_export_blocks(vh_volume* v, int32_t* keys, float* tsdf, float* weight, int64_t capacity, int64_t* n) {
+  if (!v || !n || capacity < 0) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  FrameCtl c{};
+  if ((rc = read_ctl(v, &c))) return rc;
+  const int64_t nb = std::min<int64_t>(c.pool_count, v->cfg.pool_blocks);
+  *n = nb;
+  if (nb > capacity) return fail(VH_ERR_TRUNCATED, "export buffer too small");
+  if (nb == 0) return VH_OK;
+  std::vector<unsigned long long> bk((size_t)nb);
+  VH_HIP(hipMemcpy(bk.data(), v->block_keys, (size_t)nb * 8, hipMemcpyDeviceToHost));
+  if (keys)
+    for (int64_t b = 0; b < nb; ++b) unpack_key(bk[(size_t)b], keys[3 * b], keys[3 * b + 1], keys[3 * b + 2]);
+  if (tsdf || weight) {
+    const size_t nv = (size_t)nb * kB3;
+    if (v->cfg.storage == VH_STORAGE_F32) {
+      std::vector<float> raw(nv * 2);
+      VH_HIP(hipMemcpy(raw.data(), v->voxels, nv * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nv; ++i) {
+        if (tsdf) tsdf[i] = raw[2 * i];
+        if (weight) weight[i] = raw[2 * i + 1];
+      }
+    } else {
+      std::vector<uint32_t> raw(nv);
+      VH_HIP(hipMemcpy(raw.data(), v->voxels, nv * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nv; ++i) {
+        if (tsdf) tsdf[i] = f16_bits_to_f32((uint16_t)(raw[i] & 0xffffu));
+        if (weight) weight[i] = (float)(raw[i] >> 16);
+      }
+    }
+  }
+  return VH_OK;
+}
+
+int vh_query_sdf_device(vh_volume* v, const float* d_pts, int64_t n, float* d_sdf, uint8_t* d_valid) {
+  if (!v || n < 0 || (n > 0 && !d_pts)) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  if (n == 0) return VH_OK;
+  VH_HIP(launch_query(query_args(v), v->cfg.storage, d_pts, n, d_sdf, d_valid, v->stream));
+  return VH_OK;
+}
+
+int vh_query_sdf(vh_volume* v, const float* pts, int64_t n, float* sdf, uint8_t* valid) {
+  if (!v || n < 0 || (n > 0 && (!pts || !sdf))) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (n == 0) return VH_OK;
+  int rc = set_device(v);
+  if (rc) return rc;
+  float* d_pts = nullptr;
+  float* d_sdf = nullptr;
+  uint8_t* d_valid = nullptr;
+  if ((rc = dev_alloc(&d_pts, (size_t)n * 12)) || (rc = dev_alloc(&d_sdf, (size_t)n * 4)) ||
+      (rc = dev_alloc(&d_valid, (size_t)n))) {
+    if (d_pts) (void)hipFree(d_pts);
+    if (d_sdf) (void)hipFree(d_sdf);
+    return rc;
+  }
+  struct Guard {
+    void* p[3];
+    ~Guard() { for (void* q : p) if (q) (void)hipFree(q); }
+  } guard{{d_pts, d_sdf, d_valid}};
+  VH_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 12, hipMemcpyHostToDevice, v->stream));
+  rc = vh_query_sdf_device(v, d_pts, n, d_sdf, d_valid);
+  if (rc) return rc;
+  VH_HIP(hipMemcpyAsync(sdf, d_sdf, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+  if (valid) VH_HIP(hipMemcpyAsync(valid, d_valid, (size_t)n, hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return VH_OK;
+}
+
+int vh_query_sdf_backward_device(vh_volume* v, const float* d_pts, int64_t n, const float* c2w, const float* d_grad_r,
+                                 float* d_r, uint8_t* d_valid, float* d_J, double* d_pose_grad, float* d_voxel_grad) {
+  if (!v || !c2w || n < 0 || (n > 0 && !d_pts)) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if ((d_pose_grad || d_voxel_grad) && !d_grad_r) return fail(VH_ERR_INVALID_ARG, "gradients need d_grad_r");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  if ((rc = ensure_gn_scratch(v))) return rc;
+  float* d_T = v->bw_pose;
+  VH_HIP(hipMemcpyAsync(d_T, c2w, 12 * sizeof(float), hipMemcpyHostToDevice, v->stream));
+  VH_HIP(launch_backward(query_args(v), v->cfg.storage, d_pts, n, d_T, d_grad_r, d_r, d_valid, d_J,
+                         d_pose_grad ? v->gn_partials : nullptr, kGnGrid, d_voxel_grad, v->stream));
+  if (d_pose_grad) VH_HIP(launch_reduce6(v->gn_partials, kGnGrid, d_pose_grad, v->stream));
+  // the scratch pose must outlive the kernels before the next call reuses it
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return VH_OK;
+}
+
+int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int32_t iters, vh_gn_stats* stats) {
+  if (!v || !pts || !c2w || n <= 0 || iters < 0 || iters > 64) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(c2w[k])) return fail(VH_ERR_INVALID_ARG, "pose has a non-finite entry");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  if ((rc = ensure_gn_scratch(v))) return rc;
+  float* d_pts = nullptr;
+  if ((rc = dev_alloc(&d_pts, (size_t)n * 12))) return rc;
+  struct Guard {
+    void* p;
+    ~Guard() { if (p) (void)hipFree(p); }
+  } guard{d_pts};
+  VH_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 12, hipMemcpyHostToDevice, v->stream));
+  VH_HIP(hipMemcpyAsync(v->gn_pose, c2w, 12 * sizeof(double), hipMemcpyHostToDevice, v->stream));
+  GnArgs a{};
+  a.q = query_args(v);
+  a.pts = d_pts;
+  a.n = n;
+  a.pose = v->gn_pose;
+  a.partials = v->gn_partials;
+  for (int it = 0; it < iters; ++it) {
+    VH_HIP(launch_gn_accum(a, v->cfg.storage, kGnGrid, v->stream));
+    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_pose, v->gn_stats + 3 * it, v->stream));
+  }
+  double host_stats[3 * 64];
+  VH_HIP(hipMemcpyAsync(c2w, v->gn_pose, 12 * sizeof(double), hipMemcpyDeviceToHost, v->stream));
+  if (iters) VH_HIP(hipMemcpyAsync(host_stats, v->gn_stats, 3 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  c2w[12] = c2w[13] = c2w[14] = 0.0;
+  c2w[15] = 1.0;
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->iters = iters;
+    for (int it = 0; it < iters; ++it) {
+      stats->n_valid[it] = (int32_t)host_stats[3 * it];
+      stats->rms[it] = host_stats[3 * it + 1];
+      stats->step_norm[it] = host_stats[3 * it + 2];
+    }
+  }
+  return VH_OK;
+}
+
+int vh_pool_view(vh_volume* v, void** d_voxels, void** d_block_keys, int64_t* n_blocks) {
+  if (!v) return fail(VH_ERR_INVALID_ARG, "null volume");
+  if (d_voxels) *d_voxels = v->voxels;
+  if (d_block_keys) *d_block_keys = v->block_keys;
+  if (n_blocks) {
+    std::lock_guard<std::mutex> lk(v->mu);
+    int rc = set_device(v);
+    if (rc) return rc;
+    FrameCtl c{};
+    if ((rc = read_ctl(v, &c))) return rc;
+    *n_blocks = std::min<int64_t>(c.pool_count, v->cfg.pool_blocks);
+  }
+  return VH_OK;
+}
+
+}  // extern "C"

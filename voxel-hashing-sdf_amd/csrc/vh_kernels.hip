@@ -1,0 +1,725 @@
+// vh_kernels.hip - gfx950 kernels of libvh.
+//
+// Per frame (one captured graph): k_alloc (one lane per pixel: truncation-band DDA, hash
+// insert, first-touch dedup, compaction of the touched list) -> k_integrate (one 256-lane
+// workgroup per touched block, grid-stride; the last workgroup to finish closes the frame).
+// Extraction: k_mc (one workgroup per pool block; 9^3 halo in LDS; wave prefix sums).
+// Queries: k_query / k_backward / k_gn_accum + k_gn_solve (FP64 reductions).
+//
+// Kernel code follows the op order of vh_math.h exactly and is compiled with
+// -ffp-contract=off, so FP32 results match a CPU twin of the same algorithm bit-for-bit.
+#include <hip/hip_runtime.h>
+
+#include "vh.h"
+#include "vh_internal.h"
+#include "vh_math.h"
+#include "vh_mc_table.h"
+
+namespace vh {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint32_t read_lane_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+__device__ __forceinline__ void raise_flag(FrameCtl* ctl, uint32_t f) { atomicOr(&ctl->flags, f); }
+
+// Read-only lookup (no concurrent inserts): pool index of `key`, or kNoBlock.
+// Slots of a bucket fill in order, so the first empty slot ends the search.
+__device__ __forceinline__ uint32_t find_block(const HashView& h, uint64_t key) {
+  uint64_t b = bucket_of(key, h.log2_buckets);
+  for (int p = 0; p < kMaxProbe; ++p) {
+    const ulonglong2* bp = reinterpret_cast<const ulonglong2*>(h.keys + b * kSlotsPerBucket);
+    const ulonglong2 k01 = bp[0], k23 = bp[1];
+    const uint64_t ks[4] = {k01.x, k01.y, k23.x, k23.y};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (ks[s] == key) return (uint32_t)h.vals[b * kSlotsPerBucket + s];
+      if (ks[s] == kEmptyKey) return kNoBlock;
+    }
+    b = (b + 1) & h.bucket_mask;
+  }
+  return kNoBlock;
+}
+
+// Lock-free find-or-insert. A key is written once into the first empty slot of its probe
+// sequence by a compare-and-swap; a stale cached read can only show EMPTY where a key now
+// lives, and the CAS then returns the real occupant, so no lane ever spins on another.
+// Returns the slot id (or -1 when the probe limit is exhausted); `inserted` is set for the
+// lane whose CAS created the key.
+__device__ __forceinline__ int64_t find_or_insert(const HashView& h, uint64_t key, bool& inserted) {
+  uint64_t b = bucket_of(key, h.log2_buckets);
+  for (int p = 0; p < kMaxProbe; ++p) {
+    unsigned long long* bk = h.keys + b * kSlotsPerBucket;
+    const ulonglong2* bp = reinterpret_cast<const ulonglong2*>(bk);
+    const ulonglong2 k01 = bp[0], k23 = bp[1];
+    const uint64_t ks[4] = {k01.x, k01.y, k23.x, k23.y};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (ks[s] == key) return (int64_t)(b * kSlotsPerBucket + s);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (ks[s] != kEmptyKey) continue;
+      const unsigned long long old = atomicCAS(bk + s, (unsigned long long)kEmptyKey, (unsigned long long)key);
+      if (old == kEmptyKey) {
+        inserted = true;
+        return (int64_t)(b * kSlotsPerBucket + s);
+      }
+      if (old == key) return (int64_t)(b * kSlotsPerBucket + s);
+    }
+    b = (b + 1) & h.bucket_mask;
+  }
+  return -1;
+}
+
+template <bool kU16>
+__device__ __forceinline__ float depth_at(const void* base, float scale, int pix) {
+  if (kU16) return (float)reinterpret_cast<const uint16_t*>(base)[pix] * scale;
+  return reinterpret_cast<const float*>(base)[pix];
+}
+
+__device__ __forceinline__ uint32_t frame_slot(const FrameArgs& a) {
+  return a.ctl->cursor[a.cursor_id] % a.n_seq;
+}
+
+// ============================================================================ allocation
+// Workgroup = 16x16 pixels, each wave an 8x8 tile, so the lanes of a wave cross mostly the
+// same blocks: hash lines are shared and the first-touch atomics are deduplicated per wave.
+template <bool kU16>
+__global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tiles_x = (a.W + 15) >> 4;
+  const int u = ((int)(blockIdx.x % tiles_x) << 4) + ((wave & 1) << 3) + (lane & 7);
+  const int v = ((int)(blockIdx.x / tiles_x) << 4) + ((wave >> 1) << 3) + (lane >> 3);
+  FrameCtl* ctl = a.ctl;
+  const uint32_t frame = ctl->frame_id;
+  const uint32_t fs = frame_slot(a);
+  const float* Tp = a.poses + 16 * (size_t)fs;
+  float T[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) T[k] = Tp[k];
+  const size_t elem = kU16 ? 2 : 4;
+  const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
+
+  BandDDA s;
+  bool active = false;
+  if (u < a.W && v < a.H) {
+    const float d = depth_at<kU16>(base, a.depth_scale, v * a.W + u);
+    if (depth_valid(d, a.max_depth)) {
+      active = band_setup(T, a.K, a.bsz, a.mu, a.min_depth, u, v, d, s);
+      if (!active) raise_flag(ctl, kFlagRange);
+    }
+  }
+
+  while (__ballot(active)) {
+    const uint64_t key = active ? pack_key(s.ix, s.iy, s.iz) : kEmptyKey;
+    bool inserted = false;
+    int64_t slot = -1;
+    if (active) {
+      slot = find_or_insert(a.hash, key, inserted);
+      if (slot < 0) raise_flag(ctl, kFlagHashFull);
+    }
+    // pool indices for the keys this wave created: one atomic per wave
+    const uint64_t im = __ballot(inserted);
+    if (im) {
+      const int leader = first_lane(im);
+      uint32_t pbase = 0;
+      if (lane == leader) pbase = atomicAdd(&ctl->pool_count, (uint32_t)__popcll(im));
+      pbase = read_lane_u32(pbase, leader);
+      if (inserted) {
+        const uint32_t idx = pbase + lanes_below(im);
+        uint32_t stored = kNoBlock;
+        if (idx < a.pool_cap) {
+          a.block_keys[idx] = key;
+          stored = idx;
+        } else {
+          raise_flag(ctl, kFlagPoolFull);
+        }
+        a.hash.vals[slot] = (unsigned long long)stored | ((unsigned long long)frame << 32);
+      }
+    }
+    // first touch of the block in this frame: one exchange per distinct slot per wave
+    bool need = false;
+    if (slot >= 0) need = a.stamps[slot] != frame;
+    bool first = false;
+    uint64_t nm = __ballot(need);
+    while (nm) {
+      const int leader = first_lane(nm);
+      const uint32_t ls_lo = read_lane_u32((uint32_t)slot, leader);
+      if (lane == leader) first = atomicExch(&a.stamps[slot], frame) != frame;
+      if (need && (uint32_t)slot == ls_lo) need = false;
+      nm = __ballot(need);
+    }
+    const uint64_t fm = __ballot(first);
+    if (fm) {
+      const int leader = first_lane(fm);
+      uint32_t tbase = 0;
+      if (lane == leader) tbase = atomicAdd(&ctl->touched_count, (uint32_t)__popcll(fm));
+      tbase = read_lane_u32(tbase, leader);
+      if (first) {
+        const uint32_t pos = tbase + lanes_below(fm);
+        if (pos < a.pool_cap) a.touched[pos] = (uint32_t)slot;
+        else raise_flag(ctl, kFlagPoolFull);
+      }
+    }
+    if (active) active = band_next(s);
+  }
+}
+
+// ============================================================================ integration
+template <int kStorage>
+struct VoxelIO;
+
+template <>
+struct VoxelIO<VH_STORAGE_F32> {
+  using Word = float2;
+  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) { s = w.x; wt = w.y; }
+  static __device__ __forceinline__ Word pack(float s, float wt) { return make_float2(s, wt); }
+  static __device__ __forceinline__ Word zero() { return make_float2(0.0f, 0.0f); }
+};
+template <>
+struct VoxelIO<VH_STORAGE_F16> {
+  using Word = uint32_t;
+  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
+    s = (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    wt = (float)(w >> 16);
+  }
+  static __device__ __forceinline__ Word pack(float s, float wt) {
+    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
+    return (uint32_t)h | ((uint32_t)wt << 16);
+  }
+  static __device__ __forceinline__ Word zero() { return 0u; }
+};
+
+template <int kStorage, bool kU16>
+__global__ __launch_bounds__(256) void k_integrate(FrameArgs a) {
+  using IO = VoxelIO<kStorage>;
+  using Word = typename IO::Word;
+  FrameCtl* ctl = a.ctl;
+  const uint32_t n = min(ctl->touched_count, a.pool_cap);
+  const uint32_t frame = ctl->frame_id;
+  const uint32_t fs = frame_slot(a);
+  const float* Tp = a.poses + 16 * (size_t)fs;
+  float T[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) T[k] = Tp[k];
+  const size_t elem = kU16 ? 2 : 4;
+  const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
+  auto dep = [&](int pix) { return depth_at<kU16>(base, a.depth_scale, pix); };
+  const int tid = threadIdx.x;
+
+  for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
+    const uint32_t sl = a.touched[t];
+    const unsigned long long val = a.hash.vals[sl];
+    const uint32_t idx = (uint32_t)val;
+    if (idx >= a.pool_cap) {
+      if (tid == 0) raise_flag(ctl, kFlagPoolFull);
+      continue;
+    }
+    const bool fresh = (uint32_t)(val >> 32) == frame;
+    int bx, by, bz;
+    unpack_key(a.hash.keys[sl], bx, by, bz);
+    Word* vp = reinterpret_cast<Word*>(a.voxels) + (size_t)idx * kB3;
+    Word old[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) old[h] = fresh ? IO::zero() : vp[tid + 256 * h];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = tid + 256 * h;
+      const int lx = i >> 6, ly = (i >> 3) & 7, lz = i & 7;
+      float dist;
+      if (observe_voxel(T, a.K, a.W, a.H, a.vox, a.max_depth, a.mu, bx * kB + lx, by * kB + ly,
+                        bz * kB + lz, dep, dist)) {
+        float sv, wv;
+        IO::unpack(old[h], sv, wv);
+        tsdf_update(sv, wv, dist, a.cap);
+        vp[i] = IO::pack(sv, wv);
+      } else if (fresh) {
+        vp[i] = IO::zero();
+      }
+    }
+  }
+
+  // the last workgroup to finish closes the frame
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(&ctl->done_wgs, 1u);
+    if (prev == gridDim.x - 1) {
+      __threadfence();
+      const uint32_t pc = atomicAdd(&ctl->pool_count, 0u);
+      ctl->touched_last = n;
+      ctl->touched_total += n;
+      ctl->frames += 1ull;
+      ctl->new_total += (unsigned long long)(pc - ctl->pool_at_frame_start);
+      ctl->pool_at_frame_start = pc;
+      ctl->cursor[a.cursor_id] += 1u;
+      ctl->frame_id = frame + 1u;
+      ctl->touched_count = 0u;
+      ctl->done_wgs = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// ============================================================================ marching cubes
+constexpr int kHalo = 9;
+constexpr int kHalo3 = kHalo * kHalo * kHalo;
+
+// Triangles of one cube whose corners are read from the 9^3 LDS tile at (cx,cy,cz); lattice
+// origin (gx,gy,gz). Emit(tri_in_case, case, P[9]) for each non-degenerate triangle
+// (reference marchingCubeHashKernel, src/tsdf.cu:978-1061, with the intended degenerate test).
+template <class Emit>
+__device__ __forceinline__ void cube_tris(const float* sv, const uint8_t* so, int cx, int cy, int cz,
+                                          int gx, int gy, int gz, Emit emit) {
+  float v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int e = (cx + corner_dx(c)) * 81 + (cy + corner_dy(c)) * 9 + (cz + corner_dz(c));
+    if (!so[e]) return;
+    v[c] = sv[e];
+  }
+  int ci = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (v[c] < 0.0f) ci |= 1 << c;
+  if (ci == 0 || ci == 255) return;
+  const uint64_t pk = kTriTable[ci];
+  const int nt = (int)(pk >> 60);
+  for (int t = 0; t < nt; ++t) {
+    float P[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
+      const int ca = edge_a(e), cb = edge_b(e);
+      vertex_interp((float)(gx + corner_dx(ca)), (float)(gy + corner_dy(ca)), (float)(gz + corner_dz(ca)),
+                    (float)(gx + corner_dx(cb)), (float)(gy + corner_dy(cb)), (float)(gz + corner_dz(cb)),
+                    v[ca], v[cb], P[3 * j], P[3 * j + 1], P[3 * j + 2]);
+    }
+    const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
+    const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
+    const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
+    if (d01 || d12 || d20) continue;
+    emit(t, ci, P);
+  }
+}
+
+template <int kStorage>
+__global__ __launch_bounds__(256) void k_mc(McArgs a) {
+  using IO = VoxelIO<kStorage>;
+  using Word = typename IO::Word;
+  __shared__ float s_val[kHalo3];
+  __shared__ uint8_t s_obs[kHalo3 + 7];
+  __shared__ uint32_t s_nb[8];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_base;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t npool = min(atomicAdd(&a.ctl->pool_count, 0u), a.pool_cap);
+  const Word* vox = reinterpret_cast<const Word*>(a.voxels);
+
+  for (uint32_t b = blockIdx.x; b < npool; b += gridDim.x) {
+    int bx, by, bz;
+    unpack_key(a.block_keys[b], bx, by, bz);
+    if (tid < 8) {
+      uint32_t nb = b;
+      if (tid > 0) {
+        const int nx = bx + (tid >> 2), ny = by + ((tid >> 1) & 1), nz = bz + (tid & 1);
+        nb = key_in_range(nx, ny, nz) ? find_block(a.hash, pack_key(nx, ny, nz)) : kNoBlock;
+        if (nb >= a.pool_cap) nb = kNoBlock;
+      }
+      s_nb[tid] = nb;
+    }
+    __syncthreads();
+    for (int e = tid; e < kHalo3; e += 256) {
+      const int x = e / 81, y = (e / 9) % 9, z = e % 9;
+      const uint32_t pb = s_nb[((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3)];
+      float sv = 0.0f, wv = 0.0f;
+      if (pb != kNoBlock) IO::unpack(vox[(size_t)pb * kB3 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7))], sv, wv);
+      s_val[e] = sv;
+      s_obs[e] = (pb != kNoBlock && wv > 0.0f) ? 1 : 0;
+    }
+    __syncthreads();
+
+    const int gx0 = bx * kB, gy0 = by * kB, gz0 = bz * kB;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = tid + 256 * h;
+      const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+      cube_tris(s_val, s_obs, cx, cy, cz, gx0 + cx, gy0 + cy, gz0 + cz,
+                [&](int, int, const float*) { ++cnt; });
+    }
+    // workgroup exclusive scan of the per-thread triangle counts
+    uint32_t x = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t wpre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (w < wave) wpre += s_wsum[w];
+      total += s_wsum[w];
+    }
+    if (tid == 0) s_base = total ? atomicAdd(&a.ctl->mc_tri_count, total) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + wpre + x - cnt;
+    if (cnt) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int c = tid + 256 * h;
+        const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+        cube_tris(s_val, s_obs, cx, cy, cz, gx0 + cx, gy0 + cy, gz0 + cz,
+                  [&](int t, int ci, const float* P) {
+                    if (pos < a.capacity) {
+                      float* o = a.tris + (size_t)pos * 9;
+#pragma unroll
+                      for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
+                      if (a.cubes) {
+                        int32_t* q = a.cubes + (size_t)pos * 4;
+                        q[0] = gx0 + cx; q[1] = gy0 + cy; q[2] = gz0 + cz; q[3] = (ci << 8) | t;
+                      }
+                    }
+                    ++pos;
+                  });
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ============================================================================ queries
+// Trilinear sample of the normalised TSDF at world point p; false unless all 8 lattice
+// corners are allocated and observed. vid/w receive the pool voxel ids and weights.
+template <int kStorage>
+__device__ __forceinline__ bool sample_sdf(const QueryArgs& q, float px, float py, float pz,
+                                           float& sdf, float g[3], uint32_t vid[8], float w[8]) {
+  using IO = VoxelIO<kStorage>;
+  using Word = typename IO::Word;
+  const float gxf = px / q.vox, gyf = py / q.vox, gzf = pz / q.vox;
+  if (!(fabsf(gxf) < 8.0e6f && fabsf(gyf) < 8.0e6f && fabsf(gzf) < 8.0e6f)) return false;
+  const float flx = floorf(gxf), fly = floorf(gyf), flz = floorf(gzf);
+  const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+  const float fx = gxf - flx, fy = gyf - fly, fz = gzf - flz;
+  const Word* vox = reinterpret_cast<const Word*>(q.voxels);
+  uint64_t last_key = kEmptyKey;
+  uint32_t blk = kNoBlock;
+  float s[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int cx = ix + (k >> 2), cy = iy + ((k >> 1) & 1), cz = iz + (k & 1);
+    const uint64_t key = pack_key(cx >> 3, cy >> 3, cz >> 3);
+    if (key != last_key) {
+      last_key = key;
+      blk = find_block(q.hash, key);
+    }
+    if (blk >= q.pool_cap) return false;
+    const uint32_t id = blk * kB3 + (uint32_t)((((cx & 7) * 8) + (cy & 7)) * 8 + (cz & 7));
+    float sv, wv;
+    IO::unpack(vox[id], sv, wv);
+    if (!(wv > 0.0f)) return false;
+    s[k] = sv;
+    vid[k] = id;
+  }
+  trilinear_weights(fx, fy, fz, w);
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc = acc + w[k] * s[k];
+  sdf = acc;
+  trilinear_dfrac(fx, fy, fz, s, g);
+  g[0] = g[0] / q.vox;
+  g[1] = g[1] / q.vox;
+  g[2] = g[2] / q.vox;
+  return true;
+}
+
+template <int kStorage>
+__global__ __launch_bounds__(256) void k_query(QueryArgs q, const float* pts, int64_t n, float* sdf,
+                                               uint8_t* valid) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float r = 0.0f, g[3], w[8];
+    uint32_t vid[8];
+    const bool ok = sample_sdf<kStorage>(q, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], r, g, vid, w);
+    if (sdf) sdf[i] = ok ? r : 0.0f;
+    if (valid) valid[i] = ok ? 1 : 0;
+  }
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __shfl_xor((int)(b & 0xffffffffll), m);
+  const int hi = __shfl_xor((int)(b >> 32), m);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sums acc[0..N) over the workgroup (256 lanes) into out[0..N); fixed tree, deterministic.
+template <int N>
+__device__ __forceinline__ void wg_reduce_store(double (&acc)[N], double* out) {
+  __shared__ double s_part[4][N];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
+    acc[k] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) s_part[wave][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int k = threadIdx.x;
+    out[k] = ((s_part[0][k] + s_part[1][k]) + s_part[2][k]) + s_part[3][k];
+  }
+  __syncthreads();
+}
+
+// Residuals r_i = sdf(R p_i + t) and their derivatives with respect to a left se(3)
+// perturbation (rho, phi): dr/drho = grad, dr/dphi = q x grad.
+template <int kStorage>
+__global__ __launch_bounds__(256) void k_backward(QueryArgs q, const float* pts, int64_t n, const float* T12,
+                                                  const float* grad_r, float* r_out, uint8_t* valid, float* J,
+                                                  double* partials, float* voxel_grad) {
+  float T[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) T[k] = T12[k];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float qx, qy, qz;
+    cam_to_world(T, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], qx, qy, qz);
+    float r = 0.0f, g[3] = {0, 0, 0}, w[8];
+    uint32_t vid[8];
+    const bool ok = sample_sdf<kStorage>(q, qx, qy, qz, r, g, vid, w);
+    float j[6] = {0, 0, 0, 0, 0, 0};
+    if (ok) {
+      j[0] = g[0]; j[1] = g[1]; j[2] = g[2];
+      j[3] = qy * g[2] - qz * g[1];
+      j[4] = qz * g[0] - qx * g[2];
+      j[5] = qx * g[1] - qy * g[0];
+    }
+    if (r_out) r_out[i] = ok ? r : 0.0f;
+    if (valid) valid[i] = ok ? 1 : 0;
+    if (J) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) J[6 * i + k] = j[k];
+    }
+    const float up = grad_r ? grad_r[i] : 0.0f;
+    if (ok && grad_r) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k] += (double)up * (double)j[k];
+      if (voxel_grad) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) atomicAdd(voxel_grad + vid[k], up * w[k]);
+      }
+    }
+  }
+  if (partials) wg_reduce_store<6>(acc, partials + 6 * (size_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_reduce6(const double* partials, int grid, double* out6) {
+  if (threadIdx.x < 6) {
+    double s = 0.0;
+    for (int g = 0; g < grid; ++g) s += partials[6 * (size_t)g + threadIdx.x];
+    out6[threadIdx.x] = s;
+  }
+}
+
+// ============================================================================ Gauss-Newton
+template <int kStorage>
+__global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
+  float T[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) T[k] = (float)a.pose[k];
+  double acc[kGnAcc];
+#pragma unroll
+  for (int k = 0; k < kGnAcc; ++k) acc[k] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    float qx, qy, qz;
+    cam_to_world(T, a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2], qx, qy, qz);
+    float r, g[3], w[8];
+    uint32_t vid[8];
+    if (!sample_sdf<kStorage>(a.q, qx, qy, qz, r, g, vid, w)) continue;
+    const double j[6] = {(double)g[0], (double)g[1], (double)g[2], (double)(qy * g[2] - qz * g[1]),
+                         (double)(qz * g[0] - qx * g[2]), (double)(qx * g[1] - qy * g[0])};
+    int m = 0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+      for (int c = p; c < 6; ++c) acc[m++] += j[p] * j[c];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) acc[21 + p] += j[p] * (double)r;
+    acc[27] += (double)r * (double)r;
+    acc[28] += 1.0;
+  }
+  wg_reduce_store<kGnAcc>(acc, a.partials + kGnAcc * (size_t)blockIdx.x);
+}
+
+// One workgroup: deterministic sum of the partials, 6x6 Cholesky solve of
+// (J^T J) delta = -J^T r, and the update T <- exp(delta^) T, all in FP64.
+// stats_row: [n_valid, rms, |delta|].
+__global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int grid, double* pose, double* stats_row) {
+  __shared__ double tot[kGnAcc];
+  if (threadIdx.x < kGnAcc) {
+    double s = 0.0;
+    for (int g = 0; g < grid; ++g) s += partials[kGnAcc * (size_t)g + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double A[6][6], b[6];
+  int m = 0;
+  for (int p = 0; p < 6; ++p)
+    for (int c = p; c < 6; ++c) { A[p][c] = tot[m]; A[c][p] = tot[m]; ++m; }
+  for (int p = 0; p < 6; ++p) b[p] = -tot[21 + p];
+  const double cnt = tot[28];
+  stats_row[0] = cnt;
+  stats_row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
+  stats_row[2] = 0.0;
+  if (cnt < 6.0) return;
+  // Cholesky A = L L^T (in place, lower)
+  double L[6][6];
+  for (int p = 0; p < 6; ++p)
+    for (int c = 0; c < 6; ++c) L[p][c] = 0.0;
+  for (int c = 0; c < 6; ++c) {
+    double d = A[c][c];
+    for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
+    if (!(d > 0.0)) return;  // not positive definite: keep the pose
+    L[c][c] = sqrt(d);
+    for (int p = c + 1; p < 6; ++p) {
+      double s = A[p][c];
+      for (int k = 0; k < c; ++k) s -= L[p][k] * L[c][k];
+      L[p][c] = s / L[c][c];
+    }
+  }
+  double y[6], x[6];
+  for (int p = 0; p < 6; ++p) {
+    double s = b[p];
+    for (int k = 0; k < p; ++k) s -= L[p][k] * y[k];
+    y[p] = s / L[p][p];
+  }
+  for (int p = 5; p >= 0; --p) {
+    double s = y[p];
+    for (int k = p + 1; k < 6; ++k) s -= L[k][p] * x[k];
+    x[p] = s / L[p][p];
+  }
+  // exp of (rho, phi)
+  const double rho[3] = {x[0], x[1], x[2]}, ph[3] = {x[3], x[4], x[5]};
+  const double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
+  const double th = sqrt(th2);
+  double A1, B1, C1;  // sin/th, (1-cos)/th^2, (th - sin)/th^3
+  if (th < 1e-5) {
+    A1 = 1.0 - th2 / 6.0;
+    B1 = 0.5 - th2 / 24.0;
+    C1 = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    A1 = sin(th) / th;
+    B1 = (1.0 - cos(th)) / th2;
+    C1 = (th - sin(th)) / (th2 * th);
+  }
+  const double K[3][3] = {{0.0, -ph[2], ph[1]}, {ph[2], 0.0, -ph[0]}, {-ph[1], ph[0], 0.0}};
+  double K2[3][3], R[3][3], V[3][3];
+  for (int p = 0; p < 3; ++p)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += K[p][k] * K[k][c];
+      K2[p][c] = s;
+    }
+  for (int p = 0; p < 3; ++p)
+    for (int c = 0; c < 3; ++c) {
+      const double I = p == c ? 1.0 : 0.0;
+      R[p][c] = I + A1 * K[p][c] + B1 * K2[p][c];
+      V[p][c] = I + B1 * K[p][c] + C1 * K2[p][c];
+    }
+  double P[12];
+  for (int p = 0; p < 3; ++p) {
+    for (int c = 0; c < 3; ++c) P[4 * p + c] = R[p][0] * pose[c] + R[p][1] * pose[4 + c] + R[p][2] * pose[8 + c];
+    P[4 * p + 3] = R[p][0] * pose[3] + R[p][1] * pose[7] + R[p][2] * pose[11] +
+                   (V[p][0] * rho[0] + V[p][1] * rho[1] + V[p][2] * rho[2]);
+  }
+  for (int k = 0; k < 12; ++k) pose[k] = P[k];
+  stats_row[2] = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + th2);
+}
+
+}  // namespace
+
+// ============================================================================ launchers
+hipError_t launch_alloc(const FrameArgs& a, int depth_u16, hipStream_t st) {
+  const int tiles = ((a.W + 15) >> 4) * ((a.H + 15) >> 4);
+  if (depth_u16)
+    hipLaunchKernelGGL(k_alloc<true>, dim3(tiles), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_alloc<false>, dim3(tiles), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
+  if (storage == VH_STORAGE_F16) {
+    if (depth_u16) hipLaunchKernelGGL((k_integrate<VH_STORAGE_F16, true>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_integrate<VH_STORAGE_F16, false>), dim3(grid), dim3(256), 0, st, a);
+  } else {
+    if (depth_u16) hipLaunchKernelGGL((k_integrate<VH_STORAGE_F32, true>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_integrate<VH_STORAGE_F32, false>), dim3(grid), dim3(256), 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mc(const McArgs& a, int storage, int grid, hipStream_t st) {
+  if (storage == VH_STORAGE_F16) hipLaunchKernelGGL(k_mc<VH_STORAGE_F16>, dim3(grid), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_mc<VH_STORAGE_F32>, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+static int query_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf, uint8_t* valid,
+                        hipStream_t st) {
+  const int g = query_grid(n);
+  if (storage == VH_STORAGE_F16) hipLaunchKernelGGL(k_query<VH_STORAGE_F16>, dim3(g), dim3(256), 0, st, q, pts, n, sdf, valid);
+  else hipLaunchKernelGGL(k_query<VH_STORAGE_F32>, dim3(g), dim3(256), 0, st, q, pts, n, sdf, valid);
+  return hipGetLastError();
+}
+
+hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, int64_t n, const float* T12,
+                           const float* grad_r, float* r, uint8_t* valid, float* J, double* partials, int grid,
+                           float* voxel_grad, hipStream_t st) {
+  if (storage == VH_STORAGE_F16)
+    hipLaunchKernelGGL(k_backward<VH_STORAGE_F16>, dim3(grid), dim3(256), 0, st, q, pts, n, T12, grad_r, r, valid, J,
+                       partials, voxel_grad);
+  else
+    hipLaunchKernelGGL(k_backward<VH_STORAGE_F32>, dim3(grid), dim3(256), 0, st, q, pts, n, T12, grad_r, r, valid, J,
+                       partials, voxel_grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce6, dim3(1), dim3(64), 0, st, partials, grid, out6);
+  return hipGetLastError();
+}
+
+hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st) {
+  if (storage == VH_STORAGE_F16) hipLaunchKernelGGL(k_gn_accum<VH_STORAGE_F16>, dim3(grid), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_gn_accum<VH_STORAGE_F32>, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_gn_solve(const double* partials, int grid, double* pose, double* stats_row, hipStream_t st) {
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, st, partials, grid, pose, stats_row);
+  return hipGetLastError();
+}
+
+}  // namespace vh

@@ -1,0 +1,282 @@
+// vh_math.h - scalar math shared by the gfx950 kernels and the host side of libvh.
+//
+// Everything here is written so that a CPU build with -ffp-contract=off and the HIP
+// build with -ffp-contract=off evaluate the same IEEE-754 operations in the same order:
+// no FMA contraction, correctly-rounded division, floorf/roundf (both exact). That is
+// what makes the allocated block set and the FP32 voxel values reproducible bit-for-bit
+// by a CPU twin of the algorithm.
+//
+// Lattice conventions (reference src/tsdf.cu:24-64):
+//   voxel (i,j,k) sits at world (i*vox, j*vox, k*vox) - corner lattice, no origin offset;
+//   block = floor(voxel / 8); in-block linear index = (x*8 + y)*8 + z, z fastest.
+// Camera conventions (reference src/tsdf.cu:66-116, PointCloudGenerator.cpp:118-125):
+//   c2w is a row-major 4x4 camera-to-world; world->camera is R^T (p - t);
+//   projection rounds to the nearest pixel with roundf.
+#pragma once
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define VH_HD __host__ __device__ __forceinline__
+#else
+#define VH_HD static inline
+#endif
+
+namespace vh {
+
+constexpr int kB = 8;          // voxels per block edge
+constexpr int kB3 = 512;       // voxels per block
+constexpr int kSlotsPerBucket = 4;
+constexpr int kKeyBits = 21;   // bits per axis in a packed key
+constexpr int kKeyBias = 1 << 20;
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr uint32_t kNoBlock = 0xffffffffu;
+
+// ---------------------------------------------------------------- keys and hashing
+// A block coordinate is packed as three biased 21-bit fields; bit 63 is always 0, so the
+// all-ones word can never be a valid key and serves as the empty-slot sentinel.
+// (Replaces the reference's int3 key with the {999999,999999,999999} empty value,
+// src/tsdf.cu:1488, and BlockEqual, include/tsdf.cuh:158-165.)
+VH_HD bool key_in_range(int x, int y, int z) {
+  return x >= -kKeyBias && x < kKeyBias && y >= -kKeyBias && y < kKeyBias &&
+         z >= -kKeyBias && z < kKeyBias;
+}
+VH_HD uint64_t pack_key(int x, int y, int z) {
+  return ((uint64_t)(uint32_t)(x + kKeyBias) << 42) | ((uint64_t)(uint32_t)(y + kKeyBias) << 21) |
+         (uint64_t)(uint32_t)(z + kKeyBias);
+}
+VH_HD void unpack_key(uint64_t k, int& x, int& y, int& z) {
+  const uint64_t m = (1ull << kKeyBits) - 1;
+  x = (int)((k >> 42) & m) - kKeyBias;
+  y = (int)((k >> 21) & m) - kKeyBias;
+  z = (int)(k & m) - kKeyBias;
+}
+// 64-bit finaliser (MurmurHash3 fmix64). The bucket is the top log2_buckets bits, so the
+// table size is a power of two and no modulo is needed (the reference reduces a
+// Teschner xor-hash modulo 100001, include/tsdf.cuh:144-156, include/vhashing.h:151-152).
+VH_HD uint64_t mix_key(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+VH_HD uint64_t bucket_of(uint64_t key, int log2_buckets) {
+  return mix_key(key) >> (64 - log2_buckets);
+}
+
+// ---------------------------------------------------------------- intrinsics / pose
+struct Intr {
+  float fx, fy, cx, cy;
+};
+
+// world -> camera, R^T (p - t) with c2w row-major (reference base2cam, src/tsdf.cu:81-93).
+VH_HD void world_to_cam(const float* T, float px, float py, float pz, float& x, float& y,
+                        float& z) {
+  const float dx = px - T[3], dy = py - T[7], dz = pz - T[11];
+  x = T[0] * dx + T[4] * dy + T[8] * dz;
+  y = T[1] * dx + T[5] * dy + T[9] * dz;
+  z = T[2] * dx + T[6] * dy + T[10] * dz;
+}
+// camera -> world, R p + t (reference cam2base, src/tsdf.cu:95-105).
+VH_HD void cam_to_world(const float* T, float x, float y, float z, float& px, float& py,
+                        float& pz) {
+  px = x * T[0] + y * T[1] + z * T[2] + T[3];
+  py = x * T[4] + y * T[5] + z * T[6] + T[7];
+  pz = x * T[8] + y * T[9] + z * T[10] + T[11];
+}
+
+// ---------------------------------------------------------------- integration (one voxel)
+// Observation test and truncated distance for the voxel at global lattice index (gx,gy,gz)
+// (reference IntegrateHashKernel skip rules, src/tsdf.cu:613-738). Returns false when the
+// voxel is skipped; otherwise dist = min(1, (d - z)/mu) in (-1, 1].
+// DepthAt(int pixel) -> float metres.
+template <class DepthAt>
+VH_HD bool observe_voxel(const float* T, const Intr& K, int W, int H, float vox, float max_depth,
+                         float mu, int gx, int gy, int gz, DepthAt depth_at, float& dist) {
+  const float px = (float)gx * vox, py = (float)gy * vox, pz = (float)gz * vox;
+  float x, y, z;
+  world_to_cam(T, px, py, pz, x, y, z);
+  if (!(z > 0.0f)) return false;
+  const float uf = roundf(K.fx * (x / z) + K.cx);
+  const float vf = roundf(K.fy * (y / z) + K.cy);
+  if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) return false;
+  const float d = depth_at((int)vf * W + (int)uf);
+  if (!(d > 0.0f && d <= max_depth)) return false;
+  const float diff = d - z;
+  if (diff <= -mu) return false;
+  dist = fminf(1.0f, diff / mu);
+  return true;
+}
+
+// Running average with the old weight (the intent of src/tsdf.cu:855-858; the hash kernel
+// at :739-742 assigns the new weight first, which is a defect and is not reproduced).
+// cap > 0 bounds the stored weight, turning the mean into an exponential average.
+VH_HD void tsdf_update(float& s, float& w, float dist, float cap) {
+  const float w_new = w + 1.0f;
+  s = (s * w + dist) / w_new;
+  w = (cap > 0.0f && w_new > cap) ? cap : w_new;
+}
+
+// ---------------------------------------------------------------- binary16 conversion
+// Round-to-nearest-even float -> binary16 bits, and exact binary16 -> float. Used on the
+// host (exports, tests); kernels use the hardware conversion, which rounds the same way.
+VH_HD uint16_t f32_to_f16_bits(float f) {
+  union { float f; uint32_t u; } c;
+  c.f = f;
+  const uint32_t u = c.u;
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const uint32_t a = u & 0x7fffffffu;
+  if (a >= 0x7f800000u) return (uint16_t)(sign | (a > 0x7f800000u ? 0x7e00u : 0x7c00u));
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+  if (a < 0x38800000u) {                                      // result is subnormal or zero
+    if (a < 0x33000000u) return (uint16_t)sign;               // < 2^-25 rounds to 0
+    const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+    const int e = (int)(a >> 23);                             // 102..112
+    const int sh = 126 - e;                                   // 14..24
+    uint32_t r = m >> sh;
+    const uint32_t rem = m & ((1u << sh) - 1u);
+    const uint32_t half = 1u << (sh - 1);
+    if (rem > half || (rem == half && (r & 1u))) ++r;
+    return (uint16_t)(sign | r);
+  }
+  uint32_t r = ((a >> 13) - (112u << 10));
+  const uint32_t rem = a & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+  return (uint16_t)(sign | r);
+}
+VH_HD float f16_bits_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  union { float f; uint32_t u; } c;
+  if (e == 0) {
+    c.f = (float)m * 5.9604644775390625e-08f;  // m * 2^-24, exact
+    c.u |= sign;
+    return c.f;
+  }
+  if (e == 31) {
+    c.u = sign | 0x7f800000u | (m << 13);
+    return c.f;
+  }
+  c.u = sign | ((e + 112u) << 23) | (m << 13);
+  return c.f;
+}
+
+// ---------------------------------------------------------------- truncation-band DDA
+// Enumerates every block whose cube is crossed by the ray segment of pixel (u,v) between
+// camera depths max(d-mu, min_depth) and d+mu (Amanatides-Woo traversal in block units).
+// This replaces the reference's whole-ray DDA over every 10th pixel (src/tsdf.cu:2088-2238),
+// which ignores the truncation band. Steps are chosen among the axes that still have to
+// move, smallest crossing parameter first, ties to the lower axis; the walk therefore ends
+// exactly at the end block after |e - i|_1 steps. band_setup returns false if the band
+// leaves the key range.
+struct BandDDA {
+  float ax, ay, az, dx, dy, dz;  // start point and direction, block units
+  int ix, iy, iz;                // current block
+  int ex, ey, ez;                // last block
+};
+
+VH_HD bool band_setup(const float* T, const Intr& K, float bsz, float mu, float min_depth, int u,
+                      int v, float d, BandDDA& s) {
+  float t0 = d - mu;
+  if (t0 < min_depth) t0 = min_depth;
+  const float t1 = d + mu;
+  const float ru = (float)u - K.cx, rv = (float)v - K.cy;
+  const float cax = ru * t0 / K.fx, cay = rv * t0 / K.fy;
+  const float cbx = ru * t1 / K.fx, cby = rv * t1 / K.fy;
+  float wax, way, waz, wbx, wby, wbz;
+  cam_to_world(T, cax, cay, t0, wax, way, waz);
+  cam_to_world(T, cbx, cby, t1, wbx, wby, wbz);
+  s.ax = wax / bsz; s.ay = way / bsz; s.az = waz / bsz;
+  const float bx = wbx / bsz, by = wby / bsz, bz = wbz / bsz;
+  const float lim = 1048000.0f;  // keeps every block inside the 21-bit key range
+  if (!(fabsf(s.ax) < lim && fabsf(s.ay) < lim && fabsf(s.az) < lim && fabsf(bx) < lim &&
+        fabsf(by) < lim && fabsf(bz) < lim))
+    return false;
+  s.dx = bx - s.ax; s.dy = by - s.ay; s.dz = bz - s.az;
+  s.ix = (int)floorf(s.ax); s.iy = (int)floorf(s.ay); s.iz = (int)floorf(s.az);
+  s.ex = (int)floorf(bx); s.ey = (int)floorf(by); s.ez = (int)floorf(bz);
+  return true;
+}
+// Crossing parameter of the next block face along one axis (only for axes still moving).
+VH_HD float band_t(int i, int e, float a, float dir) {
+  const float bound = (float)(e > i ? i + 1 : i);
+  return (bound - a) / dir;
+}
+// Advances to the next block; returns false when the current block was the last one.
+VH_HD bool band_next(BandDDA& s) {
+  const bool mx = s.ix != s.ex, my = s.iy != s.ey, mz = s.iz != s.ez;
+  if (!(mx || my || mz)) return false;
+  const float tx = mx ? band_t(s.ix, s.ex, s.ax, s.dx) : 0.0f;
+  const float ty = my ? band_t(s.iy, s.ey, s.ay, s.dy) : 0.0f;
+  const float tz = mz ? band_t(s.iz, s.ez, s.az, s.dz) : 0.0f;
+  // smallest crossing parameter first; ties go to the lower axis
+  const bool take_x = mx && (!my || !(ty < tx)) && (!mz || !(tz < tx));
+  const bool take_y = !take_x && my && (!mz || !(tz < ty));
+  if (take_x) s.ix += s.ex > s.ix ? 1 : -1;
+  else if (take_y) s.iy += s.ey > s.iy ? 1 : -1;
+  else s.iz += s.ez > s.iz ? 1 : -1;
+  return true;
+}
+VH_HD bool depth_valid(float d, float max_depth) { return d > 0.0f && d <= max_depth; }
+
+// ---------------------------------------------------------------- marching cubes
+// Corner order of the reference (include/tsdf.cuh:234-241): x/y swapped against Bourke's
+// numbering, which fixes the triangle winding. Edge e joins corners kEdgeA[e], kEdgeB[e]
+// (reference VertexInterp calls, src/tsdf.cu:1001-1037).
+VH_HD int corner_dx(int c) { return (c == 2 || c == 3 || c == 6 || c == 7) ? 1 : 0; }
+VH_HD int corner_dy(int c) { return (c == 1 || c == 2 || c == 5 || c == 6) ? 1 : 0; }
+VH_HD int corner_dz(int c) { return c >= 4 ? 1 : 0; }
+VH_HD int edge_a(int e) { return e < 8 ? e : e - 8; }
+VH_HD int edge_b(int e) { return e < 8 ? ((e & 3) == 3 ? e - 3 : e + 1) : e - 4; }
+
+// |v| < 1e-5 evaluated as the reference does it (float |v| promoted to double against the
+// double literal, src/tsdf.cu:1645-1650): equivalent to |v| <= 1e-5f in float.
+VH_HD bool near_iso(float v) { return fabsf(v) <= 1e-5f; }
+
+// Interpolates along one cube edge in voxel units (reference VertexInterp, iso = 0,
+// src/tsdf.cu:1640-1660). p1/p2 are integer lattice coordinates held as floats.
+VH_HD void vertex_interp(float p1x, float p1y, float p1z, float p2x, float p2y, float p2z,
+                         float v1, float v2, float& x, float& y, float& z) {
+  if (near_iso(v1)) {
+    x = p1x; y = p1y; z = p1z;
+    return;
+  }
+  if (near_iso(v2)) {
+    x = p2x; y = p2y; z = p2z;
+    return;
+  }
+  if (near_iso(v1 - v2)) {
+    x = p1x; y = p1y; z = p1z;
+    return;
+  }
+  const float mu = (0.0f - v1) / (v2 - v1);
+  x = p1x + mu * (p2x - p1x);
+  y = p1y + mu * (p2y - p1y);
+  z = p1z + mu * (p2z - p1z);
+}
+
+// ---------------------------------------------------------------- trilinear weights
+// Corner k = (dx<<2)|(dy<<1)|dz of the lattice cell containing g; weight products are
+// formed as (wx*wy)*wz and sums run in k order, so CPU and GPU agree bit-for-bit.
+VH_HD void trilinear_weights(float fx, float fy, float fz, float w[8]) {
+  const float ax[2] = {1.0f - fx, fx}, ay[2] = {1.0f - fy, fy}, az[2] = {1.0f - fz, fz};
+  for (int k = 0; k < 8; ++k) w[k] = (ax[k >> 2] * ay[(k >> 1) & 1]) * az[k & 1];
+}
+// d(sum_k w_k s_k)/d(fx,fy,fz) for the cell values s.
+VH_HD void trilinear_dfrac(float fx, float fy, float fz, const float s[8], float g[3]) {
+  const float ax[2] = {1.0f - fx, fx}, ay[2] = {1.0f - fy, fy}, az[2] = {1.0f - fz, fz};
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  for (int k = 0; k < 8; ++k) {
+    const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+    const float sx = dx ? 1.0f : -1.0f, sy = dy ? 1.0f : -1.0f, sz = dz ? 1.0f : -1.0f;
+    gx = gx + ((sx * ay[dy]) * az[dz]) * s[k];
+    gy = gy + ((ax[dx] * sy) * az[dz]) * s[k];
+    gz = gz + ((ax[dx] * ay[dy]) * sz) * s[k];
+  }
+  g[0] = gx; g[1] = gy; g[2] = gz;
+}
+
+}  // namespace vh

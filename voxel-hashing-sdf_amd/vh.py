@@ -1,0 +1,378 @@
+"""ctypes binding of libvh (include/vh.h, include/vh_synth.h).
+
+Python-side mirror of the engine's host interface for tests, benchmarks and scripts. The
+reference's host API is the C++ class ark::GpuTsdfGenerator (include/tsdf.cuh:548-683);
+`Volume` keeps its three operations - construct, integrate(depth, pose), extract mesh -
+and adds the query / backward / Gauss-Newton extensions.
+
+Loading order matters: PyTorch wheels carry their own HIP runtime (SONAME
+libamdhip64.so.7). Importing torch before libvh makes both resolve to that one runtime, so
+device pointers from torch tensors are valid inside libvh. `load()` does this and refuses to
+run with two HIP runtimes mapped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvh.so")
+
+VH_OK = 0
+VH_ERR_INVALID_ARG = 1
+VH_ERR_OOM = 2
+VH_ERR_POOL_FULL = 3
+VH_ERR_HASH_FULL = 4
+VH_ERR_DEVICE = 5
+VH_ERR_RANGE = 6
+VH_ERR_TRUNCATED = 7
+VH_ERR_IO = 8
+VH_ERR_NO_DEVICE = 9
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "POOL_FULL", 4: "HASH_FULL", 5: "DEVICE",
+                6: "RANGE", 7: "TRUNCATED", 8: "IO", 9: "NO_DEVICE"}
+
+STORAGE_F32 = 0
+STORAGE_F16 = 1
+DEPTH_U16 = 0
+DEPTH_F32 = 1
+
+
+class VhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libvh status {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("fx", C.c_float), ("fy", C.c_float),
+                ("cx", C.c_float), ("cy", C.c_float), ("max_depth", C.c_float), ("min_depth", C.c_float),
+                ("vox_size", C.c_float), ("trunc", C.c_float), ("log2_buckets", C.c_int32),
+                ("pool_blocks", C.c_int32), ("storage", C.c_int32), ("weight_cap", C.c_int32),
+                ("device", C.c_int32), ("origin", C.c_float * 3), ("vox_dim", C.c_int32 * 3),
+                ("reserved", C.c_int32 * 8)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("depth_format", C.c_int32), ("depth_scale", C.c_float),
+                ("pitch", C.c_int64), ("rgb", C.c_void_p), ("c2w", C.c_float * 16)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("frames", C.c_int64), ("blocks_allocated", C.c_int64), ("pool_capacity", C.c_int64),
+                ("slots_used", C.c_int64), ("touched_last", C.c_int64), ("touched_total", C.c_int64),
+                ("new_total", C.c_int64), ("storage", C.c_int32), ("flags", C.c_int32),
+                ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("n_tris", C.c_int64), ("xyz", C.POINTER(C.c_float)), ("cubes", C.POINTER(C.c_int32))]
+
+
+class GnStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("n_valid", C.c_int32 * 64), ("rms", C.c_double * 64),
+                ("step_norm", C.c_double * 64)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("n_frames", C.c_int32), ("fx", C.c_float),
+                ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float), ("extent", C.c_float * 3),
+                ("n_spheres", C.c_int32), ("n_boxes", C.c_int32), ("dropout", C.c_float)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported function
+_PROTOS = [
+    ("vh_config_default", None, [C.POINTER(Config)]),
+    ("vh_create", C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    ("vh_destroy", C.c_int, [C.c_void_p]),
+    ("vh_reset", C.c_int, [C.c_void_p]),
+    ("vh_last_error", C.c_char_p, []),
+    ("vh_abi_version", C.c_int, []),
+    ("vh_integrate", C.c_int, [C.c_void_p, C.POINTER(Frame)]),
+    ("vh_bind_sequence", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
+    ("vh_integrate_next", C.c_int, [C.c_void_p, C.c_int32]),
+    ("vh_sync", C.c_int, [C.c_void_p]),
+    ("vh_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
+    ("vh_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("vh_stream", C.c_void_p, [C.c_void_p]),
+    ("vh_extract_mesh", C.c_int, [C.c_void_p, C.POINTER(Mesh)]),
+    ("vh_mesh_free", C.c_int, [C.POINTER(Mesh)]),
+    ("vh_extract_mesh_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    ("vh_save_ply", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("vh_export_blocks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                   C.POINTER(C.c_int64)]),
+    ("vh_query_sdf", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("vh_query_sdf_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("vh_query_sdf_backward_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("vh_gn_refine", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.POINTER(GnStats)]),
+    ("vh_pool_view", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    ("vh_scene_create", C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("vh_scene_destroy", C.c_int, [C.c_void_p]),
+    ("vh_scene_info_get", C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
+    ("vh_scene_pose", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    ("vh_scene_render_u16", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("vh_scene_render_clean", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("vh_scene_render_range", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+]
+
+
+def _hip_runtimes_mapped():
+    try:
+        with open("/proc/self/maps") as f:
+            paths = {line.split()[-1] for line in f if "libamdhip64" in line}
+    except OSError:
+        return set()
+    return {os.path.realpath(p) for p in paths}
+
+
+def load():
+    """Loads libvh.so (after torch, so that one HIP runtime serves both)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
+    except ImportError:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise VhError(VH_ERR_DEVICE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in _PROTOS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    runtimes = _hip_runtimes_mapped()
+    if len(runtimes) > 1:
+        raise VhError(VH_ERR_DEVICE, f"two HIP runtimes are mapped ({sorted(runtimes)}); import torch first")
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != VH_OK:
+        raise VhError(rc, _lib.vh_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _tptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def make_config(**kw) -> Config:
+    lib = load()
+    c = Config()
+    lib.vh_config_default(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+def config_from_workload(w: dict, **over) -> Config:
+    keys = ("width", "height", "fx", "fy", "cx", "cy", "max_depth", "min_depth", "vox_size", "trunc",
+            "log2_buckets", "pool_blocks", "storage", "weight_cap")
+    kw = {k: w[k] for k in keys if k in w}
+    kw.update(over)
+    return make_config(**kw)
+
+
+class Volume:
+    """One TSDF volume on one GPU (the engine behind ark::GpuTsdfGenerator)."""
+
+    def __init__(self, cfg: Config):
+        lib = load()
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(lib.vh_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    # -- lifetime
+    def close(self):
+        if self._h:
+            _check(_lib.vh_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        _check(_lib.vh_reset(self._h))
+
+    # -- integration
+    def integrate(self, depth: np.ndarray, c2w, scale: float = 0.001):
+        depth = np.ascontiguousarray(depth)
+        f = Frame()
+        f.depth = depth.ctypes.data_as(C.c_void_p)
+        if depth.dtype == np.uint16:
+            f.depth_format = DEPTH_U16
+            f.depth_scale = scale
+        elif depth.dtype == np.float32:
+            f.depth_format = DEPTH_F32
+            f.depth_scale = 1.0
+        else:
+            raise TypeError("depth must be uint16 or float32")
+        f.pitch = depth.strides[0]
+        f.rgb = None
+        T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32).reshape(16))
+        for k in range(16):
+            f.c2w[k] = float(T[k])
+        _check(_lib.vh_integrate(self._h, C.byref(f)))
+
+    def bind_sequence(self, d_depth, fmt: int, scale: float, d_c2w, n: int):
+        """d_depth / d_c2w: torch device tensors (or integer device addresses)."""
+        dp = d_depth if isinstance(d_depth, int) else d_depth.data_ptr()
+        pp = d_c2w if isinstance(d_c2w, int) else d_c2w.data_ptr()
+        _check(_lib.vh_bind_sequence(self._h, C.c_void_p(dp), fmt, scale, C.c_void_p(pp), n))
+
+    def integrate_next(self, n: int = 1):
+        _check(_lib.vh_integrate_next(self._h, n))
+
+    def sync(self):
+        _check(_lib.vh_sync(self._h))
+
+    def set_profiling(self, on: bool):
+        _check(_lib.vh_set_profiling(self._h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(_lib.vh_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def stream(self) -> int:
+        return _lib.vh_stream(self._h) or 0
+
+    # -- blocks / mesh
+    def export_blocks(self):
+        """(keys (N,3) int32, tsdf (N,512) float32, weight (N,512) float32), pool order."""
+        n = C.c_int64(0)
+        rc = _lib.vh_export_blocks(self._h, None, None, None, 0, C.byref(n))
+        if rc not in (VH_OK, VH_ERR_TRUNCATED):
+            _check(rc)
+        nb = n.value
+        keys = np.zeros((nb, 3), np.int32)
+        tsdf = np.zeros((nb, 512), np.float32)
+        w = np.zeros((nb, 512), np.float32)
+        _check(_lib.vh_export_blocks(self._h, _ptr(keys), _ptr(tsdf), _ptr(w), nb, C.byref(n)))
+        return keys, tsdf, w
+
+    def extract_mesh(self):
+        """(xyz (T,9) float32 metres, cubes (T,4) int32)."""
+        m = Mesh()
+        _check(_lib.vh_extract_mesh(self._h, C.byref(m)))
+        try:
+            n = m.n_tris
+            xyz = np.ctypeslib.as_array(m.xyz, shape=(max(n, 1) * 9,))[: n * 9].reshape(n, 9).copy()
+            cubes = np.ctypeslib.as_array(m.cubes, shape=(max(n, 1) * 4,))[: n * 4].reshape(n, 4).copy()
+        finally:
+            _lib.vh_mesh_free(C.byref(m))
+        return xyz, cubes
+
+    def extract_mesh_device(self, d_xyz, d_cubes, capacity: int) -> int:
+        n = C.c_int64(0)
+        rc = _lib.vh_extract_mesh_device(self._h, _tptr(d_xyz), _tptr(d_cubes), capacity, C.byref(n))
+        if rc not in (VH_OK, VH_ERR_TRUNCATED):
+            _check(rc)
+        return n.value
+
+    def save_ply(self, path: str):
+        _check(_lib.vh_save_ply(self._h, path.encode()))
+
+    # -- queries
+    def query(self, pts: np.ndarray):
+        pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        n = pts.shape[0]
+        sdf = np.zeros(n, np.float32)
+        valid = np.zeros(n, np.uint8)
+        _check(_lib.vh_query_sdf(self._h, _ptr(pts), n, _ptr(sdf), _ptr(valid)))
+        return sdf, valid.astype(bool)
+
+    def query_device(self, d_pts, n, d_sdf, d_valid):
+        _check(_lib.vh_query_sdf_device(self._h, _tptr(d_pts), n, _tptr(d_sdf), _tptr(d_valid)))
+
+    def backward_device(self, d_pts, n, c2w, d_grad_r=None, d_r=None, d_valid=None, d_J=None, d_pose_grad=None,
+                        d_voxel_grad=None):
+        T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32).reshape(16))
+        _check(_lib.vh_query_sdf_backward_device(self._h, _tptr(d_pts), n, _ptr(T), _tptr(d_grad_r), _tptr(d_r),
+                                                 _tptr(d_valid), _tptr(d_J), _tptr(d_pose_grad),
+                                                 _tptr(d_voxel_grad)))
+
+    def gn_refine(self, pts_cam: np.ndarray, c2w, iters: int):
+        pts = np.ascontiguousarray(pts_cam, dtype=np.float32).reshape(-1, 3)
+        T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float64).reshape(16)).copy()
+        st = GnStats()
+        _check(_lib.vh_gn_refine(self._h, _ptr(pts), pts.shape[0], _ptr(T), iters, C.byref(st)))
+        stats = {"n_valid": list(st.n_valid[:iters]), "rms": list(st.rms[:iters]),
+                 "step_norm": list(st.step_norm[:iters])}
+        return T.reshape(4, 4), stats
+
+    def pool_view(self):
+        vox = C.c_void_p()
+        keys = C.c_void_p()
+        n = C.c_int64()
+        _check(_lib.vh_pool_view(self._h, C.byref(vox), C.byref(keys), C.byref(n)))
+        return vox.value, keys.value, n.value
+
+
+class Scene:
+    """Seeded synthetic depth sequence (include/vh_synth.h)."""
+
+    def __init__(self, preset: int, seed: int):
+        lib = load()
+        h = C.c_void_p()
+        rc = lib.vh_scene_create(preset, seed, C.byref(h))
+        if rc:
+            raise VhError(rc, f"vh_scene_create({preset}) failed")
+        self._h = h
+        info = SceneInfo()
+        lib.vh_scene_info_get(h, C.byref(info))
+        self.width, self.height, self.n_frames = info.width, info.height, info.n_frames
+        self.intr = (info.fx, info.fy, info.cx, info.cy)
+        self.extent = tuple(info.extent)
+        self.n_spheres, self.n_boxes = info.n_spheres, info.n_boxes
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.vh_scene_destroy(self._h)
+            self._h = None
+
+    def pose(self, f: int) -> np.ndarray:
+        T = np.zeros(16, np.float32)
+        _check(_lib.vh_scene_pose(self._h, f, _ptr(T)))
+        return T.reshape(4, 4)
+
+    def depth(self, f: int, c2w=None) -> np.ndarray:
+        d = np.zeros((self.height, self.width), np.uint16)
+        Tp = None if c2w is None else np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        _check(_lib.vh_scene_render_u16(self._h, f, _ptr(Tp), _ptr(d)))
+        return d
+
+    def clean_depth(self, f: int, c2w=None) -> np.ndarray:
+        d = np.zeros((self.height, self.width), np.float32)
+        Tp = None if c2w is None else np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        _check(_lib.vh_scene_render_clean(self._h, f, _ptr(Tp), _ptr(d)))
+        return d
+
+    def frames(self, f0: int, n: int):
+        """(depth (n,H,W) uint16, c2w (n,4,4) float32)."""
+        d = np.zeros((n, self.height, self.width), np.uint16)
+        T = np.zeros((n, 16), np.float32)
+        _check(_lib.vh_scene_render_range(self._h, f0, n, _ptr(d), _ptr(T)))
+        return d, T.reshape(n, 4, 4)
