@@ -48,8 +48,11 @@ typedef enum vh_status {
 } vh_status;
 
 typedef enum vh_storage {
-  VH_STORAGE_F32 = 0, /* float tsdf + float weight, 8 B per voxel */
-  VH_STORAGE_F16 = 1  /* binary16 tsdf + uint16 weight, 4 B per voxel; updates in FP32 */
+  VH_STORAGE_F32 = 0,  /* float tsdf + float weight, 8 B per voxel */
+  VH_STORAGE_F16 = 1,  /* binary16 tsdf + uint16 weight, 4 B per voxel; updates in FP32 */
+  VH_STORAGE_F16R = 2  /* binary16 tsdf + uint8 weight + int8 rounding residual (units of
+                          ulp/256 of the binary16 value), 4 B per voxel; the residual carries
+                          the rounding error of each store into the next update */
 } vh_storage;
 
 typedef enum vh_depth_format { VH_DEPTH_U16 = 0, VH_DEPTH_F32 = 1 } vh_depth_format;
@@ -71,7 +74,7 @@ typedef struct vh_config {
   int32_t log2_buckets;  /* 2^log2_buckets hash buckets of 4 slots each (1..30) */
   int32_t pool_blocks;   /* voxel-block pool capacity */
   int32_t storage;       /* vh_storage */
-  int32_t weight_cap;    /* 0 = uncapped; VH_STORAGE_F16 requires 1..65535 */
+  int32_t weight_cap;    /* 0 = uncapped; VH_STORAGE_F16 requires 1..65535, F16R 1..255 */
   int32_t device;        /* HIP device ordinal */
   float origin[3];       /* ignored */
   int32_t vox_dim[3];    /* ignored */
@@ -169,7 +172,7 @@ int vh_save_ply(vh_volume* vol, const char* path);
 
 /* ------------------------------------------------------------------ block export */
 /* Copies out every allocated block: keys (3 ints each), tsdf and weight (512 floats each, in
- * (x*8+y)*8+z order, binary16 values widened exactly). *n receives the block count
+ * (x*8+y)*8+z order; 16-bit formats are decoded exactly as the kernels read them). *n receives the block count
  * (VH_ERR_TRUNCATED if it exceeds capacity; nothing is written then). Any pointer may be
  * NULL to skip that array. */
 int vh_export_blocks(vh_volume* vol, int32_t* keys, float* tsdf, float* weight,
@@ -204,7 +207,8 @@ int vh_gn_refine(vh_volume* vol, const float* pts, int64_t n, double c2w[16], in
 
 /* Pool layout for callers that wrap the voxel memory (e.g. as a tensor): device pointer of
  * the voxel pool (pool_blocks*512 voxels; FP32: float2 {tsdf, weight}; FP16: uint32
- * {binary16 tsdf | uint16 weight << 16}) and of the block keys (pool_blocks packed keys). */
+ * {binary16 tsdf | uint16 weight << 16}; F16R: uint32 {binary16 tsdf | uint8 weight << 16 |
+ * int8 residual << 24}) and of the block keys (pool_blocks packed keys). */
 int vh_pool_view(vh_volume* vol, void** d_voxels, void** d_block_keys, int64_t* n_blocks);
 
 #ifdef __cplusplus

@@ -96,6 +96,18 @@ float from_half(uint16_t h) {
   return (float)(sign ? -v : v);
 }
 
+// Compensated binary16: value = h + r * ulp(h)/256, r an int8 residual of the rounding.
+double comp_scale(uint16_t h) {
+  const int e = (h >> 10) & 31;
+  return std::ldexp(1.0, e == 0 ? -32 : e - 33);
+}
+float decode_comp(uint16_t h, int8_t r) { return (float)((double)from_half(h) + (double)r * comp_scale(h)); }
+void encode_comp(float s, uint16_t& h, int8_t& r) {
+  h = to_half(s);
+  const double q = std::round(((double)s - (double)from_half(h)) / comp_scale(h));
+  r = (int8_t)std::max(-128.0, std::min(127.0, q));
+}
+
 // ------------------------------------------------------------------ keys
 inline int64_t key_of(int x, int y, int z) {
   return ((int64_t)(x + (1 << 20)) << 42) | ((int64_t)(y + (1 << 20)) << 21) | (int64_t)(z + (1 << 20));
@@ -110,12 +122,13 @@ inline int floor_div8(int v) { return v >= 0 ? v / 8 : -((-v + 7) / 8); }
 struct Oracle {
   int W, H;
   float fx, fy, cx, cy, max_depth, min_depth, vox, mu;
-  int storage;  // 0: FP32 twin, 1: FP16 twin
+  int storage;  // 0: FP32 twin, 1: FP16 twin, 2: compensated-FP16 twin
   int cap;      // 0 = uncapped
   std::unordered_map<int64_t, int64_t> index;  // key -> block
   std::vector<int64_t> keys;                   // block -> key
   std::vector<float> s32, w32;                 // FP32 twin
-  std::vector<uint16_t> s16, w16;              // FP16 twin
+  std::vector<uint16_t> s16, w16;              // FP16 twins (binary16 bits, weight)
+  std::vector<int8_t> r8;                      // compensated FP16: rounding residual
   std::vector<double> s64, w64;                // FP64 shadow
   std::vector<int64_t> touched;                // last frame, sorted keys
   int64_t frames = 0;
@@ -128,6 +141,7 @@ struct Oracle {
     w32.resize(w32.size() + 512, 0.0f);
     s16.resize(s16.size() + 512, 0);
     w16.resize(w16.size() + 512, 0);
+    r8.resize(r8.size() + 512, 0);
     s64.resize(s64.size() + 512, 0.0);
     w64.resize(w64.size() + 512, 0.0);
     return b;
@@ -139,7 +153,10 @@ struct Oracle {
   // twin voxel value (FP32 or widened FP16) and weight
   void twin(int64_t b, int i, float& s, float& w) const {
     const size_t v = (size_t)b * 512 + i;
-    if (storage == 1) {
+    if (storage == 2) {
+      s = decode_comp(s16[v], r8[v]);
+      w = (float)w16[v];
+    } else if (storage == 1) {
       s = from_half(s16[v]);
       w = (float)w16[v];
     } else {
@@ -264,7 +281,14 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
       if (diff <= -o.mu) continue;
       const float dist = std::fmin(1.0f, diff / o.mu);
       // twin update: mean with the old weight, then the cap
-      if (o.storage == 1) {
+      if (o.storage == 2) {
+        float s = decode_comp(o.s16[base + i], o.r8[base + i]), w = (float)o.w16[base + i];
+        const float wn = w + 1.0f;
+        s = (s * w + dist) / wn;
+        w = (capf > 0.0f && wn > capf) ? capf : wn;
+        encode_comp(s, o.s16[base + i], o.r8[base + i]);
+        o.w16[base + i] = (uint16_t)w;
+      } else if (o.storage == 1) {
         float s = from_half(o.s16[base + i]), w = (float)o.w16[base + i];
         const float wn = w + 1.0f;
         s = (s * w + dist) / wn;

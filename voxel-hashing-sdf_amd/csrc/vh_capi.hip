@@ -93,7 +93,7 @@ int set_device(vh_volume* v) {
   return VH_OK;
 }
 
-size_t voxel_bytes(int storage) { return storage == VH_STORAGE_F16 ? 4 : 8; }
+size_t voxel_bytes(int storage) { return storage == VH_STORAGE_F32 ? 8 : 4; }
 
 FrameArgs frame_args(vh_volume* v, int which) {
   FrameArgs a{};
@@ -260,10 +260,13 @@ int validate(const vh_config* c) {
   if (!(c->max_depth > 0.0f) || !(c->min_depth >= 0.0f)) return fail(VH_ERR_INVALID_ARG, "bad depth range");
   if (c->log2_buckets < 1 || c->log2_buckets > 28) return fail(VH_ERR_INVALID_ARG, "log2_buckets must be in 1..28");
   if (c->pool_blocks <= 0 || c->pool_blocks > (1 << 27)) return fail(VH_ERR_INVALID_ARG, "pool_blocks must be in 1..2^27");
-  if (c->storage != VH_STORAGE_F32 && c->storage != VH_STORAGE_F16) return fail(VH_ERR_INVALID_ARG, "unknown storage");
+  if (c->storage != VH_STORAGE_F32 && c->storage != VH_STORAGE_F16 && c->storage != VH_STORAGE_F16R)
+    return fail(VH_ERR_INVALID_ARG, "unknown storage");
   if (c->weight_cap < 0) return fail(VH_ERR_INVALID_ARG, "weight_cap must be >= 0");
   if (c->storage == VH_STORAGE_F16 && (c->weight_cap < 1 || c->weight_cap > 65535))
     return fail(VH_ERR_INVALID_ARG, "FP16 storage needs weight_cap in 1..65535");
+  if (c->storage == VH_STORAGE_F16R && (c->weight_cap < 1 || c->weight_cap > 255))
+    return fail(VH_ERR_INVALID_ARG, "F16R storage needs weight_cap in 1..255");
   return VH_OK;
 }
 
@@ -651,9 +654,11 @@ int vh_export_blocks(vh_volume* v, int32_t* keys, float* tsdf, float* weight, in
     } else {
       std::vector<uint32_t> raw(nv);
       VH_HIP(hipMemcpy(raw.data(), v->voxels, nv * 4, hipMemcpyDeviceToHost));
+      const bool comp = v->cfg.storage == VH_STORAGE_F16R;
       for (size_t i = 0; i < nv; ++i) {
-        if (tsdf) tsdf[i] = f16_bits_to_f32((uint16_t)(raw[i] & 0xffffu));
-        if (weight) weight[i] = (float)(raw[i] >> 16);
+        const uint16_t h = (uint16_t)(raw[i] & 0xffffu);
+        if (tsdf) tsdf[i] = comp ? f16r_decode(h, (int)(int8_t)(raw[i] >> 24)) : f16_bits_to_f32(h);
+        if (weight) weight[i] = comp ? (float)((raw[i] >> 16) & 0xffu) : (float)(raw[i] >> 16);
       }
     }
   }

@@ -10,6 +10,8 @@
 // -ffp-contract=off, so FP32 results match a CPU twin of the same algorithm bit-for-bit.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "vh.h"
 #include "vh_internal.h"
 #include "vh_math.h"
@@ -192,6 +194,23 @@ struct VoxelIO<VH_STORAGE_F16> {
   static __device__ __forceinline__ Word pack(float s, float wt) {
     const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
     return (uint32_t)h | ((uint32_t)wt << 16);
+  }
+  static __device__ __forceinline__ Word zero() { return 0u; }
+};
+
+template <>
+struct VoxelIO<VH_STORAGE_F16R> {
+  using Word = uint32_t;
+  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
+    const uint16_t h = (uint16_t)(w & 0xffffu);
+    s = (float)__builtin_bit_cast(_Float16, h) + (float)(int)(int8_t)(w >> 24) * f16r_scale_from_bits(h);
+    wt = (float)((w >> 16) & 0xffu);
+  }
+  static __device__ __forceinline__ Word pack(float s, float wt) {
+    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
+    const float d = (s - (float)__builtin_bit_cast(_Float16, h)) * f16r_inv_scale_from_bits(h);
+    const float r = fminf(127.0f, fmaxf(-128.0f, roundf(d)));
+    return (uint32_t)h | ((uint32_t)wt << 16) | ((uint32_t)(uint8_t)(int8_t)(int)r << 24);
   }
   static __device__ __forceinline__ Word zero() { return 0u; }
 };
@@ -662,20 +681,28 @@ hipError_t launch_alloc(const FrameArgs& a, int depth_u16, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
-  if (storage == VH_STORAGE_F16) {
-    if (depth_u16) hipLaunchKernelGGL((k_integrate<VH_STORAGE_F16, true>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_integrate<VH_STORAGE_F16, false>), dim3(grid), dim3(256), 0, st, a);
-  } else {
-    if (depth_u16) hipLaunchKernelGGL((k_integrate<VH_STORAGE_F32, true>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_integrate<VH_STORAGE_F32, false>), dim3(grid), dim3(256), 0, st, a);
+// Calls f(std::integral_constant<int, storage>) for the runtime storage format.
+template <class F>
+static void with_storage(int storage, F f) {
+  switch (storage) {
+    case VH_STORAGE_F16: f(std::integral_constant<int, VH_STORAGE_F16>()); break;
+    case VH_STORAGE_F16R: f(std::integral_constant<int, VH_STORAGE_F16R>()); break;
+    default: f(std::integral_constant<int, VH_STORAGE_F32>()); break;
   }
+}
+
+hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
+  with_storage(storage, [&](auto S) {
+    if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), dim3(grid), dim3(256), 0, st, a);
+  });
   return hipGetLastError();
 }
 
 hipError_t launch_mc(const McArgs& a, int storage, int grid, hipStream_t st) {
-  if (storage == VH_STORAGE_F16) hipLaunchKernelGGL(k_mc<VH_STORAGE_F16>, dim3(grid), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(k_mc<VH_STORAGE_F32>, dim3(grid), dim3(256), 0, st, a);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_mc<decltype(S)::value>, dim3(grid), dim3(256), 0, st, a);
+  });
   return hipGetLastError();
 }
 
@@ -689,20 +716,19 @@ static int query_grid(int64_t n) {
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf, uint8_t* valid,
                         hipStream_t st) {
   const int g = query_grid(n);
-  if (storage == VH_STORAGE_F16) hipLaunchKernelGGL(k_query<VH_STORAGE_F16>, dim3(g), dim3(256), 0, st, q, pts, n, sdf, valid);
-  else hipLaunchKernelGGL(k_query<VH_STORAGE_F32>, dim3(g), dim3(256), 0, st, q, pts, n, sdf, valid);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_query<decltype(S)::value>, dim3(g), dim3(256), 0, st, q, pts, n, sdf, valid);
+  });
   return hipGetLastError();
 }
 
 hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, int64_t n, const float* T12,
                            const float* grad_r, float* r, uint8_t* valid, float* J, double* partials, int grid,
                            float* voxel_grad, hipStream_t st) {
-  if (storage == VH_STORAGE_F16)
-    hipLaunchKernelGGL(k_backward<VH_STORAGE_F16>, dim3(grid), dim3(256), 0, st, q, pts, n, T12, grad_r, r, valid, J,
-                       partials, voxel_grad);
-  else
-    hipLaunchKernelGGL(k_backward<VH_STORAGE_F32>, dim3(grid), dim3(256), 0, st, q, pts, n, T12, grad_r, r, valid, J,
-                       partials, voxel_grad);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_backward<decltype(S)::value>, dim3(grid), dim3(256), 0, st, q, pts, n, T12, grad_r, r,
+                       valid, J, partials, voxel_grad);
+  });
   return hipGetLastError();
 }
 
@@ -712,8 +738,9 @@ hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStr
 }
 
 hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st) {
-  if (storage == VH_STORAGE_F16) hipLaunchKernelGGL(k_gn_accum<VH_STORAGE_F16>, dim3(grid), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(k_gn_accum<VH_STORAGE_F32>, dim3(grid), dim3(256), 0, st, a);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_gn_accum<decltype(S)::value>, dim3(grid), dim3(256), 0, st, a);
+  });
   return hipGetLastError();
 }
 

@@ -164,6 +164,32 @@ VH_HD float f16_bits_to_f32(uint16_t h) {
   return c.f;
 }
 
+// Compensated binary16 (storage F16R): value = binary16 h + r * ulp(h)/256 with r an int8.
+// ulp(h)/256 is a power of two, so decoding is exact in FP32 (11 + 8 significant bits) and the
+// encoder's residual is exact before its final rounding to an integer.
+VH_HD float f16r_scale_from_bits(uint16_t h) {  // ulp(h) / 256
+  const uint32_t e = (h >> 10) & 0x1fu;
+  union { uint32_t u; float f; } c;
+  c.u = (e == 0 ? 95u : e + 94u) << 23;  // 2^-32 for subnormals, else 2^(e-33)
+  return c.f;
+}
+VH_HD float f16r_inv_scale_from_bits(uint16_t h) {
+  const uint32_t e = (h >> 10) & 0x1fu;
+  union { uint32_t u; float f; } c;
+  c.u = (e == 0 ? 159u : 160u - e) << 23;
+  return c.f;
+}
+VH_HD float f16r_decode(uint16_t h, int r) {
+  return f16_bits_to_f32(h) + (float)r * f16r_scale_from_bits(h);
+}
+// Residual of s against its binary16 rounding h, in units of ulp(h)/256, clamped to int8.
+VH_HD int f16r_residual(float s, uint16_t h) {
+  const float d = (s - f16_bits_to_f32(h)) * f16r_inv_scale_from_bits(h);
+  float r = roundf(d);
+  r = r > 127.0f ? 127.0f : (r < -128.0f ? -128.0f : r);
+  return (int)r;
+}
+
 // ---------------------------------------------------------------- truncation-band DDA
 // Enumerates every block whose cube is crossed by the ray segment of pixel (u,v) between
 // camera depths max(d-mu, min_depth) and d+mu (Amanatides-Woo traversal in block units).
