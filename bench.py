@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="short run for hardware-counter passes: one FP32 integration pass over the "
+                         "sequence plus a 1 GiB device copy that calibrates FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the volume is not sharded across GPUs in this build "
@@ -89,6 +92,21 @@ def main():
     cap = 48 << 20  # triangles (1.7 GB of float xyz)
     d_xyz = torch.empty(cap * 9, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
+
+    if args.counters:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        vol = vh.Volume(vh.config_from_workload(w))
+        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False)
+        st = vol.stats()
+        vol.close()
+        log(f"bench: counters run: {nf} frames, {st['touched_total']} touched blocks in total, "
+            f"copy calibration 1 GiB read + 1 GiB written")
+        print(json.dumps({"counters_run": True, "frames": nf, "touched_total": st["touched_total"],
+                          "copy_bytes": 1 << 30}), flush=True)
+        return
 
     results = {}
     variants = [("f32", {})]

@@ -257,29 +257,42 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
   const double T64[12] = {T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11]};
   const double vox64 = o.vox, mu64 = o.mu, scale64 = scale;
   const float capf = (float)o.cap;
+  const float inv_mu = 1.0f / o.mu;
 #pragma omp parallel for schedule(dynamic, 4)
   for (long k = 0; k < (long)t.size(); ++k) {
     int bx, by, bz;
     key_xyz(t[(size_t)k], bx, by, bz);
     const size_t base = (size_t)blk[(size_t)k] * 512;
+    float cb[3];  // camera coordinates of the block origin, R^T (o - t)
+    {
+      const float d0 = (float)(bx * 8) * o.vox - T[3], d1 = (float)(by * 8) * o.vox - T[7],
+                  d2 = (float)(bz * 8) * o.vox - T[11];
+      for (int r = 0; r < 3; ++r) cb[r] = T[r] * d0 + T[4 + r] * d1 + T[8 + r] * d2;
+    }
     for (int i = 0; i < 512; ++i) {
       const int gx = bx * 8 + (i >> 6), gy = by * 8 + ((i >> 3) & 7), gz = bz * 8 + (i & 7);
-      // FP32 geometry: decides observation, pixel and the FP32 distance
-      const float px = (float)gx * o.vox, py = (float)gy * o.vox, pz = (float)gz * o.vox;
-      const float q0 = px - T[3], q1 = py - T[7], q2 = pz - T[11];
-      const float xc = T[0] * q0 + T[4] * q1 + T[8] * q2;
-      const float yc = T[1] * q0 + T[5] * q1 + T[9] * q2;
-      const float zc = T[2] * q0 + T[6] * q1 + T[10] * q2;
+      // FP32 geometry: decides observation, pixel and the FP32 distance. The camera point is
+      // formed per block origin and voxel step, R^T(o - t) + vox R^T (k, j, i), as three
+      // partial sums; one reciprocal of z feeds both image coordinates, which round to the
+      // nearest pixel with ties to even; the truncation multiplies by 1/mu. (The reference
+      // rotates each voxel's p - t, divides x/z, y/z and by mu, and rounds ties away from
+      // zero: src/tsdf.cu:75-93,738, compiled with -use_fast_math.)
+      const int vx = i >> 6, vy = (i >> 3) & 7, vz = i & 7;
+      float cam[3];
+      for (int r = 0; r < 3; ++r)
+        cam[r] = (cb[r] + ((float)vy * (T[4 + r] * o.vox) + (float)vz * (T[8 + r] * o.vox))) + (float)vx * (T[r] * o.vox);
+      const float xc = cam[0], yc = cam[1], zc = cam[2];
       if (!(zc > 0.0f)) continue;
-      const float uf = std::round(o.fx * (xc / zc) + o.cx);
-      const float vf = std::round(o.fy * (yc / zc) + o.cy);
+      const float rz = 1.0f / zc;
+      const float uf = std::nearbyint(o.fx * (xc * rz) + o.cx);
+      const float vf = std::nearbyint(o.fy * (yc * rz) + o.cy);
       if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
       const size_t pix = (size_t)vf * W + (size_t)uf;
       const float dv = d[pix];
       if (!(dv > 0.0f && dv <= o.max_depth)) continue;
       const float diff = dv - zc;
-      if (diff <= -o.mu) continue;
-      const float dist = std::fmin(1.0f, diff / o.mu);
+      if (!(diff > -o.mu)) continue;
+      const float dist = std::fmin(1.0f, diff * inv_mu);
       // twin update: mean with the old weight, then the cap
       if (o.storage == 2) {
         float s = decode_comp(o.s16[base + i], o.r8[base + i]), w = (float)o.w16[base + i];

@@ -38,9 +38,21 @@ int fail(int code, const std::string& msg) {
       return fail(VH_ERR_DEVICE, std::string(#call " failed: ") + hipGetErrorString(e_));      \
   } while (0)
 
-constexpr int kIntegrateGrid = 1024;  // persistent integration workgroups (4 per CU)
-constexpr int kMcGrid = 2048;
+constexpr int kIntegrateGrid = 8192;  // single-wave workgroups: 32 per CU, a multiple of kStripes
 constexpr int kGnGrid = 512;
+
+// Launch-shape tuning: VH_INTEGRATE_GRID overrides the integration grid (rounded to a
+// multiple of kStripes).
+int integrate_grid() {
+  static const int g = [] {
+    const char* e = std::getenv("VH_INTEGRATE_GRID");
+    long v = e ? std::strtol(e, nullptr, 10) : kIntegrateGrid;
+    if (v < kStripes) v = kStripes;
+    if (v > (1 << 20)) v = 1 << 20;
+    return (int)(v / kStripes * kStripes);
+  }();
+  return g;
+}
 
 }  // namespace
 
@@ -56,6 +68,7 @@ struct vh_volume {
   unsigned long long* block_keys = nullptr;
   void* voxels = nullptr;
   uint32_t* touched = nullptr;
+  uint32_t stripe_cap = 0;
   FrameCtl* ctl = nullptr;
   // staged host input (one frame, both formats) and its pose
   void* stage_depth = nullptr;
@@ -73,10 +86,13 @@ struct vh_volume {
   std::vector<hipEvent_t> ev_pending;  // groups of 3: start, after alloc, after integrate
   double alloc_ms = 0.0, integrate_ms = 0.0;
   int64_t profiled_frames = 0;
-  // marching-cubes output buffer (grown on demand)
+  // marching-cubes output buffer (grown on demand) and scan scratch
   float* mc_tris = nullptr;
   int32_t* mc_cubes = nullptr;
   int64_t mc_cap = 0;
+  uint32_t* mc_counts = nullptr;   // pool_blocks
+  uint32_t* mc_chunks = nullptr;   // pool_blocks / kScanChunk + 1
+  uint32_t* mc_total = nullptr;
   // scratch for queries / Gauss-Newton
   double* gn_partials = nullptr;
   double* gn_pose = nullptr;
@@ -109,7 +125,7 @@ FrameArgs frame_args(vh_volume* v, int which) {
     a.poses = v->stage_pose;
     a.n_seq = 1;
     a.cursor_id = 0;
-    a.depth_scale = 0.0f;  // set per frame through stage_pose[16] (see below)
+    a.depth_scale = 1.0f;  // the caller sets the uint16 scale of a staged frame
   }
   a.W = c.width;
   a.H = c.height;
@@ -119,12 +135,14 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.vox = c.vox_size;
   a.bsz = c.vox_size * (float)kB;
   a.mu = c.trunc;
+  a.inv_mu = 1.0f / c.trunc;
   a.cap = (float)c.weight_cap;
   a.hash = HashView{v->keys, v->vals, c.log2_buckets, ((uint64_t)1 << c.log2_buckets) - 1};
   a.stamps = v->stamps;
   a.block_keys = v->block_keys;
   a.pool_cap = (uint32_t)c.pool_blocks;
   a.touched = v->touched;
+  a.stripe_cap = v->stripe_cap;
   a.voxels = v->voxels;
   a.ctl = v->ctl;
   return a;
@@ -140,7 +158,7 @@ int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale) {
     a.depth_scale = u16 ? staged_scale : 1.0f;
   }
   VH_HIP(launch_alloc(a, u16, v->stream));
-  VH_HIP(launch_integrate(a, v->cfg.storage, u16, kIntegrateGrid, v->stream));
+  VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
   return VH_OK;
 }
 
@@ -165,16 +183,13 @@ int launch_frame(vh_volume* v, int which, float staged_scale) {
     VH_HIP(hipEventRecord(e0, v->stream));
     VH_HIP(launch_alloc(a, u16, v->stream));
     VH_HIP(hipEventRecord(e1, v->stream));
-    VH_HIP(launch_integrate(a, v->cfg.storage, u16, kIntegrateGrid, v->stream));
+    VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
     VH_HIP(hipEventRecord(e2, v->stream));
     v->ev_pending.push_back(e0);
     v->ev_pending.push_back(e1);
     v->ev_pending.push_back(e2);
     return VH_OK;
   }
-  // The staged graphs bake the depth scale into their kernel arguments; a different scale
-  // re-captures them.
-  static_assert(sizeof(float) == 4, "");
   hipGraphExec_t& g = v->graph[which];
   if (!g) {
     hipGraph_t graph = nullptr;
@@ -234,6 +249,8 @@ int sync_and_check(vh_volume* v) {
   VH_HIP(hipMemcpy(&v->ctl->flags, &zero, sizeof zero, hipMemcpyHostToDevice));
   if (flags & kFlagPoolFull)
     return fail(VH_ERR_POOL_FULL, "voxel-block pool is full: blocks were not integrated");
+  if (flags & kFlagTouchedFull)
+    return fail(VH_ERR_POOL_FULL, "a frame touched more blocks than the per-frame list holds (sized from pool_blocks)");
   if (flags & kFlagHashFull)
     return fail(VH_ERR_HASH_FULL, "hash table probe limit reached: blocks were not allocated");
   return fail(VH_ERR_RANGE, "a truncation band left the representable block range");
@@ -290,8 +307,8 @@ void free_volume(vh_volume* v) {
   for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
-                  v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->gn_partials, v->gn_pose, v->gn_stats,
-                  v->bw_pose};
+                  v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
+                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -308,20 +325,34 @@ QueryArgs query_args(vh_volume* v) {
 }
 
 int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
+  if (!v->mc_counts) {
+    int rc;
+    if ((rc = dev_alloc(&v->mc_counts, (size_t)v->cfg.pool_blocks * 4)) ||
+        (rc = dev_alloc(&v->mc_chunks, ((size_t)v->cfg.pool_blocks / kScanChunk + 1) * 4)) ||
+        (rc = dev_alloc(&v->mc_total, 4)))
+      return rc;
+  }
+  FrameCtl c{};
+  int rc = read_ctl(v, &c);
+  if (rc) return rc;
   McArgs a{};
   a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
   a.block_keys = v->block_keys;
   a.voxels = v->voxels;
   a.pool_cap = (uint32_t)v->cfg.pool_blocks;
-  a.ctl = v->ctl;
+  a.n_blocks = std::min<uint32_t>(c.pool_count, (uint32_t)v->cfg.pool_blocks);
   a.vox = v->cfg.vox_size;
+  a.counts = v->mc_counts;
+  a.chunk_sums = v->mc_chunks;
+  a.total = v->mc_total;
   a.tris = d_xyz;
   a.cubes = d_cubes;
   a.capacity = (uint32_t)std::min<int64_t>(capacity, 0xffffffffll);
-  VH_HIP(hipMemsetAsync(&v->ctl->mc_tri_count, 0, sizeof(uint32_t), v->stream));
-  VH_HIP(launch_mc(a, v->cfg.storage, kMcGrid, v->stream));
   uint32_t cnt = 0;
-  VH_HIP(hipMemcpyAsync(&cnt, &v->ctl->mc_tri_count, sizeof cnt, hipMemcpyDeviceToHost, v->stream));
+  if (a.n_blocks) {
+    VH_HIP(launch_mc(a, v->cfg.storage, v->stream));
+    VH_HIP(hipMemcpyAsync(&cnt, v->mc_total, sizeof cnt, hipMemcpyDeviceToHost, v->stream));
+  }
   VH_HIP(hipStreamSynchronize(v->stream));
   *n_tris = cnt;
   if ((int64_t)cnt > capacity) return fail(VH_ERR_TRUNCATED, "mesh larger than the output buffer");
@@ -402,12 +433,15 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
     return fail(VH_ERR_DEVICE, "hipStreamCreate failed");
   }
   v->n_slots = (uint64_t)kSlotsPerBucket << cfg->log2_buckets;
+  // touched blocks per frame, per stripe: room for twice an even share of the whole pool
+  v->stripe_cap = (uint32_t)(cfg->pool_blocks / (kStripes / 2) + 4096);
   const size_t pool = (size_t)cfg->pool_blocks;
   const size_t frame_px = (size_t)cfg->width * cfg->height;
   if ((rc = dev_alloc(&v->keys, v->n_slots * 8)) || (rc = dev_alloc(&v->vals, v->n_slots * 8)) ||
       (rc = dev_alloc(&v->stamps, v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
       (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||
-      (rc = dev_alloc(&v->touched, pool * 4)) || (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
+      (rc = dev_alloc(&v->touched, (size_t)kStripes * v->stripe_cap * 4)) ||
+      (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
       (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float)))) {
     std::string msg = g_last_error;
     free_volume(v);

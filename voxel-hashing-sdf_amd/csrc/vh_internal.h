@@ -8,8 +8,12 @@
 //   stamps[4 * 2^log2_buckets]      u32 per slot: last frame that touched the block;
 //   block_keys[pool_blocks]         u64 key of each pool block (pool order);
 //   voxels[pool_blocks * 512]       FP32: float2 {tsdf, weight}; FP16: u32 {f16 tsdf | u16 w << 16};
-//   touched[pool_blocks]            u32 slot ids of the blocks crossed by this frame's bands;
+//   touched[kStripes * stripe_cap]  u32 slot ids of the blocks crossed by this frame's bands,
+//                                   in kStripes sub-lists with separate counters;
 //   ctl                             counters of the frame pipeline (FrameCtl).
+//
+// Device-scope atomics on one address serialise (about 11 ns each on MI355X), so every
+// counter that many workgroups update is striped: kStripes copies, one per 128-byte line.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -18,23 +22,25 @@
 
 namespace vh {
 
-enum : uint32_t { kFlagPoolFull = 1u, kFlagHashFull = 2u, kFlagRange = 4u };
+enum : uint32_t { kFlagPoolFull = 1u, kFlagHashFull = 2u, kFlagRange = 4u, kFlagTouchedFull = 8u };
 constexpr int kMaxProbe = 64;  // buckets examined before a key is declared unplaceable
+constexpr int kStripes = 64;   // striped counters / touched sub-lists
+constexpr int kLine = 32;      // u32 per 128-byte line
 
 struct FrameCtl {
   uint32_t frame_id;       // stamp of the frame being integrated (starts at 1)
-  uint32_t touched_count;  // distinct blocks appended to `touched` this frame
-  uint32_t done_wgs;       // integration workgroups finished this frame
   uint32_t pool_count;     // keys inserted (pool indices handed out, may pass capacity)
   uint32_t flags;          // kFlag* raised since the host last read them
   uint32_t touched_last;
   uint32_t cursor[2];      // next frame of the staged input (0) and bound sequence (1)
+  uint32_t done_top;       // integration stripes finished this frame
+  uint32_t pool_at_frame_start;
   unsigned long long touched_total;
   unsigned long long frames;
   unsigned long long new_total;
-  uint32_t pool_at_frame_start;
-  uint32_t mc_tri_count;   // marching-cubes output counter
-  uint32_t pad[4];
+  uint32_t pad0[18];
+  uint32_t tcount[kStripes * kLine];  // touched sub-list lengths (one line each)
+  uint32_t done[kStripes * kLine];    // integration workgroups finished, per stripe
 };
 
 struct HashView {
@@ -54,13 +60,14 @@ struct FrameArgs {
   // camera and volume parameters
   int32_t W, H;
   Intr K;
-  float max_depth, min_depth, vox, bsz, mu, cap;
+  float max_depth, min_depth, vox, bsz, mu, inv_mu, cap;
   // tables
   HashView hash;
   uint32_t* stamps;
   unsigned long long* block_keys;
   uint32_t pool_cap;
   uint32_t* touched;
+  uint32_t stripe_cap;
   void* voxels;
   FrameCtl* ctl;
 };
@@ -70,12 +77,16 @@ struct McArgs {
   const unsigned long long* block_keys;
   const void* voxels;
   uint32_t pool_cap;
-  FrameCtl* ctl;
+  uint32_t n_blocks;   // pool blocks to extract
   float vox;
-  float* tris;       // capacity * 9
-  int32_t* cubes;    // capacity * 4, may be null
+  uint32_t* counts;    // per pool block: triangles (count pass), then exclusive offsets
+  uint32_t* chunk_sums;// per 1024 pool blocks
+  uint32_t* total;     // total triangles
+  float* tris;         // capacity * 9
+  int32_t* cubes;      // capacity * 4, may be null
   uint32_t capacity;
 };
+constexpr int kScanChunk = 1024;
 
 struct QueryArgs {
   HashView hash;
@@ -99,7 +110,8 @@ struct GnArgs {
 hipError_t launch_alloc(const FrameArgs& a, int depth_u16, hipStream_t st);
 hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid,
                             hipStream_t st);
-hipError_t launch_mc(const McArgs& a, int storage, int grid, hipStream_t st);
+// count pass, two-level exclusive scan, write pass (all on `st`)
+hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,
                         uint8_t* valid, hipStream_t st);
 hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, int64_t n,

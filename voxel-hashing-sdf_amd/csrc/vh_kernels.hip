@@ -90,14 +90,73 @@ __device__ __forceinline__ uint32_t frame_slot(const FrameArgs& a) {
 }
 
 // ============================================================================ allocation
-// Workgroup = 16x16 pixels, each wave an 8x8 tile, so the lanes of a wave cross mostly the
-// same blocks: hash lines are shared and the first-touch atomics are deduplicated per wave.
+// Workgroup = 16x16 pixels. Phase 1: every lane walks its pixel's band and inserts the block
+// keys into an LDS hash set, so each distinct block of the tile is kept once (a tile's ~200
+// pixel-block visits typically reduce to ~20 keys). Phase 2: the distinct keys go to the
+// global hash (find or insert), take a pool index if new, claim the block for this frame
+// with an exchange on its stamp, and the claimers append the slot to the workgroup's stripe
+// of the touched list.
+constexpr int kSetSlots = 1024;  // LDS key set (8 KB)
+constexpr int kUniqCap = 1024;   // distinct keys kept per tile (8 KB)
+
+__device__ __forceinline__ uint32_t set_hash(uint64_t key) {
+  return (uint32_t)((key * 0x9e3779b97f4a7c15ull) >> 54) & (kSetSlots - 1);
+}
+
+// Global part for the active lanes' keys: find-or-insert, pool index for new keys (one
+// atomic per wave), first touch of the frame, append to the stripe (one atomic per wave).
+__device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t frame, int lane,
+                           uint32_t stripe) {
+  FrameCtl* ctl = a.ctl;
+  bool inserted = false;
+  int64_t slot = -1;
+  if (act) {
+    slot = find_or_insert(a.hash, key, inserted);
+    if (slot < 0) raise_flag(ctl, kFlagHashFull);
+  }
+  const uint64_t im = __ballot(inserted);
+  if (im) {
+    const int leader = first_lane(im);
+    uint32_t pbase = 0;
+    if (lane == leader) pbase = atomicAdd(&ctl->pool_count, (uint32_t)__popcll(im));
+    pbase = read_lane_u32(pbase, leader);
+    if (inserted) {
+      const uint32_t idx = pbase + lanes_below(im);
+      uint32_t stored = kNoBlock;
+      if (idx < a.pool_cap) {
+        a.block_keys[idx] = key;
+        stored = idx;
+      } else {
+        raise_flag(ctl, kFlagPoolFull);
+      }
+      a.hash.vals[slot] = (unsigned long long)stored | ((unsigned long long)frame << 32);
+    }
+  }
+  const bool first = slot >= 0 && atomicExch(&a.stamps[slot], frame) != frame;
+  const uint64_t fm = __ballot(first);
+  if (fm) {
+    const int leader = first_lane(fm);
+    uint32_t tbase = 0;
+    if (lane == leader) tbase = atomicAdd(&ctl->tcount[stripe * kLine], (uint32_t)__popcll(fm));
+    tbase = read_lane_u32(tbase, leader);
+    if (first) {
+      const uint32_t pos = tbase + lanes_below(fm);
+      if (pos < a.stripe_cap) a.touched[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
+      else raise_flag(ctl, kFlagTouchedFull);
+    }
+  }
+}
+
 template <bool kU16>
 __global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ unsigned long long s_set[kSetSlots];
+  __shared__ unsigned long long s_uniq[kUniqCap];
+  __shared__ uint32_t s_nuniq;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tiles_x = (a.W + 15) >> 4;
   const int u = ((int)(blockIdx.x % tiles_x) << 4) + ((wave & 1) << 3) + (lane & 7);
   const int v = ((int)(blockIdx.x / tiles_x) << 4) + ((wave >> 1) << 3) + (lane >> 3);
+  const uint32_t stripe = blockIdx.x & (kStripes - 1);
   FrameCtl* ctl = a.ctl;
   const uint32_t frame = ctl->frame_id;
   const uint32_t fs = frame_slot(a);
@@ -108,6 +167,10 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
   const size_t elem = kU16 ? 2 : 4;
   const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
 
+  for (int i = tid; i < kSetSlots; i += 256) s_set[i] = kEmptyKey;
+  if (tid == 0) s_nuniq = 0;
+  __syncthreads();
+
   BandDDA s;
   bool active = false;
   if (u < a.W && v < a.H) {
@@ -117,59 +180,37 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
       if (!active) raise_flag(ctl, kFlagRange);
     }
   }
-
+  // phase 1: distinct keys of the tile into LDS; keys that do not fit are claimed directly
   while (__ballot(active)) {
-    const uint64_t key = active ? pack_key(s.ix, s.iy, s.iz) : kEmptyKey;
-    bool inserted = false;
-    int64_t slot = -1;
+    bool spill = false;
+    uint64_t key = kEmptyKey;
     if (active) {
-      slot = find_or_insert(a.hash, key, inserted);
-      if (slot < 0) raise_flag(ctl, kFlagHashFull);
-    }
-    // pool indices for the keys this wave created: one atomic per wave
-    const uint64_t im = __ballot(inserted);
-    if (im) {
-      const int leader = first_lane(im);
-      uint32_t pbase = 0;
-      if (lane == leader) pbase = atomicAdd(&ctl->pool_count, (uint32_t)__popcll(im));
-      pbase = read_lane_u32(pbase, leader);
-      if (inserted) {
-        const uint32_t idx = pbase + lanes_below(im);
-        uint32_t stored = kNoBlock;
-        if (idx < a.pool_cap) {
-          a.block_keys[idx] = key;
-          stored = idx;
-        } else {
-          raise_flag(ctl, kFlagPoolFull);
+      key = pack_key(s.ix, s.iy, s.iz);
+      uint32_t h = set_hash(key);
+      spill = true;
+      for (int p = 0; p < 32; ++p) {
+        const unsigned long long old = atomicCAS(&s_set[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (old == key) { spill = false; break; }
+        if (old == kEmptyKey) {
+          const uint32_t n = atomicAdd(&s_nuniq, 1u);
+          if (n < kUniqCap) { s_uniq[n] = key; spill = false; }
+          break;
         }
-        a.hash.vals[slot] = (unsigned long long)stored | ((unsigned long long)frame << 32);
+        h = (h + 1) & (kSetSlots - 1);
       }
     }
-    // first touch of the block in this frame: one exchange per distinct slot per wave
-    bool need = false;
-    if (slot >= 0) need = a.stamps[slot] != frame;
-    bool first = false;
-    uint64_t nm = __ballot(need);
-    while (nm) {
-      const int leader = first_lane(nm);
-      const uint32_t ls_lo = read_lane_u32((uint32_t)slot, leader);
-      if (lane == leader) first = atomicExch(&a.stamps[slot], frame) != frame;
-      if (need && (uint32_t)slot == ls_lo) need = false;
-      nm = __ballot(need);
-    }
-    const uint64_t fm = __ballot(first);
-    if (fm) {
-      const int leader = first_lane(fm);
-      uint32_t tbase = 0;
-      if (lane == leader) tbase = atomicAdd(&ctl->touched_count, (uint32_t)__popcll(fm));
-      tbase = read_lane_u32(tbase, leader);
-      if (first) {
-        const uint32_t pos = tbase + lanes_below(fm);
-        if (pos < a.pool_cap) a.touched[pos] = (uint32_t)slot;
-        else raise_flag(ctl, kFlagPoolFull);
-      }
-    }
+    if (__ballot(spill)) claim_keys(a, key, spill, frame, lane, stripe);
     if (active) active = band_next(s);
+  }
+  __syncthreads();
+  // phase 2: the tile's distinct keys
+  const uint32_t n = min(s_nuniq, (uint32_t)kUniqCap);
+  for (uint32_t b0 = 0; b0 < n; b0 += 256) {
+    const uint32_t i = b0 + tid;
+    const bool act = i < n;
+    const uint32_t wave_first = b0 + (uint32_t)(wave << 6);
+    if (wave_first >= n) break;  // whole wave idle (uniform per wave)
+    claim_keys(a, act ? s_uniq[i] : kEmptyKey, act, frame, lane, stripe);
   }
 }
 
@@ -215,12 +256,28 @@ struct VoxelIO<VH_STORAGE_F16R> {
   static __device__ __forceinline__ Word zero() { return 0u; }
 };
 
+// One wave per touched block: lane l owns voxels l + 64k (x = k, y = l >> 3, z = l & 7), so
+// each of its 8 loads is one coalesced 512-byte wave access and all 8 are in flight at once.
+// Workgroup b walks stripe b % kStripes of the touched list. The last workgroup to finish
+// (two-level striped counter) closes the frame.
 template <int kStorage, bool kU16>
-__global__ __launch_bounds__(256) void k_integrate(FrameArgs a) {
+__global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
-  const uint32_t n = min(ctl->touched_count, a.pool_cap);
+  const int lane = threadIdx.x;
+  // The 64 stripes of the touched list, seen as one index space: lane s holds stripe s's
+  // length and exclusive offset, so every workgroup gets an equal share whatever the
+  // stripe sizes.
+  const uint32_t my_len = min(ctl->tcount[lane * kLine], a.stripe_cap);
+  uint32_t incl = my_len;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+    if (lane >= off) incl += y;
+  }
+  const uint32_t my_off = incl - my_len;
+  const uint32_t n = read_lane_u32(incl, 63);
   const uint32_t frame = ctl->frame_id;
   const uint32_t fs = frame_slot(a);
   const float* Tp = a.poses + 16 * (size_t)fs;
@@ -230,58 +287,102 @@ __global__ __launch_bounds__(256) void k_integrate(FrameArgs a) {
   const size_t elem = kU16 ? 2 : 4;
   const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
   auto dep = [&](int pix) { return depth_at<kU16>(base, a.depth_scale, pix); };
-  const int tid = threadIdx.x;
+  BlockFrame bf;
+  block_frame(T, a.vox, bf);
 
-  for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
-    const uint32_t sl = a.touched[t];
-    const unsigned long long val = a.hash.vals[sl];
+  // Lane j fetches the list entry, slot value and key of this workgroup's j-th block, so the
+  // per-block chain is a single round trip (voxels + depth).
+  for (uint32_t t0 = blockIdx.x; t0 < n; t0 += 64u * gridDim.x) {
+    const uint32_t tj = t0 + (uint32_t)lane * gridDim.x;
+    const uint32_t tq = min(tj, n - 1u);
+    int lo = 0, hi = 63;  // stripe of entry tq: the last stripe whose offset is <= tq
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {  // all lanes active: the shuffles read defined values
+      const int mid = (lo + hi + 1) >> 1;
+      const uint32_t om = (uint32_t)__shfl((int)my_off, mid);
+      if (om <= tq) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t stripe_off = (uint32_t)__shfl((int)my_off, lo);
+    unsigned long long jval = ~0ull, jkey = 0ull;
+    if (tj < n) {
+      const uint32_t sl = a.touched[(size_t)lo * a.stripe_cap + (tj - stripe_off)];
+      jval = a.hash.vals[sl];
+      jkey = a.hash.keys[sl];
+    }
+    const uint32_t nj = min(64u, (n - t0 + gridDim.x - 1) / gridDim.x);
+    for (uint32_t j = 0; j < nj; ++j) {
+    const unsigned long long val = ((unsigned long long)read_lane_u32((uint32_t)(jval >> 32), j) << 32) |
+                                   read_lane_u32((uint32_t)jval, j);
+    const unsigned long long keyj = ((unsigned long long)read_lane_u32((uint32_t)(jkey >> 32), j) << 32) |
+                                    read_lane_u32((uint32_t)jkey, j);
     const uint32_t idx = (uint32_t)val;
     if (idx >= a.pool_cap) {
-      if (tid == 0) raise_flag(ctl, kFlagPoolFull);
+      if (lane == 0) raise_flag(ctl, kFlagPoolFull);
       continue;
     }
     const bool fresh = (uint32_t)(val >> 32) == frame;
     int bx, by, bz;
-    unpack_key(a.hash.keys[sl], bx, by, bz);
-    Word* vp = reinterpret_cast<Word*>(a.voxels) + (size_t)idx * kB3;
-    Word old[2];
+    unpack_key(keyj, bx, by, bz);
+    Word* vp = reinterpret_cast<Word*>(a.voxels) + (size_t)idx * kB3 + lane;
+    Word old[8];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) old[h] = fresh ? IO::zero() : vp[tid + 256 * h];
+    for (int k = 0; k < 8; ++k) old[k] = fresh ? IO::zero() : vp[64 * k];
+    float cb[3], cl[3];
+    block_origin_cam(T, a.vox, bx, by, bz, cb);
+    column_cam(bf, cb, lane >> 3, lane & 7, cl);
+    // all projections, then all depth fetches, then the updates: the 8 fetches are in flight
+    // together instead of one round trip per voxel
+    float zc[8];
+    int pix[8];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int i = tid + 256 * h;
-      const int lx = i >> 6, ly = (i >> 3) & 7, lz = i & 7;
+    for (int k = 0; k < 8; ++k) {
+      const float fk = (float)k;
+      zc[k] = cl[2] + fk * bf.ax[2];
+      pix[k] = project_cam(cl[0] + fk * bf.ax[0], cl[1] + fk * bf.ax[1], zc[k], a.K, a.W, a.H);
+    }
+    float dv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dv[k] = dep(pix[k] < 0 ? 0 : pix[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
       float dist;
-      if (observe_voxel(T, a.K, a.W, a.H, a.vox, a.max_depth, a.mu, bx * kB + lx, by * kB + ly,
-                        bz * kB + lz, dep, dist)) {
-        float sv, wv;
-        IO::unpack(old[h], sv, wv);
-        tsdf_update(sv, wv, dist, a.cap);
-        vp[i] = IO::pack(sv, wv);
-      } else if (fresh) {
-        vp[i] = IO::zero();
-      }
+      const bool ok = truncate_obs(pix[k], dv[k], zc[k], a.max_depth, a.mu, a.inv_mu, dist);
+      float sv, wv;
+      IO::unpack(old[k], sv, wv);
+      tsdf_update(sv, wv, dist, a.cap);
+      if (ok) vp[64 * k] = IO::pack(sv, wv);
+      else if (fresh) vp[64 * k] = IO::zero();
+    }
     }
   }
 
-  // the last workgroup to finish closes the frame
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    const uint32_t prev = atomicAdd(&ctl->done_wgs, 1u);
-    if (prev == gridDim.x - 1) {
-      __threadfence();
-      const uint32_t pc = atomicAdd(&ctl->pool_count, 0u);
-      ctl->touched_last = n;
-      ctl->touched_total += n;
-      ctl->frames += 1ull;
-      ctl->new_total += (unsigned long long)(pc - ctl->pool_at_frame_start);
-      ctl->pool_at_frame_start = pc;
-      ctl->cursor[a.cursor_id] += 1u;
-      ctl->frame_id = frame + 1u;
-      ctl->touched_count = 0u;
-      ctl->done_wgs = 0u;
-      __threadfence();
+  // No fence before the counters: the closing workgroup reads no voxel data, and the voxel
+  // writes are published by the kernel boundary (a device-scope fence per workgroup would
+  // write back the L2 thousands of times per frame).
+  const uint32_t group = blockIdx.x & (kStripes - 1);
+  const uint32_t per_group = gridDim.x / kStripes;
+  if (lane == 0) {
+    const uint32_t prev = atomicAdd(&ctl->done[group * kLine], 1u);
+    if (prev == per_group - 1) {
+      const uint32_t top = atomicAdd(&ctl->done_top, 1u);
+      if (top == kStripes - 1) {
+        uint32_t total = 0;
+        for (int k = 0; k < kStripes; ++k) {
+          total += min(atomicAdd(&ctl->tcount[k * kLine], 0u), a.stripe_cap);
+          ctl->tcount[k * kLine] = 0u;
+          ctl->done[k * kLine] = 0u;
+        }
+        const uint32_t pc = atomicAdd(&ctl->pool_count, 0u);
+        ctl->touched_last = total;
+        ctl->touched_total += total;
+        ctl->frames += 1ull;
+        ctl->new_total += (unsigned long long)(pc - ctl->pool_at_frame_start);
+        ctl->pool_at_frame_start = pc;
+        ctl->cursor[a.cursor_id] += 1u;
+        ctl->frame_id = frame + 1u;
+        ctl->done_top = 0u;
+      }
     }
   }
 }
@@ -328,89 +429,268 @@ __device__ __forceinline__ void cube_tris(const float* sv, const uint8_t* so, in
   }
 }
 
+// One wave per pool block. The block's 8^3 voxels and the +x/+y/+z faces of its 7
+// neighbours (one hash lookup each) are staged as a 9^3 tile in LDS. The count pass writes
+// the block's triangle count; a two-level exclusive scan turns counts into offsets; the
+// write pass recomputes the triangles and stores them at their offsets. No output atomics:
+// the mesh is ordered by pool block, then cube, then table position.
 template <int kStorage>
-__global__ __launch_bounds__(256) void k_mc(McArgs a) {
+__device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, float* s_val, uint8_t* s_obs, uint32_t* s_nb,
+                                         int& gx0, int& gy0, int& gz0) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
+  const int lane = threadIdx.x;
+  int bx, by, bz;
+  unpack_key(a.block_keys[b], bx, by, bz);
+  if (lane < 8) {
+    uint32_t nb = b;
+    if (lane > 0) {
+      const int nx = bx + (lane >> 2), ny = by + ((lane >> 1) & 1), nz = bz + (lane & 1);
+      nb = key_in_range(nx, ny, nz) ? find_block(a.hash, pack_key(nx, ny, nz)) : kNoBlock;
+      if (nb >= a.pool_cap) nb = kNoBlock;
+    }
+    s_nb[lane] = nb;
+  }
+  __syncthreads();
+  const Word* vox = reinterpret_cast<const Word*>(a.voxels);
+  for (int e = lane; e < kHalo3; e += 64) {
+    const int x = e / 81, y = (e / 9) % 9, z = e % 9;
+    const uint32_t pb = s_nb[((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3)];
+    float sv = 0.0f, wv = 0.0f;
+    if (pb != kNoBlock) IO::unpack(vox[(size_t)pb * kB3 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7))], sv, wv);
+    s_val[e] = sv;
+    s_obs[e] = (pb != kNoBlock && wv > 0.0f) ? 1 : 0;
+  }
+  __syncthreads();
+  gx0 = bx * kB;
+  gy0 = by * kB;
+  gz0 = bz * kB;
+  return true;
+}
+
+// Dense per-wave triangle work list of the staged block. Lane l classifies the 8 cubes of its
+// (y, z) column (x = 0..7), sharing each x-face's corners with the next cube; the wave then
+// walks the list of (cube, table triangle) items 64 at a time, one item per lane. Emit
+// receives (valid, item, cube x/y/z, case, t, P[9]) for every item of the round.
+struct McList {
+  uint8_t cases[512];     // case per cube (cube = x*64 + y*8 + z), 0 = no triangles
+  uint32_t lane_off[65];  // exclusive prefix of table triangles per lane
+};
+
+__device__ __forceinline__ void mc_classify(const float* sv, const uint8_t* so, McList* L, uint32_t& total) {
+  const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
+  auto at = [&](int x, int y, int z) { return x * 81 + y * 9 + z; };
+  float f0[4];
+  bool o0[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = at(0, cy + (k >> 1), cz + (k & 1));
+    f0[k] = sv[e];
+    o0[k] = so[e] != 0;
+  }
+  uint32_t ntri = 0;
+  for (int cx = 0; cx < 8; ++cx) {
+    float f1[4];
+    bool o1[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = at(cx + 1, cy + (k >> 1), cz + (k & 1));
+      f1[k] = sv[e];
+      o1[k] = so[e] != 0;
+    }
+    // corner c -> (dx, dy, dz); face slot k = (dy << 1) | dz
+    const float val[8] = {f0[0], f0[2], f1[2], f1[0], f0[1], f0[3], f1[3], f1[1]};
+    const bool obs = o0[0] && o0[1] && o0[2] && o0[3] && o1[0] && o1[1] && o1[2] && o1[3];
+    int ci = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (val[c] < 0.0f) ci |= 1 << c;
+    if (!obs || ci == 255) ci = 0;
+    L->cases[cx * 64 + cy * 8 + cz] = (uint8_t)ci;
+    ntri += (uint32_t)(kTriTable[ci] >> 60);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      f0[k] = f1[k];
+      o0[k] = o1[k];
+    }
+  }
+  uint32_t x = ntri;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  L->lane_off[lane] = x - ntri;
+  if (lane == 63) L->lane_off[64] = x;
+  __syncthreads();
+  total = L->lane_off[64];
+}
+
+template <class Emit>
+__device__ __forceinline__ void mc_items(const float* sv, const McList* L, uint32_t total, int gx0, int gy0, int gz0,
+                                         Emit emit) {
+  const int lane = threadIdx.x;
+  for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+    const uint32_t item = r0 + lane;
+    bool valid = false;
+    int ci = 0, t = 0, cx = 0, cy = 0, cz = 0;
+    float P[9];
+    if (item < total) {
+      // owning lane: last lane whose offset <= item
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (L->lane_off[mid] <= item) lo = mid;
+        else hi = mid - 1;
+      }
+      cy = lo >> 3;
+      cz = lo & 7;
+      uint32_t rem = item - L->lane_off[lo];
+      for (cx = 0; cx < 8; ++cx) {
+        ci = L->cases[cx * 64 + lo];
+        const uint32_t n = (uint32_t)(kTriTable[ci] >> 60);
+        if (rem < n) break;
+        rem -= n;
+      }
+      t = (int)rem;
+      const uint64_t pk = kTriTable[ci];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
+        const int ca = edge_a(e), cb = edge_b(e);
+        const float va = sv[(cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca))];
+        const float vb = sv[(cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb))];
+        vertex_interp((float)(gx0 + cx + corner_dx(ca)), (float)(gy0 + cy + corner_dy(ca)),
+                      (float)(gz0 + cz + corner_dz(ca)), (float)(gx0 + cx + corner_dx(cb)),
+                      (float)(gy0 + cy + corner_dy(cb)), (float)(gz0 + cz + corner_dz(cb)), va, vb, P[3 * j],
+                      P[3 * j + 1], P[3 * j + 2]);
+      }
+      const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
+      const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
+      const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
+      valid = !(d01 || d12 || d20);
+    }
+    emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P);
+  }
+}
+
+template <int kStorage>
+__global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
   __shared__ float s_val[kHalo3];
   __shared__ uint8_t s_obs[kHalo3 + 7];
   __shared__ uint32_t s_nb[8];
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_base;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t npool = min(atomicAdd(&a.ctl->pool_count, 0u), a.pool_cap);
-  const Word* vox = reinterpret_cast<const Word*>(a.voxels);
-
-  for (uint32_t b = blockIdx.x; b < npool; b += gridDim.x) {
-    int bx, by, bz;
-    unpack_key(a.block_keys[b], bx, by, bz);
-    if (tid < 8) {
-      uint32_t nb = b;
-      if (tid > 0) {
-        const int nx = bx + (tid >> 2), ny = by + ((tid >> 1) & 1), nz = bz + (tid & 1);
-        nb = key_in_range(nx, ny, nz) ? find_block(a.hash, pack_key(nx, ny, nz)) : kNoBlock;
-        if (nb >= a.pool_cap) nb = kNoBlock;
-      }
-      s_nb[tid] = nb;
-    }
-    __syncthreads();
-    for (int e = tid; e < kHalo3; e += 256) {
-      const int x = e / 81, y = (e / 9) % 9, z = e % 9;
-      const uint32_t pb = s_nb[((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3)];
-      float sv = 0.0f, wv = 0.0f;
-      if (pb != kNoBlock) IO::unpack(vox[(size_t)pb * kB3 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7))], sv, wv);
-      s_val[e] = sv;
-      s_obs[e] = (pb != kNoBlock && wv > 0.0f) ? 1 : 0;
-    }
-    __syncthreads();
-
-    const int gx0 = bx * kB, gy0 = by * kB, gz0 = bz * kB;
+  __shared__ McList s_list;
+  const int lane = threadIdx.x;
+  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    int gx0, gy0, gz0;
+    mc_stage<kStorage>(a, b, s_val, s_obs, s_nb, gx0, gy0, gz0);
+    uint32_t total;
+    mc_classify(s_val, s_obs, &s_list, total);
     uint32_t cnt = 0;
+    mc_items(s_val, &s_list, total, gx0, gy0, gz0,
+             [&](bool valid, int, int, int, int, int, const float*) { cnt += (uint32_t)__popcll(__ballot(valid)); });
+    if (lane == 0) a.counts[b] = cnt;
+    __syncthreads();
+  }
+}
+
+// Exclusive scan of counts within chunks of kScanChunk blocks (one workgroup of 256 each,
+// 4 counts per lane); chunk totals go to chunk_sums.
+__global__ __launch_bounds__(256) void k_scan_chunks(McArgs a) {
+  __shared__ uint32_t s_w[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t c0 = blockIdx.x * kScanChunk;
+  uint32_t v[4], sum = 0;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = tid + 256 * h;
-      const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
-      cube_tris(s_val, s_obs, cx, cy, cz, gx0 + cx, gy0 + cy, gz0 + cz,
-                [&](int, int, const float*) { ++cnt; });
-    }
-    // workgroup exclusive scan of the per-thread triangle counts
-    uint32_t x = cnt;
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = c0 + 4 * tid + k;
+    v[k] = i < a.n_blocks ? a.counts[i] : 0u;
+    sum += v[k];
+  }
+  uint32_t x = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) pre += s_w[w];
+    tot += s_w[w];
+  }
+  uint32_t run = pre + x - sum;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = c0 + 4 * tid + k;
+    if (i < a.n_blocks) a.counts[i] = run;
+    run += v[k];
+  }
+  if (tid == 0) a.chunk_sums[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the chunk totals (one workgroup, sequential over groups of 256).
+__global__ __launch_bounds__(256) void k_scan_top(McArgs a, uint32_t n_chunks) {
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < n_chunks; b0 += 256) {
+    const uint32_t i = b0 + tid;
+    const uint32_t v = i < n_chunks ? a.chunk_sums[i] : 0u;
+    uint32_t x = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t y = (uint32_t)__shfl_up((int)x, off);
       if (lane >= off) x += y;
     }
-    if (lane == 63) s_wsum[wave] = x;
+    if (lane == 63) s_w[wave] = x;
     __syncthreads();
-    uint32_t wpre = 0, total = 0;
+    uint32_t pre = 0, tot = 0;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
-      if (w < wave) wpre += s_wsum[w];
-      total += s_wsum[w];
+      if (w < wave) pre += s_w[w];
+      tot += s_w[w];
     }
-    if (tid == 0) s_base = total ? atomicAdd(&a.ctl->mc_tri_count, total) : 0u;
+    const uint32_t carry = s_carry;
+    if (i < n_chunks) a.chunk_sums[i] = carry + pre + x - v;
     __syncthreads();
-    uint32_t pos = s_base + wpre + x - cnt;
-    if (cnt) {
+    if (tid == 0) s_carry = carry + tot;
+    __syncthreads();
+  }
+  if (tid == 0) *a.total = s_carry;
+}
+
+template <int kStorage>
+__global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
+  __shared__ float s_val[kHalo3];
+  __shared__ uint8_t s_obs[kHalo3 + 7];
+  __shared__ uint32_t s_nb[8];
+  __shared__ McList s_list;
+  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    int gx0, gy0, gz0;
+    mc_stage<kStorage>(a, b, s_val, s_obs, s_nb, gx0, gy0, gz0);
+    uint32_t total;
+    mc_classify(s_val, s_obs, &s_list, total);
+    uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
+    mc_items(s_val, &s_list, total, gx0, gy0, gz0,
+             [&](bool valid, int x, int y, int z, int ci, int t, const float* P) {
+               const uint64_t m = __ballot(valid);
+               const uint32_t p = pos + lanes_below(m);
+               if (valid && p < a.capacity) {
+                 float* o = a.tris + (size_t)p * 9;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int c = tid + 256 * h;
-        const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
-        cube_tris(s_val, s_obs, cx, cy, cz, gx0 + cx, gy0 + cy, gz0 + cz,
-                  [&](int t, int ci, const float* P) {
-                    if (pos < a.capacity) {
-                      float* o = a.tris + (size_t)pos * 9;
-#pragma unroll
-                      for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
-                      if (a.cubes) {
-                        int32_t* q = a.cubes + (size_t)pos * 4;
-                        q[0] = gx0 + cx; q[1] = gy0 + cy; q[2] = gz0 + cz; q[3] = (ci << 8) | t;
-                      }
-                    }
-                    ++pos;
-                  });
-      }
-    }
+                 for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
+                 if (a.cubes) {
+                   int32_t* q = a.cubes + (size_t)p * 4;
+                   q[0] = x; q[1] = y; q[2] = z; q[3] = (ci << 8) | t;
+                 }
+               }
+               pos += (uint32_t)__popcll(m);
+             });
     __syncthreads();
   }
 }
@@ -693,15 +973,23 @@ static void with_storage(int storage, F f) {
 
 hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
   with_storage(storage, [&](auto S) {
-    if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), dim3(grid), dim3(256), 0, st, a);
+    if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
   });
   return hipGetLastError();
 }
 
-hipError_t launch_mc(const McArgs& a, int storage, int grid, hipStream_t st) {
+hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
+  if (a.n_blocks == 0) return hipSuccess;
+  const uint32_t grid = min(a.n_blocks, 8192u);
+  const uint32_t n_chunks = (a.n_blocks + kScanChunk - 1) / kScanChunk;
   with_storage(storage, [&](auto S) {
-    hipLaunchKernelGGL(k_mc<decltype(S)::value>, dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_mc_count<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
+  });
+  hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, a, n_chunks);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_mc_write<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
   });
   return hipGetLastError();
 }

@@ -89,26 +89,56 @@ VH_HD void cam_to_world(const float* T, float x, float y, float z, float& px, fl
 }
 
 // ---------------------------------------------------------------- integration (one voxel)
-// Observation test and truncated distance for the voxel at global lattice index (gx,gy,gz)
-// (reference IntegrateHashKernel skip rules, src/tsdf.cu:613-738). Returns false when the
-// voxel is skipped; otherwise dist = min(1, (d - z)/mu) in (-1, 1].
-// DepthAt(int pixel) -> float metres.
-template <class DepthAt>
-VH_HD bool observe_voxel(const float* T, const Intr& K, int W, int H, float vox, float max_depth,
-                         float mu, int gx, int gy, int gz, DepthAt depth_at, float& dist) {
-  const float px = (float)gx * vox, py = (float)gy * vox, pz = (float)gz * vox;
-  float x, y, z;
-  world_to_cam(T, px, py, pz, x, y, z);
-  if (!(z > 0.0f)) return false;
-  const float uf = roundf(K.fx * (x / z) + K.cx);
-  const float vf = roundf(K.fy * (y / z) + K.cy);
-  if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) return false;
-  const float d = depth_at((int)vf * W + (int)uf);
-  if (!(d > 0.0f && d <= max_depth)) return false;
+// Camera coordinates of the voxels of one block, in the reference's subtract-then-rotate
+// form (base2cam, src/tsdf.cu:81-93) evaluated once per block origin o = 8*b*vox:
+//   c = R^T (o - t),  then voxel (k, j, i) of the block: c + vox*R^T (k, j, i).
+// With a = R^T e_x vox, b = R^T e_y vox, c = R^T e_z vox (per frame) a lane holding one
+// (j, i) column adds its offset once: c_l = c + (j*b + i*c), and voxel k along x is
+// c_l + k*a. Every voxel therefore costs 3 multiplies and 3 adds.
+struct BlockFrame {
+  float ax[3], ay[3], az[3];  // per-frame step vectors (camera coordinates per voxel step)
+};
+VH_HD void block_frame(const float* T, float vox, BlockFrame& f) {
+  for (int i = 0; i < 3; ++i) {
+    f.ax[i] = T[i] * vox;
+    f.ay[i] = T[4 + i] * vox;
+    f.az[i] = T[8 + i] * vox;
+  }
+}
+// Camera coordinates of the block origin (block coordinates bx, by, bz).
+VH_HD void block_origin_cam(const float* T, float vox, int bx, int by, int bz, float c[3]) {
+  const float d0 = (float)(bx * kB) * vox - T[3];
+  const float d1 = (float)(by * kB) * vox - T[7];
+  const float d2 = (float)(bz * kB) * vox - T[11];
+  for (int i = 0; i < 3; ++i) c[i] = T[i] * d0 + T[4 + i] * d1 + T[8 + i] * d2;
+}
+// Camera coordinates of the column (j, i) of a block: c + (j*b + i*c).
+VH_HD void column_cam(const BlockFrame& f, const float c[3], int j, int i, float out[3]) {
+  const float fj = (float)j, fi = (float)i;
+  for (int r = 0; r < 3; ++r) out[r] = c[r] + (fj * f.ay[r] + fi * f.az[r]);
+}
+
+// Observation test and truncated distance of the voxel at camera point (x, y, z)
+// (reference IntegrateHashKernel skip rules, src/tsdf.cu:706-738), in two steps so that a
+// caller can issue all depth fetches of a block before using any of them:
+//   project_cam: pixel index of the voxel, or -1 when it is behind the camera or outside
+//     the image. One correctly rounded reciprocal 1/z serves both image coordinates,
+//     fx*(x*(1/z)) + cx, rounded to the nearest pixel with ties to even (rintf); the
+//     reference divides per coordinate and rounds ties away from zero (cam2frame,
+//     src/tsdf.cu:75-79), compiled with -use_fast_math.
+//   truncate_obs: with the pixel's depth d, true iff 0 < d <= max_depth and d - z > -mu;
+//     dist = min(1, (d - z) * inv_mu) with inv_mu = 1/mu (reference: divides by mu, :738).
+VH_HD int project_cam(float x, float y, float z, const Intr& K, int W, int H) {
+  const float iz = 1.0f / z;
+  const float uf = rintf(K.fx * (x * iz) + K.cx);
+  const float vf = rintf(K.fy * (y * iz) + K.cy);
+  const bool inside = (z > 0.0f) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
+  return inside ? (int)vf * W + (int)uf : -1;
+}
+VH_HD bool truncate_obs(int pix, float d, float z, float max_depth, float mu, float inv_mu, float& dist) {
   const float diff = d - z;
-  if (diff <= -mu) return false;
-  dist = fminf(1.0f, diff / mu);
-  return true;
+  dist = fminf(1.0f, diff * inv_mu);
+  return (pix >= 0) & (d > 0.0f) & (d <= max_depth) & (diff > -mu);
 }
 
 // Running average with the old weight (the intent of src/tsdf.cu:855-858; the hash kernel
