@@ -116,9 +116,9 @@ typedef struct vh_mesh {
 
 typedef struct vh_gn_stats {
   int32_t iters;
-  int32_t n_valid[64];  /* points with a valid trilinear sample, per iteration */
-  double rms[64];       /* residual RMS (normalised TSDF units) before each step */
-  double step_norm[64]; /* |delta xi| of each step */
+  int32_t n_valid[64];  /* points with a valid trilinear sample at each evaluated candidate */
+  double rms[64];       /* residual RMS (normalised TSDF units) over those points */
+  double step_norm[64]; /* |delta xi| proposed after the evaluation (0 for the last) */
 } vh_gn_stats;
 
 /* ------------------------------------------------------------------ lifetime */
@@ -200,8 +200,11 @@ int vh_query_sdf_backward_device(vh_volume* vol, const float* d_pts, int64_t n,
                                  float* d_voxel_grad);
 /* Gauss-Newton refinement of c2w (host, row-major 4x4, updated in place, FP64) so that the
  * camera-frame points p_i (host, n x 3) land on the zero level set: minimises
- * sum_i sdf(R p_i + t)^2. J^T J and J^T r are reduced in FP64 on the GPU and the 6x6
- * system is solved there (Cholesky). stats may be NULL. */
+ * sum_i sdf(R p_i + t)^2, counting a point without a valid sample as a residual of 1.
+ * Each of the `iters` (<= 63) steps reduces J^T J, J^T r and the cost in FP64 on the GPU
+ * and solves the damped 6x6 system (Levenberg-Marquardt, Cholesky) there; a last
+ * evaluation picks the best accepted pose. stats (may be NULL) holds iters + 1 entries.
+ * No reference counterpart (the reference takes poses as given). */
 int vh_gn_refine(vh_volume* vol, const float* pts, int64_t n, double c2w[16], int32_t iters,
                  vh_gn_stats* stats);
 

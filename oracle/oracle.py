@@ -45,6 +45,7 @@ def load():
     lib.vho_tri_case.argtypes = [C.c_int]
     lib.vho_query.argtypes = [P, P, C.c_int64, P, P, P, P]
     lib.vho_query64.argtypes = [P, P, C.c_int64, P, P, P]
+    lib.vho_query_twin.argtypes = [P, P, C.c_int64, P, P, P]
     lib.vho_band.restype = C.c_int
     lib.vho_band.argtypes = [P, P, C.c_int, C.c_int, C.c_float, P, C.c_int]
     lib.vho_f32_to_f16.restype = C.c_uint16
@@ -131,6 +132,16 @@ class Oracle:
         g = np.zeros((n, 3), np.float64)
         valid = np.zeros(n, np.uint8)
         _lib.vho_query64(self._h, _p(pts), n, _p(s), _p(g), _p(valid))
+        return s, g, valid.astype(bool)
+
+    def query_twin(self, pts64):
+        """FP32 value and FP32 gradient as the GPU kernels compute them, at FP64 points."""
+        pts = np.ascontiguousarray(pts64, np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        s = np.zeros(n, np.float32)
+        g = np.zeros((n, 3), np.float32)
+        valid = np.zeros(n, np.uint8)
+        _lib.vho_query_twin(self._h, _p(pts), n, _p(s), _p(g), _p(valid))
         return s, g, valid.astype(bool)
 
     def band(self, c2w, u, v, d):

@@ -385,17 +385,20 @@ void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& o
 }
 
 // ------------------------------------------------------------------ trilinear query
+// The lattice position q/vox and the cell fraction are formed in FP64 for both precisions;
+// R is the precision of the blend (float: twin of the GPU kernels; double: FP64 reference).
 template <class R>
-bool sample(const Oracle& o, R qx, R qy, R qz, R& sdf, R g[3]) {
+bool sample(const Oracle& o, double qx, double qy, double qz, R& sdf, R g[3]) {
   const R vox = (R)o.vox;
-  const R gq[3] = {qx / vox, qy / vox, qz / vox};
+  const double vd = (double)o.vox;
+  const double gq[3] = {qx / vd, qy / vd, qz / vd};
   int ix[3];
   R f[3];
   for (int k = 0; k < 3; ++k) {
-    if (!(std::fabs(gq[k]) < (R)8.0e6)) return false;
-    const R fl = std::floor(gq[k]);
+    if (!(std::fabs(gq[k]) < 8.0e6)) return false;
+    const double fl = std::floor(gq[k]);
     ix[k] = (int)fl;
-    f[k] = gq[k] - fl;
+    f[k] = (R)(gq[k] - fl);
   }
   R s[8];
   for (int k = 0; k < 8; ++k) {
@@ -547,7 +550,7 @@ void vho_query(void* p, const float* pts, int64_t n, float* sdf32, double* sdf64
   for (long i = 0; i < (long)n; ++i) {
     float s = 0.0f, g[3];
     double s6 = 0.0, g6[3] = {0, 0, 0};
-    const bool ok = sample<float>(*o, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], s, g);
+    const bool ok = sample<float>(*o, (double)pts[3 * i], (double)pts[3 * i + 1], (double)pts[3 * i + 2], s, g);
     if (ok) sample<double>(*o, (double)pts[3 * i], (double)pts[3 * i + 1], (double)pts[3 * i + 2], s6, g6);
     if (sdf32) sdf32[i] = ok ? s : 0.0f;
     if (sdf64) sdf64[i] = ok ? s6 : 0.0;
@@ -556,6 +559,20 @@ void vho_query(void* p, const float* pts, int64_t n, float* sdf32, double* sdf64
     if (valid) valid[i] = ok ? 1 : 0;
   }
 }
+// FP32 twin of the kernels' sample: FP32 value and FP32 gradient w.r.t. q at FP64 points.
+void vho_query_twin(void* p, const double* pts, int64_t n, float* sdf, float* grad, uint8_t* valid) {
+  Oracle* o = static_cast<Oracle*>(p);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    float s = 0.0f, g[3] = {0, 0, 0};
+    const bool ok = sample<float>(*o, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], s, g);
+    if (sdf) sdf[i] = ok ? s : 0.0f;
+    if (grad)
+      for (int k = 0; k < 3; ++k) grad[3 * i + k] = ok ? g[k] : 0.0f;
+    if (valid) valid[i] = ok ? 1 : 0;
+  }
+}
+
 // FP64 trilinear at double-precision world points (for finite differences).
 void vho_query64(void* p, const double* pts, int64_t n, double* sdf, double* grad, uint8_t* valid) {
   Oracle* o = static_cast<Oracle*>(p);

@@ -97,6 +97,7 @@ struct vh_volume {
   double* gn_partials = nullptr;
   double* gn_pose = nullptr;
   double* gn_stats = nullptr;
+  GnState* gn_state = nullptr;
   float* bw_pose = nullptr;
   float graph_scale = -1.0f;  // depth scale baked into the staged uint16 graph
   std::mutex mu;
@@ -308,7 +309,7 @@ void free_volume(vh_volume* v) {
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
-                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose};
+                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -382,7 +383,9 @@ int ensure_gn_scratch(vh_volume* v) {
   if (rc) return rc;
   rc = dev_alloc(&v->bw_pose, 16 * sizeof(float));
   if (rc) return rc;
-  return dev_alloc(&v->gn_stats, 3 * 64 * sizeof(double));
+  rc = dev_alloc(&v->gn_state, sizeof(GnState));
+  if (rc) return rc;
+  return dev_alloc(&v->gn_stats, (size_t)kGnStats * 65 * sizeof(double));
 }
 
 }  // namespace
@@ -755,7 +758,7 @@ int vh_query_sdf_backward_device(vh_volume* v, const float* d_pts, int64_t n, co
 }
 
 int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int32_t iters, vh_gn_stats* stats) {
-  if (!v || !pts || !c2w || n <= 0 || iters < 0 || iters > 64) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (!v || !pts || !c2w || n <= 0 || iters < 0 || iters > 63) return fail(VH_ERR_INVALID_ARG, "bad argument");
   for (int k = 0; k < 16; ++k)
     if (!std::isfinite(c2w[k])) return fail(VH_ERR_INVALID_ARG, "pose has a non-finite entry");
   std::lock_guard<std::mutex> lk(v->mu);
@@ -774,25 +777,32 @@ int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int3
   a.q = query_args(v);
   a.pts = d_pts;
   a.n = n;
-  a.pose = v->gn_pose;
+  a.pose = v->gn_pose;  // the candidate pose: each accumulation evaluates it
   a.partials = v->gn_partials;
-  for (int it = 0; it < iters; ++it) {
+  GnState init{};
+  init.lambda = 1e-4;
+  VH_HIP(hipMemcpyAsync(v->gn_state, &init, sizeof init, hipMemcpyHostToDevice, v->stream));
+  // iters LM steps, then one evaluation of the last candidate; the result is the best
+  // accepted pose
+  for (int it = 0; it <= iters; ++it) {
     VH_HIP(launch_gn_accum(a, v->cfg.storage, kGnGrid, v->stream));
-    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_pose, v->gn_stats + 3 * it, v->stream));
+    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_state, v->gn_pose, (double)n, it == iters ? 1 : 0,
+                           v->gn_stats + kGnStats * it, v->stream));
   }
-  double host_stats[3 * 64];
+  std::vector<double> host_stats((size_t)kGnStats * (iters + 1));
   VH_HIP(hipMemcpyAsync(c2w, v->gn_pose, 12 * sizeof(double), hipMemcpyDeviceToHost, v->stream));
-  if (iters) VH_HIP(hipMemcpyAsync(host_stats, v->gn_stats, 3 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipMemcpyAsync(host_stats.data(), v->gn_stats, host_stats.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
   c2w[12] = c2w[13] = c2w[14] = 0.0;
   c2w[15] = 1.0;
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->iters = iters;
-    for (int it = 0; it < iters; ++it) {
-      stats->n_valid[it] = (int32_t)host_stats[3 * it];
-      stats->rms[it] = host_stats[3 * it + 1];
-      stats->step_norm[it] = host_stats[3 * it + 2];
+    for (int it = 0; it <= iters && it < 64; ++it) {
+      stats->n_valid[it] = (int32_t)host_stats[kGnStats * it];
+      stats->rms[it] = host_stats[kGnStats * it + 1];
+      stats->step_norm[it] = host_stats[kGnStats * it + 2];
     }
   }
   return VH_OK;

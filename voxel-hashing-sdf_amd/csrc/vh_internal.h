@@ -98,6 +98,17 @@ struct QueryArgs {
 // Gauss-Newton / backward accumulators: 21 (J^T J upper) + 6 (J^T r) + 1 (r^2) + 1 (count).
 constexpr int kGnAcc = 29;
 
+// Levenberg-Marquardt state kept on the device between iterations.
+struct GnState {
+  double pose[12];  // accepted pose (row-major 3x4)
+  double Ab[27];    // J^T J (upper, 21) and J^T r (6) at the accepted pose
+  double cost;      // its cost
+  double lambda;
+  int has;          // an accepted pose exists
+  int pad;
+};
+constexpr int kGnStats = 5;  // per-iteration stats: n_valid, rms, |delta|, lambda, accepted
+
 struct GnArgs {
   QueryArgs q;
   const float* pts;   // camera-frame points
@@ -120,7 +131,7 @@ hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, in
                            hipStream_t st);
 hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStream_t st);
 hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st);
-hipError_t launch_gn_solve(const double* partials, int grid, double* pose, double* stats_row,
-                           hipStream_t st);
+hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand,
+                           double n_points, int final_step, double* stats_row, hipStream_t st);
 
 }  // namespace vh

@@ -696,19 +696,22 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
 }
 
 // ============================================================================ queries
-// Trilinear sample of the normalised TSDF at world point p; false unless all 8 lattice
-// corners are allocated and observed. vid/w receive the pool voxel ids and weights.
+// Trilinear sample of the normalised TSDF at world point q; false unless all 8 lattice
+// corners are allocated and observed. The lattice position q/vox and its cell fraction are
+// formed in FP64 (an FP32 q/vox at |q| ~ 3 m is off by ~2e-5 of a voxel), the blend runs in
+// FP32. vid/w receive the pool voxel ids and weights; g the gradient w.r.t. q.
 template <int kStorage>
-__device__ __forceinline__ bool sample_sdf(const QueryArgs& q, float px, float py, float pz,
+__device__ __forceinline__ bool sample_sdf(const QueryArgs& q, double px, double py, double pz,
                                            float& sdf, float g[3], uint32_t vid[8], float w[8]) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
-  const float gxf = px / q.vox, gyf = py / q.vox, gzf = pz / q.vox;
-  if (!(fabsf(gxf) < 8.0e6f && fabsf(gyf) < 8.0e6f && fabsf(gzf) < 8.0e6f)) return false;
-  const float flx = floorf(gxf), fly = floorf(gyf), flz = floorf(gzf);
+  const double vox = (double)q.vox;
+  const double gxd = px / vox, gyd = py / vox, gzd = pz / vox;
+  if (!(fabs(gxd) < 8.0e6 && fabs(gyd) < 8.0e6 && fabs(gzd) < 8.0e6)) return false;
+  const double flx = floor(gxd), fly = floor(gyd), flz = floor(gzd);
   const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-  const float fx = gxf - flx, fy = gyf - fly, fz = gzf - flz;
-  const Word* vox = reinterpret_cast<const Word*>(q.voxels);
+  const float fx = (float)(gxd - flx), fy = (float)(gyd - fly), fz = (float)(gzd - flz);
+  const Word* voxw = reinterpret_cast<const Word*>(q.voxels);
   uint64_t last_key = kEmptyKey;
   uint32_t blk = kNoBlock;
   float s[8];
@@ -723,7 +726,7 @@ __device__ __forceinline__ bool sample_sdf(const QueryArgs& q, float px, float p
     if (blk >= q.pool_cap) return false;
     const uint32_t id = blk * kB3 + (uint32_t)((((cx & 7) * 8) + (cy & 7)) * 8 + (cz & 7));
     float sv, wv;
-    IO::unpack(vox[id], sv, wv);
+    IO::unpack(voxw[id], sv, wv);
     if (!(wv > 0.0f)) return false;
     s[k] = sv;
     vid[k] = id;
@@ -746,10 +749,20 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const float* pts, in
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float r = 0.0f, g[3], w[8];
     uint32_t vid[8];
-    const bool ok = sample_sdf<kStorage>(q, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], r, g, vid, w);
+    const bool ok = sample_sdf<kStorage>(q, (double)pts[3 * i], (double)pts[3 * i + 1], (double)pts[3 * i + 2], r,
+                                         g, vid, w);
     if (sdf) sdf[i] = ok ? r : 0.0f;
     if (valid) valid[i] = ok ? 1 : 0;
   }
+}
+
+// q = R p + t in FP64 from a row-major 3x4 pose (float or double entries).
+template <class P>
+__device__ __forceinline__ void pose_apply_d(const P* T, float x, float y, float z, double& qx, double& qy, double& qz) {
+  const double X = x, Y = y, Z = z;
+  qx = ((X * (double)T[0] + Y * (double)T[1]) + Z * (double)T[2]) + (double)T[3];
+  qy = ((X * (double)T[4] + Y * (double)T[5]) + Z * (double)T[6]) + (double)T[7];
+  qz = ((X * (double)T[8] + Y * (double)T[9]) + Z * (double)T[10]) + (double)T[11];
 }
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) {
@@ -789,22 +802,20 @@ template <int kStorage>
 __global__ __launch_bounds__(256) void k_backward(QueryArgs q, const float* pts, int64_t n, const float* T12,
                                                   const float* grad_r, float* r_out, uint8_t* valid, float* J,
                                                   double* partials, float* voxel_grad) {
-  float T[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) T[k] = T12[k];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    float qx, qy, qz;
-    cam_to_world(T, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], qx, qy, qz);
+    double qx, qy, qz;
+    pose_apply_d(T12, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], qx, qy, qz);
     float r = 0.0f, g[3] = {0, 0, 0}, w[8];
     uint32_t vid[8];
     const bool ok = sample_sdf<kStorage>(q, qx, qy, qz, r, g, vid, w);
     float j[6] = {0, 0, 0, 0, 0, 0};
     if (ok) {
+      const double gx = g[0], gy = g[1], gz = g[2];
       j[0] = g[0]; j[1] = g[1]; j[2] = g[2];
-      j[3] = qy * g[2] - qz * g[1];
-      j[4] = qz * g[0] - qx * g[2];
-      j[5] = qx * g[1] - qy * g[0];
+      j[3] = (float)(qy * gz - qz * gy);
+      j[4] = (float)(qz * gx - qx * gz);
+      j[5] = (float)(qx * gy - qy * gx);
     }
     if (r_out) r_out[i] = ok ? r : 0.0f;
     if (valid) valid[i] = ok ? 1 : 0;
@@ -836,20 +847,17 @@ __global__ __launch_bounds__(64) void k_reduce6(const double* partials, int grid
 // ============================================================================ Gauss-Newton
 template <int kStorage>
 __global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
-  float T[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) T[k] = (float)a.pose[k];
   double acc[kGnAcc];
 #pragma unroll
   for (int k = 0; k < kGnAcc; ++k) acc[k] = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
-    float qx, qy, qz;
-    cam_to_world(T, a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2], qx, qy, qz);
+    double qx, qy, qz;
+    pose_apply_d(a.pose, a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2], qx, qy, qz);
     float r, g[3], w[8];
     uint32_t vid[8];
     if (!sample_sdf<kStorage>(a.q, qx, qy, qz, r, g, vid, w)) continue;
-    const double j[6] = {(double)g[0], (double)g[1], (double)g[2], (double)(qy * g[2] - qz * g[1]),
-                         (double)(qz * g[0] - qx * g[2]), (double)(qx * g[1] - qy * g[0])};
+    const double gx = g[0], gy = g[1], gz = g[2];
+    const double j[6] = {gx, gy, gz, qy * gz - qz * gy, qz * gx - qx * gz, qx * gy - qy * gx};
     int m = 0;
 #pragma unroll
     for (int p = 0; p < 6; ++p)
@@ -863,55 +871,15 @@ __global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
   wg_reduce_store<kGnAcc>(acc, a.partials + kGnAcc * (size_t)blockIdx.x);
 }
 
-// One workgroup: deterministic sum of the partials, 6x6 Cholesky solve of
-// (J^T J) delta = -J^T r, and the update T <- exp(delta^) T, all in FP64.
-// stats_row: [n_valid, rms, |delta|].
-__global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int grid, double* pose, double* stats_row) {
-  __shared__ double tot[kGnAcc];
-  if (threadIdx.x < kGnAcc) {
-    double s = 0.0;
-    for (int g = 0; g < grid; ++g) s += partials[kGnAcc * (size_t)g + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double A[6][6], b[6];
-  int m = 0;
-  for (int p = 0; p < 6; ++p)
-    for (int c = p; c < 6; ++c) { A[p][c] = tot[m]; A[c][p] = tot[m]; ++m; }
-  for (int p = 0; p < 6; ++p) b[p] = -tot[21 + p];
-  const double cnt = tot[28];
-  stats_row[0] = cnt;
-  stats_row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
-  stats_row[2] = 0.0;
-  if (cnt < 6.0) return;
-  // Cholesky A = L L^T (in place, lower)
-  double L[6][6];
-  for (int p = 0; p < 6; ++p)
-    for (int c = 0; c < 6; ++c) L[p][c] = 0.0;
-  for (int c = 0; c < 6; ++c) {
-    double d = A[c][c];
-    for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
-    if (!(d > 0.0)) return;  // not positive definite: keep the pose
-    L[c][c] = sqrt(d);
-    for (int p = c + 1; p < 6; ++p) {
-      double s = A[p][c];
-      for (int k = 0; k < c; ++k) s -= L[p][k] * L[c][k];
-      L[p][c] = s / L[c][c];
-    }
-  }
-  double y[6], x[6];
-  for (int p = 0; p < 6; ++p) {
-    double s = b[p];
-    for (int k = 0; k < p; ++k) s -= L[p][k] * y[k];
-    y[p] = s / L[p][p];
-  }
-  for (int p = 5; p >= 0; --p) {
-    double s = y[p];
-    for (int k = p + 1; k < 6; ++k) s -= L[k][p] * x[k];
-    x[p] = s / L[p][p];
-  }
-  // exp of (rho, phi)
+// One workgroup, FP64: deterministic sum of the partials at the candidate pose, then a
+// Levenberg-Marquardt decision and step.
+//   cost(candidate) = sum r^2 over valid points + 1 per invalid point (a saturated residual),
+//   so costs at different poses compare. The candidate is accepted if its cost does not
+//   exceed the accepted one (lambda /= 3) and rejected otherwise (lambda *= 4). Unless
+//   `final`, the next candidate is exp(delta^) T_acc with
+//   (A + lambda diag(A)) delta = -b, A = J^T J and b = J^T r at the accepted pose,
+//   delta = (rho, phi), T <- exp(delta^) T. stats_row: [n_valid, rms, |delta|, lambda, accepted].
+__device__ void se3_left_update(const double* x, const double* pose, double* out) {
   const double rho[3] = {x[0], x[1], x[2]}, ph[3] = {x[3], x[4], x[5]};
   const double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
   const double th = sqrt(th2);
@@ -929,9 +897,9 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int gri
   double K2[3][3], R[3][3], V[3][3];
   for (int p = 0; p < 3; ++p)
     for (int c = 0; c < 3; ++c) {
-      double s = 0.0;
-      for (int k = 0; k < 3; ++k) s += K[p][k] * K[k][c];
-      K2[p][c] = s;
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k) acc += K[p][k] * K[k][c];
+      K2[p][c] = acc;
     }
   for (int p = 0; p < 3; ++p)
     for (int c = 0; c < 3; ++c) {
@@ -939,14 +907,82 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int gri
       R[p][c] = I + A1 * K[p][c] + B1 * K2[p][c];
       V[p][c] = I + B1 * K[p][c] + C1 * K2[p][c];
     }
-  double P[12];
   for (int p = 0; p < 3; ++p) {
-    for (int c = 0; c < 3; ++c) P[4 * p + c] = R[p][0] * pose[c] + R[p][1] * pose[4 + c] + R[p][2] * pose[8 + c];
-    P[4 * p + 3] = R[p][0] * pose[3] + R[p][1] * pose[7] + R[p][2] * pose[11] +
-                   (V[p][0] * rho[0] + V[p][1] * rho[1] + V[p][2] * rho[2]);
+    for (int c = 0; c < 3; ++c) out[4 * p + c] = R[p][0] * pose[c] + R[p][1] * pose[4 + c] + R[p][2] * pose[8 + c];
+    out[4 * p + 3] = R[p][0] * pose[3] + R[p][1] * pose[7] + R[p][2] * pose[11] +
+                     (V[p][0] * rho[0] + V[p][1] * rho[1] + V[p][2] * rho[2]);
   }
-  for (int k = 0; k < 12; ++k) pose[k] = P[k];
-  stats_row[2] = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + th2);
+}
+
+__global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int grid, GnState* st, double* cand,
+                                                 double n_points, int final_step, double* stats_row) {
+  __shared__ double tot[kGnAcc];
+  if (threadIdx.x < kGnAcc) {
+    double s = 0.0;
+    for (int g = 0; g < grid; ++g) s += partials[kGnAcc * (size_t)g + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double cnt = tot[28];
+  const double cost = tot[27] + (n_points - cnt);
+  stats_row[0] = cnt;
+  stats_row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
+  stats_row[2] = 0.0;
+  const bool accept = !st->has || cost <= st->cost;
+  if (accept) {
+    for (int k = 0; k < 12; ++k) st->pose[k] = cand[k];
+    for (int k = 0; k < 27; ++k) st->Ab[k] = tot[k];
+    st->cost = cost;
+    st->lambda = st->has ? fmax(st->lambda / 3.0, 1e-9) : st->lambda;
+    st->has = 1;
+  } else {
+    st->lambda = fmin(st->lambda * 4.0, 1e9);
+  }
+  stats_row[3] = st->lambda;
+  stats_row[4] = accept ? 1.0 : 0.0;
+  if (final_step) {
+    for (int k = 0; k < 12; ++k) cand[k] = st->pose[k];
+    return;
+  }
+  double A[6][6], b[6];
+  int m = 0;
+  for (int p = 0; p < 6; ++p)
+    for (int c = p; c < 6; ++c) { A[p][c] = st->Ab[m]; A[c][p] = st->Ab[m]; ++m; }
+  for (int p = 0; p < 6; ++p) {
+    b[p] = -st->Ab[21 + p];
+    A[p][p] += st->lambda * A[p][p] + 1e-12;
+  }
+  double L[6][6];
+  for (int p = 0; p < 6; ++p)
+    for (int c = 0; c < 6; ++c) L[p][c] = 0.0;
+  for (int c = 0; c < 6; ++c) {
+    double d = A[c][c];
+    for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
+    if (!(d > 0.0)) {  // not positive definite: propose the accepted pose again
+      for (int k = 0; k < 12; ++k) cand[k] = st->pose[k];
+      return;
+    }
+    L[c][c] = sqrt(d);
+    for (int p = c + 1; p < 6; ++p) {
+      double acc = A[p][c];
+      for (int k = 0; k < c; ++k) acc -= L[p][k] * L[c][k];
+      L[p][c] = acc / L[c][c];
+    }
+  }
+  double y[6], x[6];
+  for (int p = 0; p < 6; ++p) {
+    double acc = b[p];
+    for (int k = 0; k < p; ++k) acc -= L[p][k] * y[k];
+    y[p] = acc / L[p][p];
+  }
+  for (int p = 5; p >= 0; --p) {
+    double acc = y[p];
+    for (int k = p + 1; k < 6; ++k) acc -= L[k][p] * x[k];
+    x[p] = acc / L[p][p];
+  }
+  se3_left_update(x, st->pose, cand);
+  stats_row[2] = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
 }
 
 }  // namespace
@@ -1032,8 +1068,10 @@ hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_gn_solve(const double* partials, int grid, double* pose, double* stats_row, hipStream_t st) {
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, st, partials, grid, pose, stats_row);
+hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand, double n_points,
+                           int final_step, double* stats_row, hipStream_t st) {
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, st, partials, grid, state, cand, n_points, final_step,
+                     stats_row);
   return hipGetLastError();
 }
 

@@ -319,8 +319,8 @@ class Volume:
         T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float64).reshape(16)).copy()
         st = GnStats()
         _check(_lib.vh_gn_refine(self._h, _ptr(pts), pts.shape[0], _ptr(T), iters, C.byref(st)))
-        stats = {"n_valid": list(st.n_valid[:iters]), "rms": list(st.rms[:iters]),
-                 "step_norm": list(st.step_norm[:iters])}
+        k = iters + 1
+        stats = {"n_valid": list(st.n_valid[:k]), "rms": list(st.rms[:k]), "step_norm": list(st.step_norm[:k])}
         return T.reshape(4, 4), stats
 
     def pool_view(self):
