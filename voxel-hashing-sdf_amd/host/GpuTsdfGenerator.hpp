@@ -1,0 +1,164 @@
+// GpuTsdfGenerator.hpp - header-only C++ drop-in for the reference's ark::GpuTsdfGenerator
+// (include/tsdf.cuh:548-683), implemented over the C ABI of libvh (include/vh.h).
+//
+// Kept from the reference: the namespace, the 15-argument constructor (tsdf.cuh:605-607;
+// its defaults, which the reference only gives on the out-of-line definition at
+// src/tsdf.cu:1249-1252, are on the declaration here), processFrame(depth, rgb, c2w)
+// (tsdf.cuh:609-610) with metric float depth and a row-major camera-to-world matrix,
+// SavePLY(filename) writing an ASCII PLY of welded vertices and triangle faces
+// (tsdf2mesh, src/tsdf.cu:1760-1888), render(), Shutdown(), getVertices()/getFaces().
+// Changed: errors throw std::runtime_error (the reference exits the process, tsdf.cuh:46-61);
+// render() draws nothing (no GL in this library); the mesh comes from marching cubes over
+// every allocated block at the time of the call; colour is not integrated.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vh.h"
+
+namespace ark {
+
+struct Vertex {
+  float x, y, z;
+  unsigned char r, g, b;
+};
+
+struct Face {
+  int vIdx[3];
+};
+
+class GpuTsdfGenerator {
+ public:
+  GpuTsdfGenerator(int width, int height, float fx, float fy, float cx, float cy, float max_depth,
+                   float origin_x = -1.5f, float origin_y = -1.5f, float origin_z = 0.5f, float vox_size = 0.006f,
+                   float trunc_m = 0.03f, int vox_dim_x = 500, int vox_dim_y = 500, int vox_dim_z = 500) {
+    vh_config c;
+    vh_config_default(&c);
+    c.width = width;
+    c.height = height;
+    c.fx = fx;
+    c.fy = fy;
+    c.cx = cx;
+    c.cy = cy;
+    c.max_depth = max_depth;
+    c.origin[0] = origin_x;
+    c.origin[1] = origin_y;
+    c.origin[2] = origin_z;
+    c.vox_size = vox_size;
+    c.trunc = trunc_m;
+    c.vox_dim[0] = vox_dim_x;
+    c.vox_dim[1] = vox_dim_y;
+    c.vox_dim[2] = vox_dim_z;
+    c.log2_buckets = 20;
+    c.pool_blocks = 1 << 19;
+    open(c);
+  }
+  // Full control over hash and pool sizes and the voxel storage format.
+  explicit GpuTsdfGenerator(const vh_config& cfg) { open(cfg); }
+
+  GpuTsdfGenerator(const GpuTsdfGenerator&) = delete;
+  GpuTsdfGenerator& operator=(const GpuTsdfGenerator&) = delete;
+  ~GpuTsdfGenerator() { Shutdown(); }
+
+  // depth: height x width metres (0 = invalid); rgb: may be null (not integrated);
+  // c2w: row-major 4x4 camera-to-world.
+  void processFrame(float* depth, unsigned char* rgb, float* c2w) {
+    vh_frame f;
+    std::memset(&f, 0, sizeof f);
+    f.depth = depth;
+    f.depth_format = VH_DEPTH_F32;
+    f.depth_scale = 1.0f;
+    f.pitch = 0;
+    f.rgb = rgb;
+    std::memcpy(f.c2w, c2w, sizeof f.c2w);
+    check(vh_integrate(vol_, &f), "processFrame");
+  }
+
+  // ASCII PLY: "x y z r g b" vertices (metres, colour 0) welded on identical coordinates,
+  // then "3 i j k" faces.
+  void SavePLY(std::string filename) {
+    extract();
+    FILE* fp = std::fopen(filename.c_str(), "w");
+    if (!fp) throw std::runtime_error("SavePLY: cannot open " + filename);
+    std::fprintf(fp,
+                 "ply\nformat ascii 1.0\ncomment vh marching cubes\nelement vertex %zu\nproperty float x\n"
+                 "property float y\nproperty float z\nproperty uchar red\nproperty uchar green\n"
+                 "property uchar blue\nelement face %zu\nproperty list uchar int vertex_index\nend_header\n",
+                 vertices_.size(), faces_.size());
+    for (const Vertex& v : vertices_)
+      std::fprintf(fp, "%.9g %.9g %.9g %d %d %d\n", v.x, v.y, v.z, (int)v.r, (int)v.g, (int)v.b);
+    for (const Face& f : faces_) std::fprintf(fp, "3 %d %d %d\n", f.vIdx[0], f.vIdx[1], f.vIdx[2]);
+    if (std::fclose(fp) != 0) throw std::runtime_error("SavePLY: write failed");
+  }
+
+  void render() {}
+
+  void Shutdown() {
+    if (vol_) {
+      vh_destroy(vol_);
+      vol_ = nullptr;
+    }
+  }
+
+  // The welded mesh of the last SavePLY/extract call.
+  std::vector<Vertex>* getVertices() { return &vertices_; }
+  std::vector<Face>* getFaces() { return &faces_; }
+
+  // Recomputes the welded mesh from the current volume.
+  void extract() {
+    vh_mesh m;
+    check(vh_extract_mesh(vol_, &m), "extract mesh");
+    vertices_.clear();
+    faces_.clear();
+    struct Key {
+      uint32_t b[3];
+      bool operator==(const Key& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
+    };
+    struct KeyHash {
+      size_t operator()(const Key& k) const {
+        uint64_t h = (uint64_t)k.b[0] * 0x9e3779b97f4a7c15ull ^ (uint64_t)k.b[1] * 0xc2b2ae3d27d4eb4full ^
+                     (uint64_t)k.b[2] * 0x165667b19e3779f9ull;
+        return (size_t)(h ^ (h >> 29));
+      }
+    };
+    std::unordered_map<Key, int, KeyHash> index;
+    index.reserve((size_t)m.n_tris * 2);
+    faces_.reserve((size_t)m.n_tris);
+    for (int64_t t = 0; t < m.n_tris; ++t) {
+      Face f;
+      for (int j = 0; j < 3; ++j) {
+        const float* p = m.xyz + 9 * t + 3 * j;
+        Key k;
+        std::memcpy(k.b, p, 12);
+        auto it = index.find(k);
+        if (it == index.end()) {
+          it = index.emplace(k, (int)vertices_.size()).first;
+          vertices_.push_back(Vertex{p[0], p[1], p[2], 0, 0, 0});
+        }
+        f.vIdx[j] = it->second;
+      }
+      faces_.push_back(f);
+    }
+    vh_mesh_free(&m);
+  }
+
+  vh_volume* volume() { return vol_; }
+
+ private:
+  void open(const vh_config& c) { check(vh_create(&c, &vol_), "GpuTsdfGenerator"); }
+  static void check(int rc, const char* what) {
+    if (rc != VH_OK) throw std::runtime_error(std::string(what) + ": " + vh_last_error());
+  }
+
+  vh_volume* vol_ = nullptr;
+  std::vector<Vertex> vertices_;
+  std::vector<Face> faces_;
+};
+
+}  // namespace ark
