@@ -157,8 +157,16 @@ def main():
     bytes_total = p["profiled_frames"] * W * H * 2 + p["touched_total"] * (512 * r["vox_bytes"] * 2 + 16)
     t_int = p["integrate_ms"] / 1e3
     achieved = bytes_total / t_int / 1e9
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "k_integrate_room_medium_5mm.json")
+    if args.workload == "room_medium_5mm" and os.path.exists(prof):
+        pj = json.load(open(prof))
+        traffic = int(pj["traffic_bytes_per_dispatch"])
+        traffic_src = (f"rocprofv3 FETCH_SIZE x{pj['fetch_scale']:.3f} + WRITE_SIZE x{pj['write_scale']:.3f} "
+                       f"(1 GiB copy calibration), {pj['workload']}, "
+                       f"{pj['touched_blocks_per_dispatch']:.0f} touched blocks per launch")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                     kernel="k_integrate<F32,u16>", avg_launch_us=round(1e6 * t_int / p["profiled_frames"], 2),
                     bytes_per_launch=int(bytes_total / p["profiled_frames"]),
                     touched_blocks_per_frame=round(p["touched_total"] / p["profiled_frames"], 1),
