@@ -1,8 +1,8 @@
 """ctypes loader of the CPU oracle (oracle/libvh_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
-CPU-baseline leg of bench.py, always as the checker of libvh, never as the thing measured
-or shipped. See vh_oracle.cpp for the reference lines each function restates.
+CPU-baseline leg of bench.py, always as the oracle that libvh's results are compared with,
+never as the thing measured or shipped. See vh_oracle.cpp for the reference lines each function restates.
 """
 from __future__ import annotations
 

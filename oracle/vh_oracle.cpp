@@ -1,7 +1,7 @@
 // vh_oracle.cpp - CPU restatement of the reference's voxel-hashing TSDF path.
 //
-// TEST INFRASTRUCTURE ONLY. This library is the checker for libvh: it is loaded by the
-// tests, by the smoke check and by the CPU-baseline leg of the benchmark, and by nothing
+// TEST INFRASTRUCTURE ONLY. This library is the oracle of libvh's results: it is loaded by
+// the tests, by the smoke check and by the CPU-baseline leg of the benchmark, and by nothing
 // that ships. It shares no source with libvh; it restates the algorithm from the
 // reference (citations below) with the documented corrections of the reference's defects,
 // evaluating every FP32 operation in the same order as the GPU kernels so that an FP32
