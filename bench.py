@@ -12,6 +12,12 @@ Reported beside the headline FP32 figure: the same pass with binary16 TSDF + uin
 storage, the FP16/FP32 speedup and the FP16 error against the oracle's FP64 shadow, the
 integration kernel's achieved HBM bandwidth (algorithmic bytes / kernel time, measured with
 HIP events on the volume's stream), and the oracle's CPU throughput on a bounded sample.
+Two further blocks: "gn" (pose refinement on the FP16 volume: 10 Levenberg-Marquardt
+iterations over the ~300k back-projected pixels of a held-out frame from a seeded
+2 deg / 3 cm pose error, the pose error after every evaluation) and "tracked" (the
+floor_large_4mm workload: 2000 frames of a 20x20x4 m floor plan, 4 mm voxels, FP16, each
+frame's pose - the previous refined pose times a noisy odometry increment - refined by 5
+solver iterations against the volume before it is integrated, final marching cubes).
 """
 import argparse
 import json
@@ -30,6 +36,167 @@ METRIC = "frames/s integrated + MC blocks/s, FP32 vs FP16-store at ≤1e-3 TSDF 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def se3_exp(xi):
+    """exp of xi = (rho, phi) in se(3) as a 4x4 (FP64 closed form)."""
+    rho, phi = np.asarray(xi[:3], float), np.asarray(xi[3:], float)
+    th = np.linalg.norm(phi)
+    K = np.array([[0, -phi[2], phi[1]], [phi[2], 0, -phi[0]], [-phi[1], phi[0], 0]])
+    if th < 1e-9:
+        R, V = np.eye(3) + K, np.eye(3) + 0.5 * K
+    else:
+        R = np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K
+        V = np.eye(3) + (1 - np.cos(th)) / th ** 2 * K + (th - np.sin(th)) / th ** 3 * K @ K
+    T = np.eye(4)
+    T[:3, :3], T[:3, 3] = R, V @ rho
+    return T
+
+
+def pose_err(T, T_true):
+    """(rotation error in degrees, translation error in millimetres)."""
+    T, T_true = np.asarray(T, np.float64), np.asarray(T_true, np.float64)
+    D = T[:3, :3] @ T_true[:3, :3].T
+    return (float(np.degrees(np.arccos(np.clip((np.trace(D) - 1) / 2, -1, 1)))),
+            float(1e3 * np.linalg.norm(T[:3, 3] - T_true[:3, 3])))
+
+
+def backproject(depth_u16, w, stride=1, scale=0.001, zmax=None):
+    """Camera-frame FP32 points ((u - cx) d / fx, (v - cy) d / fy, d) of the pixels with
+    0 < d <= zmax (default max_depth)."""
+    f32 = np.float32
+    d = depth_u16[::stride, ::stride].astype(f32) * f32(scale)
+    v, u = np.mgrid[0:depth_u16.shape[0]:stride, 0:depth_u16.shape[1]:stride]
+    ok = (d > 0) & (d <= f32(zmax or w["max_depth"]))
+    x = ((u.astype(f32) - f32(w["cx"])) * d) / f32(w["fx"])
+    y = ((v.astype(f32) - f32(w["cy"])) * d) / f32(w["fy"])
+    return np.stack([x[ok], y[ok], d[ok]], 1).astype(f32)
+
+
+def gn_block(vh, wl, w, vol, nf):
+    """Pose refinement on the FP16 volume after the full sequence: the held-out frame nf + 17
+    of the camera path (never integrated), all its valid pixels as points (and, separately,
+    those up to 3 m: far surfaces of a TSDF fused from noisy depth sit a few mm off, which
+    biases the objective), a seeded camera-frame pose error xi ~ N(0, diag(3 cm, 2 deg)) (and
+    the same error scaled by 1/4), 10 Levenberg-Marquardt iterations; pose error after every
+    evaluation."""
+    import torch
+
+    scene = vh.Scene(w["scene"], wl.SEED)
+    f = nf + 17
+    d, T = scene.frames(f, 1)
+    T_true = T[0].astype(np.float64)
+    rng = np.random.default_rng(wl.SEED + 2)
+    xi = np.concatenate([rng.normal(0, 0.03, 3), rng.normal(0, np.deg2rad(2.0), 3)])
+    out = {"frame": f, "iterations": 10, "storage": "fp16 (binary16 + uint16 weight)",
+           "perturbation": "camera frame, rho ~ N(0, 3 cm) and phi ~ N(0, 2 deg) per axis, seed SEED+2", "runs": []}
+    for scale, zmax in ((1.0, None), (0.25, None), (1.0, 3.0), (0.25, 3.0)):
+        pts = backproject(d[0], w, zmax=zmax)
+        T0 = T_true @ se3_exp(scale * xi)
+        vol.gn_refine(pts, T0, 10)  # warm (scratch allocation, code paths)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        T1, st = vol.gn_refine(pts, T0, 10)
+        dt = time.perf_counter() - t
+        errs = [pose_err(P, T_true) for P in st["pose"]]
+        out["runs"].append({"error_scale": scale, "max_point_depth_m": zmax or w["max_depth"], "points": int(len(pts)),
+                            "initial_err_deg_mm": [round(x, 4) for x in pose_err(T0, T_true)],
+                            "err_deg_mm_per_eval": [[round(a, 4), round(b, 3)] for a, b in errs],
+                            "rms_per_eval": [round(r, 5) for r in st["rms"]],
+                            "n_valid_per_eval": st["n_valid"], "wall_ms": round(1e3 * dt, 3)})
+        log(f"bench: gn x{scale} z<={zmax or w['max_depth']}: {len(pts)} points, {pose_err(T0, T_true)} -> {errs[-1]} "
+            f"in {1e3 * dt:.2f} ms")
+    return out
+
+
+def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.0005, sigma_deg=0.05):
+    """floor_large_4mm with tracking from odometry: frame 0 at its true pose; frame f > 0
+    starts from the refined pose of frame f-1 times a noisy motion increment
+    inc_f = (T_{f-1}^-1 T_f) exp(xi_f), xi_f ~ N(0, sigma_t) and N(0, sigma_deg) per axis
+    (camera frame, seeded), and is refined by gn_iters solver iterations on every stride-th
+    pixel up to zmax metres before integration. Dead reckoning (chaining the increments
+    without refinement) is reported beside it."""
+    import torch
+
+    w = dict(wl.WORKLOADS["floor_large_4mm"])
+    nf = n_frames or w["frames"]
+    gn_iters = w.get("gn_iters", gn_iters)
+    scene = vh.Scene(w["scene"], wl.SEED)
+    t = time.perf_counter()
+    depth, poses = scene.frames(0, nf)
+    log(f"bench: tracked: generated {nf} floor_large_4mm frames in {time.perf_counter() - t:.1f} s")
+    rng = np.random.default_rng(wl.SEED + 2)
+    P = poses.astype(np.float64)
+    inc = np.empty_like(P)
+    inc[0] = P[0]
+    for f in range(1, nf):
+        xi = np.concatenate([rng.normal(0, sigma_t, 3), rng.normal(0, np.deg2rad(sigma_deg), 3)])
+        inc[f] = np.linalg.inv(P[f - 1]) @ P[f] @ se3_exp(xi)
+    inc = inc.astype(np.float32)
+    dead = [inc[0].astype(np.float64)]
+    for f in range(1, nf):
+        dead.append(dead[-1] @ inc[f].astype(np.float64))
+    d_depth = torch.from_numpy(depth.view(np.int16)).cuda()
+    d_inc = torch.from_numpy(inc.reshape(nf, 16)).cuda()
+    d_out = torch.zeros(nf, 16, dtype=torch.float32, device="cuda")
+    d_stats = torch.zeros(nf * (gn_iters + 1) * vh.GN_STATS, dtype=torch.float64, device="cuda")
+    del depth
+    vol = vh.Volume(vh.config_from_workload(w))
+    mode = vh.PRIOR_INCREMENT
+    res = {"workload": "floor_large_4mm", "frames": nf, "gn_iters_per_frame": gn_iters, "point_stride": stride,
+           "max_point_depth_m": zmax, "prior": "refined pose of the previous frame x noisy odometry increment "
+           f"(camera frame, N(0, {sigma_t * 1e3:g} mm) and N(0, {sigma_deg:g} deg) per axis per frame, seed SEED+2)"}
+    try:
+        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode)
+        vol.integrate_next(1)  # captures the per-frame graph
+        vol.sync()
+        vol.reset()
+        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        vol.integrate_next(nf)
+        vol.sync()
+        dt = time.perf_counter() - t
+        st = vol.stats()
+        cap = 64 << 20
+        d_xyz = torch.empty(cap * 9, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ntri = vol.extract_mesh_device(d_xyz, None, cap)
+        mc_s = time.perf_counter() - t
+        del d_xyz
+        out = d_out.cpu().numpy().reshape(nf, 4, 4)
+        e_dead = np.array([pose_err(dead[f], P[f]) for f in range(1, nf)])
+        e_trk = np.array([pose_err(out[f], P[f]) for f in range(1, nf)])
+        summ = lambda e: {"mean": round(float(e.mean()), 4), "median": round(float(np.median(e)), 4),
+                          "p95": round(float(np.percentile(e, 95)), 4), "max": round(float(e.max()), 4),
+                          "last": round(float(e[-1]), 4)}
+        res.update({"fps": round(nf / dt, 2), "ms_per_frame": round(1e3 * dt / nf, 4),
+                    "blocks": st["blocks_allocated"], "pool_blocks": w["pool_blocks"],
+                    "mc_triangles": ntri, "mc_ms": round(1e3 * mc_s, 3),
+                    "mc_blocks_per_s": round(st["blocks_allocated"] / mc_s, 1),
+                    "rot_err_deg": {"dead_reckoning": summ(e_dead[:, 0]), "tracked": summ(e_trk[:, 0])},
+                    "trans_err_mm": {"dead_reckoning": summ(e_dead[:, 1]), "tracked": summ(e_trk[:, 1])}})
+        # per-stage device time (eager launches with events, same sequence)
+        vol.reset()
+        vol.set_profiling(True)
+        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode)
+        vol.integrate_next(nf)
+        vol.sync()
+        p = vol.stats()
+        vol.set_profiling(False)
+        n = max(1, p["profiled_frames"])
+        res["device_us_per_frame"] = {"track": round(1e3 * p["track_ms"] / n, 2),
+                                      "alloc": round(1e3 * p["alloc_ms"] / n, 2),
+                                      "integrate": round(1e3 * p["integrate_ms"] / n, 2)}
+        log(f"bench: tracked: {res['fps']} frames/s, {st['blocks_allocated']} blocks; mean error dead reckoning "
+            f"{res['rot_err_deg']['dead_reckoning']['mean']} deg {res['trans_err_mm']['dead_reckoning']['mean']} mm, "
+            f"tracked {res['rot_err_deg']['tracked']['mean']} deg {res['trans_err_mm']['tracked']['mean']} mm")
+    except vh.VhError as e:
+        res["error"] = str(e)
+        log(f"bench: tracked: {e}")
+    vol.close()
+    return res
 
 
 def run_pass(vol, vh, seq, nf, mc_every, d_xyz, cap, final_mc=True):
@@ -64,6 +231,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--no-gn", action="store_true", help="skip the pose-refinement block")
+    ap.add_argument("--no-tracked", action="store_true", help="skip the tracked floor_large_4mm block")
+    ap.add_argument("--tracked-frames", type=int, default=0, help="override the tracked sequence length")
     ap.add_argument("--counters", action="store_true",
                     help="short run for hardware-counter passes: one FP32 integration pass over the "
                          "sequence plus a 1 GiB device copy that calibrates FETCH_SIZE/WRITE_SIZE")
@@ -143,6 +313,8 @@ def main():
         run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False)
         pst = vol.stats()
         vol.set_profiling(False)
+        if name == "f16" and not args.no_gn:  # the volume now holds the whole sequence
+            results["gn"] = gn_block(vh, wl, w, vol, nf)
         results[name] = dict(fps=args.steps * nf / dt, ms_per_step=1e3 * dt / args.steps, fps_integrate=nf / dt_int,
                              blocks=st["blocks_allocated"], tris=ntri, mc_ms=1e3 * mc_s,
                              mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst,
@@ -184,6 +356,8 @@ def main():
            "roofline": roofline}
     labels = {"f16r": ("fp16", "binary16 tsdf + uint8 weight (cap 255) + int8 rounding residual, 4 B/voxel"),
               "f16": ("fp16_plain", "binary16 tsdf + uint16 weight (cap 255), 4 B/voxel")}
+    if "gn" in results:
+        out["gn"] = results.pop("gn")
     for name, (key, desc) in labels.items():
         if name not in results:
             continue
@@ -198,6 +372,10 @@ def main():
                     "integrate_kernel_gbs": round(ah, 1),
                     "integrate_avg_launch_us": round(1e6 * ph["integrate_ms"] / 1e3 / ph["profiled_frames"], 2)}
 
+    if not args.no_tracked:
+        del d_xyz, d_depth, d_pose
+        torch.cuda.empty_cache()
+        out["tracked"] = tracked_block(vh, wl, args.tracked_frames)
     if not args.no_cpu:
         out["cpu_baseline"], errs = cpu_baseline(vh, w, depth, poses, args.cpu_seconds, args.no_f16)
         for key, err in errs.items():

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 1
+#define VH_ABI_VERSION 2
 
 typedef enum vh_status {
   VH_OK = 0,
@@ -105,7 +105,26 @@ typedef struct vh_stats {
   double integrate_ms;        /* profiling mode: summed integration-kernel time */
   double alloc_ms;            /* profiling mode: summed allocation-kernel time */
   int64_t profiled_frames;
+  double track_ms;            /* profiling mode: summed tracking time (back-projection + solver) */
 } vh_stats;
+
+typedef enum vh_prior_mode {
+  VH_PRIOR_ABSOLUTE = 0,  /* d_prior_c2w[f] is frame f's starting pose */
+  VH_PRIOR_INCREMENT = 1  /* d_prior_c2w[f] is the motion from frame f-1 (f > 0) */
+} vh_prior_mode;
+
+/* Tracking of a bound sequence (vh_bind_tracking). Device buffers stay owned by the caller
+ * and must outlive the frames that use them. */
+typedef struct vh_track_config {
+  const float* d_prior_c2w; /* n_frames x 16 row-major (device) */
+  float* d_out_c2w;         /* n_frames x 16 row-major refined poses (device) */
+  double* d_stats;          /* n_frames x (gn_iters + 1) x 17 (device) or NULL */
+  int32_t gn_iters;         /* 0..63 */
+  int32_t point_stride;     /* 1..64 */
+  float max_point_depth;    /* metres; <= 0: the volume's max_depth */
+  int32_t prior_mode;       /* vh_prior_mode */
+  int32_t reserved[4];      /* zero */
+} vh_track_config;
 
 /* Triangle soup in host memory, owned by the library (freed by vh_mesh_free). */
 typedef struct vh_mesh {
@@ -119,6 +138,7 @@ typedef struct vh_gn_stats {
   int32_t n_valid[64];  /* points with a valid trilinear sample at each evaluated candidate */
   double rms[64];       /* residual RMS (normalised TSDF units) over those points */
   double step_norm[64]; /* |delta xi| proposed after the evaluation (0 for the last) */
+  double pose[64][12];  /* accepted pose after each evaluation (row-major 3x4) */
 } vh_gn_stats;
 
 /* ------------------------------------------------------------------ lifetime */
@@ -144,6 +164,21 @@ int vh_integrate(vh_volume* vol, const vh_frame* frame);
  * (device memory). Later vh_integrate_next calls consume them in order. */
 int vh_bind_sequence(vh_volume* vol, const void* d_depth, int32_t depth_format,
                      float depth_scale, const float* d_c2w, int32_t n_frames);
+/* Device-resident input with tracking: like vh_bind_sequence, but each frame's pose is
+ * first refined against the current volume (gn_iters Levenberg-Marquardt steps of
+ * vh_gn_refine over the frame's depth back-projected on every point_stride-th pixel of every
+ * point_stride-th row; pixels with depth outside (0, min(max_depth, max_point_depth)] are
+ * skipped, max_point_depth <= 0 meaning max_depth), then the frame is integrated at the
+ * refined pose, which is also written (FP32, row-major 4x4) to d_out_c2w[f]. The starting
+ * pose of frame f is d_prior_c2w[f] (VH_PRIOR_ABSOLUTE), or, with VH_PRIOR_INCREMENT and
+ * f > 0, the refined pose of frame f-1 times d_prior_c2w[f] (a camera-frame motion, e.g.
+ * odometry); frame 0 always starts from d_prior_c2w[0]. d_stats (device, may be NULL)
+ * receives per frame (gn_iters + 1) rows of 17 doubles: n_valid, rms, |delta xi|, lambda,
+ * accepted, accepted pose (12). On an empty volume no point has a sample and the starting
+ * pose is kept.
+ * No reference counterpart (the reference takes poses as given, PointCloudGenerator.cpp:93-128). */
+int vh_bind_tracking(vh_volume* vol, const void* d_depth, int32_t depth_format, float depth_scale,
+                     int32_t n_frames, const vh_track_config* track);
 /* Enqueues the next n frames of the bound sequence on the volume's stream (one captured
  * HIP graph launch per frame unless profiling is on). Asynchronous; see vh_sync. */
 int vh_integrate_next(vh_volume* vol, int32_t n);
@@ -200,7 +235,8 @@ int vh_query_sdf_backward_device(vh_volume* vol, const float* d_pts, int64_t n,
                                  float* d_voxel_grad);
 /* Gauss-Newton refinement of c2w (host, row-major 4x4, updated in place, FP64) so that the
  * camera-frame points p_i (host, n x 3) land on the zero level set: minimises
- * sum_i sdf(R p_i + t)^2, counting a point without a valid sample as a residual of 1.
+ * sum_i sdf(R p_i + t)^2, counting a point without a valid sample as a residual of 1;
+ * points with a non-finite coordinate are skipped.
  * Each of the `iters` (<= 63) steps reduces J^T J, J^T r and the cost in FP64 on the GPU
  * and solves the damped 6x6 system (Levenberg-Marquardt, Cholesky) there; a last
  * evaluation picks the best accepted pose. stats (may be NULL) holds iters + 1 entries.

@@ -282,7 +282,7 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
       for (int r = 0; r < 3; ++r)
         cam[r] = (cb[r] + ((float)vy * (T[4 + r] * o.vox) + (float)vz * (T[8 + r] * o.vox))) + (float)vx * (T[r] * o.vox);
       const float xc = cam[0], yc = cam[1], zc = cam[2];
-      if (!(zc > 0.0f)) continue;
+      if (!(zc >= 0x1p-64f && zc <= 0x1p64f)) continue;  // depth range of the projection
       const float rz = 1.0f / zc;
       const float uf = std::nearbyint(o.fx * (xc * rz) + o.cx);
       const float vf = std::nearbyint(o.fy * (yc * rz) + o.cy);

@@ -38,7 +38,7 @@ def test_every_declared_function_is_exported(vh):
 
 def test_abi_version_and_defaults(vh):
     lib = vh.load()
-    assert lib.vh_abi_version() == 1
+    assert lib.vh_abi_version() == vh.ABI_VERSION == 2
     c = vh.make_config()
     assert (c.width, c.height, c.log2_buckets, c.storage) == (640, 480, 18, 0)
     assert C.sizeof(vh.Config) == 4 * (2 + 8 + 5 + 3 + 3 + 8)
@@ -98,3 +98,32 @@ def test_dropin_wrapper_builds_and_fails_cleanly_without_gpu(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-I",
                         os.path.join(PKG, "host"), str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_the_header(vh, tmp_path):
+    """The ctypes mirrors of the ABI structs have the C compiler's sizes and field offsets."""
+    import shutil
+
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"vh_config": vh.Config, "vh_frame": vh.Frame, "vh_stats": vh.Stats, "vh_mesh": vh.Mesh,
+               "vh_gn_stats": vh.GnStats, "vh_track_config": vh.TrackConfig}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vh.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -10,38 +10,9 @@ import numpy as np
 import pytest
 
 from conftest import look_down_z, plane_depth
+from gn_reference import lm_reference, pose_error, se3_exp, skew, world_points  # noqa: F401
 
 pytestmark = pytest.mark.gpu
-
-
-def skew(v):
-    return np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
-
-
-def se3_exp(xi):
-    """exp of xi = (rho, phi) as a 4x4 (FP64)."""
-    rho, phi = np.asarray(xi[:3], float), np.asarray(xi[3:], float)
-    th = np.linalg.norm(phi)
-    K = skew(phi)
-    if th < 1e-9:
-        R, V = np.eye(3) + K, np.eye(3) + 0.5 * K
-    else:
-        R = np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K
-        V = np.eye(3) + (1 - np.cos(th)) / th ** 2 * K + (th - np.sin(th)) / th ** 3 * K @ K
-    T = np.eye(4)
-    T[:3, :3], T[:3, 3] = R, V @ rho
-    return T
-
-
-def world_points(T, pts):
-    """q = R p + t in FP64 with the kernels' operation order ((x*T0 + y*T1) + z*T2) + T3, from
-    FP32 points and pose entries."""
-    T = np.asarray(np.asarray(T, np.float32), np.float64)
-    p = np.asarray(np.asarray(pts, np.float32), np.float64)
-    q = np.empty_like(p)
-    for r in range(3):
-        q[:, r] = ((p[:, 0] * T[r, 0] + p[:, 1] * T[r, 1]) + p[:, 2] * T[r, 2]) + T[r, 3]
-    return q
 
 
 def backproject(depth_m, T, step=2, fx=525.0, fy=525.0, cx=319.5, cy=239.5):
@@ -272,42 +243,6 @@ def corner_view(pos, yaw_deg, pitch_deg):
     M = np.eye(4)
     M[:3, 0], M[:3, 1], M[:3, 2], M[:3, 3] = r, np.cross(f, r), f, pos
     return M.astype(np.float32).astype(np.float64)
-
-
-def lm_reference(ref, pts, T0, iters, lam0=1e-4):
-    """CPU FP64 restatement of the GPU solver: J from the FP32 twin of the sample, FP64 sums,
-    cost = sum r^2 + 1 per invalid point, Levenberg-Marquardt accept/reject, T <- exp(d^) T."""
-    n = len(pts)
-
-    def evaluate(T):
-        q = world_points(T, pts)
-        s, g, v = ref.query_twin(q)
-        g64 = g.astype(np.float64)
-        J = np.concatenate([g64, np.cross(q, g64)], 1)[v]
-        r = s.astype(np.float64)[v]
-        return (r ** 2).sum() + (n - v.sum()), J.T @ J, J.T @ r
-
-    cand, acc, lam, out = T0.copy(), None, lam0, []
-    for it in range(iters + 1):
-        cost, A, b = evaluate(cand)
-        if acc is None or cost <= acc[1]:
-            lam = lam if acc is None else max(lam / 3, 1e-9)
-            acc = (cand, cost, A, b)
-        else:
-            lam = min(lam * 4, 1e9)
-        if it == iters:
-            break
-        Ad = acc[2] + np.diag(lam * np.diag(acc[2]) + 1e-12)
-        x = np.linalg.solve(Ad, -acc[3])
-        cand = se3_exp(x) @ acc[0]
-        out.append(acc[0])
-    return acc[0], out
-
-
-def pose_error(T, T_true):
-    D = T @ np.linalg.inv(T_true)
-    return (np.degrees(np.arccos(np.clip((np.trace(D[:3, :3]) - 1) / 2, -1, 1))),
-            float(np.linalg.norm(T[:3, 3] - T_true[:3, 3])))
 
 
 def test_gauss_newton_matches_cpu_reference_and_converges(vh, oracle, wl, room):

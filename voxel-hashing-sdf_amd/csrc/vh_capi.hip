@@ -1,9 +1,11 @@
 // vh_capi.hip - extern "C" entry points of libvh (see include/vh.h).
 //
 // One volume owns one HIP stream (default priority) and all of its device memory. The
-// per-frame pipeline (k_alloc -> k_integrate) is captured once into a HIP graph per input
-// kind (host-staged frames, bound device sequence); its kernels read the frame index and
-// counters from device memory, so a launch needs no host-side parameter updates.
+// per-frame pipeline (k_alloc -> k_integrate, preceded for tracked sequences by the
+// back-projection and the Gauss-Newton iterations) is captured once into a HIP graph per
+// input kind (host-staged frames, bound device sequence, tracked sequence); its kernels read
+// the frame index and counters from device memory, so a launch needs no host-side parameter
+// updates.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,13 +80,23 @@ struct vh_volume {
   const float* seq_pose = nullptr;
   int32_t seq_n = 0, seq_format = VH_DEPTH_U16;
   float seq_scale = 0.001f;
-  // graphs: [0] staged u16, [1] staged f32, [2] bound sequence
-  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+  // tracking of the bound sequence (vh_bind_tracking)
+  bool track = false;
+  const float* trk_prior = nullptr;
+  float* trk_out = nullptr;
+  double* trk_stats = nullptr;
+  int32_t trk_iters = 0, trk_stride = 1, trk_increment = 0;
+  float trk_max_depth = 0.0f;  // depth limit of the tracking points
+  float* trk_pts = nullptr;
+  size_t trk_pts_cap = 0;  // points the buffer holds
+  // graphs: [0] staged u16, [1] staged f32, [2] bound sequence, [3] tracked sequence
+  static constexpr int kGraphs = 4;
+  hipGraphExec_t graph[kGraphs] = {nullptr, nullptr, nullptr, nullptr};
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<hipEvent_t> ev_pending;  // groups of 3: start, after alloc, after integrate
-  double alloc_ms = 0.0, integrate_ms = 0.0;
+  std::vector<hipEvent_t> ev_pending;  // groups of 4: start, after tracking, after alloc, after integrate
+  double alloc_ms = 0.0, integrate_ms = 0.0, track_ms = 0.0;
   int64_t profiled_frames = 0;
   // marching-cubes output buffer (grown on demand) and scan scratch
   float* mc_tris = nullptr;
@@ -115,9 +127,9 @@ size_t voxel_bytes(int storage) { return storage == VH_STORAGE_F32 ? 8 : 4; }
 FrameArgs frame_args(vh_volume* v, int which) {
   FrameArgs a{};
   const vh_config& c = v->cfg;
-  if (which == 2) {
+  if (which >= 2) {
     a.depth = v->seq_depth;
-    a.poses = v->seq_pose;
+    a.poses = which == 3 ? v->trk_out : v->seq_pose;
     a.n_seq = (uint32_t)v->seq_n;
     a.cursor_id = 1;
     a.depth_scale = v->seq_format == VH_DEPTH_U16 ? v->seq_scale : 1.0f;
@@ -149,14 +161,71 @@ FrameArgs frame_args(vh_volume* v, int which) {
   return a;
 }
 
+QueryArgs query_args(vh_volume* v) {
+  QueryArgs q{};
+  q.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
+  q.voxels = v->voxels;
+  q.pool_cap = (uint32_t)v->cfg.pool_blocks;
+  q.vox = v->cfg.vox_size;
+  return q;
+}
+
+TrackArgs track_args(vh_volume* v) {
+  const vh_config& c = v->cfg;
+  TrackArgs t{};
+  t.depth = v->seq_depth;
+  t.depth_u16 = v->seq_format == VH_DEPTH_U16;
+  t.depth_scale = t.depth_u16 ? v->seq_scale : 1.0f;
+  t.prior = v->trk_prior;
+  t.increment = v->trk_increment;
+  t.out = v->trk_out;
+  t.stats = v->trk_stats;
+  t.eval_stats = v->gn_stats;
+  t.n_seq = (uint32_t)v->seq_n;
+  t.ctl = v->ctl;
+  t.W = c.width;
+  t.H = c.height;
+  t.stride = v->trk_stride;
+  t.pw = (c.width + v->trk_stride - 1) / v->trk_stride;
+  t.ph = (c.height + v->trk_stride - 1) / v->trk_stride;
+  t.K = Intr{c.fx, c.fy, c.cx, c.cy};
+  t.max_depth = v->trk_max_depth;
+  t.pts = v->trk_pts;
+  t.pose = v->gn_pose;
+  t.state = v->gn_state;
+  t.iters = v->trk_iters;
+  return t;
+}
+
+// Tracking part of a frame of the tracked sequence: back-projection, iters + 1 solver
+// evaluations (the last only picks the best accepted pose), the pose write-out.
+int enqueue_track_kernels(vh_volume* v) {
+  const TrackArgs t = track_args(v);
+  VH_HIP(launch_track_begin(t, v->stream));
+  GnArgs g{};
+  g.q = query_args(v);
+  g.pts = t.pts;
+  g.n = (int64_t)t.pw * t.ph;
+  g.pose = v->gn_pose;
+  g.partials = v->gn_partials;
+  for (int it = 0; it <= t.iters; ++it) {
+    VH_HIP(launch_gn_accum(g, v->cfg.storage, kGnGrid, v->stream));
+    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_state, v->gn_pose, it == t.iters ? 1 : 0,
+                           v->gn_stats + kGnStats * it, v->stream));
+  }
+  VH_HIP(launch_track_end(t, v->stream));
+  return VH_OK;
+}
+
+int frame_u16(vh_volume* v, int which) { return which >= 2 ? v->seq_format == VH_DEPTH_U16 : which == 0; }
+
 int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale) {
   FrameArgs a = frame_args(v, which);
-  int u16;
-  if (which == 2) {
-    u16 = v->seq_format == VH_DEPTH_U16;
-  } else {
-    u16 = which == 0;
-    a.depth_scale = u16 ? staged_scale : 1.0f;
+  const int u16 = frame_u16(v, which);
+  if (which < 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+  if (which == 3) {
+    int rc = enqueue_track_kernels(v);
+    if (rc) return rc;
   }
   VH_HIP(launch_alloc(a, u16, v->stream));
   VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
@@ -176,19 +245,22 @@ hipEvent_t take_event(vh_volume* v) {
 
 int launch_frame(vh_volume* v, int which, float staged_scale) {
   if (v->profiling) {
-    hipEvent_t e0 = take_event(v), e1 = take_event(v), e2 = take_event(v);
-    if (!e0 || !e1 || !e2) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
+    hipEvent_t e[4] = {take_event(v), take_event(v), take_event(v), take_event(v)};
+    if (!e[0] || !e[1] || !e[2] || !e[3]) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
     FrameArgs a = frame_args(v, which);
-    int u16 = which == 2 ? v->seq_format == VH_DEPTH_U16 : which == 0;
-    if (which != 2) a.depth_scale = u16 ? staged_scale : 1.0f;
-    VH_HIP(hipEventRecord(e0, v->stream));
+    const int u16 = frame_u16(v, which);
+    if (which < 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+    VH_HIP(hipEventRecord(e[0], v->stream));
+    if (which == 3) {
+      int rc = enqueue_track_kernels(v);
+      if (rc) return rc;
+    }
+    VH_HIP(hipEventRecord(e[1], v->stream));
     VH_HIP(launch_alloc(a, u16, v->stream));
-    VH_HIP(hipEventRecord(e1, v->stream));
+    VH_HIP(hipEventRecord(e[2], v->stream));
     VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
-    VH_HIP(hipEventRecord(e2, v->stream));
-    v->ev_pending.push_back(e0);
-    v->ev_pending.push_back(e1);
-    v->ev_pending.push_back(e2);
+    VH_HIP(hipEventRecord(e[3], v->stream));
+    for (hipEvent_t x : e) v->ev_pending.push_back(x);
     return VH_OK;
   }
   hipGraphExec_t& g = v->graph[which];
@@ -211,7 +283,7 @@ int launch_frame(vh_volume* v, int which, float staged_scale) {
 }
 
 void drop_graphs(vh_volume* v, int which_mask) {
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < vh_volume::kGraphs; ++k)
     if ((which_mask >> k) & 1 && v->graph[k]) {
       (void)hipGraphExecDestroy(v->graph[k]);
       v->graph[k] = nullptr;
@@ -219,10 +291,12 @@ void drop_graphs(vh_volume* v, int which_mask) {
 }
 
 int collect_profile(vh_volume* v) {
-  for (size_t k = 0; k + 2 < v->ev_pending.size(); k += 3) {
-    float a = 0.0f, b = 0.0f;
-    VH_HIP(hipEventElapsedTime(&a, v->ev_pending[k], v->ev_pending[k + 1]));
-    VH_HIP(hipEventElapsedTime(&b, v->ev_pending[k + 1], v->ev_pending[k + 2]));
+  for (size_t k = 0; k + 3 < v->ev_pending.size(); k += 4) {
+    float t = 0.0f, a = 0.0f, b = 0.0f;
+    VH_HIP(hipEventElapsedTime(&t, v->ev_pending[k], v->ev_pending[k + 1]));
+    VH_HIP(hipEventElapsedTime(&a, v->ev_pending[k + 1], v->ev_pending[k + 2]));
+    VH_HIP(hipEventElapsedTime(&b, v->ev_pending[k + 2], v->ev_pending[k + 3]));
+    v->track_ms += t;
     v->alloc_ms += a;
     v->integrate_ms += b;
     v->profiled_frames += 1;
@@ -304,25 +378,16 @@ void free_volume(vh_volume* v) {
   if (!v) return;
   (void)hipSetDevice(v->device);
   if (v->stream) (void)hipStreamSynchronize(v->stream);
-  drop_graphs(v, 7);
+  drop_graphs(v, (1 << vh_volume::kGraphs) - 1);
   for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
-                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state};
+                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
   delete v;
-}
-
-QueryArgs query_args(vh_volume* v) {
-  QueryArgs q{};
-  q.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
-  q.voxels = v->voxels;
-  q.pool_cap = (uint32_t)v->cfg.pool_blocks;
-  q.vox = v->cfg.vox_size;
-  return q;
 }
 
 int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
@@ -524,7 +589,63 @@ int vh_bind_sequence(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   v->seq_n = n;
   v->seq_format = fmt;
   v->seq_scale = scale;
-  if (!same) drop_graphs(v, 4);  // the captured graph holds the sequence pointers
+  v->track = false;
+  if (!same) drop_graphs(v, 4 | 8);  // the captured graphs hold the sequence pointers
+  const uint32_t zero = 0;
+  VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  return VH_OK;
+}
+
+int vh_bind_tracking(vh_volume* v, const void* d_depth, int32_t fmt, float scale, int32_t n,
+                     const vh_track_config* tc) {
+  if (!v || !d_depth || !tc || !tc->d_prior_c2w || !tc->d_out_c2w || n <= 0)
+    return fail(VH_ERR_INVALID_ARG, "bad sequence");
+  if (tc->gn_iters < 0 || tc->gn_iters > 63) return fail(VH_ERR_INVALID_ARG, "gn_iters must be in 0..63");
+  if (tc->point_stride < 1 || tc->point_stride > 64)
+    return fail(VH_ERR_INVALID_ARG, "point_stride must be in 1..64");
+  if (std::isnan(tc->max_point_depth)) return fail(VH_ERR_INVALID_ARG, "max_point_depth is NaN");
+  if (tc->prior_mode != VH_PRIOR_ABSOLUTE && tc->prior_mode != VH_PRIOR_INCREMENT)
+    return fail(VH_ERR_INVALID_ARG, "unknown prior_mode");
+  for (int32_t r : tc->reserved)
+    if (r) return fail(VH_ERR_INVALID_ARG, "reserved fields must be zero");
+  if (fmt != VH_DEPTH_U16 && fmt != VH_DEPTH_F32) return fail(VH_ERR_INVALID_ARG, "unknown depth format");
+  if (fmt == VH_DEPTH_U16 && !(scale > 0.0f)) return fail(VH_ERR_INVALID_ARG, "depth_scale must be > 0");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  VH_HIP(hipStreamSynchronize(v->stream));
+  if ((rc = ensure_gn_scratch(v))) return rc;
+  const int stride = tc->point_stride;
+  const size_t npts = (size_t)((v->cfg.width + stride - 1) / stride) * ((v->cfg.height + stride - 1) / stride);
+  if (npts > v->trk_pts_cap) {
+    if (v->trk_pts) (void)hipFree(v->trk_pts);
+    v->trk_pts = nullptr;
+    v->trk_pts_cap = 0;
+    drop_graphs(v, 8);
+    if ((rc = dev_alloc(&v->trk_pts, npts * 3 * sizeof(float)))) return rc;
+    v->trk_pts_cap = npts;
+  }
+  const float zmax = std::min(v->cfg.max_depth, tc->max_point_depth > 0.0f ? tc->max_point_depth : v->cfg.max_depth);
+  const int incr = tc->prior_mode == VH_PRIOR_INCREMENT;
+  const bool same = v->track && v->seq_depth == d_depth && v->trk_prior == tc->d_prior_c2w &&
+                    v->trk_out == tc->d_out_c2w && v->trk_stats == tc->d_stats && v->seq_n == n &&
+                    v->seq_format == fmt && v->seq_scale == scale && v->trk_iters == tc->gn_iters &&
+                    v->trk_stride == stride && v->trk_max_depth == zmax && v->trk_increment == incr;
+  v->seq_depth = d_depth;
+  v->seq_pose = tc->d_prior_c2w;
+  v->seq_n = n;
+  v->seq_format = fmt;
+  v->seq_scale = scale;
+  v->track = true;
+  v->trk_prior = tc->d_prior_c2w;
+  v->trk_out = tc->d_out_c2w;
+  v->trk_stats = tc->d_stats;
+  v->trk_iters = tc->gn_iters;
+  v->trk_stride = stride;
+  v->trk_max_depth = zmax;
+  v->trk_increment = incr;
+  if (!same) drop_graphs(v, 4 | 8);
   const uint32_t zero = 0;
   VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
@@ -538,7 +659,7 @@ int vh_integrate_next(vh_volume* v, int32_t n) {
   int rc = set_device(v);
   if (rc) return rc;
   for (int k = 0; k < n; ++k) {
-    rc = launch_frame(v, 2, v->seq_scale);
+    rc = launch_frame(v, v->track ? 3 : 2, v->seq_scale);
     if (rc) return rc;
   }
   return VH_OK;
@@ -561,7 +682,7 @@ int vh_set_profiling(vh_volume* v, int32_t on) {
   rc = collect_profile(v);
   if (rc) return rc;
   v->profiling = on != 0;
-  v->alloc_ms = v->integrate_ms = 0.0;
+  v->alloc_ms = v->integrate_ms = v->track_ms = 0.0;
   v->profiled_frames = 0;
   return VH_OK;
 }
@@ -588,6 +709,7 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->integrate_ms = v->integrate_ms;
   s->alloc_ms = v->alloc_ms;
   s->profiled_frames = v->profiled_frames;
+  s->track_ms = v->track_ms;
   return VH_OK;
 }
 
@@ -786,7 +908,7 @@ int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int3
   // accepted pose
   for (int it = 0; it <= iters; ++it) {
     VH_HIP(launch_gn_accum(a, v->cfg.storage, kGnGrid, v->stream));
-    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_state, v->gn_pose, (double)n, it == iters ? 1 : 0,
+    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_state, v->gn_pose, it == iters ? 1 : 0,
                            v->gn_stats + kGnStats * it, v->stream));
   }
   std::vector<double> host_stats((size_t)kGnStats * (iters + 1));
@@ -803,6 +925,7 @@ int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int3
       stats->n_valid[it] = (int32_t)host_stats[kGnStats * it];
       stats->rms[it] = host_stats[kGnStats * it + 1];
       stats->step_norm[it] = host_stats[kGnStats * it + 2];
+      for (int k = 0; k < 12; ++k) stats->pose[it][k] = host_stats[kGnStats * it + 5 + k];
     }
   }
   return VH_OK;

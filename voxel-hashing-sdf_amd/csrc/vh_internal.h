@@ -22,6 +22,15 @@
 
 namespace vh {
 
+// 1/z as the hardware reciprocal refined by one Newton step (3 instructions instead of the
+// 11 of the IEEE division sequence). For every normal z whose reciprocal is normal
+// (z < 2^126) this equals the correctly rounded quotient 1.0f / z bit for bit; the check
+// over all such z is tests/device_checks/recip_exact.hip. Callers bound z to [2^-64, 2^64].
+__device__ __forceinline__ float recip_rn(float z) {
+  const float r0 = __builtin_amdgcn_rcpf(z);
+  return __builtin_fmaf(__builtin_fmaf(-z, r0, 1.0f), r0, r0);
+}
+
 enum : uint32_t { kFlagPoolFull = 1u, kFlagHashFull = 2u, kFlagRange = 4u, kFlagTouchedFull = 8u };
 constexpr int kMaxProbe = 64;  // buckets examined before a key is declared unplaceable
 constexpr int kStripes = 64;   // striped counters / touched sub-lists
@@ -95,8 +104,9 @@ struct QueryArgs {
   float vox;
 };
 
-// Gauss-Newton / backward accumulators: 21 (J^T J upper) + 6 (J^T r) + 1 (r^2) + 1 (count).
-constexpr int kGnAcc = 29;
+// Gauss-Newton accumulators: 21 (J^T J upper) + 6 (J^T r) + 1 (r^2) + 1 (valid samples) +
+// 1 (points considered: points with a non-finite coordinate are skipped).
+constexpr int kGnAcc = 30;
 
 // Levenberg-Marquardt state kept on the device between iterations.
 struct GnState {
@@ -107,14 +117,39 @@ struct GnState {
   int has;          // an accepted pose exists
   int pad;
 };
-constexpr int kGnStats = 5;  // per-iteration stats: n_valid, rms, |delta|, lambda, accepted
+// per-evaluation stats: n_valid, rms, |delta|, lambda, accepted, accepted pose (12)
+constexpr int kGnStats = 17;
 
 struct GnArgs {
   QueryArgs q;
-  const float* pts;   // camera-frame points
+  const float* pts;   // camera-frame points (a non-finite coordinate marks a point to skip)
   int64_t n;
   const double* pose; // 12 doubles, row-major 3x4 camera-to-world
   double* partials;   // gridDim.x * kGnAcc
+};
+
+// Tracking over a bound sequence: frame f (the sequence cursor) starts Gauss-Newton from
+// prior[f] (or, with `increment` and f > 0, from out[f-1] * prior[f]), back-projects its
+// depth on a stride grid, and writes the accepted pose to out[f], which the frame's
+// allocation and integration then use.
+struct TrackArgs {
+  const void* depth;      // n_seq frames of W*H
+  int32_t depth_u16;
+  float depth_scale;
+  const float* prior;     // n_seq x 16 row-major camera-to-world (or motion from frame f-1)
+  int32_t increment;      // prior[f] is a motion from the refined pose of frame f-1
+  float* out;             // n_seq x 16
+  double* stats;          // n_seq x (iters + 1) x kGnStats, or null
+  const double* eval_stats; // the solver's rows for the current frame ((iters + 1) x kGnStats)
+  uint32_t n_seq;
+  FrameCtl* ctl;
+  int32_t W, H, stride, pw, ph;  // image, stride, point grid (ceil(W/stride) x ceil(H/stride))
+  Intr K;
+  float max_depth;        // points need 0 < d <= max_depth
+  float* pts;             // pw * ph * 3
+  double* pose;           // Gauss-Newton candidate (12)
+  GnState* state;
+  int32_t iters;
 };
 
 // launchers (vh_kernels.hip); all asynchronous on `st`
@@ -132,6 +167,10 @@ hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, in
 hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStream_t st);
 hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st);
 hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand,
-                           double n_points, int final_step, double* stats_row, hipStream_t st);
+                           int final_step, double* stats_row, hipStream_t st);
+// Gauss-Newton state reset + prior pose of the current frame; depth back-projection.
+hipError_t launch_track_begin(const TrackArgs& t, hipStream_t st);
+// Accepted pose (and the frame's solver stats) to the per-frame outputs.
+hipError_t launch_track_end(const TrackArgs& t, hipStream_t st);
 
 }  // namespace vh

@@ -31,6 +31,7 @@ __device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned
 
 __device__ __forceinline__ void raise_flag(FrameCtl* ctl, uint32_t f) { atomicOr(&ctl->flags, f); }
 
+
 // Read-only lookup (no concurrent inserts): pool index of `key`, or kNoBlock.
 // Slots of a bucket fill in order, so the first empty slot ends the search.
 __device__ __forceinline__ uint32_t find_block(const HashView& h, uint64_t key) {
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
     for (int k = 0; k < 8; ++k) {
       const float fk = (float)k;
       zc[k] = cl[2] + fk * bf.ax[2];
-      pix[k] = project_cam(cl[0] + fk * bf.ax[0], cl[1] + fk * bf.ax[1], zc[k], a.K, a.W, a.H);
+      pix[k] = project_cam(cl[0] + fk * bf.ax[0], cl[1] + fk * bf.ax[1], zc[k], recip_rn(zc[k]), a.K, a.W, a.H);
     }
     float dv[8];
 #pragma unroll
@@ -851,8 +852,11 @@ __global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
 #pragma unroll
   for (int k = 0; k < kGnAcc; ++k) acc[k] = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float px = a.pts[3 * i], py = a.pts[3 * i + 1], pz = a.pts[3 * i + 2];
+    if (!(isfinite(px) && isfinite(py) && isfinite(pz))) continue;  // no depth at this pixel
+    acc[29] += 1.0;
     double qx, qy, qz;
-    pose_apply_d(a.pose, a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2], qx, qy, qz);
+    pose_apply_d(a.pose, px, py, pz, qx, qy, qz);
     float r, g[3], w[8];
     uint32_t vid[8];
     if (!sample_sdf<kStorage>(a.q, qx, qy, qz, r, g, vid, w)) continue;
@@ -874,11 +878,12 @@ __global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
 // One workgroup, FP64: deterministic sum of the partials at the candidate pose, then a
 // Levenberg-Marquardt decision and step.
 //   cost(candidate) = sum r^2 over valid points + 1 per invalid point (a saturated residual),
-//   so costs at different poses compare. The candidate is accepted if its cost does not
-//   exceed the accepted one (lambda /= 3) and rejected otherwise (lambda *= 4). Unless
-//   `final`, the next candidate is exp(delta^) T_acc with
+//   so costs at different poses compare; skipped points count neither way. The candidate is
+//   accepted if its cost does not exceed the accepted one (lambda /= 3) and rejected
+//   otherwise (lambda *= 4). Unless `final`, the next candidate is exp(delta^) T_acc with
 //   (A + lambda diag(A)) delta = -b, A = J^T J and b = J^T r at the accepted pose,
-//   delta = (rho, phi), T <- exp(delta^) T. stats_row: [n_valid, rms, |delta|, lambda, accepted].
+//   delta = (rho, phi), T <- exp(delta^) T.
+//   stats_row: [n_valid, rms, |delta|, lambda, accepted, accepted pose (12)].
 __device__ void se3_left_update(const double* x, const double* pose, double* out) {
   const double rho[3] = {x[0], x[1], x[2]}, ph[3] = {x[3], x[4], x[5]};
   const double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
@@ -914,18 +919,22 @@ __device__ void se3_left_update(const double* x, const double* pose, double* out
   }
 }
 
-__global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int grid, GnState* st, double* cand,
-                                                 double n_points, int final_step, double* stats_row) {
+// The partials are summed by all 256 lanes at once (each lane a strided subset of the
+// workgroups, then the fixed workgroup tree), so the loads are in flight together.
+__global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int grid, GnState* st, double* cand,
+                                                  int final_step, double* stats_row) {
   __shared__ double tot[kGnAcc];
-  if (threadIdx.x < kGnAcc) {
-    double s = 0.0;
-    for (int g = 0; g < grid; ++g) s += partials[kGnAcc * (size_t)g + threadIdx.x];
-    tot[threadIdx.x] = s;
+  double acc[kGnAcc];
+#pragma unroll
+  for (int k = 0; k < kGnAcc; ++k) acc[k] = 0.0;
+  for (int g = threadIdx.x; g < grid; g += 256) {
+#pragma unroll
+    for (int k = 0; k < kGnAcc; ++k) acc[k] += partials[kGnAcc * (size_t)g + k];
   }
-  __syncthreads();
+  wg_reduce_store<kGnAcc>(acc, tot);
   if (threadIdx.x != 0) return;
   const double cnt = tot[28];
-  const double cost = tot[27] + (n_points - cnt);
+  const double cost = tot[27] + (tot[29] - cnt);
   stats_row[0] = cnt;
   stats_row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
   stats_row[2] = 0.0;
@@ -941,6 +950,7 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int gri
   }
   stats_row[3] = st->lambda;
   stats_row[4] = accept ? 1.0 : 0.0;
+  for (int k = 0; k < 12; ++k) stats_row[5 + k] = st->pose[k];
   if (final_step) {
     for (int k = 0; k < 12; ++k) cand[k] = st->pose[k];
     return;
@@ -983,6 +993,68 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* partials, int gri
   }
   se3_left_update(x, st->pose, cand);
   stats_row[2] = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+}
+
+// ============================================================================ tracking
+__device__ __forceinline__ uint32_t track_slot(const TrackArgs& t) { return t.ctl->cursor[1] % t.n_seq; }
+
+// One workgroup: the frame's starting pose becomes the candidate; the solver state is
+// cleared (a kernel, not a memset node, so the captured graph replays it exactly). With
+// increments, start = out[f-1] * prior[f] in FP64, each entry summed in column order
+// ((a0 b0 + a1 b1) + a2 b2) + a3 b3.
+__global__ __launch_bounds__(64) void k_track_begin(TrackArgs t) {
+  const uint32_t fs = track_slot(t);
+  const int k = threadIdx.x;
+  const float* P = t.prior + 16 * (size_t)fs;
+  if (k < 12) {
+    if (t.increment && fs > 0) {
+      const float* A = t.out + 16 * (size_t)(fs - 1);
+      const int r = k >> 2, c = k & 3;
+      t.pose[k] = (((double)A[4 * r] * (double)P[c] + (double)A[4 * r + 1] * (double)P[4 + c]) +
+                   (double)A[4 * r + 2] * (double)P[8 + c]) + (double)A[4 * r + 3] * (double)P[12 + c];
+    } else {
+      t.pose[k] = (double)P[k];
+    }
+  }
+  if (k == 0) {
+    GnState z{};
+    z.lambda = 1e-4;
+    *t.state = z;
+  }
+}
+
+// Camera-frame point of every stride-th pixel: ((u - cx) d / fx, (v - cy) d / fy, d) in FP32
+// with this operation order; pixels without a valid depth give NaN (skipped by the solver).
+template <bool kU16>
+__global__ __launch_bounds__(256) void k_backproject(TrackArgs t) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= t.pw * t.ph) return;
+  const uint32_t fs = track_slot(t);
+  const int u = (i % t.pw) * t.stride, v = (i / t.pw) * t.stride;
+  const size_t elem = kU16 ? 2 : 4;
+  const void* base = reinterpret_cast<const char*>(t.depth) + (size_t)fs * t.W * t.H * elem;
+  const float d = depth_at<kU16>(base, t.depth_scale, v * t.W + u);
+  float x = __builtin_nanf(""), y = x, z = x;
+  if (depth_valid(d, t.max_depth)) {
+    x = (((float)u - t.K.cx) * d) / t.K.fx;
+    y = (((float)v - t.K.cy) * d) / t.K.fy;
+    z = d;
+  }
+  t.pts[3 * i] = x;
+  t.pts[3 * i + 1] = y;
+  t.pts[3 * i + 2] = z;
+}
+
+// One workgroup: the accepted pose, rounded to FP32, is the frame's pose; the solver's rows
+// of this frame go to the per-frame stats.
+__global__ __launch_bounds__(64) void k_track_end(TrackArgs t) {
+  const uint32_t fs = track_slot(t);
+  const int k = threadIdx.x;
+  if (k < 16) t.out[16 * (size_t)fs + k] = k < 12 ? (float)t.state->pose[k] : (k == 15 ? 1.0f : 0.0f);
+  if (t.stats) {
+    const int rows = (t.iters + 1) * kGnStats;
+    for (int j = k; j < rows; j += 64) t.stats[(size_t)fs * rows + j] = t.eval_stats[j];
+  }
 }
 
 }  // namespace
@@ -1068,10 +1140,24 @@ hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand, double n_points,
-                           int final_step, double* stats_row, hipStream_t st) {
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, st, partials, grid, state, cand, n_points, final_step,
-                     stats_row);
+hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand, int final_step,
+                           double* stats_row, hipStream_t st) {
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(256), 0, st, partials, grid, state, cand, final_step, stats_row);
+  return hipGetLastError();
+}
+
+hipError_t launch_track_begin(const TrackArgs& t, hipStream_t st) {
+  hipLaunchKernelGGL(k_track_begin, dim3(1), dim3(64), 0, st, t);
+  const int n = t.pw * t.ph;
+  if (t.depth_u16)
+    hipLaunchKernelGGL(k_backproject<true>, dim3((n + 255) / 256), dim3(256), 0, st, t);
+  else
+    hipLaunchKernelGGL(k_backproject<false>, dim3((n + 255) / 256), dim3(256), 0, st, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_track_end(const TrackArgs& t, hipStream_t st) {
+  hipLaunchKernelGGL(k_track_end, dim3(1), dim3(64), 0, st, t);
   return hipGetLastError();
 }
 

@@ -121,18 +121,20 @@ VH_HD void column_cam(const BlockFrame& f, const float c[3], int j, int i, float
 // Observation test and truncated distance of the voxel at camera point (x, y, z)
 // (reference IntegrateHashKernel skip rules, src/tsdf.cu:706-738), in two steps so that a
 // caller can issue all depth fetches of a block before using any of them:
-//   project_cam: pixel index of the voxel, or -1 when it is behind the camera or outside
-//     the image. One correctly rounded reciprocal 1/z serves both image coordinates,
-//     fx*(x*(1/z)) + cx, rounded to the nearest pixel with ties to even (rintf); the
-//     reference divides per coordinate and rounds ties away from zero (cam2frame,
-//     src/tsdf.cu:75-79), compiled with -use_fast_math.
+//   project_cam: pixel index of the voxel, or -1 when it is outside the depth range
+//     2^-64 <= z <= 2^64 or outside the image. One correctly rounded reciprocal iz = 1/z
+//     (passed in by the caller) serves both image coordinates, fx*(x*iz) + cx, rounded to
+//     the nearest pixel with ties to even (rintf); the reference divides per coordinate and
+//     rounds ties away from zero (cam2frame, src/tsdf.cu:75-79), compiled with
+//     -use_fast_math. The range bound keeps z normal and 1/z normal, where the kernels'
+//     two-instruction reciprocal equals the IEEE quotient.
 //   truncate_obs: with the pixel's depth d, true iff 0 < d <= max_depth and d - z > -mu;
 //     dist = min(1, (d - z) * inv_mu) with inv_mu = 1/mu (reference: divides by mu, :738).
-VH_HD int project_cam(float x, float y, float z, const Intr& K, int W, int H) {
-  const float iz = 1.0f / z;
+constexpr float kMinZ = 0x1p-64f, kMaxZ = 0x1p64f;
+VH_HD int project_cam(float x, float y, float z, float iz, const Intr& K, int W, int H) {
   const float uf = rintf(K.fx * (x * iz) + K.cx);
   const float vf = rintf(K.fy * (y * iz) + K.cy);
-  const bool inside = (z > 0.0f) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
+  const bool inside = (z >= kMinZ) & (z <= kMaxZ) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
   return inside ? (int)vf * W + (int)uf : -1;
 }
 VH_HD bool truncate_obs(int pix, float d, float z, float max_depth, float mu, float inv_mu, float& dist) {

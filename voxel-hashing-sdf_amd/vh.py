@@ -35,8 +35,14 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "POOL_FULL", 4: "HASH_FU
 
 STORAGE_F32 = 0
 STORAGE_F16 = 1
+STORAGE_F16R = 2
 DEPTH_U16 = 0
 DEPTH_F32 = 1
+PRIOR_ABSOLUTE = 0
+PRIOR_INCREMENT = 1
+ABI_VERSION = 2
+# per-evaluation solver stats: n_valid, rms, |delta xi|, lambda, accepted, accepted pose (12)
+GN_STATS = 17
 
 
 class VhError(RuntimeError):
@@ -63,16 +69,23 @@ class Stats(C.Structure):
     _fields_ = [("frames", C.c_int64), ("blocks_allocated", C.c_int64), ("pool_capacity", C.c_int64),
                 ("slots_used", C.c_int64), ("touched_last", C.c_int64), ("touched_total", C.c_int64),
                 ("new_total", C.c_int64), ("storage", C.c_int32), ("flags", C.c_int32),
-                ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64)]
+                ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64),
+                ("track_ms", C.c_double)]
 
 
 class Mesh(C.Structure):
     _fields_ = [("n_tris", C.c_int64), ("xyz", C.POINTER(C.c_float)), ("cubes", C.POINTER(C.c_int32))]
 
 
+class TrackConfig(C.Structure):
+    _fields_ = [("d_prior_c2w", C.c_void_p), ("d_out_c2w", C.c_void_p), ("d_stats", C.c_void_p),
+                ("gn_iters", C.c_int32), ("point_stride", C.c_int32), ("max_point_depth", C.c_float),
+                ("prior_mode", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
 class GnStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("n_valid", C.c_int32 * 64), ("rms", C.c_double * 64),
-                ("step_norm", C.c_double * 64)]
+                ("step_norm", C.c_double * 64), ("pose", (C.c_double * 12) * 64)]
 
 
 class SceneInfo(C.Structure):
@@ -93,6 +106,8 @@ _PROTOS = [
     ("vh_abi_version", C.c_int, []),
     ("vh_integrate", C.c_int, [C.c_void_p, C.POINTER(Frame)]),
     ("vh_bind_sequence", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
+    ("vh_bind_tracking", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
+                                   C.POINTER(TrackConfig)]),
     ("vh_integrate_next", C.c_int, [C.c_void_p, C.c_int32]),
     ("vh_sync", C.c_int, [C.c_void_p]),
     ("vh_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
@@ -242,6 +257,22 @@ class Volume:
         pp = d_c2w if isinstance(d_c2w, int) else d_c2w.data_ptr()
         _check(_lib.vh_bind_sequence(self._h, C.c_void_p(dp), fmt, scale, C.c_void_p(pp), n))
 
+    def bind_tracking(self, d_depth, fmt: int, scale: float, n: int, d_prior, d_out, d_stats=None,
+                      gn_iters: int = 5, point_stride: int = 4, max_point_depth: float = 0.0,
+                      prior_mode: int = PRIOR_ABSOLUTE):
+        """Tracked sequence: per frame, gn_iters solver steps from its starting pose (d_prior[f],
+        or with PRIOR_INCREMENT the refined pose of frame f-1 times d_prior[f]) over the
+        frame's back-projected depth (points up to max_point_depth, 0 = the volume's
+        max_depth), then integration at the refined pose (written to d_out[f]).
+        d_stats: None or a float64 device tensor of n * (gn_iters + 1) * GN_STATS values."""
+        tc = TrackConfig()
+        tc.d_prior_c2w = d_prior.data_ptr()
+        tc.d_out_c2w = d_out.data_ptr()
+        tc.d_stats = None if d_stats is None else d_stats.data_ptr()
+        tc.gn_iters, tc.point_stride = gn_iters, point_stride
+        tc.max_point_depth, tc.prior_mode = max_point_depth, prior_mode
+        _check(_lib.vh_bind_tracking(self._h, _tptr(d_depth), fmt, scale, n, C.byref(tc)))
+
     def integrate_next(self, n: int = 1):
         _check(_lib.vh_integrate_next(self._h, n))
 
@@ -320,7 +351,9 @@ class Volume:
         st = GnStats()
         _check(_lib.vh_gn_refine(self._h, _ptr(pts), pts.shape[0], _ptr(T), iters, C.byref(st)))
         k = iters + 1
-        stats = {"n_valid": list(st.n_valid[:k]), "rms": list(st.rms[:k]), "step_norm": list(st.step_norm[:k])}
+        stats = {"n_valid": list(st.n_valid[:k]), "rms": list(st.rms[:k]), "step_norm": list(st.step_norm[:k]),
+                 "pose": [np.vstack([np.array(st.pose[i][:], np.float64).reshape(3, 4), [0, 0, 0, 1]])
+                          for i in range(k)]}
         return T.reshape(4, 4), stats
 
     def pool_view(self):
