@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="short run for hardware-counter passes: one FP32 integration pass over the "
                          "sequence plus a 1 GiB device copy that calibrates FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--counters-mc", action="store_true",
+                    help="with --counters: one marching-cubes extraction after the integration pass")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the volume is not sharded across GPUs in this build "
@@ -269,7 +271,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         vol = vh.Volume(vh.config_from_workload(w))
-        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False)
+        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=args.counters_mc)
         st = vol.stats()
         vol.close()
         log(f"bench: counters run: {nf} frames, {st['touched_total']} touched blocks in total, "

@@ -391,53 +391,29 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
 // ============================================================================ marching cubes
 constexpr int kHalo = 9;
 constexpr int kHalo3 = kHalo * kHalo * kHalo;
+constexpr int kItemCap = 1024;  // (cube, triangle) items listed per round
 
-// Triangles of one cube whose corners are read from the 9^3 LDS tile at (cx,cy,cz); lattice
-// origin (gx,gy,gz). Emit(tri_in_case, case, P[9]) for each non-degenerate triangle
-// (reference marchingCubeHashKernel, src/tsdf.cu:978-1061, with the intended degenerate test).
-template <class Emit>
-__device__ __forceinline__ void cube_tris(const float* sv, const uint8_t* so, int cx, int cy, int cz,
-                                          int gx, int gy, int gz, Emit emit) {
-  float v[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int e = (cx + corner_dx(c)) * 81 + (cy + corner_dy(c)) * 9 + (cz + corner_dz(c));
-    if (!so[e]) return;
-    v[c] = sv[e];
-  }
-  int ci = 0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c)
-    if (v[c] < 0.0f) ci |= 1 << c;
-  if (ci == 0 || ci == 255) return;
-  const uint64_t pk = kTriTable[ci];
-  const int nt = (int)(pk >> 60);
-  for (int t = 0; t < nt; ++t) {
-    float P[9];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
-      const int ca = edge_a(e), cb = edge_b(e);
-      vertex_interp((float)(gx + corner_dx(ca)), (float)(gy + corner_dy(ca)), (float)(gz + corner_dz(ca)),
-                    (float)(gx + corner_dx(cb)), (float)(gy + corner_dy(cb)), (float)(gz + corner_dz(cb)),
-                    v[ca], v[cb], P[3 * j], P[3 * j + 1], P[3 * j + 2]);
-    }
-    const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
-    const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
-    const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
-    if (d01 || d12 || d20) continue;
-    emit(t, ci, P);
-  }
+// LDS of one extraction wave (about 6 KB).
+struct McShared {
+  float val[kHalo3];         // 9^3 tile (x*81 + y*9 + z): tsdf, NaN where unobserved
+  uint32_t nb[8];            // pool blocks of the neighbourhood, index (dx << 2) | (dy << 1) | dz
+  uint8_t cnt[256];          // triangles of each case (table count)
+  uint8_t cases[512];        // case per cube (x*64 + y*8 + z); 0 = no triangles
+  uint32_t lane_off[65];     // exclusive prefix of table triangles per lane
+  uint16_t items[kItemCap];  // (cube << 3) | t of the current round
+};
+
+__device__ __forceinline__ void mc_load_counts(McShared& S) {
+  for (int c = threadIdx.x; c < 256; c += 64) S.cnt[c] = (uint8_t)(kTriTable[c] >> 60);
 }
 
-// One wave per pool block. The block's 8^3 voxels and the +x/+y/+z faces of its 7
-// neighbours (one hash lookup each) are staged as a 9^3 tile in LDS. The count pass writes
-// the block's triangle count; a two-level exclusive scan turns counts into offsets; the
-// write pass recomputes the triangles and stores them at their offsets. No output atomics:
-// the mesh is ordered by pool block, then cube, then table position.
+// One wave per pool block. The block's 8^3 voxels and the +x/+y/+z faces, edges and corner
+// of its 7 neighbours (one hash lookup each) are staged as a 9^3 tile in LDS: eight
+// coalesced rows of the block itself, one load per face, one for the three edges and the
+// corner, so no lane divides an index. A corner is observed iff its block exists and its
+// weight is > 0; unobserved corners hold NaN.
 template <int kStorage>
-__device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, float* s_val, uint8_t* s_obs, uint32_t* s_nb,
-                                         int& gx0, int& gy0, int& gz0) {
+__device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& S, int& gx0, int& gy0, int& gz0) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   const int lane = threadIdx.x;
@@ -450,70 +426,77 @@ __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, float* s_v
       nb = key_in_range(nx, ny, nz) ? find_block(a.hash, pack_key(nx, ny, nz)) : kNoBlock;
       if (nb >= a.pool_cap) nb = kNoBlock;
     }
-    s_nb[lane] = nb;
+    S.nb[lane] = nb;
   }
   __syncthreads();
   const Word* vox = reinterpret_cast<const Word*>(a.voxels);
-  for (int e = lane; e < kHalo3; e += 64) {
-    const int x = e / 81, y = (e / 9) % 9, z = e % 9;
-    const uint32_t pb = s_nb[((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3)];
-    float sv = 0.0f, wv = 0.0f;
-    if (pb != kNoBlock) IO::unpack(vox[(size_t)pb * kB3 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7))], sv, wv);
-    s_val[e] = sv;
-    s_obs[e] = (pb != kNoBlock && wv > 0.0f) ? 1 : 0;
-  }
+  const float nan = __builtin_nanf("");
+  auto value = [&](Word w) {
+    float sv, wv;
+    IO::unpack(w, sv, wv);
+    return wv > 0.0f ? sv : nan;
+  };
+  const int hi = lane >> 3, lo = lane & 7;
+  // this block: voxel lane + 64k = (k, hi, lo)
+  const Word* own = vox + (size_t)b * kB3 + lane;
+  Word w[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) w[k] = own[64 * k];
+  // faces x = 8 (voxel (0, hi, lo) of +x), y = 8 ((hi, 0, lo) of +y), z = 8 ((hi, lo, 0) of +z)
+  const uint32_t nfx = S.nb[4], nfy = S.nb[2], nfz = S.nb[1];
+  const Word wfx = nfx != kNoBlock ? vox[(size_t)nfx * kB3 + lane] : IO::zero();
+  const Word wfy = nfy != kNoBlock ? vox[(size_t)nfy * kB3 + hi * 64 + lo] : IO::zero();
+  const Word wfz = nfz != kNoBlock ? vox[(size_t)nfz * kB3 + hi * 64 + lo * 8] : IO::zero();
+  // edges (8, 8, z) of +x+y, (8, y, 8) of +x+z, (x, 8, 8) of +y+z (8 lanes each), corner
+  // (8, 8, 8) of +x+y+z (lane 24)
+  int edst = 0;
+  uint32_t enb = kNoBlock;
+  int esrc = 0;
+  if (lane < 8) { enb = S.nb[6]; esrc = lane; edst = 8 * 81 + 8 * 9 + lane; }
+  else if (lane < 16) { enb = S.nb[5]; esrc = (lane - 8) * 8; edst = 8 * 81 + (lane - 8) * 9 + 8; }
+  else if (lane < 24) { enb = S.nb[3]; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; }
+  else if (lane == 24) { enb = S.nb[7]; esrc = 0; edst = kHalo3 - 1; }
+  const Word we = enb != kNoBlock ? vox[(size_t)enb * kB3 + esrc] : IO::zero();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) S.val[k * 81 + hi * 9 + lo] = value(w[k]);
+  S.val[8 * 81 + hi * 9 + lo] = nfx != kNoBlock ? value(wfx) : nan;
+  S.val[hi * 81 + 8 * 9 + lo] = nfy != kNoBlock ? value(wfy) : nan;
+  S.val[hi * 81 + lo * 9 + 8] = nfz != kNoBlock ? value(wfz) : nan;
+  if (lane < 25) S.val[edst] = enb != kNoBlock ? value(we) : nan;
   __syncthreads();
   gx0 = bx * kB;
   gy0 = by * kB;
   gz0 = bz * kB;
-  return true;
 }
 
-// Dense per-wave triangle work list of the staged block. Lane l classifies the 8 cubes of its
-// (y, z) column (x = 0..7), sharing each x-face's corners with the next cube; the wave then
-// walks the list of (cube, table triangle) items 64 at a time, one item per lane. Emit
-// receives (valid, item, cube x/y/z, case, t, P[9]) for every item of the round.
-struct McList {
-  uint8_t cases[512];     // case per cube (cube = x*64 + y*8 + z), 0 = no triangles
-  uint32_t lane_off[65];  // exclusive prefix of table triangles per lane
-};
-
-__device__ __forceinline__ void mc_classify(const float* sv, const uint8_t* so, McList* L, uint32_t& total) {
+// Lane l classifies the 8 cubes of its (y, z) column (x = 0..7), sharing each x-face's
+// corners with the next cube. A cube has triangles iff its 8 corners are observed and its
+// case is not 0 or 255 (bit c set iff corner c < 0, reference corner order). Returns the
+// block's table triangle count; lane_off holds the per-lane exclusive prefix.
+__device__ __forceinline__ uint32_t mc_classify(McShared& S) {
   const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
-  auto at = [&](int x, int y, int z) { return x * 81 + y * 9 + z; };
   float f0[4];
-  bool o0[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int e = at(0, cy + (k >> 1), cz + (k & 1));
-    f0[k] = sv[e];
-    o0[k] = so[e] != 0;
-  }
+  for (int k = 0; k < 4; ++k) f0[k] = S.val[(cy + (k >> 1)) * 9 + cz + (k & 1)];
   uint32_t ntri = 0;
   for (int cx = 0; cx < 8; ++cx) {
     float f1[4];
-    bool o1[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int e = at(cx + 1, cy + (k >> 1), cz + (k & 1));
-      f1[k] = sv[e];
-      o1[k] = so[e] != 0;
-    }
+    for (int k = 0; k < 4; ++k) f1[k] = S.val[(cx + 1) * 81 + (cy + (k >> 1)) * 9 + cz + (k & 1)];
     // corner c -> (dx, dy, dz); face slot k = (dy << 1) | dz
     const float val[8] = {f0[0], f0[2], f1[2], f1[0], f0[1], f0[3], f1[3], f1[1]};
-    const bool obs = o0[0] && o0[1] && o0[2] && o0[3] && o1[0] && o1[1] && o1[2] && o1[3];
+    bool obs = true;
     int ci = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
+    for (int c = 0; c < 8; ++c) {
+      obs = obs && !__builtin_isnan(val[c]);
       if (val[c] < 0.0f) ci |= 1 << c;
-    if (!obs || ci == 255) ci = 0;
-    L->cases[cx * 64 + cy * 8 + cz] = (uint8_t)ci;
-    ntri += (uint32_t)(kTriTable[ci] >> 60);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      f0[k] = f1[k];
-      o0[k] = o1[k];
     }
+    if (!obs || ci == 255) ci = 0;
+    S.cases[cx * 64 + lane] = (uint8_t)ci;
+    ntri += S.cnt[ci];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f0[k] = f1[k];
   }
   uint32_t x = ntri;
 #pragma unroll
@@ -521,74 +504,80 @@ __device__ __forceinline__ void mc_classify(const float* sv, const uint8_t* so, 
     const uint32_t y = (uint32_t)__shfl_up((int)x, off);
     if (lane >= off) x += y;
   }
-  L->lane_off[lane] = x - ntri;
-  if (lane == 63) L->lane_off[64] = x;
+  S.lane_off[lane] = x - ntri;
+  if (lane == 63) S.lane_off[64] = x;
   __syncthreads();
-  total = L->lane_off[64];
+  return S.lane_off[64];
 }
 
+// The block's (cube, table triangle) items in order lane, x, t, in rounds of kItemCap: each
+// lane lists its own items of the round in LDS, then the wave takes them one per lane.
+// Emit receives (valid, cube x/y/z, case, t, P[9]) per item slot of a 64-item step; valid
+// is false for empty slots and for degenerate triangles (two equal vertices; the
+// reference's test at src/tsdf.cu:1055-1058, with its typo corrected).
 template <class Emit>
-__device__ __forceinline__ void mc_items(const float* sv, const McList* L, uint32_t total, int gx0, int gy0, int gz0,
-                                         Emit emit) {
+__device__ __forceinline__ void mc_items(McShared& S, uint32_t total, int gx0, int gy0, int gz0, Emit emit) {
   const int lane = threadIdx.x;
-  for (uint32_t r0 = 0; r0 < total; r0 += 64) {
-    const uint32_t item = r0 + lane;
-    bool valid = false;
-    int ci = 0, t = 0, cx = 0, cy = 0, cz = 0;
-    float P[9];
-    if (item < total) {
-      // owning lane: last lane whose offset <= item
-      int lo = 0, hi = 63;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (L->lane_off[mid] <= item) lo = mid;
-        else hi = mid - 1;
+  const uint32_t my0 = S.lane_off[lane], my1 = S.lane_off[lane + 1];
+  for (uint32_t r0 = 0; r0 < total; r0 += kItemCap) {
+    const uint32_t r1 = min(total, r0 + (uint32_t)kItemCap);
+    if (my0 < r1 && my1 > r0) {
+      uint32_t idx = my0;
+      for (int cx = 0; cx < 8 && idx < r1; ++cx) {
+        const int cube = cx * 64 + lane;
+        const int n = S.cnt[S.cases[cube]];
+        for (int t = 0; t < n; ++t, ++idx)
+          if (idx >= r0 && idx < r1) S.items[idx - r0] = (uint16_t)((cube << 3) | t);
       }
-      cy = lo >> 3;
-      cz = lo & 7;
-      uint32_t rem = item - L->lane_off[lo];
-      for (cx = 0; cx < 8; ++cx) {
-        ci = L->cases[cx * 64 + lo];
-        const uint32_t n = (uint32_t)(kTriTable[ci] >> 60);
-        if (rem < n) break;
-        rem -= n;
-      }
-      t = (int)rem;
-      const uint64_t pk = kTriTable[ci];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
-        const int ca = edge_a(e), cb = edge_b(e);
-        const float va = sv[(cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca))];
-        const float vb = sv[(cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb))];
-        vertex_interp((float)(gx0 + cx + corner_dx(ca)), (float)(gy0 + cy + corner_dy(ca)),
-                      (float)(gz0 + cz + corner_dz(ca)), (float)(gx0 + cx + corner_dx(cb)),
-                      (float)(gy0 + cy + corner_dy(cb)), (float)(gz0 + cz + corner_dz(cb)), va, vb, P[3 * j],
-                      P[3 * j + 1], P[3 * j + 2]);
-      }
-      const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
-      const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
-      const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
-      valid = !(d01 || d12 || d20);
     }
-    emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P);
+    __syncthreads();
+    for (uint32_t i0 = r0; i0 < r1; i0 += 64) {
+      const uint32_t item = i0 + lane;
+      bool valid = false;
+      int ci = 0, t = 0, cx = 0, cy = 0, cz = 0;
+      float P[9];
+      if (item < r1) {
+        const int it = S.items[item - r0];
+        const int cube = it >> 3;
+        t = it & 7;
+        cx = cube >> 6;
+        cy = (cube >> 3) & 7;
+        cz = cube & 7;
+        ci = S.cases[cube];
+        const uint64_t pk = kTriTable[ci];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
+          const int ca = edge_a(e), cb = edge_b(e);
+          const float va = S.val[(cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca))];
+          const float vb = S.val[(cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb))];
+          vertex_interp((float)(gx0 + cx + corner_dx(ca)), (float)(gy0 + cy + corner_dy(ca)),
+                        (float)(gz0 + cz + corner_dz(ca)), (float)(gx0 + cx + corner_dx(cb)),
+                        (float)(gy0 + cy + corner_dy(cb)), (float)(gz0 + cz + corner_dz(cb)), va, vb, P[3 * j],
+                        P[3 * j + 1], P[3 * j + 2]);
+        }
+        const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
+        const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
+        const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
+        valid = !(d01 || d12 || d20);
+      }
+      emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P);
+    }
+    __syncthreads();
   }
 }
 
 template <int kStorage>
 __global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
-  __shared__ float s_val[kHalo3];
-  __shared__ uint8_t s_obs[kHalo3 + 7];
-  __shared__ uint32_t s_nb[8];
-  __shared__ McList s_list;
+  __shared__ McShared S;
   const int lane = threadIdx.x;
+  mc_load_counts(S);
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
-    mc_stage<kStorage>(a, b, s_val, s_obs, s_nb, gx0, gy0, gz0);
-    uint32_t total;
-    mc_classify(s_val, s_obs, &s_list, total);
+    mc_stage<kStorage>(a, b, S, gx0, gy0, gz0);
+    const uint32_t total = mc_classify(S);
     uint32_t cnt = 0;
-    mc_items(s_val, &s_list, total, gx0, gy0, gz0,
+    mc_items(S, total, gx0, gy0, gz0,
              [&](bool valid, int, int, int, int, int, const float*) { cnt += (uint32_t)__popcll(__ballot(valid)); });
     if (lane == 0) a.counts[b] = cnt;
     __syncthreads();
@@ -667,31 +656,27 @@ __global__ __launch_bounds__(256) void k_scan_top(McArgs a, uint32_t n_chunks) {
 
 template <int kStorage>
 __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
-  __shared__ float s_val[kHalo3];
-  __shared__ uint8_t s_obs[kHalo3 + 7];
-  __shared__ uint32_t s_nb[8];
-  __shared__ McList s_list;
+  __shared__ McShared S;
+  mc_load_counts(S);
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
-    mc_stage<kStorage>(a, b, s_val, s_obs, s_nb, gx0, gy0, gz0);
-    uint32_t total;
-    mc_classify(s_val, s_obs, &s_list, total);
+    mc_stage<kStorage>(a, b, S, gx0, gy0, gz0);
+    const uint32_t total = mc_classify(S);
     uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
-    mc_items(s_val, &s_list, total, gx0, gy0, gz0,
-             [&](bool valid, int x, int y, int z, int ci, int t, const float* P) {
-               const uint64_t m = __ballot(valid);
-               const uint32_t p = pos + lanes_below(m);
-               if (valid && p < a.capacity) {
-                 float* o = a.tris + (size_t)p * 9;
+    mc_items(S, total, gx0, gy0, gz0, [&](bool valid, int x, int y, int z, int ci, int t, const float* P) {
+      const uint64_t m = __ballot(valid);
+      const uint32_t p = pos + lanes_below(m);
+      if (valid && p < a.capacity) {
+        float* o = a.tris + (size_t)p * 9;
 #pragma unroll
-                 for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
-                 if (a.cubes) {
-                   int32_t* q = a.cubes + (size_t)p * 4;
-                   q[0] = x; q[1] = y; q[2] = z; q[3] = (ci << 8) | t;
-                 }
-               }
-               pos += (uint32_t)__popcll(m);
-             });
+        for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
+        if (a.cubes) {
+          int32_t* q = a.cubes + (size_t)p * 4;
+          q[0] = x; q[1] = y; q[2] = z; q[3] = (ci << 8) | t;
+        }
+      }
+      pos += (uint32_t)__popcll(m);
+    });
     __syncthreads();
   }
 }
