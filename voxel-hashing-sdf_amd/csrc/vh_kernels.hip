@@ -291,8 +291,12 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
   BlockFrame bf;
   block_frame(T, a.vox, bf);
 
-  // Lane j fetches the list entry, slot value and key of this workgroup's j-th block, so the
-  // per-block chain is a single round trip (voxels + depth).
+  // Lane j fetches the list entry, slot value and key of this workgroup's j-th block and
+  // parks them in LDS, so the per-block chain is a single round trip (voxels + depth). The
+  // block loop reads them back from LDS: a use of the loaded registers inside the loop would
+  // make the compiler wait for every outstanding vector memory operation, the previous
+  // block's stores included (vmcnt counts stores on this architecture).
+  __shared__ unsigned long long s_val[64], s_key[64];
   for (uint32_t t0 = blockIdx.x; t0 < n; t0 += 64u * gridDim.x) {
     const uint32_t tj = t0 + (uint32_t)lane * gridDim.x;
     const uint32_t tq = min(tj, n - 1u);
@@ -311,12 +315,13 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
       jval = a.hash.vals[sl];
       jkey = a.hash.keys[sl];
     }
+    s_val[lane] = jval;
+    s_key[lane] = jkey;
+    __syncthreads();
     const uint32_t nj = min(64u, (n - t0 + gridDim.x - 1) / gridDim.x);
     for (uint32_t j = 0; j < nj; ++j) {
-    const unsigned long long val = ((unsigned long long)read_lane_u32((uint32_t)(jval >> 32), j) << 32) |
-                                   read_lane_u32((uint32_t)jval, j);
-    const unsigned long long keyj = ((unsigned long long)read_lane_u32((uint32_t)(jkey >> 32), j) << 32) |
-                                    read_lane_u32((uint32_t)jkey, j);
+    const unsigned long long val = s_val[j];
+    const unsigned long long keyj = s_key[j];
     const uint32_t idx = (uint32_t)val;
     if (idx >= a.pool_cap) {
       if (lane == 0) raise_flag(ctl, kFlagPoolFull);
@@ -356,6 +361,7 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
       else if (fresh) vp[64 * k] = IO::zero();
     }
     }
+    __syncthreads();  // the next batch overwrites s_val / s_key
   }
 
   // No fence before the counters: the closing workgroup reads no voxel data, and the voxel
@@ -363,27 +369,31 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
   // write back the L2 thousands of times per frame).
   const uint32_t group = blockIdx.x & (kStripes - 1);
   const uint32_t per_group = gridDim.x / kStripes;
+  uint32_t closes = 0;
   if (lane == 0) {
     const uint32_t prev = atomicAdd(&ctl->done[group * kLine], 1u);
-    if (prev == per_group - 1) {
-      const uint32_t top = atomicAdd(&ctl->done_top, 1u);
-      if (top == kStripes - 1) {
-        uint32_t total = 0;
-        for (int k = 0; k < kStripes; ++k) {
-          total += min(atomicAdd(&ctl->tcount[k * kLine], 0u), a.stripe_cap);
-          ctl->tcount[k * kLine] = 0u;
-          ctl->done[k * kLine] = 0u;
-        }
-        const uint32_t pc = atomicAdd(&ctl->pool_count, 0u);
-        ctl->touched_last = total;
-        ctl->touched_total += total;
-        ctl->frames += 1ull;
-        ctl->new_total += (unsigned long long)(pc - ctl->pool_at_frame_start);
-        ctl->pool_at_frame_start = pc;
-        ctl->cursor[a.cursor_id] += 1u;
-        ctl->frame_id = frame + 1u;
-        ctl->done_top = 0u;
-      }
+    if (prev == per_group - 1) closes = atomicAdd(&ctl->done_top, 1u) == kStripes - 1;
+  }
+  if (read_lane_u32(closes, 0)) {
+    // The stripe lengths and the pool count were final before this kernel started (the
+    // allocation kernel wrote them): each lane reads and clears one stripe, so the close
+    // costs one round trip instead of one per stripe.
+    const uint32_t c = min(ctl->tcount[lane * kLine], a.stripe_cap);
+    ctl->tcount[lane * kLine] = 0u;
+    ctl->done[lane * kLine] = 0u;
+    uint32_t total = c;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) total += (uint32_t)__shfl_xor((int)total, m);
+    if (lane == 0) {
+      const uint32_t pc = ctl->pool_count;
+      ctl->touched_last = total;
+      ctl->touched_total += total;
+      ctl->frames += 1ull;
+      ctl->new_total += (unsigned long long)(pc - ctl->pool_at_frame_start);
+      ctl->pool_at_frame_start = pc;
+      ctl->cursor[a.cursor_id] += 1u;
+      ctl->frame_id = frame + 1u;
+      ctl->done_top = 0u;
     }
   }
 }
