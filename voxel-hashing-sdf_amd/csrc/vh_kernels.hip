@@ -115,11 +115,16 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
     slot = find_or_insert(a.hash, key, inserted);
     if (slot < 0) raise_flag(ctl, kFlagHashFull);
   }
+  // The pool reservation and the first-touch exchange depend only on the slot: both are in
+  // flight together, and the slot-value stores go out just before the stripe atomic, so
+  // the chain is bucket load -> (CAS) -> pool + stamp -> stripe.
   const uint64_t im = __ballot(inserted);
+  const int leader = im ? first_lane(im) : 0;
+  uint32_t pbase = 0;
+  if (im && lane == leader) pbase = atomicAdd(&ctl->pool_count, (uint32_t)__popcll(im));
+  uint32_t old_stamp = frame;
+  if (slot >= 0) old_stamp = atomicExch(&a.stamps[slot], frame);
   if (im) {
-    const int leader = first_lane(im);
-    uint32_t pbase = 0;
-    if (lane == leader) pbase = atomicAdd(&ctl->pool_count, (uint32_t)__popcll(im));
     pbase = read_lane_u32(pbase, leader);
     if (inserted) {
       const uint32_t idx = pbase + lanes_below(im);
@@ -133,7 +138,7 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
       a.hash.vals[slot] = (unsigned long long)stored | ((unsigned long long)frame << 32);
     }
   }
-  const bool first = slot >= 0 && atomicExch(&a.stamps[slot], frame) != frame;
+  const bool first = old_stamp != frame;
   const uint64_t fm = __ballot(first);
   if (fm) {
     const int leader = first_lane(fm);
