@@ -272,20 +272,23 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
     for (int i = 0; i < 512; ++i) {
       const int gx = bx * 8 + (i >> 6), gy = by * 8 + ((i >> 3) & 7), gz = bz * 8 + (i & 7);
       // FP32 geometry: decides observation, pixel and the FP32 distance. The camera point is
-      // formed per block origin and voxel step, R^T(o - t) + vox R^T (k, j, i), as three
-      // partial sums; one reciprocal of z feeds both image coordinates, which round to the
+      // formed per block origin and voxel step, R^T(o - t) + vox R^T (k, j, i): the column
+      // offset as fma(j, b, i*c) added to the origin, then fma(k, a, column) per voxel; one
+      // reciprocal of z feeds both image coordinates, fma(f, x/z, c), which round to the
       // nearest pixel with ties to even; the truncation multiplies by 1/mu. (The reference
       // rotates each voxel's p - t, divides x/z, y/z and by mu, and rounds ties away from
       // zero: src/tsdf.cu:75-93,738, compiled with -use_fast_math.)
       const int vx = i >> 6, vy = (i >> 3) & 7, vz = i & 7;
       float cam[3];
-      for (int r = 0; r < 3; ++r)
-        cam[r] = (cb[r] + ((float)vy * (T[4 + r] * o.vox) + (float)vz * (T[8 + r] * o.vox))) + (float)vx * (T[r] * o.vox);
+      for (int r = 0; r < 3; ++r) {
+        const float col = cb[r] + std::fmaf((float)vy, T[4 + r] * o.vox, (float)vz * (T[8 + r] * o.vox));
+        cam[r] = std::fmaf((float)vx, T[r] * o.vox, col);
+      }
       const float xc = cam[0], yc = cam[1], zc = cam[2];
       if (!(zc >= 0x1p-64f && zc <= 0x1p64f)) continue;  // depth range of the projection
       const float rz = 1.0f / zc;
-      const float uf = std::nearbyint(o.fx * (xc * rz) + o.cx);
-      const float vf = std::nearbyint(o.fy * (yc * rz) + o.cy);
+      const float uf = std::nearbyint(std::fmaf(o.fx, xc * rz, o.cx));
+      const float vf = std::nearbyint(std::fmaf(o.fy, yc * rz, o.cy));
       if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
       const size_t pix = (size_t)vf * W + (size_t)uf;
       const float dv = d[pix];

@@ -40,7 +40,7 @@ int fail(int code, const std::string& msg) {
       return fail(VH_ERR_DEVICE, std::string(#call " failed: ") + hipGetErrorString(e_));      \
   } while (0)
 
-constexpr int kIntegrateGrid = 8192;  // single-wave workgroups: 32 per CU, a multiple of kStripes
+constexpr int kIntegrateGrid = 8192;  // integration workers (waves, four per workgroup), a multiple of kStripes
 constexpr int kGnGrid = 512;
 
 // Launch-shape tuning: VH_INTEGRATE_GRID overrides the integration grid (rounded to a

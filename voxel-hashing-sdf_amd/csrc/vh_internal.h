@@ -31,6 +31,19 @@ __device__ __forceinline__ float recip_rn(float z) {
   return __builtin_fmaf(__builtin_fmaf(-z, r0, 1.0f), r0, r0);
 }
 
+// a / b correctly rounded, for 1 <= b <= 2^25: y = RN(1/b) (recip_rn), q0 = RN(a*y), then
+// one correction q0 + (a - b*q0)*y with the residual exact under FMA (Markstein's theorem:
+// with y within half an ulp of 1/b and q0 within one ulp of a/b the corrected quotient is
+// RN(a/b) unless the quotient is subnormal). 6 instructions instead of 11; for 0 < |a| <
+// 2^-76 (where a/b could be subnormal) the IEEE division is taken instead. The check over
+// all a for b = 1..256 is tests/device_checks/recip_exact.hip.
+__device__ __forceinline__ float div_rn(float a, float b) {
+  if (__builtin_expect(__builtin_fabsf(a) < 0x1p-76f && a != 0.0f, 0)) return a / b;
+  const float y = recip_rn(b);
+  const float q0 = a * y;
+  return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+}
+
 enum : uint32_t { kFlagPoolFull = 1u, kFlagHashFull = 2u, kFlagRange = 4u, kFlagTouchedFull = 8u };
 constexpr int kMaxProbe = 64;  // buckets examined before a key is declared unplaceable
 constexpr int kStripes = 64;   // striped counters / touched sub-lists

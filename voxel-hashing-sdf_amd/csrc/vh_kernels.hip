@@ -82,8 +82,8 @@ __device__ __forceinline__ int64_t find_or_insert(const HashView& h, uint64_t ke
 
 template <bool kU16>
 __device__ __forceinline__ float depth_at(const void* base, float scale, int pix) {
-  if (kU16) return (float)reinterpret_cast<const uint16_t*>(base)[pix] * scale;
-  return reinterpret_cast<const float*>(base)[pix];
+  if (kU16) return (float)reinterpret_cast<const uint16_t*>(base)[(uint32_t)pix] * scale;
+  return reinterpret_cast<const float*>(base)[(uint32_t)pix];
 }
 
 __device__ __forceinline__ uint32_t frame_slot(const FrameArgs& a) {
@@ -262,16 +262,27 @@ struct VoxelIO<VH_STORAGE_F16R> {
   static __device__ __forceinline__ Word zero() { return 0u; }
 };
 
+// tsdf_update (vh_math.h) with the quotient by div_rn: the same correctly rounded result.
+__device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap) {
+  const float w_new = w + 1.0f;
+  s = div_rn(s * w + dist, w_new);
+  w = (cap > 0.0f && w_new > cap) ? cap : w_new;
+}
+
 // One wave per touched block: lane l owns voxels l + 64k (x = k, y = l >> 3, z = l & 7), so
 // each of its 8 loads is one coalesced 512-byte wave access and all 8 are in flight at once.
-// Workgroup b walks stripe b % kStripes of the touched list. The last workgroup to finish
-// (two-level striped counter) closes the frame.
+// The waves of a workgroup are independent workers (worker id = 4 * workgroup + wave; a
+// workgroup holds four so that more than 16 waves fit on a CU) and synchronise only within
+// themselves. The last worker to finish (two-level striped counter) closes the frame.
+constexpr int kIntegrateWaves = 4;
 template <int kStorage, bool kU16>
-__global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
+__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t worker = blockIdx.x * kIntegrateWaves + (uint32_t)wave;
+  const uint32_t workers = gridDim.x * kIntegrateWaves;
   // The 64 stripes of the touched list, seen as one index space: lane s holds stripe s's
   // length and exclusive offset, so every workgroup gets an equal share whatever the
   // stripe sizes.
@@ -301,9 +312,11 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
   // block loop reads them back from LDS: a use of the loaded registers inside the loop would
   // make the compiler wait for every outstanding vector memory operation, the previous
   // block's stores included (vmcnt counts stores on this architecture).
-  __shared__ unsigned long long s_val[64], s_key[64];
-  for (uint32_t t0 = blockIdx.x; t0 < n; t0 += 64u * gridDim.x) {
-    const uint32_t tj = t0 + (uint32_t)lane * gridDim.x;
+  __shared__ unsigned long long s_vals[kIntegrateWaves][64], s_keys[kIntegrateWaves][64];
+  unsigned long long* s_val = s_vals[wave];
+  unsigned long long* s_key = s_keys[wave];
+  for (uint32_t t0 = worker; t0 < n; t0 += 64u * workers) {
+    const uint32_t tj = t0 + (uint32_t)lane * workers;
     const uint32_t tq = min(tj, n - 1u);
     int lo = 0, hi = 63;  // stripe of entry tq: the last stripe whose offset is <= tq
 #pragma unroll
@@ -322,8 +335,8 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
     }
     s_val[lane] = jval;
     s_key[lane] = jkey;
-    __syncthreads();
-    const uint32_t nj = min(64u, (n - t0 + gridDim.x - 1) / gridDim.x);
+    __builtin_amdgcn_wave_barrier();  // one wave owns this slice: it only needs its own writes
+    const uint32_t nj = min(64u, (n - t0 + workers - 1) / workers);
     for (uint32_t j = 0; j < nj; ++j) {
     const unsigned long long val = s_val[j];
     const unsigned long long keyj = s_key[j];
@@ -349,8 +362,9 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const float fk = (float)k;
-      zc[k] = cl[2] + fk * bf.ax[2];
-      pix[k] = project_cam(cl[0] + fk * bf.ax[0], cl[1] + fk * bf.ax[1], zc[k], recip_rn(zc[k]), a.K, a.W, a.H);
+      zc[k] = __builtin_fmaf(fk, bf.ax[2], cl[2]);
+      pix[k] = project_cam(__builtin_fmaf(fk, bf.ax[0], cl[0]), __builtin_fmaf(fk, bf.ax[1], cl[1]), zc[k],
+                           recip_rn(zc[k]), a.K, a.W, a.H);
     }
     float dv[8];
 #pragma unroll
@@ -361,19 +375,19 @@ __global__ __launch_bounds__(64) void k_integrate(FrameArgs a) {
       const bool ok = truncate_obs(pix[k], dv[k], zc[k], a.max_depth, a.mu, a.inv_mu, dist);
       float sv, wv;
       IO::unpack(old[k], sv, wv);
-      tsdf_update(sv, wv, dist, a.cap);
+      mean_update(sv, wv, dist, a.cap);
       if (ok) vp[64 * k] = IO::pack(sv, wv);
       else if (fresh) vp[64 * k] = IO::zero();
     }
     }
-    __syncthreads();  // the next batch overwrites s_val / s_key
+    __builtin_amdgcn_wave_barrier();  // the next batch overwrites s_val / s_key
   }
 
   // No fence before the counters: the closing workgroup reads no voxel data, and the voxel
   // writes are published by the kernel boundary (a device-scope fence per workgroup would
   // write back the L2 thousands of times per frame).
-  const uint32_t group = blockIdx.x & (kStripes - 1);
-  const uint32_t per_group = gridDim.x / kStripes;
+  const uint32_t group = worker & (kStripes - 1);
+  const uint32_t per_group = workers / kStripes;
   uint32_t closes = 0;
   if (lane == 0) {
     const uint32_t prev = atomicAdd(&ctl->done[group * kLine], 1u);
@@ -1081,8 +1095,9 @@ static void with_storage(int storage, F f) {
 
 hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
   with_storage(storage, [&](auto S) {
-    if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
+    const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
+    if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), g, b, 0, st, a);
   });
   return hipGetLastError();
 }

@@ -93,8 +93,9 @@ VH_HD void cam_to_world(const float* T, float x, float y, float z, float& px, fl
 // form (base2cam, src/tsdf.cu:81-93) evaluated once per block origin o = 8*b*vox:
 //   c = R^T (o - t),  then voxel (k, j, i) of the block: c + vox*R^T (k, j, i).
 // With a = R^T e_x vox, b = R^T e_y vox, c = R^T e_z vox (per frame) a lane holding one
-// (j, i) column adds its offset once: c_l = c + (j*b + i*c), and voxel k along x is
-// c_l + k*a. Every voxel therefore costs 3 multiplies and 3 adds.
+// (j, i) column adds its offset once: c_l = c + fma(j, b, i*c), and voxel k along x is
+// fma(k, a, c_l) (one rounding per coordinate). Every voxel therefore costs 3 fused
+// multiply-adds.
 struct BlockFrame {
   float ax[3], ay[3], az[3];  // per-frame step vectors (camera coordinates per voxel step)
 };
@@ -112,10 +113,10 @@ VH_HD void block_origin_cam(const float* T, float vox, int bx, int by, int bz, f
   const float d2 = (float)(bz * kB) * vox - T[11];
   for (int i = 0; i < 3; ++i) c[i] = T[i] * d0 + T[4 + i] * d1 + T[8 + i] * d2;
 }
-// Camera coordinates of the column (j, i) of a block: c + (j*b + i*c).
+// Camera coordinates of the column (j, i) of a block: c + fma(j, b, i*c).
 VH_HD void column_cam(const BlockFrame& f, const float c[3], int j, int i, float out[3]) {
   const float fj = (float)j, fi = (float)i;
-  for (int r = 0; r < 3; ++r) out[r] = c[r] + (fj * f.ay[r] + fi * f.az[r]);
+  for (int r = 0; r < 3; ++r) out[r] = c[r] + fmaf(fj, f.ay[r], fi * f.az[r]);
 }
 
 // Observation test and truncated distance of the voxel at camera point (x, y, z)
@@ -123,19 +124,20 @@ VH_HD void column_cam(const BlockFrame& f, const float c[3], int j, int i, float
 // caller can issue all depth fetches of a block before using any of them:
 //   project_cam: pixel index of the voxel, or -1 when it is outside the depth range
 //     2^-64 <= z <= 2^64 or outside the image. One correctly rounded reciprocal iz = 1/z
-//     (passed in by the caller) serves both image coordinates, fx*(x*iz) + cx, rounded to
-//     the nearest pixel with ties to even (rintf); the reference divides per coordinate and
-//     rounds ties away from zero (cam2frame, src/tsdf.cu:75-79), compiled with
-//     -use_fast_math. The range bound keeps z normal and 1/z normal, where the kernels'
-//     two-instruction reciprocal equals the IEEE quotient.
+//     (passed in by the caller) serves both image coordinates, fma(fx, x*iz, cx), rounded
+//     to the nearest pixel with ties to even (rintf); the reference divides per coordinate
+//     and rounds ties away from zero (cam2frame, src/tsdf.cu:75-79), compiled with
+//     -use_fast_math (which contracts to the same fused multiply-add). The range bound keeps
+//     z normal and 1/z normal, where the kernels' two-instruction reciprocal equals the IEEE
+//     quotient.
 //   truncate_obs: with the pixel's depth d, true iff 0 < d <= max_depth and d - z > -mu;
 //     dist = min(1, (d - z) * inv_mu) with inv_mu = 1/mu (reference: divides by mu, :738).
 constexpr float kMinZ = 0x1p-64f, kMaxZ = 0x1p64f;
 VH_HD int project_cam(float x, float y, float z, float iz, const Intr& K, int W, int H) {
-  const float uf = rintf(K.fx * (x * iz) + K.cx);
-  const float vf = rintf(K.fy * (y * iz) + K.cy);
+  const float uf = rintf(fmaf(K.fx, x * iz, K.cx));
+  const float vf = rintf(fmaf(K.fy, y * iz, K.cy));
   const bool inside = (z >= kMinZ) & (z <= kMaxZ) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
-  return inside ? (int)vf * W + (int)uf : -1;
+  return inside ? (int)((uint32_t)vf * (uint32_t)W + (uint32_t)uf) : -1;
 }
 VH_HD bool truncate_obs(int pix, float d, float z, float max_depth, float mu, float inv_mu, float& dist) {
   const float diff = d - z;
