@@ -120,6 +120,46 @@ def test_device_sequence_graph_matches_host_path(vh, oracle, wl, small_frames):
     vol2.close()
 
 
+def test_sequence_chunks_overlap_allocation_and_integration(vh, oracle, wl, small_frames):
+    """Bound sequences run one launch per frame that integrates frame f and allocates frame
+    f+1; chunked calls (1, 5, 0, then 6 frames, a host-staged frame in between) must give the
+    same volume, statistics and mesh as the oracle frame by frame."""
+    import torch
+
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    n = 12
+    print("room_small_1cm: 12 sequence frames in chunks 1, 5, 0, 6 plus one staged frame", flush=True)
+    d_depth = torch.from_numpy(depth[:n].view(np.int16).copy()).cuda()
+    d_pose = torch.from_numpy(poses[:n].copy()).cuda()
+    vol = vh.Volume(vh.config_from_workload(w))
+    ref = oracle.Oracle(w)
+    vol.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, n)
+    touched = []
+    done = 0
+    for chunk in (1, 5, 0):
+        vol.integrate_next(chunk)
+        vol.sync()
+        for k in range(done, done + chunk):
+            touched.append(ref.integrate(depth[k], poses[k]))
+        done += chunk
+        if chunk:
+            assert vol.stats()["touched_last"] == touched[-1]
+    # a host-staged frame between chunks (frame 20 of the same camera path)
+    vol.integrate(depth[20], poses[20])
+    touched.append(ref.integrate(depth[20], poses[20]))
+    vol.integrate_next(6)
+    vol.sync()
+    for k in range(done, n):
+        touched.append(ref.integrate(depth[k], poses[k]))
+    st = vol.stats()
+    assert st["frames"] == n + 1 and st["touched_last"] == touched[-1] and st["touched_total"] == sum(touched)
+    assert st["new_total"] == ref.num_blocks()
+    assert_same_volume(oracle, vol, ref)
+    assert_same_mesh(vol, ref)
+    vol.close()
+
+
 def test_float_depth_input_matches_u16(vh, oracle, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames

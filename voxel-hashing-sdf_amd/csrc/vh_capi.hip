@@ -89,9 +89,11 @@ struct vh_volume {
   float trk_max_depth = 0.0f;  // depth limit of the tracking points
   float* trk_pts = nullptr;
   size_t trk_pts_cap = 0;  // points the buffer holds
-  // graphs: [0] staged u16, [1] staged f32, [2] bound sequence, [3] tracked sequence
-  static constexpr int kGraphs = 4;
-  hipGraphExec_t graph[kGraphs] = {nullptr, nullptr, nullptr, nullptr};
+  // graphs: [0] staged u16, [1] staged f32, [2] bound sequence step (integrate frame f +
+  // allocate frame f+1), [3] tracked sequence, [4] bound sequence first allocation, [5] bound
+  // sequence last integration
+  static constexpr int kGraphs = 6;
+  hipGraphExec_t graph[kGraphs] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -219,16 +221,24 @@ int enqueue_track_kernels(vh_volume* v) {
 
 int frame_u16(vh_volume* v, int which) { return which >= 2 ? v->seq_format == VH_DEPTH_U16 : which == 0; }
 
+// Kinds 0, 1, 3: allocation then integration of one frame. Bound sequences run as kind 4
+// (allocate the first frame), kind 2 per further frame (integrate the previous frame and
+// allocate this one in one launch) and kind 5 (integrate the last frame).
 int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale) {
-  FrameArgs a = frame_args(v, which);
-  const int u16 = frame_u16(v, which);
-  if (which < 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+  const int fa = which == 4 || which == 5 ? 2 : which;
+  FrameArgs a = frame_args(v, fa);
+  const int u16 = frame_u16(v, fa);
+  if (fa < 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+  if (which == 2) {
+    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, integrate_grid(), v->stream));
+    return VH_OK;
+  }
   if (which == 3) {
     int rc = enqueue_track_kernels(v);
     if (rc) return rc;
   }
-  VH_HIP(launch_alloc(a, u16, v->stream));
-  VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
+  if (which != 5) VH_HIP(launch_alloc(a, u16, v->stream));
+  if (which != 4) VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
   return VH_OK;
 }
 
@@ -508,7 +518,7 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
   if ((rc = dev_alloc(&v->keys, v->n_slots * 8)) || (rc = dev_alloc(&v->vals, v->n_slots * 8)) ||
       (rc = dev_alloc(&v->stamps, v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
       (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||
-      (rc = dev_alloc(&v->touched, (size_t)kStripes * v->stripe_cap * 4)) ||
+      (rc = dev_alloc(&v->touched, (size_t)2 * kStripes * v->stripe_cap * 4)) ||
       (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
       (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float)))) {
     std::string msg = g_last_error;
@@ -590,7 +600,7 @@ int vh_bind_sequence(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   v->seq_format = fmt;
   v->seq_scale = scale;
   v->track = false;
-  if (!same) drop_graphs(v, 4 | 8);  // the captured graphs hold the sequence pointers
+  if (!same) drop_graphs(v, 4 | 8 | 16 | 32);  // the captured graphs hold the sequence pointers
   const uint32_t zero = 0;
   VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
@@ -645,7 +655,7 @@ int vh_bind_tracking(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   v->trk_stride = stride;
   v->trk_max_depth = zmax;
   v->trk_increment = incr;
-  if (!same) drop_graphs(v, 4 | 8);
+  if (!same) drop_graphs(v, 4 | 8 | 16 | 32);
   const uint32_t zero = 0;
   VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
@@ -658,11 +668,21 @@ int vh_integrate_next(vh_volume* v, int32_t n) {
   std::lock_guard<std::mutex> lk(v->mu);
   int rc = set_device(v);
   if (rc) return rc;
-  for (int k = 0; k < n; ++k) {
-    rc = launch_frame(v, v->track ? 3 : 2, v->seq_scale);
-    if (rc) return rc;
+  if (v->track || v->profiling) {
+    // per frame: (tracking,) allocation, integration
+    for (int k = 0; k < n; ++k) {
+      rc = launch_frame(v, v->track ? 3 : 2, v->seq_scale);
+      if (rc) return rc;
+    }
+    return VH_OK;
   }
-  return VH_OK;
+  // allocation of frame k+1 overlaps the integration of frame k; nothing stays allocated
+  // but unintegrated when the call returns
+  if (n == 0) return VH_OK;
+  if ((rc = launch_frame(v, 4, v->seq_scale))) return rc;
+  for (int k = 1; k < n; ++k)
+    if ((rc = launch_frame(v, 2, v->seq_scale))) return rc;
+  return launch_frame(v, 5, v->seq_scale);
 }
 
 int vh_sync(vh_volume* v) {
@@ -703,7 +723,7 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->slots_used = c.pool_count;
   s->touched_last = c.touched_last;
   s->touched_total = (int64_t)c.touched_total;
-  s->new_total = (int64_t)c.new_total;
+  s->new_total = s->blocks_allocated;
   s->storage = v->cfg.storage;
   s->flags = (int32_t)c.flags;
   s->integrate_ms = v->integrate_ms;

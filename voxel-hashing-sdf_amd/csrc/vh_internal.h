@@ -8,8 +8,9 @@
 //   stamps[4 * 2^log2_buckets]      u32 per slot: last frame that touched the block;
 //   block_keys[pool_blocks]         u64 key of each pool block (pool order);
 //   voxels[pool_blocks * 512]       FP32: float2 {tsdf, weight}; FP16: u32 {f16 tsdf | u16 w << 16};
-//   touched[kStripes * stripe_cap]  u32 slot ids of the blocks crossed by this frame's bands,
-//                                   in kStripes sub-lists with separate counters;
+//   touched[2][kStripes * stripe_cap] u32 slot ids of the blocks crossed by a frame's bands,
+//                                   in kStripes sub-lists with separate counters, one list
+//                                   per frame parity;
 //   ctl                             counters of the frame pipeline (FrameCtl).
 //
 // Device-scope atomics on one address serialise (about 11 ns each on MI355X), so every
@@ -61,8 +62,8 @@ struct FrameCtl {
   unsigned long long frames;
   unsigned long long new_total;
   uint32_t pad0[18];
-  uint32_t tcount[kStripes * kLine];  // touched sub-list lengths (one line each)
-  uint32_t done[kStripes * kLine];    // integration workgroups finished, per stripe
+  uint32_t tcount[2 * kStripes * kLine];  // touched sub-list lengths (one line each), per frame parity
+  uint32_t done[kStripes * kLine];        // workgroups of the frame launch finished, per stripe
 };
 
 struct HashView {
@@ -169,6 +170,10 @@ struct TrackArgs {
 hipError_t launch_alloc(const FrameArgs& a, int depth_u16, hipStream_t st);
 hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid,
                             hipStream_t st);
+// Integration of the current frame and allocation of the next one in one launch (bound
+// sequences; the next frame's allocation must not have run).
+hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16, int grid,
+                                  hipStream_t st);
 // count pass, two-level exclusive scan, write pass (all on `st`)
 hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,

@@ -104,6 +104,15 @@ __device__ __forceinline__ uint32_t set_hash(uint64_t key) {
   return (uint32_t)((key * 0x9e3779b97f4a7c15ull) >> 54) & (kSetSlots - 1);
 }
 
+// The touched list and its stripe lengths are double-buffered by frame parity, so the
+// allocation of frame f+1 can run while frame f is integrated.
+__device__ __forceinline__ uint32_t* tcount_of(FrameCtl* ctl, uint32_t frame) {
+  return ctl->tcount + (frame & 1u) * (kStripes * kLine);
+}
+__device__ __forceinline__ uint32_t* touched_of(const FrameArgs& a, uint32_t frame) {
+  return a.touched + (size_t)(frame & 1u) * kStripes * a.stripe_cap;
+}
+
 // Global part for the active lanes' keys: find-or-insert, pool index for new keys (one
 // atomic per wave), first touch of the frame, append to the stripe (one atomic per wave).
 __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t frame, int lane,
@@ -143,29 +152,28 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
   if (fm) {
     const int leader = first_lane(fm);
     uint32_t tbase = 0;
-    if (lane == leader) tbase = atomicAdd(&ctl->tcount[stripe * kLine], (uint32_t)__popcll(fm));
+    if (lane == leader) tbase = atomicAdd(&tcount_of(ctl, frame)[stripe * kLine], (uint32_t)__popcll(fm));
     tbase = read_lane_u32(tbase, leader);
     if (first) {
       const uint32_t pos = tbase + lanes_below(fm);
-      if (pos < a.stripe_cap) a.touched[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
+      if (pos < a.stripe_cap) touched_of(a, frame)[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
       else raise_flag(ctl, kFlagTouchedFull);
     }
   }
 }
 
+// Allocation of one 16x16 tile of frame `frame` (input slot fs); 256 lanes.
 template <bool kU16>
-__global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
+__device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, uint32_t fs) {
   __shared__ unsigned long long s_set[kSetSlots];
   __shared__ unsigned long long s_uniq[kUniqCap];
   __shared__ uint32_t s_nuniq;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tiles_x = (a.W + 15) >> 4;
-  const int u = ((int)(blockIdx.x % tiles_x) << 4) + ((wave & 1) << 3) + (lane & 7);
-  const int v = ((int)(blockIdx.x / tiles_x) << 4) + ((wave >> 1) << 3) + (lane >> 3);
-  const uint32_t stripe = blockIdx.x & (kStripes - 1);
+  const int u = ((int)(tile % tiles_x) << 4) + ((wave & 1) << 3) + (lane & 7);
+  const int v = ((int)(tile / tiles_x) << 4) + ((wave >> 1) << 3) + (lane >> 3);
+  const uint32_t stripe = tile & (kStripes - 1);
   FrameCtl* ctl = a.ctl;
-  const uint32_t frame = ctl->frame_id;
-  const uint32_t fs = frame_slot(a);
   const float* Tp = a.poses + 16 * (size_t)fs;
   float T[12];
 #pragma unroll
@@ -220,6 +228,11 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
   }
 }
 
+template <bool kU16>
+__global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
+  alloc_tile<kU16>(a, blockIdx.x, a.ctl->frame_id, frame_slot(a));
+}
+
 // ============================================================================ integration
 template <int kStorage>
 struct VoxelIO;
@@ -271,22 +284,20 @@ __device__ __forceinline__ void mean_update(float& s, float& w, float dist, floa
 
 // One wave per touched block: lane l owns voxels l + 64k (x = k, y = l >> 3, z = l & 7), so
 // each of its 8 loads is one coalesced 512-byte wave access and all 8 are in flight at once.
-// The waves of a workgroup are independent workers (worker id = 4 * workgroup + wave; a
-// workgroup holds four so that more than 16 waves fit on a CU) and synchronise only within
-// themselves. The last worker to finish (two-level striped counter) closes the frame.
-constexpr int kIntegrateWaves = 4;
+// Worker w of `workers` (one wave each) takes an equal share of frame `frame`'s touched list
+// (input slot fs), its 64 stripes seen as one index space; workers synchronise only within
+// their own wave.
 template <int kStorage, bool kU16>
-__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a) {
+__device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t frame, uint32_t fs,
+                                 unsigned long long* s_val, unsigned long long* s_key) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t worker = blockIdx.x * kIntegrateWaves + (uint32_t)wave;
-  const uint32_t workers = gridDim.x * kIntegrateWaves;
-  // The 64 stripes of the touched list, seen as one index space: lane s holds stripe s's
-  // length and exclusive offset, so every workgroup gets an equal share whatever the
-  // stripe sizes.
-  const uint32_t my_len = min(ctl->tcount[lane * kLine], a.stripe_cap);
+  const int lane = threadIdx.x & 63;
+  const uint32_t* tcount = tcount_of(ctl, frame);
+  const uint32_t* touched = touched_of(a, frame);
+  // lane s holds stripe s's length and exclusive offset
+  const uint32_t my_len = min(tcount[lane * kLine], a.stripe_cap);
   uint32_t incl = my_len;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -295,8 +306,6 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a)
   }
   const uint32_t my_off = incl - my_len;
   const uint32_t n = read_lane_u32(incl, 63);
-  const uint32_t frame = ctl->frame_id;
-  const uint32_t fs = frame_slot(a);
   const float* Tp = a.poses + 16 * (size_t)fs;
   float T[12];
 #pragma unroll
@@ -307,14 +316,11 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a)
   BlockFrame bf;
   block_frame(T, a.vox, bf);
 
-  // Lane j fetches the list entry, slot value and key of this workgroup's j-th block and
-  // parks them in LDS, so the per-block chain is a single round trip (voxels + depth). The
-  // block loop reads them back from LDS: a use of the loaded registers inside the loop would
-  // make the compiler wait for every outstanding vector memory operation, the previous
-  // block's stores included (vmcnt counts stores on this architecture).
-  __shared__ unsigned long long s_vals[kIntegrateWaves][64], s_keys[kIntegrateWaves][64];
-  unsigned long long* s_val = s_vals[wave];
-  unsigned long long* s_key = s_keys[wave];
+  // Lane j fetches the list entry, slot value and key of this worker's j-th block and parks
+  // them in LDS, so the per-block chain is a single round trip (voxels + depth). The block
+  // loop reads them back from LDS: a use of the loaded registers inside the loop would make
+  // the compiler wait for every outstanding vector memory operation, the previous block's
+  // stores included (vmcnt counts stores on this architecture).
   for (uint32_t t0 = worker; t0 < n; t0 += 64u * workers) {
     const uint32_t tj = t0 + (uint32_t)lane * workers;
     const uint32_t tq = min(tj, n - 1u);
@@ -329,7 +335,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a)
     const uint32_t stripe_off = (uint32_t)__shfl((int)my_off, lo);
     unsigned long long jval = ~0ull, jkey = 0ull;
     if (tj < n) {
-      const uint32_t sl = a.touched[(size_t)lo * a.stripe_cap + (tj - stripe_off)];
+      const uint32_t sl = touched[(size_t)lo * a.stripe_cap + (tj - stripe_off)];
       jval = a.hash.vals[sl];
       jkey = a.hash.keys[sl];
     }
@@ -338,83 +344,133 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a)
     __builtin_amdgcn_wave_barrier();  // one wave owns this slice: it only needs its own writes
     const uint32_t nj = min(64u, (n - t0 + workers - 1) / workers);
     for (uint32_t j = 0; j < nj; ++j) {
-    const unsigned long long val = s_val[j];
-    const unsigned long long keyj = s_key[j];
-    const uint32_t idx = (uint32_t)val;
-    if (idx >= a.pool_cap) {
-      if (lane == 0) raise_flag(ctl, kFlagPoolFull);
-      continue;
-    }
-    const bool fresh = (uint32_t)(val >> 32) == frame;
-    int bx, by, bz;
-    unpack_key(keyj, bx, by, bz);
-    Word* vp = reinterpret_cast<Word*>(a.voxels) + (size_t)idx * kB3 + lane;
-    Word old[8];
+      const unsigned long long val = s_val[j];
+      const unsigned long long keyj = s_key[j];
+      const uint32_t idx = (uint32_t)val;
+      if (idx >= a.pool_cap) {
+        if (lane == 0) raise_flag(ctl, kFlagPoolFull);
+        continue;
+      }
+      const bool fresh = (uint32_t)(val >> 32) == frame;
+      int bx, by, bz;
+      unpack_key(keyj, bx, by, bz);
+      Word* vp = reinterpret_cast<Word*>(a.voxels) + (size_t)idx * kB3 + lane;
+      Word old[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) old[k] = fresh ? IO::zero() : vp[64 * k];
-    float cb[3], cl[3];
-    block_origin_cam(T, a.vox, bx, by, bz, cb);
-    column_cam(bf, cb, lane >> 3, lane & 7, cl);
-    // all projections, then all depth fetches, then the updates: the 8 fetches are in flight
-    // together instead of one round trip per voxel
-    float zc[8];
-    int pix[8];
+      for (int k = 0; k < 8; ++k) old[k] = fresh ? IO::zero() : vp[64 * k];
+      float cb[3], cl[3];
+      block_origin_cam(T, a.vox, bx, by, bz, cb);
+      column_cam(bf, cb, lane >> 3, lane & 7, cl);
+      // all projections, then all depth fetches, then the updates: the 8 fetches are in
+      // flight together instead of one round trip per voxel
+      float zc[8];
+      int pix[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const float fk = (float)k;
-      zc[k] = __builtin_fmaf(fk, bf.ax[2], cl[2]);
-      pix[k] = project_cam(__builtin_fmaf(fk, bf.ax[0], cl[0]), __builtin_fmaf(fk, bf.ax[1], cl[1]), zc[k],
-                           recip_rn(zc[k]), a.K, a.W, a.H);
-    }
-    float dv[8];
+      for (int k = 0; k < 8; ++k) {
+        const float fk = (float)k;
+        zc[k] = __builtin_fmaf(fk, bf.ax[2], cl[2]);
+        pix[k] = project_cam(__builtin_fmaf(fk, bf.ax[0], cl[0]), __builtin_fmaf(fk, bf.ax[1], cl[1]), zc[k],
+                             recip_rn(zc[k]), a.K, a.W, a.H);
+      }
+      float dv[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) dv[k] = dep(pix[k] < 0 ? 0 : pix[k]);
+      for (int k = 0; k < 8; ++k) dv[k] = dep(pix[k] < 0 ? 0 : pix[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      float dist;
-      const bool ok = truncate_obs(pix[k], dv[k], zc[k], a.max_depth, a.mu, a.inv_mu, dist);
-      float sv, wv;
-      IO::unpack(old[k], sv, wv);
-      mean_update(sv, wv, dist, a.cap);
-      if (ok) vp[64 * k] = IO::pack(sv, wv);
-      else if (fresh) vp[64 * k] = IO::zero();
-    }
+      for (int k = 0; k < 8; ++k) {
+        float dist;
+        const bool ok = truncate_obs(pix[k], dv[k], zc[k], a.max_depth, a.mu, a.inv_mu, dist);
+        float sv, wv;
+        IO::unpack(old[k], sv, wv);
+        mean_update(sv, wv, dist, a.cap);
+        if (ok) vp[64 * k] = IO::pack(sv, wv);
+        else if (fresh) vp[64 * k] = IO::zero();
+      }
     }
     __builtin_amdgcn_wave_barrier();  // the next batch overwrites s_val / s_key
   }
+}
 
-  // No fence before the counters: the closing workgroup reads no voxel data, and the voxel
-  // writes are published by the kernel boundary (a device-scope fence per workgroup would
-  // write back the L2 thousands of times per frame).
-  const uint32_t group = worker & (kStripes - 1);
-  const uint32_t per_group = workers / kStripes;
-  uint32_t closes = 0;
-  if (lane == 0) {
-    const uint32_t prev = atomicAdd(&ctl->done[group * kLine], 1u);
-    if (prev == per_group - 1) closes = atomicAdd(&ctl->done_top, 1u) == kStripes - 1;
+// End of a workgroup of a frame launch (all 256 lanes call it): the last of `groups`
+// workgroups to finish (two-level striped counter) closes frame `frame`: its stripe lengths
+// are summed into the statistics and cleared, the frame id and the input cursor advance.
+// No fence before the counters: the closing workgroup reads no voxel data, and the voxel
+// writes are published by the kernel boundary (a device-scope fence per workgroup would
+// write back the L2 thousands of times per frame).
+__device__ void frame_done(const FrameArgs& a, uint32_t groups, uint32_t frame) {
+  __shared__ uint32_t s_closes;
+  FrameCtl* ctl = a.ctl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t g = blockIdx.x & (kStripes - 1);
+    const uint32_t in_group = groups / kStripes + (g < groups % kStripes ? 1u : 0u);
+    const uint32_t live = min(groups, (uint32_t)kStripes);  // stripes holding a workgroup
+    uint32_t closes = 0;
+    if (atomicAdd(&ctl->done[g * kLine], 1u) == in_group - 1) closes = atomicAdd(&ctl->done_top, 1u) == live - 1;
+    s_closes = closes;
   }
-  if (read_lane_u32(closes, 0)) {
-    // The stripe lengths and the pool count were final before this kernel started (the
-    // allocation kernel wrote them): each lane reads and clears one stripe, so the close
-    // costs one round trip instead of one per stripe.
-    const uint32_t c = min(ctl->tcount[lane * kLine], a.stripe_cap);
-    ctl->tcount[lane * kLine] = 0u;
-    ctl->done[lane * kLine] = 0u;
-    uint32_t total = c;
+  __syncthreads();
+  if (threadIdx.x >= 64 || !s_closes) return;
+  // The stripe lengths were final before this launch started (the allocation of this frame
+  // ran earlier): each lane reads and clears one stripe, one round trip in all.
+  const int lane = threadIdx.x;
+  uint32_t* tcount = tcount_of(ctl, frame);
+  const uint32_t c = min(tcount[lane * kLine], a.stripe_cap);
+  tcount[lane * kLine] = 0u;
+  ctl->done[lane * kLine] = 0u;
+  uint32_t total = c;
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) total += (uint32_t)__shfl_xor((int)total, m);
-    if (lane == 0) {
-      const uint32_t pc = ctl->pool_count;
-      ctl->touched_last = total;
-      ctl->touched_total += total;
-      ctl->frames += 1ull;
-      ctl->new_total += (unsigned long long)(pc - ctl->pool_at_frame_start);
-      ctl->pool_at_frame_start = pc;
-      ctl->cursor[a.cursor_id] += 1u;
-      ctl->frame_id = frame + 1u;
-      ctl->done_top = 0u;
-    }
+  for (int m = 32; m >= 1; m >>= 1) total += (uint32_t)__shfl_xor((int)total, m);
+  if (lane == 0) {
+    ctl->touched_last = total;
+    ctl->touched_total += total;
+    ctl->frames += 1ull;
+    ctl->cursor[a.cursor_id] += 1u;
+    ctl->frame_id = frame + 1u;
+    ctl->done_top = 0u;
   }
+}
+
+constexpr int kIntegrateWaves = 4;  // independent workers per workgroup
+
+template <int kStorage, bool kU16>
+__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a) {
+  __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
+  const int wave = threadIdx.x >> 6;
+  const uint32_t frame = a.ctl->frame_id;
+  integrate_worker<kStorage, kU16>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, frame,
+                                   frame_slot(a), s_val[wave], s_key[wave]);
+  frame_done(a, gridDim.x, frame);
+}
+
+// Integration of frame f and allocation of frame f+1 in one launch (bound sequences): the
+// first 2 * min(I, A) workgroups alternate between the two roles so that both start at once
+// (the allocation is latency-bound, the integration throughput-bound); the rest take the
+// remaining role. Every workgroup counts towards the close of frame f, so the frame id and
+// cursor the allocation workgroups read at their start cannot advance under them.
+template <int kStorage, bool kU16>
+__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate_alloc(FrameArgs a, uint32_t int_groups,
+                                                                         uint32_t alloc_groups) {
+  __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
+  const uint32_t b = blockIdx.x, m = min(int_groups, alloc_groups);
+  bool alloc;
+  uint32_t idx;
+  if (b < 2 * m) {
+    alloc = b & 1u;
+    idx = b >> 1;
+  } else {
+    alloc = alloc_groups > int_groups;
+    idx = b - m;
+  }
+  const uint32_t frame = a.ctl->frame_id;
+  const uint32_t cur = a.ctl->cursor[a.cursor_id];
+  if (alloc) {
+    alloc_tile<kU16>(a, idx, frame + 1u, (cur + 1u) % a.n_seq);
+  } else {
+    const int wave = threadIdx.x >> 6;
+    integrate_worker<kStorage, kU16>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, frame,
+                                     cur % a.n_seq, s_val[wave], s_key[wave]);
+  }
+  frame_done(a, gridDim.x, frame);
 }
 
 // ============================================================================ marching cubes
@@ -1098,6 +1154,17 @@ hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int 
     const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
     if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), g, b, 0, st, a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
+  const uint32_t ig = (uint32_t)(grid / kIntegrateWaves);
+  const uint32_t ag = (uint32_t)(((a.W + 15) >> 4) * ((a.H + 15) >> 4));
+  with_storage(storage, [&](auto S) {
+    const dim3 g(ig + ag), b(64 * kIntegrateWaves);
+    if (depth_u16) hipLaunchKernelGGL((k_integrate_alloc<decltype(S)::value, true>), g, b, 0, st, a, ig, ag);
+    else hipLaunchKernelGGL((k_integrate_alloc<decltype(S)::value, false>), g, b, 0, st, a, ig, ag);
   });
   return hipGetLastError();
 }
