@@ -41,7 +41,14 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kIntegrateGrid = 8192;  // integration workers (waves, four per workgroup), a multiple of kStripes
-constexpr int kGnGrid = 512;
+constexpr int kGnGrid = 512;  // most workgroups of a solver or backward reduction
+
+// Solver grid for n points: about two points per lane (a partial per workgroup is summed
+// by the solve step, so idle workgroups cost twice).
+int gn_grid(int64_t n) {
+  const int64_t g = (n + 511) / 512;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, kGnGrid));
+}
 
 // Launch-shape tuning: VH_INTEGRATE_GRID overrides the integration grid (rounded to a
 // multiple of kStripes).
@@ -211,8 +218,8 @@ int enqueue_track_kernels(vh_volume* v) {
   g.pose = v->gn_pose;
   g.partials = v->gn_partials;
   for (int it = 0; it <= t.iters; ++it) {
-    VH_HIP(launch_gn_accum(g, v->cfg.storage, kGnGrid, v->stream));
-    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_state, v->gn_pose, it == t.iters ? 1 : 0,
+    VH_HIP(launch_gn_accum(g, v->cfg.storage, gn_grid(g.n), v->stream));
+    VH_HIP(launch_gn_solve(v->gn_partials, gn_grid(g.n), v->gn_state, v->gn_pose, it == t.iters ? 1 : 0,
                            v->gn_stats + kGnStats * it, v->stream));
   }
   VH_HIP(launch_track_end(t, v->stream));
@@ -927,8 +934,8 @@ int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int3
   // iters LM steps, then one evaluation of the last candidate; the result is the best
   // accepted pose
   for (int it = 0; it <= iters; ++it) {
-    VH_HIP(launch_gn_accum(a, v->cfg.storage, kGnGrid, v->stream));
-    VH_HIP(launch_gn_solve(v->gn_partials, kGnGrid, v->gn_state, v->gn_pose, it == iters ? 1 : 0,
+    VH_HIP(launch_gn_accum(a, v->cfg.storage, gn_grid(n), v->stream));
+    VH_HIP(launch_gn_solve(v->gn_partials, gn_grid(n), v->gn_state, v->gn_pose, it == iters ? 1 : 0,
                            v->gn_stats + kGnStats * it, v->stream));
   }
   std::vector<double> host_stats((size_t)kGnStats * (iters + 1));
