@@ -158,6 +158,13 @@ int vh_abi_version(void);
  * every block crossed by a valid pixel's truncation band and integrates those blocks.
  * Host buffers are copied before return; the call is synchronous. */
 int vh_integrate(vh_volume* vol, const vh_frame* frame);
+/* Asynchronous variant of vh_integrate (no reference counterpart; processFrame blocks,
+ * src/tsdf.cu:1495-1497): the depth is copied on a second stream into one of 4 device slots
+ * while earlier frames integrate, and the frame's allocation and integration wait for the
+ * copy. The call returns after queuing; the depth buffer must stay unchanged until vh_sync
+ * (page-locked memory, e.g. from hipHostMalloc, gives an asynchronous copy; pageable memory
+ * is staged by the runtime before the call returns). Overflows are reported by vh_sync. */
+int vh_integrate_async(vh_volume* vol, const vh_frame* frame);
 
 /* Device-resident input: binds n_frames depth images (tightly packed, device memory,
  * frame f at d_depth + f*width*height*elem) and n_frames row-major 4x4 poses

@@ -160,6 +160,42 @@ def test_sequence_chunks_overlap_allocation_and_integration(vh, oracle, wl, smal
     vol.close()
 
 
+def test_async_host_frames_match_oracle(vh, oracle, wl, small_frames):
+    """vh_integrate_async: frames copied on a second stream into a 4-slot device ring (more
+    frames than slots, so slots are reused), page-locked and pageable sources, float depth,
+    and a synchronous staged frame in between; volume and mesh equal to the oracle's."""
+    import torch
+
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    print("room_small_1cm: 14 host frames through the asynchronous copy ring", flush=True)
+    pinned = torch.from_numpy(depth[:10].view(np.int16).copy()).pin_memory()
+    vol = vh.Volume(vh.config_from_workload(w))
+    ref = oracle.Oracle(w)
+    for k in range(10):
+        vol.integrate_async(pinned[k], poses[k])
+    vol.integrate(depth[10], poses[10])  # staged, synchronous, between queued frames
+    for k in (11, 12):
+        vol.integrate_async(depth[k], poses[k])  # pageable numpy
+    vol.integrate_async(depth[13].astype(np.float32) * np.float32(0.001), poses[13])
+    vol.sync()
+    for k in range(14):
+        ref.integrate(depth[k], poses[k])
+    assert vol.stats()["frames"] == 14
+    assert_same_volume(oracle, vol, ref)
+    assert_same_mesh(vol, ref)
+    # after a reset the ring restarts in step with the device
+    vol.reset()
+    for k in range(6):
+        vol.integrate_async(pinned[k], poses[k])
+    vol.sync()
+    ref2 = oracle.Oracle(w)
+    for k in range(6):
+        ref2.integrate(depth[k], poses[k])
+    assert_same_volume(oracle, vol, ref2)
+    vol.close()
+
+
 def test_float_depth_input_matches_u16(vh, oracle, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames

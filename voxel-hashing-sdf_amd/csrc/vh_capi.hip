@@ -96,11 +96,21 @@ struct vh_volume {
   float trk_max_depth = 0.0f;  // depth limit of the tracking points
   float* trk_pts = nullptr;
   size_t trk_pts_cap = 0;  // points the buffer holds
+  // host frames queued by vh_integrate_async: copied on copy_stream into kRing device slots;
+  // a frame's kernels wait for its copy, a slot's next copy waits for the frame that read it
+  static constexpr int kRing = 4;
+  hipStream_t copy_stream = nullptr;
+  void* ring_depth = nullptr;       // kRing frames of W*H elements
+  float* ring_pose = nullptr;       // kRing x 16 (device)
+  float* ring_pose_host = nullptr;  // kRing x 16 (page-locked)
+  hipEvent_t ring_copied[kRing] = {}, ring_done[kRing] = {};
+  uint64_t ring_next = 0;           // frames queued since create/reset (= device cursor 2)
   // graphs: [0] staged u16, [1] staged f32, [2] bound sequence step (integrate frame f +
   // allocate frame f+1), [3] tracked sequence, [4] bound sequence first allocation, [5] bound
-  // sequence last integration
-  static constexpr int kGraphs = 6;
-  hipGraphExec_t graph[kGraphs] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // sequence last integration, [6] ring u16, [7] ring f32
+  static constexpr int kGraphs = 8;
+  hipGraphExec_t graph[kGraphs] = {};
+  float ring_scale = -1.0f;  // depth scale baked into the ring uint16 graph
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -136,7 +146,13 @@ size_t voxel_bytes(int storage) { return storage == VH_STORAGE_F32 ? 8 : 4; }
 FrameArgs frame_args(vh_volume* v, int which) {
   FrameArgs a{};
   const vh_config& c = v->cfg;
-  if (which >= 2) {
+  if (which == 6 || which == 7) {
+    a.depth = v->ring_depth;
+    a.poses = v->ring_pose;
+    a.n_seq = vh_volume::kRing;
+    a.cursor_id = 2;
+    a.depth_scale = which == 6 ? v->ring_scale : 1.0f;
+  } else if (which >= 2) {
     a.depth = v->seq_depth;
     a.poses = which == 3 ? v->trk_out : v->seq_pose;
     a.n_seq = (uint32_t)v->seq_n;
@@ -226,7 +242,10 @@ int enqueue_track_kernels(vh_volume* v) {
   return VH_OK;
 }
 
-int frame_u16(vh_volume* v, int which) { return which >= 2 ? v->seq_format == VH_DEPTH_U16 : which == 0; }
+int frame_u16(vh_volume* v, int which) {
+  if (which == 6 || which == 7) return which == 6;
+  return which >= 2 ? v->seq_format == VH_DEPTH_U16 : which == 0;
+}
 
 // Kinds 0, 1, 3: allocation then integration of one frame. Bound sequences run as kind 4
 // (allocate the first frame), kind 2 per further frame (integrate the previous frame and
@@ -235,7 +254,7 @@ int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale) {
   const int fa = which == 4 || which == 5 ? 2 : which;
   FrameArgs a = frame_args(v, fa);
   const int u16 = frame_u16(v, fa);
-  if (fa < 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+  if (fa < 2 || fa >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
   if (which == 2) {
     VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, integrate_grid(), v->stream));
     return VH_OK;
@@ -266,7 +285,7 @@ int launch_frame(vh_volume* v, int which, float staged_scale) {
     if (!e[0] || !e[1] || !e[2] || !e[3]) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
     FrameArgs a = frame_args(v, which);
     const int u16 = frame_u16(v, which);
-    if (which < 2) a.depth_scale = u16 ? staged_scale : 1.0f;
+    if (which < 2 || which >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
     VH_HIP(hipEventRecord(e[0], v->stream));
     if (which == 3) {
       int rc = enqueue_track_kernels(v);
@@ -349,6 +368,7 @@ int sync_and_check(vh_volume* v) {
 }
 
 int reset_tables(vh_volume* v) {
+  v->ring_next = 0;  // the device cursor of the ring restarts with the control block
   VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
   FrameCtl c{};
   VH_HIP(hipMemcpyAsync(&c, v->ctl, sizeof c, hipMemcpyDeviceToHost, v->stream));
@@ -396,11 +416,19 @@ void free_volume(vh_volume* v) {
   (void)hipSetDevice(v->device);
   if (v->stream) (void)hipStreamSynchronize(v->stream);
   drop_graphs(v, (1 << vh_volume::kGraphs) - 1);
+  if (v->copy_stream) (void)hipStreamSynchronize(v->copy_stream);
   for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
+  for (int k = 0; k < vh_volume::kRing; ++k) {
+    if (v->ring_copied[k]) (void)hipEventDestroy(v->ring_copied[k]);
+    if (v->ring_done[k]) (void)hipEventDestroy(v->ring_done[k]);
+  }
+  if (v->ring_pose_host) (void)hipHostFree(v->ring_pose_host);
+  if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
-                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts};
+                  v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
+                  v->ring_depth, v->ring_pose};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -589,6 +617,59 @@ int vh_integrate(vh_volume* v, const vh_frame* f) {
   rc = launch_frame(v, which, f->depth_scale);
   if (rc) return rc;
   return sync_and_check(v);
+}
+
+int vh_integrate_async(vh_volume* v, const vh_frame* f) {
+  if (!v || !f || !f->depth) return fail(VH_ERR_INVALID_ARG, "null argument");
+  if (f->depth_format != VH_DEPTH_U16 && f->depth_format != VH_DEPTH_F32)
+    return fail(VH_ERR_INVALID_ARG, "unknown depth format");
+  if (f->depth_format == VH_DEPTH_U16 && !(f->depth_scale > 0.0f))
+    return fail(VH_ERR_INVALID_ARG, "depth_scale must be > 0 for uint16 depth");
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(f->c2w[k])) return fail(VH_ERR_INVALID_ARG, "pose has a non-finite entry");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  constexpr int R = vh_volume::kRing;
+  const size_t elem = f->depth_format == VH_DEPTH_U16 ? 2 : 4;
+  const size_t row = (size_t)v->cfg.width * elem;
+  const size_t pitch = f->pitch ? (size_t)f->pitch : row;
+  if (pitch < row) return fail(VH_ERR_INVALID_ARG, "pitch smaller than a row");
+  if (!v->copy_stream) {
+    const size_t frame_px = (size_t)v->cfg.width * v->cfg.height;
+    if ((rc = dev_alloc(&v->ring_depth, (size_t)R * frame_px * 4)) ||
+        (rc = dev_alloc(&v->ring_pose, (size_t)R * 16 * sizeof(float))))
+      return rc;
+    VH_HIP(hipHostMalloc((void**)&v->ring_pose_host, (size_t)R * 16 * sizeof(float), hipHostMallocDefault));
+    for (int k = 0; k < R; ++k) {
+      VH_HIP(hipEventCreateWithFlags(&v->ring_copied[k], hipEventDisableTiming));
+      VH_HIP(hipEventCreateWithFlags(&v->ring_done[k], hipEventDisableTiming));
+    }
+    VH_HIP(hipStreamCreateWithFlags(&v->copy_stream, hipStreamNonBlocking));
+  }
+  const int which = f->depth_format == VH_DEPTH_U16 ? 6 : 7;
+  if (which == 6 && f->depth_scale != v->ring_scale) {  // the ring uint16 graph bakes the scale
+    VH_HIP(hipStreamSynchronize(v->stream));
+    drop_graphs(v, 1 << 6);
+    v->ring_scale = f->depth_scale;
+  }
+  const int slot = (int)(v->ring_next % R);
+  if (v->ring_next >= (uint64_t)R) {
+    VH_HIP(hipStreamWaitEvent(v->copy_stream, v->ring_done[slot], 0));  // the frame that read the slot
+    VH_HIP(hipEventSynchronize(v->ring_copied[slot]));                    // the pose staging is free
+  }
+  std::memcpy(v->ring_pose_host + 16 * slot, f->c2w, 16 * sizeof(float));
+  char* dst = static_cast<char*>(v->ring_depth) + (size_t)slot * v->cfg.width * v->cfg.height * elem;
+  VH_HIP(hipMemcpy2DAsync(dst, row, f->depth, pitch, row, (size_t)v->cfg.height, hipMemcpyHostToDevice,
+                          v->copy_stream));
+  VH_HIP(hipMemcpyAsync(v->ring_pose + 16 * slot, v->ring_pose_host + 16 * slot, 16 * sizeof(float),
+                        hipMemcpyHostToDevice, v->copy_stream));
+  VH_HIP(hipEventRecord(v->ring_copied[slot], v->copy_stream));
+  VH_HIP(hipStreamWaitEvent(v->stream, v->ring_copied[slot], 0));
+  if ((rc = launch_frame(v, which, f->depth_scale))) return rc;
+  VH_HIP(hipEventRecord(v->ring_done[slot], v->stream));
+  ++v->ring_next;
+  return VH_OK;
 }
 
 int vh_bind_sequence(vh_volume* v, const void* d_depth, int32_t fmt, float scale, const float* d_c2w, int32_t n) {

@@ -55,9 +55,9 @@ struct FrameCtl {
   uint32_t pool_count;     // keys inserted (pool indices handed out, may pass capacity)
   uint32_t flags;          // kFlag* raised since the host last read them
   uint32_t touched_last;
-  uint32_t cursor[2];      // next frame of the staged input (0) and bound sequence (1)
-  uint32_t done_top;       // integration stripes finished this frame
-  uint32_t pool_at_frame_start;
+  uint32_t cursor[3];      // next frame of the staged input (0), the bound sequence (1) and the
+                           // ring of asynchronously copied host frames (2)
+  uint32_t done_top;       // stripes whose workgroups all finished this frame launch
   unsigned long long touched_total;
   unsigned long long frames;
   unsigned long long new_total;

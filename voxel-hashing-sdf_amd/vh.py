@@ -105,6 +105,7 @@ _PROTOS = [
     ("vh_last_error", C.c_char_p, []),
     ("vh_abi_version", C.c_int, []),
     ("vh_integrate", C.c_int, [C.c_void_p, C.POINTER(Frame)]),
+    ("vh_integrate_async", C.c_int, [C.c_void_p, C.POINTER(Frame)]),
     ("vh_bind_sequence", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     ("vh_bind_tracking", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
                                    C.POINTER(TrackConfig)]),
@@ -209,12 +210,14 @@ class Volume:
         h = C.c_void_p()
         _check(lib.vh_create(C.byref(cfg), C.byref(h)))
         self._h = h
+        self._pending = []  # host frames queued by integrate_async, referenced until sync()
 
     # -- lifetime
     def close(self):
         if self._h:
             _check(_lib.vh_destroy(self._h))
             self._h = None
+            self._pending = []
 
     def __enter__(self):
         return self
@@ -251,6 +254,34 @@ class Volume:
             f.c2w[k] = float(T[k])
         _check(_lib.vh_integrate(self._h, C.byref(f)))
 
+    def integrate_async(self, depth, c2w, scale: float = 0.001):
+        """Queues one host frame (vh_integrate_async). depth: numpy uint16/float32 array or a
+        CPU torch tensor (int16 holding uint16 bits, or float32), ideally page-locked
+        (pin_memory=True); it is kept referenced until sync()."""
+        f = Frame()
+        if hasattr(depth, "data_ptr"):
+            import torch
+
+            if depth.dtype not in (torch.int16, torch.float32) or depth.is_cuda or not depth.is_contiguous():
+                raise TypeError("depth tensor must be a contiguous CPU int16 (uint16 bits) or float32 tensor")
+            f.depth = depth.data_ptr()
+            u16 = depth.dtype == torch.int16
+            f.pitch = depth.shape[-1] * (2 if u16 else 4)
+        else:
+            depth = np.ascontiguousarray(depth)
+            if depth.dtype not in (np.uint16, np.float32):
+                raise TypeError("depth must be uint16 or float32")
+            f.depth = depth.ctypes.data_as(C.c_void_p)
+            u16 = depth.dtype == np.uint16
+            f.pitch = depth.strides[0]
+        f.depth_format = DEPTH_U16 if u16 else DEPTH_F32
+        f.depth_scale = scale if u16 else 1.0
+        T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32).reshape(16))
+        for k in range(16):
+            f.c2w[k] = float(T[k])
+        _check(_lib.vh_integrate_async(self._h, C.byref(f)))
+        self._pending.append(depth)
+
     def bind_sequence(self, d_depth, fmt: int, scale: float, d_c2w, n: int):
         """d_depth / d_c2w: torch device tensors (or integer device addresses)."""
         dp = d_depth if isinstance(d_depth, int) else d_depth.data_ptr()
@@ -277,7 +308,10 @@ class Volume:
         _check(_lib.vh_integrate_next(self._h, n))
 
     def sync(self):
-        _check(_lib.vh_sync(self._h))
+        try:
+            _check(_lib.vh_sync(self._h))
+        finally:
+            self._pending = []
 
     def set_profiling(self, on: bool):
         _check(_lib.vh_set_profiling(self._h, 1 if on else 0))
