@@ -11,7 +11,8 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = [os.path.join(ROOT, "include", "vh.h"), os.path.join(ROOT, "include", "vh_synth.h")]
+HEADERS = [os.path.join(ROOT, "include", "vh.h"), os.path.join(ROOT, "include", "vh_synth.h"),
+           os.path.join(ROOT, "include", "vh_io.h")]
 PKG = os.path.join(ROOT, "voxel-hashing-sdf_amd")
 
 
@@ -107,8 +108,8 @@ def test_struct_layouts_match_the_header(vh, tmp_path):
     if not shutil.which("gcc"):
         pytest.skip("gcc not available")
     structs = {"vh_config": vh.Config, "vh_frame": vh.Frame, "vh_stats": vh.Stats, "vh_mesh": vh.Mesh,
-               "vh_gn_stats": vh.GnStats, "vh_track_config": vh.TrackConfig}
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vh.h"', "int main(void) {"]
+               "vh_gn_stats": vh.GnStats, "vh_track_config": vh.TrackConfig, "vh_settings": vh.Settings}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vh.h"', '#include "vh_io.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:

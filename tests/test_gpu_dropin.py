@@ -57,3 +57,44 @@ def test_load_frames_example_writes_welded_ply(vh, wl, small_frames, tmp_path):
     verts = np.frombuffer(raw[hdr_end:hdr_end + nv * 12], np.float32).reshape(-1, 9)
     assert np.array_equal(np.sort(verts.view(np.uint32), 0), np.sort(xyz.view(np.uint32), 0))
     vol.close()
+
+
+def test_dataset_mode_equals_synthetic_mode(vh, small_frames, tmp_path):
+    """The frames of the synthetic path written as a reference-style dataset (depth/<id>.png,
+    tcw/<id>.txt, settings file) and integrated by load_frames --dataset give the same mesh as
+    load_frames integrating them directly: the same triangles with the same vertex bits (the
+    order in the file follows pool indices, which concurrent allocation hands out in a
+    different order on every run)."""
+    exe = os.path.join(ROOT, "voxel-hashing-sdf_amd", "load_frames")
+    depth, poses = small_frames
+    n = 8
+    ds = tmp_path / "ds"
+    (ds / "depth").mkdir(parents=True)
+    (ds / "tcw").mkdir()
+    for k in range(n):
+        vh.png16_save(str(ds / "depth" / f"{k}.png"), depth[k])
+        # 9 significant digits give each float32 entry back exactly
+        (ds / "tcw" / f"{k}.txt").write_text("\n".join(" ".join(f"{x:.9g}" for x in row) for row in poses[k]) + "\n")
+    settings = tmp_path / "cam.yaml"
+    settings.write_text("%YAML:1.0\nCamera.fx: 525\nCamera.fy: 525\nCamera.cx: 319.5\nCamera.cy: 239.5\n"
+                        "Camera.width: 640\nCamera.height: 480\nDepthMapFactor: 1000\nMaxDepth: 5.0\n"
+                        "Voxel.Origin.x: -1.5\nVoxel.Origin.y: -1.5\nVoxel.Origin.z: 0.5\nVoxel.Size: 0.01\n"
+                        "Voxel.TruncMargin: 0.04\nVoxel.Dim.x: 500\nVoxel.Dim.y: 500\nVoxel.Dim.z: 500\n")
+    print(f"drop-in: load_frames --dataset, {n} frames", flush=True)
+    a, b = tmp_path / "direct.ply", tmp_path / "dataset.ply"
+    r = subprocess.run([exe, str(a), str(n)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe, str(b), "--dataset", str(ds), str(settings)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"frames {n} " in r.stdout
+
+    def triangles(path):
+        """One row of 9 float bit patterns per face (corners in the face's order), rows sorted."""
+        v, f = read_ascii_ply(path)
+        rows = v.astype(np.float32).view(np.uint32)[f].reshape(-1, 9)
+        return rows[np.lexsort(rows.T[::-1])]
+
+    ta, tb = triangles(a), triangles(b)
+    assert ta.shape == tb.shape and len(ta) > 1000
+    assert np.array_equal(ta, tb)

@@ -19,19 +19,24 @@
 #include <vector>
 
 #include "vh.h"
+#include "vh_error.h"
 #include "vh_internal.h"
 #include "vh_math.h"
 
 using namespace vh;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int vh::set_last_error(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
 
 #define VH_HIP(call)                                                                           \
   do {                                                                                         \

@@ -83,6 +83,13 @@ class TrackConfig(C.Structure):
                 ("prior_mode", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
+class Settings(C.Structure):
+    _fields_ = [("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float), ("width", C.c_int32),
+                ("height", C.c_int32), ("depth_factor", C.c_float), ("max_depth", C.c_float),
+                ("origin", C.c_float * 3), ("vox_size", C.c_float), ("trunc", C.c_float), ("vox_dim", C.c_int32 * 3),
+                ("found", C.c_uint32)]
+
+
 class GnStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("n_valid", C.c_int32 * 64), ("rms", C.c_double * 64),
                 ("step_norm", C.c_double * 64), ("pose", (C.c_double * 12) * 64)]
@@ -126,6 +133,13 @@ _PROTOS = [
                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("vh_gn_refine", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.POINTER(GnStats)]),
     ("vh_pool_view", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    ("vh_settings_load", C.c_int, [C.c_char_p, C.POINTER(Settings)]),
+    ("vh_settings_to_config", C.c_int, [C.POINTER(Settings), C.POINTER(Config)]),
+    ("vh_png16_info", C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("vh_png16_load", C.c_int, [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32]),
+    ("vh_png16_save", C.c_int, [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32]),
+    ("vh_pose_load", C.c_int, [C.c_char_p, C.c_void_p]),
+    ("vh_frame_load", C.c_int, [C.c_char_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     ("vh_scene_create", C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
     ("vh_scene_destroy", C.c_int, [C.c_void_p]),
     ("vh_scene_info_get", C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
@@ -396,6 +410,60 @@ class Volume:
         n = C.c_int64()
         _check(_lib.vh_pool_view(self._h, C.byref(vox), C.byref(keys), C.byref(n)))
         return vox.value, keys.value, n.value
+
+
+def load_settings(path: str) -> dict:
+    """vh_settings_load: the settings file's camera and voxel keys (missing keys absent)."""
+    st = Settings()
+    _check(load().vh_settings_load(os.fsencode(path), C.byref(st)))
+    out = {"found": st.found}
+    bits = [("fx", 1), ("fy", 2), ("cx", 4), ("cy", 8), ("width", 16), ("height", 32), ("depth_factor", 64),
+            ("max_depth", 128), ("origin", 256), ("vox_size", 512), ("trunc", 1024), ("vox_dim", 2048)]
+    for name, bit in bits:
+        if st.found & bit:
+            v = getattr(st, name)
+            out[name] = tuple(v) if name in ("origin", "vox_dim") else v
+    return out
+
+
+def config_from_settings(path: str, **over) -> Config:
+    """vh_config with the camera and voxel fields of a settings file (vh_settings_to_config)."""
+    lib = load()
+    st = Settings()
+    _check(lib.vh_settings_load(os.fsencode(path), C.byref(st)))
+    cfg = make_config()
+    _check(lib.vh_settings_to_config(C.byref(st), C.byref(cfg)))
+    for k, v in over.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def png16_load(path: str) -> np.ndarray:
+    lib = load()
+    w, h = C.c_int32(), C.c_int32()
+    _check(lib.vh_png16_info(os.fsencode(path), C.byref(w), C.byref(h)))
+    out = np.zeros((h.value, w.value), np.uint16)
+    _check(lib.vh_png16_load(os.fsencode(path), _ptr(out), w.value, h.value))
+    return out
+
+
+def png16_save(path: str, img: np.ndarray):
+    img = np.ascontiguousarray(img, dtype=np.uint16)
+    _check(load().vh_png16_save(os.fsencode(path), _ptr(img), img.shape[1], img.shape[0]))
+
+
+def pose_load(path: str) -> np.ndarray:
+    T = np.zeros(16, np.float32)
+    _check(load().vh_pose_load(os.fsencode(path), _ptr(T)))
+    return T.reshape(4, 4)
+
+
+def frame_load(directory: str, frame_id: int, width: int, height: int):
+    """(depth uint16 (height, width), c2w (4, 4) float32) of <dir>/depth/<id>.png, <dir>/tcw/<id>.txt."""
+    d = np.zeros((height, width), np.uint16)
+    T = np.zeros(16, np.float32)
+    _check(load().vh_frame_load(os.fsencode(directory), frame_id, _ptr(d), width, height, _ptr(T)))
+    return d, T.reshape(4, 4)
 
 
 class Scene:
