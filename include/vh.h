@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 2
+#define VH_ABI_VERSION 3
 
 typedef enum vh_status {
   VH_OK = 0,
@@ -78,7 +78,10 @@ typedef struct vh_config {
   int32_t device;        /* HIP device ordinal */
   float origin[3];       /* ignored */
   int32_t vox_dim[3];    /* ignored */
-  int32_t reserved[8];
+  int32_t color;         /* 1: integrate colour (Voxel::sdf_color, include/tsdf.cuh:122): every
+                            frame then needs its RGB image, and meshes carry vertex colours;
+                            0: depth only (4 B per voxel less traffic each way) */
+  int32_t reserved[7];   /* zero */
 } vh_config;
 
 /* One input frame in host memory (replaces processFrame's float* depth, uchar* rgb,
@@ -88,7 +91,9 @@ typedef struct vh_frame {
   int32_t depth_format;  /* vh_depth_format */
   float depth_scale;     /* metres per unit for VH_DEPTH_U16; ignored for VH_DEPTH_F32 */
   int64_t pitch;         /* bytes per depth row; 0 = tightly packed */
-  const uint8_t* rgb;    /* may be NULL; colour is not integrated by this version */
+  const uint8_t* rgb;    /* colour volumes: height x width x 3 bytes, tightly packed, the
+                            channels in the order the mesh colours are to have (the reference
+                            passes its image's bytes through unchanged); ignored otherwise */
   float c2w[16];         /* row-major camera-to-world */
 } vh_frame;
 
@@ -131,6 +136,7 @@ typedef struct vh_mesh {
   int64_t n_tris;
   float* xyz;       /* n_tris * 9 floats: three vertices in metres */
   int32_t* cubes;   /* n_tris * 4: cube's lattice x,y,z and (cube_case << 8 | tri_in_cube) */
+  uint8_t* rgb;     /* colour volumes: n_tris * 9 bytes, 3 channels per vertex; else NULL */
 } vh_mesh;
 
 typedef struct vh_gn_stats {
@@ -155,22 +161,28 @@ int vh_abi_version(void);
 
 /* ------------------------------------------------------------------ integration */
 /* GpuTsdfGenerator::processFrame(depth, rgb, c2w) (src/tsdf.cu:1484-1598): allocates
- * every block crossed by a valid pixel's truncation band and integrates those blocks.
- * Host buffers are copied before return; the call is synchronous. */
+ * every block crossed by a valid pixel's truncation band and integrates those blocks (and,
+ * in a colour volume, the frame's RGB image at the same pixels: VH_ERR_INVALID_ARG without
+ * one). Host buffers are copied before return; the call is synchronous. */
 int vh_integrate(vh_volume* vol, const vh_frame* frame);
 /* Asynchronous variant of vh_integrate (no reference counterpart; processFrame blocks,
- * src/tsdf.cu:1495-1497): the depth is copied on a second stream into one of 4 device slots
- * while earlier frames integrate, and the frame's allocation and integration wait for the
- * copy. The call returns after queuing; the depth buffer must stay unchanged until vh_sync
+ * src/tsdf.cu:1495-1497): the depth (and RGB) is copied on a second stream into one of 4
+ * device slots while earlier frames integrate, and the frame's allocation and integration
+ * wait for the copy. The call returns after queuing; the buffers must stay unchanged until vh_sync
  * (page-locked memory, e.g. from hipHostMalloc, gives an asynchronous copy; pageable memory
  * is staged by the runtime before the call returns). Overflows are reported by vh_sync. */
 int vh_integrate_async(vh_volume* vol, const vh_frame* frame);
 
 /* Device-resident input: binds n_frames depth images (tightly packed, device memory,
  * frame f at d_depth + f*width*height*elem) and n_frames row-major 4x4 poses
- * (device memory). Later vh_integrate_next calls consume them in order. */
+ * (device memory). Later vh_integrate_next calls consume them in order. A colour volume
+ * also needs the sequence's RGB images (vh_bind_sequence_rgb) before vh_integrate_next. */
 int vh_bind_sequence(vh_volume* vol, const void* d_depth, int32_t depth_format,
                      float depth_scale, const float* d_c2w, int32_t n_frames);
+/* RGB images of the sequence bound last (by vh_bind_sequence or vh_bind_tracking): frame f
+ * at d_rgb + f*width*height*3 (device memory, vh_frame.rgb's layout). Colour volumes only;
+ * each bind call clears it. */
+int vh_bind_sequence_rgb(vh_volume* vol, const uint8_t* d_rgb);
 /* Device-resident input with tracking: like vh_bind_sequence, but each frame's pose is
  * first refined against the current volume (gn_iters Levenberg-Marquardt steps of
  * vh_gn_refine over the frame's depth back-projected on every point_stride-th pixel of every
@@ -209,7 +221,13 @@ int vh_mesh_free(vh_mesh* mesh);
  * (VH_ERR_TRUNCATED if it exceeds capacity). Synchronous. */
 int vh_extract_mesh_device(vh_volume* vol, float* d_xyz, int32_t* d_cubes, int64_t capacity,
                            int64_t* n_tris);
-/* Writes the mesh as binary little-endian PLY (vertices of each triangle listed in order). */
+/* vh_extract_mesh_device with vertex colours (the reference's Vertex::r,g,b, interpolated as
+ * VertexInterp does, src/tsdf.cu:1641-1660): d_rgb (device, may be NULL) receives 9 bytes per
+ * triangle. A non-NULL d_rgb needs a colour volume. */
+int vh_extract_mesh_device_rgb(vh_volume* vol, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb,
+                               int64_t capacity, int64_t* n_tris);
+/* Writes the mesh as binary little-endian PLY (vertices of each triangle listed in order;
+ * colour volumes add uchar red, green, blue per vertex). */
 int vh_save_ply(vh_volume* vol, const char* path);
 
 /* ------------------------------------------------------------------ block export */
@@ -219,6 +237,10 @@ int vh_save_ply(vh_volume* vol, const char* path);
  * NULL to skip that array. */
 int vh_export_blocks(vh_volume* vol, int32_t* keys, float* tsdf, float* weight,
                      int64_t capacity, int64_t* n);
+/* Colour of every allocated block in vh_export_blocks' block and voxel order: 512 x 3 bytes
+ * per block (zero where the voxel was never observed). Colour volumes only; *n and
+ * VH_ERR_TRUNCATED as vh_export_blocks. */
+int vh_export_block_colors(vh_volume* vol, uint8_t* rgb, int64_t capacity, int64_t* n);
 
 /* ------------------------------------------------------------------ query (extension) */
 /* Trilinear SDF at n world points (host arrays, 3 floats each). valid[i] = 1 iff all 8

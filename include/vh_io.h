@@ -6,7 +6,8 @@
  *     reads <dir>/tcw/<id>.txt (16 numbers, row-major 4x4) which it inverts into mTcw;
  *     PointCloudGenerator::PushFrame inverts it back (src/PointCloudGenerator.cpp:139-143)
  *     before processFrame, so the file's matrix is the camera-to-world pose integrated.
- *     The colour image <dir>/RGB/<id>.jpg is not read (colour is not integrated).
+ *     The colour image <dir>/RGB/<id>.jpg is not read (no JPEG decoder here): callers with
+ *     decoded colour pass it in vh_frame.rgb.
  *   - PointCloudGenerator's constructor (src/PointCloudGenerator.cpp:20-45) reads
  *     Camera.fx/fy/cx/cy/width/height, DepthMapFactor, MaxDepth, Voxel.Origin.x/y/z,
  *     Voxel.Size, Voxel.TruncMargin and Voxel.Dim.x/y/z with cv::FileStorage from an

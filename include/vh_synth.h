@@ -48,6 +48,11 @@ int vh_scene_pose(const vh_scene* s, int32_t f, float c2w[16]);
 int vh_scene_render_u16(const vh_scene* s, int32_t f, const float* c2w, uint16_t* depth);
 /* Noise-free, dropout-free camera-z depth in metres (FP32) at pose c2w (NULL = frame f's). */
 int vh_scene_render_clean(const vh_scene* s, int32_t f, const float* c2w, float* depth);
+/* Noise-free colour image of frame f (width*height*3 bytes, RGB) at pose c2w (NULL = frame
+ * f's): each surface (room face, sphere, box) has a base colour drawn from the seed, on a
+ * 10 cm world-aligned checkerboard whose dark cells carry 5/8 of it; 0 where nothing is hit.
+ * The reference's frames carry an RGB image beside the depth (SaveFrame.cpp:154-218). */
+int vh_scene_render_rgb(const vh_scene* s, int32_t f, const float* c2w, uint8_t* rgb);
 /* Frames [f0, f0+n) into depth (n*width*height) and c2w (n*16), multithreaded. */
 int vh_scene_render_range(const vh_scene* s, int32_t f0, int32_t n, uint16_t* depth,
                           float* c2w);

@@ -53,6 +53,16 @@ def load():
     lib.vho_f16_to_f32.restype = C.c_float
     lib.vho_f16_to_f32.argtypes = [C.c_uint16]
     lib.vho_vertex_interp.argtypes = [P, P, C.c_float, C.c_float, P]
+    lib.vho_set_color.restype = C.c_int
+    lib.vho_set_color.argtypes = [P, C.c_int]
+    lib.vho_integrate_rgb.restype = C.c_int64
+    lib.vho_integrate_rgb.argtypes = [P, P, P, C.c_float, P]
+    lib.vho_export_colors.restype = C.c_int64
+    lib.vho_export_colors.argtypes = [P, P, C.c_int64]
+    lib.vho_mc_rgb.restype = C.c_int64
+    lib.vho_mc_rgb.argtypes = [P, P, P, P, C.c_int64]
+    lib.vho_vertex_interp_rgb.argtypes = [P, P, C.c_float, C.c_float, P, P, P, P]
+    lib.vho_color_update.argtypes = [P, C.c_float, P]
     _lib = lib
     return lib
 
@@ -68,7 +78,9 @@ def sort_blocks(keys, *arrays):
 
 
 class Oracle:
-    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0):
+    """color: a colour volume (integrate then needs rgb)."""
+
+    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0, color=False):
         lib = load()
         if threads:
             lib.vho_set_threads(threads)
@@ -77,15 +89,22 @@ class Oracle:
         self.w = dict(w)
         self._h = lib.vho_create(w["width"], w["height"], w["fx"], w["fy"], w["cx"], w["cy"], w["max_depth"],
                                  w["min_depth"], w["vox_size"], w["trunc"], st, cap)
+        self.color = bool(color)
+        if self.color:
+            lib.vho_set_color(self._h, 1)
 
     def __del__(self):
         if getattr(self, "_h", None):
             _lib.vho_destroy(self._h)
             self._h = None
 
-    def integrate(self, depth_u16: np.ndarray, c2w, scale=0.001) -> int:
+    def integrate(self, depth_u16: np.ndarray, c2w, scale=0.001, rgb=None) -> int:
         d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
         T = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        if self.color:
+            c = np.ascontiguousarray(rgb, dtype=np.uint8)
+            assert c.size == d.size * 3
+            return int(_lib.vho_integrate_rgb(self._h, _p(d), _p(c), scale, _p(T)))
         return int(_lib.vho_integrate(self._h, _p(d), scale, _p(T)))
 
     def num_blocks(self) -> int:
@@ -108,12 +127,22 @@ class Oracle:
         _lib.vho_export(self._h, _p(keys), _p(s), _p(w), _p(s64), _p(w64), n)
         return sort_blocks(keys, s, w, s64, w64)
 
-    def mc(self):
+    def export_rgb(self):
+        """Sorted (keys, rgb (N, 512, 3) uint8) of a colour volume."""
+        n = self.num_blocks()
+        keys = np.zeros((n, 3), np.int32)
+        _lib.vho_export(self._h, _p(keys), None, None, None, None, n)
+        rgb = np.zeros((n, 512, 3), np.uint8)
+        _lib.vho_export_colors(self._h, _p(rgb), n)
+        return sort_blocks(keys, rgb)
+
+    def mc(self, with_rgb=False):
         n = _lib.vho_mc(self._h, None, None, 0)
         xyz = np.zeros((n, 9), np.float32)
         cubes = np.zeros((n, 4), np.int32)
-        _lib.vho_mc(self._h, _p(xyz), _p(cubes), n)
-        return xyz, cubes
+        rgb = np.zeros((n, 9), np.uint8) if with_rgb else None
+        _lib.vho_mc_rgb(self._h, _p(xyz), _p(cubes), _p(rgb), n)
+        return (xyz, cubes, rgb) if with_rgb else (xyz, cubes)
 
     def query(self, pts):
         pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
@@ -178,3 +207,25 @@ def vertex_interp(a, b, va, vb):
     out = np.zeros(3, np.float32)
     _lib.vho_vertex_interp(_p(a), _p(b), va, vb, _p(out))
     return out
+
+
+def vertex_interp_rgb(a, b, va, vb, ca, cb):
+    """(position, colour) of the MC vertex between corners a and b with colours ca, cb."""
+    load()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    ca = np.ascontiguousarray(ca, np.uint8)
+    cb = np.ascontiguousarray(cb, np.uint8)
+    out = np.zeros(3, np.float32)
+    cout = np.zeros(3, np.uint8)
+    _lib.vho_vertex_interp_rgb(_p(a), _p(b), va, vb, _p(ca), _p(cb), _p(out), _p(cout))
+    return out, cout
+
+
+def color_update(c, w_old, px):
+    """One colour observation: stored colour c (3 bytes), stored weight w_old, pixel px."""
+    load()
+    c = np.array(c, np.uint8)
+    px = np.ascontiguousarray(px, np.uint8)
+    _lib.vho_color_update(_p(c), w_old, _p(px))
+    return c

@@ -13,9 +13,10 @@
 //   world<->camera transforms, projection      src/tsdf.cu:66-116 (base2cam, cam2base, cam2frame)
 //   integration skip rules, truncation         src/tsdf.cu:599-751 (IntegrateHashKernel)
 //   running average with the old weight        src/tsdf.cu:853-858 (dense Integrate: the intent)
+//   colour running average, 8-bit truncation   src/tsdf.cu:743-745, include/tsdf.cuh:122
 //   block selection along the pixel ray        src/tsdf.cu:2088-2238 (re-specified: truncation band)
 //   marching cubes corner order, sign rule     include/tsdf.cuh:234-241, src/tsdf.cu:978-1061
-//   vertex interpolation with 1e-5 snaps       src/tsdf.cu:1640-1660
+//   vertex interpolation with 1e-5 snaps       src/tsdf.cu:1640-1660 (position and colour)
 //   Bourke triangle table                      include/tsdf.cuh:278-534
 // Functions without a reference counterpart (trilinear query, its gradient, binary16
 // rounding) are restated from their textbook definitions.
@@ -130,6 +131,8 @@ struct Oracle {
   std::vector<uint16_t> s16, w16;              // FP16 twins (binary16 bits, weight)
   std::vector<int8_t> r8;                      // compensated FP16: rounding residual
   std::vector<double> s64, w64;                // FP64 shadow
+  bool color = false;
+  std::vector<uint8_t> rgb8;                   // colour volumes: 3 channels per voxel
   std::vector<int64_t> touched;                // last frame, sorted keys
   int64_t frames = 0;
 
@@ -144,6 +147,7 @@ struct Oracle {
     r8.resize(r8.size() + 512, 0);
     s64.resize(s64.size() + 512, 0.0);
     w64.resize(w64.size() + 512, 0.0);
+    if (color) rgb8.resize(rgb8.size() + 512 * 3, 0);
     return b;
   }
   int64_t find(int64_t k) const {
@@ -218,7 +222,15 @@ void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit em
 }
 
 // ------------------------------------------------------------------ integration
-void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
+// Colour of one observation (src/tsdf.cu:743-745): per channel (c * w_old + rgb) / (w_old + 1)
+// in FP32 (product and sum exact below 2^24), truncated to 8 bits; w_old is the stored weight
+// before the update.
+void color_update(uint8_t* c, float w_old, const uint8_t* px) {
+  const float wn = w_old + 1.0f;
+  for (int k = 0; k < 3; ++k) c[k] = (uint8_t)(std::fmaf((float)c[k], w_old, (float)px[k]) / wn);
+}
+
+void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, const uint8_t* rgb) {
   const int W = o.W, H = o.H;
   std::vector<float> d((size_t)W * H);
   for (size_t i = 0; i < d.size(); ++i) d[i] = (float)dep[i] * scale;
@@ -296,6 +308,11 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
       const float diff = dv - zc;
       if (!(diff > -o.mu)) continue;
       const float dist = std::fmin(1.0f, diff * inv_mu);
+      if (o.color) {
+        float s0, w0;
+        o.twin(blk[(size_t)k], i, s0, w0);
+        color_update(&o.rgb8[3 * (base + i)], w0, rgb + 3 * pix);
+      }
       // twin update: mean with the old weight, then the cap
       if (o.storage == 2) {
         float s = decode_comp(o.s16[base + i], o.r8[base + i]), w = (float)o.w16[base + i];
@@ -339,20 +356,32 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T) {
 struct Tri {
   int32_t c[4];
   float p[9];
+  uint8_t rgb[9];
 };
 
 // VertexInterp(iso = 0) in voxel units (src/tsdf.cu:1640-1660); |x| < 1e-5 compares the
 // float magnitude against the double constant, as the reference does.
-void interp(const float a[3], const float b[3], float va, float vb, float out[3]) {
-  if ((double)std::fabs(va) < 0.00001) { std::memcpy(out, a, 12); return; }
-  if ((double)std::fabs(vb) < 0.00001) { std::memcpy(out, b, 12); return; }
-  if ((double)std::fabs(va - vb) < 0.00001) { std::memcpy(out, a, 12); return; }
+// Colour (ca, cb, cout: 3 channels each, may be null) as the reference's
+// p.r = p1.r + mu*(p2.r - p1.r) (src/tsdf.cu:1655-1657): the channel difference is an int,
+// the rest FP32, the result truncated to 8 bits; a snapped vertex takes its corner's colour.
+void interp(const float a[3], const float b[3], float va, float vb, float out[3], const uint8_t* ca = nullptr,
+            const uint8_t* cb = nullptr, uint8_t* cout = nullptr) {
+  auto take = [&](const float* p, const uint8_t* c) {
+    std::memcpy(out, p, 12);
+    if (cout) std::memcpy(cout, c, 3);
+  };
+  if ((double)std::fabs(va) < 0.00001) return take(a, ca);
+  if ((double)std::fabs(vb) < 0.00001) return take(b, cb);
+  if ((double)std::fabs(va - vb) < 0.00001) return take(a, ca);
   const float m = (0.0f - va) / (vb - va);
   for (int k = 0; k < 3; ++k) out[k] = a[k] + m * (b[k] - a[k]);
+  if (cout)
+    for (int k = 0; k < 3; ++k) cout[k] = (uint8_t)((float)ca[k] + m * (float)((int)cb[k] - (int)ca[k]));
 }
 
 void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& out) {
   float val[8];
+  uint8_t col[8][3] = {};
   for (int c = 0; c < 8; ++c) {
     const int x = gx + kCorner[c][0], y = gy + kCorner[c][1], z = gz + kCorner[c][2];
     const int64_t b = o.find(key_of(floor_div8(x), floor_div8(y), floor_div8(z)));
@@ -362,6 +391,7 @@ void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& o
     o.twin(b, i, s, w);
     if (!(w > 0.0f)) return;  // unobserved corner: no surface (corrects the reference, which has no test)
     val[c] = s;
+    if (o.color) std::memcpy(col[c], &o.rgb8[3 * ((size_t)b * 512 + i)], 3);
   }
   int ci = 0;
   for (int c = 0; c < 8; ++c)
@@ -375,7 +405,7 @@ void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& o
       const int ca = kEdge[e][0], cb = kEdge[e][1];
       const float pa[3] = {(float)(gx + kCorner[ca][0]), (float)(gy + kCorner[ca][1]), (float)(gz + kCorner[ca][2])};
       const float pb[3] = {(float)(gx + kCorner[cb][0]), (float)(gy + kCorner[cb][1]), (float)(gz + kCorner[cb][2])};
-      interp(pa, pb, val[ca], val[cb], tr.p + 3 * j);
+      interp(pa, pb, val[ca], val[cb], tr.p + 3 * j, col[ca], col[cb], tr.rgb + 3 * j);
     }
     auto same = [&](int a, int b) {
       return tr.p[3 * a] == tr.p[3 * b] && tr.p[3 * a + 1] == tr.p[3 * b + 1] && tr.p[3 * a + 2] == tr.p[3 * b + 2];
@@ -465,8 +495,31 @@ int vho_max_threads(void) {
 
 int64_t vho_integrate(void* p, const uint16_t* depth, float scale, const float* c2w) {
   Oracle* o = static_cast<Oracle*>(p);
-  integrate(*o, depth, scale, c2w);
+  if (o->color) return -1;  // a colour volume needs vho_integrate_rgb
+  integrate(*o, depth, scale, c2w, nullptr);
   return (int64_t)o->touched.size();
+}
+// Makes the oracle a colour volume (before the first frame; returns 0, or -1 once blocks exist).
+int vho_set_color(void* p, int on) {
+  Oracle* o = static_cast<Oracle*>(p);
+  if (!o->keys.empty()) return -1;
+  o->color = on != 0;
+  return 0;
+}
+// Depth frame with its RGB image (H x W x 3 bytes).
+int64_t vho_integrate_rgb(void* p, const uint16_t* depth, const uint8_t* rgb, float scale, const float* c2w) {
+  Oracle* o = static_cast<Oracle*>(p);
+  if (!o->color || !rgb) return -1;
+  integrate(*o, depth, scale, c2w, rgb);
+  return (int64_t)o->touched.size();
+}
+// Colours of all blocks in insertion order (512 x 3 bytes per block).
+int64_t vho_export_colors(void* p, uint8_t* rgb, int64_t cap) {
+  Oracle* o = static_cast<Oracle*>(p);
+  const int64_t n = (int64_t)o->keys.size();
+  if (!o->color || cap < n) return n;
+  if (rgb) std::memcpy(rgb, o->rgb8.data(), (size_t)n * 512 * 3);
+  return n;
 }
 int64_t vho_num_blocks(void* p) { return (int64_t)static_cast<Oracle*>(p)->keys.size(); }
 
@@ -500,7 +553,7 @@ int64_t vho_export(void* p, int32_t* xyz, float* s, float* w, double* s64, doubl
 }
 
 // Marching cubes over every block; triangles ordered by cube then by table position.
-int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) {
+int64_t vho_mc_rgb(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap) {
   Oracle* o = static_cast<Oracle*>(p);
   std::vector<int64_t> order(o->keys.begin(), o->keys.end());
   std::sort(order.begin(), order.end());
@@ -520,10 +573,12 @@ int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) {
     for (const Tri& t : v) {
       if (xyz) std::memcpy(xyz + 9 * m, t.p, 36);
       if (cubes) std::memcpy(cubes + 4 * m, t.c, 16);
+      if (rgb) std::memcpy(rgb + 9 * m, t.rgb, 9);
       ++m;
     }
   return n;
 }
+int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) { return vho_mc_rgb(p, xyz, cubes, nullptr, cap); }
 
 // Triangles of one cube from 8 given corner values (all observed), lattice origin (0,0,0),
 // vox = 1: for exhaustive checks of the 256 cases.
@@ -604,5 +659,10 @@ int vho_band(void* p, const float* c2w, int u, int v, float d, int32_t* xyz, int
 uint16_t vho_f32_to_f16(float f) { return to_half(f); }
 float vho_f16_to_f32(uint16_t h) { return from_half(h); }
 void vho_vertex_interp(const float* a, const float* b, float va, float vb, float* out) { interp(a, b, va, vb, out); }
+void vho_vertex_interp_rgb(const float* a, const float* b, float va, float vb, const uint8_t* ca, const uint8_t* cb,
+                           float* out, uint8_t* cout) {
+  interp(a, b, va, vb, out, ca, cb, cout);
+}
+void vho_color_update(uint8_t* c, float w_old, const uint8_t* px) { color_update(c, w_old, px); }
 
 }  // extern "C"

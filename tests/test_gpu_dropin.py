@@ -12,38 +12,50 @@ from conftest import ROOT
 pytestmark = pytest.mark.gpu
 
 
-def read_ascii_ply(path):
+def read_ascii_ply(path, with_rgb=False):
     with open(path) as f:
         lines = f.read().splitlines()
     end = lines.index("end_header")
     head = lines[:end]
     nv = int([l for l in head if l.startswith("element vertex")][0].split()[-1])
     nf = int([l for l in head if l.startswith("element face")][0].split()[-1])
-    v = np.array([list(map(float, l.split()[:3])) for l in lines[end + 1:end + 1 + nv]])
+    rows = [l.split() for l in lines[end + 1:end + 1 + nv]]
+    v = np.array([list(map(float, r[:3])) for r in rows])
     f = np.array([list(map(int, l.split()[1:])) for l in lines[end + 1 + nv:end + 1 + nv + nf]])
+    if with_rgb:
+        return v, f, np.array([list(map(int, r[3:6])) for r in rows], np.int64)
     return v, f
 
 
 def test_load_frames_example_writes_welded_ply(vh, wl, small_frames, tmp_path):
     exe = os.path.join(ROOT, "voxel-hashing-sdf_amd", "load_frames")
     out = tmp_path / "room.ply"
-    print("drop-in: load_frames, 10 frames", flush=True)
+    print("drop-in: load_frames, 10 frames with colour", flush=True)
     r = subprocess.run([exe, str(out), "10"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    v, f = read_ascii_ply(out)
+    v, f, vrgb = read_ascii_ply(out, with_rgb=True)
     assert len(v) > 1000 and len(f) > 1000
     assert f.min() >= 0 and f.max() < len(v)
     # the same frames through the Python binding: as many faces as triangles, and the welded
-    # vertices are the distinct triangle corners
+    # vertices are the distinct triangle corners, each with one of the colours the library
+    # gives that position
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
-    vol = vh.Volume(vh.config_from_workload(w, log2_buckets=20, pool_blocks=1 << 19))
+    scene = vh.Scene(w["scene"], wl.SEED)
+    vol = vh.Volume(vh.config_from_workload(w, log2_buckets=20, pool_blocks=1 << 19, color=1))
     for k in range(10):
-        vol.integrate(depth[k].astype(np.float32) * np.float32(0.001), poses[k])
-    xyz, _ = vol.extract_mesh()
+        vol.integrate(depth[k].astype(np.float32) * np.float32(0.001), poses[k], rgb=scene.rgb(k, poses[k]))
+    xyz, _, trgb = vol.extract_mesh(with_rgb=True)
     assert len(xyz) == len(f)
     corners = np.unique(xyz.reshape(-1, 3), axis=0)
     assert len(corners) == len(v)
+    colours = {}
+    for p, c in zip(xyz.reshape(-1, 3).astype(np.float32).view(np.uint32), trgb.reshape(-1, 3)):
+        colours.setdefault(p.tobytes(), set()).add(tuple(int(x) for x in c))
+    vkeys = v.astype(np.float32).view(np.uint32)
+    bad = sum(tuple(vrgb[i]) not in colours[vkeys[i].tobytes()] for i in range(len(v)))
+    assert bad == 0, f"{bad} PLY vertex colours differ from the library's"
+    assert len({tuple(c) for c in vrgb}) > 100  # textured, not constant
     # each edge of a closed patch is shared by two faces at most (welded, manifold soup)
     e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), 1)
     _, counts = np.unique(e, axis=0, return_counts=True)
@@ -53,9 +65,12 @@ def test_load_frames_example_writes_welded_ply(vh, wl, small_frames, tmp_path):
     vol.save_ply(str(bpath))
     raw = bpath.read_bytes()
     hdr_end = raw.index(b"end_header\n") + len(b"end_header\n")
+    assert b"property uchar red\nproperty uchar green\nproperty uchar blue\n" in raw[:hdr_end]
     nv = len(xyz) * 3
-    verts = np.frombuffer(raw[hdr_end:hdr_end + nv * 12], np.float32).reshape(-1, 9)
-    assert np.array_equal(np.sort(verts.view(np.uint32), 0), np.sort(xyz.view(np.uint32), 0))
+    rec = np.frombuffer(raw[hdr_end:hdr_end + nv * 15], np.dtype([("p", "<f4", 3), ("c", "u1", 3)]))
+    verts = rec["p"].reshape(-1, 9)
+    assert np.array_equal(verts.view(np.uint32), xyz.view(np.uint32))
+    assert np.array_equal(rec["c"].reshape(-1, 9), trgb)
     vol.close()
 
 

@@ -58,3 +58,43 @@ def test_noise_and_dropout_model(vh, wl):
     sigma = 0.0012 + 0.0019 * (z - 0.4) ** 2
     e = (noisy[m] - z) / np.sqrt(sigma ** 2 + 0.0005 ** 2 / 3)  # + mm quantisation
     assert abs(e.mean()) < 0.02 and abs(e.std() - 1.0) < 0.05
+
+
+def test_rgb_frames_are_a_world_checkerboard_of_surface_colours(vh):
+    """vh_scene_render_rgb: each surface has one base colour (channels 48..239), dark cells
+    carry floor(5/8) of it, and the cell parity of a pixel's colour is the parity of its
+    world point (from the clean depth and the pose) on the 10 cm lattice."""
+    sc = vh.Scene(1, 20240611)
+    f = 33  # three surfaces in view
+    T = sc.pose(f).astype(np.float64)
+    rgb = sc.rgb(f).astype(np.int64)
+    z = sc.clean_depth(f).astype(np.float64)
+    H, W = z.shape
+    fx, fy, cx, cy = sc.intr
+    u, v = np.meshgrid(np.arange(W), np.arange(H))
+    d = np.stack([(u - cx) / fx, (v - cy) / fy, np.ones_like(z)], -1)
+    p = d @ T[:3, :3].T * z[..., None] + T[:3, 3]
+    # coordinates on a room face lie on a cell plane: they count as that plane's cell; the
+    # others must be clear of cell planes (the FP32 depth moves a point by ~1e-6 m)
+    q = p * 10.0
+    ext = np.array(sc.extent, np.float64) * 10.0
+    on_face = (np.abs(q) < 1e-3) | (np.abs(q - ext) < 1e-3)
+    dist = np.abs(q - np.rint(q))
+    clear = (on_face | (dist > 1e-3)).all(-1) & (on_face.sum(-1) <= 1) & (z > 0)  # not at room edges
+    cell = np.where(on_face, np.rint(q), np.floor(q)).astype(np.int64).sum(-1)
+    key = rgb[..., 0] << 16 | rgb[..., 1] << 8 | rgb[..., 2]
+    colours = np.unique(key[clear])
+    assert 2 <= len(colours) <= 2 * (6 + sc.n_spheres + sc.n_boxes)
+
+    def darken(c):
+        return ((c >> 16) * 5 // 8) << 16 | (((c >> 8) & 255) * 5 // 8) << 8 | (c & 255) * 5 // 8
+
+    # base colours are the ones whose dark shade also appears (both cell parities are seen)
+    seen = {int(c) for c in colours}
+    light = {c for c in seen if darken(c) in seen and all(48 <= (c >> s) & 255 <= 239 for s in (0, 8, 16))}
+    dark = {darken(c) for c in light}
+    assert len(light) >= 3 and not (light & dark) and seen == light | dark
+    is_dark = np.isin(key, list(dark))
+    assert np.array_equal(is_dark[clear], (cell[clear] & 1) == 1)
+    # a pose moves the image but not the colours of the world
+    assert np.array_equal(sc.rgb(f, sc.pose(f)), sc.rgb(f))

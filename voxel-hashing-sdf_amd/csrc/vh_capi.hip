@@ -81,12 +81,15 @@ struct vh_volume {
   uint32_t* stamps = nullptr;
   unsigned long long* block_keys = nullptr;
   void* voxels = nullptr;
+  uint32_t* colors = nullptr;  // colour volumes: one u32 per voxel
   uint32_t* touched = nullptr;
   uint32_t stripe_cap = 0;
   FrameCtl* ctl = nullptr;
   // staged host input (one frame, both formats) and its pose
   void* stage_depth = nullptr;
   float* stage_pose = nullptr;
+  uint8_t* stage_rgb = nullptr;  // colour volumes: W*H*3
+  const uint8_t* seq_rgb = nullptr;  // RGB of the bound sequence (vh_bind_sequence_rgb)
   // bound device sequence
   const void* seq_depth = nullptr;
   const float* seq_pose = nullptr;
@@ -106,6 +109,7 @@ struct vh_volume {
   static constexpr int kRing = 4;
   hipStream_t copy_stream = nullptr;
   void* ring_depth = nullptr;       // kRing frames of W*H elements
+  uint8_t* ring_rgb = nullptr;      // colour volumes: kRing frames of W*H*3 bytes
   float* ring_pose = nullptr;       // kRing x 16 (device)
   float* ring_pose_host = nullptr;  // kRing x 16 (page-locked)
   hipEvent_t ring_copied[kRing] = {}, ring_done[kRing] = {};
@@ -125,6 +129,7 @@ struct vh_volume {
   // marching-cubes output buffer (grown on demand) and scan scratch
   float* mc_tris = nullptr;
   int32_t* mc_cubes = nullptr;
+  uint8_t* mc_rgb = nullptr;       // colour volumes
   int64_t mc_cap = 0;
   uint32_t* mc_counts = nullptr;   // pool_blocks
   uint32_t* mc_chunks = nullptr;   // pool_blocks / kScanChunk + 1
@@ -157,18 +162,21 @@ FrameArgs frame_args(vh_volume* v, int which) {
     a.n_seq = vh_volume::kRing;
     a.cursor_id = 2;
     a.depth_scale = which == 6 ? v->ring_scale : 1.0f;
+    a.rgb = v->ring_rgb;
   } else if (which >= 2) {
     a.depth = v->seq_depth;
     a.poses = which == 3 ? v->trk_out : v->seq_pose;
     a.n_seq = (uint32_t)v->seq_n;
     a.cursor_id = 1;
     a.depth_scale = v->seq_format == VH_DEPTH_U16 ? v->seq_scale : 1.0f;
+    a.rgb = v->seq_rgb;
   } else {
     a.depth = v->stage_depth;
     a.poses = v->stage_pose;
     a.n_seq = 1;
     a.cursor_id = 0;
     a.depth_scale = 1.0f;  // the caller sets the uint16 scale of a staged frame
+    a.rgb = v->stage_rgb;
   }
   a.W = c.width;
   a.H = c.height;
@@ -187,6 +195,8 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.touched = v->touched;
   a.stripe_cap = v->stripe_cap;
   a.voxels = v->voxels;
+  a.colors = v->colors;
+  if (!a.colors) a.rgb = nullptr;
   a.ctl = v->ctl;
   return a;
 }
@@ -401,6 +411,9 @@ int validate(const vh_config* c) {
     return fail(VH_ERR_INVALID_ARG, "FP16 storage needs weight_cap in 1..65535");
   if (c->storage == VH_STORAGE_F16R && (c->weight_cap < 1 || c->weight_cap > 255))
     return fail(VH_ERR_INVALID_ARG, "F16R storage needs weight_cap in 1..255");
+  if (c->color != 0 && c->color != 1) return fail(VH_ERR_INVALID_ARG, "color must be 0 or 1");
+  for (int32_t r : c->reserved)
+    if (r) return fail(VH_ERR_INVALID_ARG, "reserved fields must be zero");
   return VH_OK;
 }
 
@@ -433,14 +446,14 @@ void free_volume(vh_volume* v) {
   void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
-                  v->ring_depth, v->ring_pose};
+                  v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
   delete v;
 }
 
-int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
+int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t capacity, int64_t* n_tris) {
   if (!v->mc_counts) {
     int rc;
     if ((rc = dev_alloc(&v->mc_counts, (size_t)v->cfg.pool_blocks * 4)) ||
@@ -463,6 +476,8 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64
   a.total = v->mc_total;
   a.tris = d_xyz;
   a.cubes = d_cubes;
+  a.colors = v->colors;
+  a.rgb = d_rgb;
   a.capacity = (uint32_t)std::min<int64_t>(capacity, 0xffffffffll);
   uint32_t cnt = 0;
   if (a.n_blocks) {
@@ -479,13 +494,16 @@ int ensure_mc_buffer(vh_volume* v, int64_t cap) {
   if (cap <= v->mc_cap) return VH_OK;
   if (v->mc_tris) (void)hipFree(v->mc_tris);
   if (v->mc_cubes) (void)hipFree(v->mc_cubes);
+  if (v->mc_rgb) (void)hipFree(v->mc_rgb);
   v->mc_tris = nullptr;
   v->mc_cubes = nullptr;
+  v->mc_rgb = nullptr;
   v->mc_cap = 0;
   int rc = dev_alloc(&v->mc_tris, (size_t)cap * 9 * sizeof(float));
   if (rc) return rc;
   rc = dev_alloc(&v->mc_cubes, (size_t)cap * 4 * sizeof(int32_t));
   if (rc) return rc;
+  if (v->colors && (rc = dev_alloc(&v->mc_rgb, (size_t)cap * 9))) return rc;
   v->mc_cap = cap;
   return VH_OK;
 }
@@ -560,7 +578,8 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
       (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||
       (rc = dev_alloc(&v->touched, (size_t)2 * kStripes * v->stripe_cap * 4)) ||
       (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
-      (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float)))) {
+      (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float))) ||
+      (cfg->color && ((rc = dev_alloc(&v->colors, pool * kB3 * 4)) || (rc = dev_alloc(&v->stage_rgb, frame_px * 3))))) {
     std::string msg = g_last_error;
     free_volume(v);
     return fail(rc, msg);
@@ -610,9 +629,13 @@ int vh_integrate(vh_volume* v, const vh_frame* f) {
   const size_t row = (size_t)v->cfg.width * elem;
   const size_t pitch = f->pitch ? (size_t)f->pitch : row;
   if (pitch < row) return fail(VH_ERR_INVALID_ARG, "pitch smaller than a row");
+  if (v->colors && !f->rgb) return fail(VH_ERR_INVALID_ARG, "a colour volume needs the frame's RGB image");
   VH_HIP(hipMemcpy2DAsync(v->stage_depth, row, f->depth, pitch, row, (size_t)v->cfg.height, hipMemcpyHostToDevice,
                           v->stream));
   VH_HIP(hipMemcpyAsync(v->stage_pose, f->c2w, 16 * sizeof(float), hipMemcpyHostToDevice, v->stream));
+  if (v->colors)
+    VH_HIP(hipMemcpyAsync(v->stage_rgb, f->rgb, (size_t)v->cfg.width * v->cfg.height * 3, hipMemcpyHostToDevice,
+                          v->stream));
   const int which = f->depth_format == VH_DEPTH_U16 ? 0 : 1;
   // the staged uint16 graph bakes the depth scale; re-capture when it changes
   if (which == 0 && f->depth_scale != v->graph_scale) {
@@ -640,10 +663,12 @@ int vh_integrate_async(vh_volume* v, const vh_frame* f) {
   const size_t row = (size_t)v->cfg.width * elem;
   const size_t pitch = f->pitch ? (size_t)f->pitch : row;
   if (pitch < row) return fail(VH_ERR_INVALID_ARG, "pitch smaller than a row");
+  if (v->colors && !f->rgb) return fail(VH_ERR_INVALID_ARG, "a colour volume needs the frame's RGB image");
+  const size_t frame_px = (size_t)v->cfg.width * v->cfg.height;
   if (!v->copy_stream) {
-    const size_t frame_px = (size_t)v->cfg.width * v->cfg.height;
     if ((rc = dev_alloc(&v->ring_depth, (size_t)R * frame_px * 4)) ||
-        (rc = dev_alloc(&v->ring_pose, (size_t)R * 16 * sizeof(float))))
+        (rc = dev_alloc(&v->ring_pose, (size_t)R * 16 * sizeof(float))) ||
+        (v->colors && (rc = dev_alloc(&v->ring_rgb, (size_t)R * frame_px * 3))))
       return rc;
     VH_HIP(hipHostMalloc((void**)&v->ring_pose_host, (size_t)R * 16 * sizeof(float), hipHostMallocDefault));
     for (int k = 0; k < R; ++k) {
@@ -669,6 +694,9 @@ int vh_integrate_async(vh_volume* v, const vh_frame* f) {
                           v->copy_stream));
   VH_HIP(hipMemcpyAsync(v->ring_pose + 16 * slot, v->ring_pose_host + 16 * slot, 16 * sizeof(float),
                         hipMemcpyHostToDevice, v->copy_stream));
+  if (v->colors)
+    VH_HIP(hipMemcpyAsync(v->ring_rgb + (size_t)slot * frame_px * 3, f->rgb, frame_px * 3, hipMemcpyHostToDevice,
+                          v->copy_stream));
   VH_HIP(hipEventRecord(v->ring_copied[slot], v->copy_stream));
   VH_HIP(hipStreamWaitEvent(v->stream, v->ring_copied[slot], 0));
   if ((rc = launch_frame(v, which, f->depth_scale))) return rc;
@@ -693,7 +721,9 @@ int vh_bind_sequence(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   v->seq_format = fmt;
   v->seq_scale = scale;
   v->track = false;
-  if (!same) drop_graphs(v, 4 | 8 | 16 | 32);  // the captured graphs hold the sequence pointers
+  // the captured graphs hold the sequence pointers
+  if (!same || v->seq_rgb) drop_graphs(v, 4 | 8 | 16 | 32);
+  v->seq_rgb = nullptr;
   const uint32_t zero = 0;
   VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
@@ -748,16 +778,34 @@ int vh_bind_tracking(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   v->trk_stride = stride;
   v->trk_max_depth = zmax;
   v->trk_increment = incr;
-  if (!same) drop_graphs(v, 4 | 8 | 16 | 32);
+  if (!same || v->seq_rgb) drop_graphs(v, 4 | 8 | 16 | 32);
+  v->seq_rgb = nullptr;
   const uint32_t zero = 0;
   VH_HIP(hipMemcpyAsync(&v->ctl->cursor[1], &zero, sizeof zero, hipMemcpyHostToDevice, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
   return VH_OK;
 }
 
+int vh_bind_sequence_rgb(vh_volume* v, const uint8_t* d_rgb) {
+  if (!v || !d_rgb) return fail(VH_ERR_INVALID_ARG, "null argument");
+  if (!v->colors) return fail(VH_ERR_INVALID_ARG, "the volume has no colour");
+  std::lock_guard<std::mutex> lk(v->mu);
+  if (!v->seq_depth) return fail(VH_ERR_INVALID_ARG, "no sequence bound");
+  int rc = set_device(v);
+  if (rc) return rc;
+  if (v->seq_rgb != d_rgb) {
+    VH_HIP(hipStreamSynchronize(v->stream));
+    drop_graphs(v, 4 | 8 | 16 | 32);
+    v->seq_rgb = d_rgb;
+  }
+  return VH_OK;
+}
+
 int vh_integrate_next(vh_volume* v, int32_t n) {
   if (!v || n < 0) return fail(VH_ERR_INVALID_ARG, "bad argument");
   if (!v->seq_depth) return fail(VH_ERR_INVALID_ARG, "no sequence bound");
+  if (v->colors && !v->seq_rgb)
+    return fail(VH_ERR_INVALID_ARG, "a colour volume needs the sequence's RGB images (vh_bind_sequence_rgb)");
   std::lock_guard<std::mutex> lk(v->mu);
   int rc = set_device(v);
   if (rc) return rc;
@@ -828,12 +876,18 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
 
 void* vh_stream(vh_volume* v) { return v ? (void*)v->stream : nullptr; }
 
-int vh_extract_mesh_device(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
+int vh_extract_mesh_device_rgb(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t capacity,
+                               int64_t* n_tris) {
   if (!v || !n_tris || capacity < 0 || (capacity > 0 && !d_xyz)) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (d_rgb && !v->colors) return fail(VH_ERR_INVALID_ARG, "the volume has no colour");
   std::lock_guard<std::mutex> lk(v->mu);
   int rc = set_device(v);
   if (rc) return rc;
-  return run_mc(v, d_xyz, d_cubes, capacity, n_tris);
+  return run_mc(v, d_xyz, d_cubes, d_rgb, capacity, n_tris);
+}
+
+int vh_extract_mesh_device(vh_volume* v, float* d_xyz, int32_t* d_cubes, int64_t capacity, int64_t* n_tris) {
+  return vh_extract_mesh_device_rgb(v, d_xyz, d_cubes, nullptr, capacity, n_tris);
 }
 
 int vh_extract_mesh(vh_volume* v, vh_mesh* out) {
@@ -844,24 +898,25 @@ int vh_extract_mesh(vh_volume* v, vh_mesh* out) {
   if (rc) return rc;
   int64_t n = 0;
   if ((rc = ensure_mc_buffer(v, std::max<int64_t>(v->mc_cap, 1 << 20)))) return rc;
-  rc = run_mc(v, v->mc_tris, v->mc_cubes, v->mc_cap, &n);
+  rc = run_mc(v, v->mc_tris, v->mc_cubes, v->mc_rgb, v->mc_cap, &n);
   if (rc == VH_ERR_TRUNCATED) {
     if ((rc = ensure_mc_buffer(v, n + (n >> 3) + 1024))) return rc;
-    rc = run_mc(v, v->mc_tris, v->mc_cubes, v->mc_cap, &n);
+    rc = run_mc(v, v->mc_tris, v->mc_cubes, v->mc_rgb, v->mc_cap, &n);
   }
   if (rc) return rc;
   out->n_tris = n;
-  out->xyz = static_cast<float*>(std::malloc((size_t)std::max<int64_t>(n, 1) * 9 * sizeof(float)));
-  out->cubes = static_cast<int32_t*>(std::malloc((size_t)std::max<int64_t>(n, 1) * 4 * sizeof(int32_t)));
-  if (!out->xyz || !out->cubes) {
-    std::free(out->xyz);
-    std::free(out->cubes);
-    std::memset(out, 0, sizeof *out);
+  const size_t nt = (size_t)std::max<int64_t>(n, 1);
+  out->xyz = static_cast<float*>(std::malloc(nt * 9 * sizeof(float)));
+  out->cubes = static_cast<int32_t*>(std::malloc(nt * 4 * sizeof(int32_t)));
+  if (v->colors) out->rgb = static_cast<uint8_t*>(std::malloc(nt * 9));
+  if (!out->xyz || !out->cubes || (v->colors && !out->rgb)) {
+    vh_mesh_free(out);
     return fail(VH_ERR_OOM, "host allocation failed");
   }
   if (n) {
     VH_HIP(hipMemcpy(out->xyz, v->mc_tris, (size_t)n * 9 * sizeof(float), hipMemcpyDeviceToHost));
     VH_HIP(hipMemcpy(out->cubes, v->mc_cubes, (size_t)n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (out->rgb) VH_HIP(hipMemcpy(out->rgb, v->mc_rgb, (size_t)n * 9, hipMemcpyDeviceToHost));
   }
   return VH_OK;
 }
@@ -870,6 +925,7 @@ int vh_mesh_free(vh_mesh* m) {
   if (!m) return fail(VH_ERR_INVALID_ARG, "null mesh");
   std::free(m->xyz);
   std::free(m->cubes);
+  std::free(m->rgb);
   std::memset(m, 0, sizeof *m);
   return VH_OK;
 }
@@ -886,9 +942,21 @@ int vh_save_ply(vh_volume* v, const char* path) {
   }
   std::fprintf(f,
                "ply\nformat binary_little_endian 1.0\nelement vertex %lld\nproperty float x\nproperty float y\n"
-               "property float z\nelement face %lld\nproperty list uchar int vertex_indices\nend_header\n",
-               (long long)(m.n_tris * 3), (long long)m.n_tris);
-  bool ok = std::fwrite(m.xyz, sizeof(float), (size_t)m.n_tris * 9, f) == (size_t)m.n_tris * 9;
+               "property float z\n%selement face %lld\nproperty list uchar int vertex_indices\nend_header\n",
+               (long long)(m.n_tris * 3), m.rgb ? "property uchar red\nproperty uchar green\nproperty uchar blue\n" : "",
+               (long long)m.n_tris);
+  bool ok = true;
+  if (m.rgb) {
+    // vertex records of 15 bytes: x, y, z, r, g, b
+    std::vector<unsigned char> rec((size_t)m.n_tris * 3 * 15);
+    for (size_t i = 0; i < (size_t)m.n_tris * 3; ++i) {
+      std::memcpy(&rec[15 * i], m.xyz + 3 * i, 12);
+      std::memcpy(&rec[15 * i + 12], m.rgb + 3 * i, 3);
+    }
+    ok = std::fwrite(rec.data(), 1, rec.size(), f) == rec.size();
+  } else {
+    ok = std::fwrite(m.xyz, sizeof(float), (size_t)m.n_tris * 9, f) == (size_t)m.n_tris * 9;
+  }
   for (int64_t t = 0; ok && t < m.n_tris; ++t) {
     unsigned char c = 3;
     int32_t idx[3] = {(int32_t)(3 * t), (int32_t)(3 * t + 1), (int32_t)(3 * t + 2)};
@@ -934,6 +1002,26 @@ int vh_export_blocks(vh_volume* v, int32_t* keys, float* tsdf, float* weight, in
       }
     }
   }
+  return VH_OK;
+}
+
+int vh_export_block_colors(vh_volume* v, uint8_t* rgb, int64_t capacity, int64_t* n) {
+  if (!v || !n || capacity < 0) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (!v->colors) return fail(VH_ERR_INVALID_ARG, "the volume has no colour");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  FrameCtl c{};
+  if ((rc = read_ctl(v, &c))) return rc;
+  const int64_t nb = std::min<int64_t>(c.pool_count, v->cfg.pool_blocks);
+  *n = nb;
+  if (nb > capacity) return fail(VH_ERR_TRUNCATED, "export buffer too small");
+  if (nb == 0 || !rgb) return VH_OK;
+  const size_t nv = (size_t)nb * kB3;
+  std::vector<uint32_t> raw(nv);
+  VH_HIP(hipMemcpy(raw.data(), v->colors, nv * 4, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nv; ++i)
+    for (int k = 0; k < 3; ++k) rgb[3 * i + k] = (uint8_t)(raw[i] >> (8 * k));
   return VH_OK;
 }
 

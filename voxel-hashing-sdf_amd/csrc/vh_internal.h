@@ -8,6 +8,7 @@
 //   stamps[4 * 2^log2_buckets]      u32 per slot: last frame that touched the block;
 //   block_keys[pool_blocks]         u64 key of each pool block (pool order);
 //   voxels[pool_blocks * 512]       FP32: float2 {tsdf, weight}; FP16: u32 {f16 tsdf | u16 w << 16};
+//   colors[pool_blocks * 512]       (colour volumes only) u32 {r | g << 8 | b << 16};
 //   touched[2][kStripes * stripe_cap] u32 slot ids of the blocks crossed by a frame's bands,
 //                                   in kStripes sub-lists with separate counters, one list
 //                                   per frame parity;
@@ -93,6 +94,10 @@ struct FrameArgs {
   uint32_t stripe_cap;
   void* voxels;
   FrameCtl* ctl;
+  // colour (both null when the volume has no colour): input frames of W*H*3 bytes (RGB,
+  // frame f at rgb + f*W*H*3) and the colour pool, one u32 {r | g << 8 | b << 16} per voxel
+  const uint8_t* rgb;
+  uint32_t* colors;
 };
 
 struct McArgs {
@@ -108,6 +113,8 @@ struct McArgs {
   float* tris;         // capacity * 9
   int32_t* cubes;      // capacity * 4, may be null
   uint32_t capacity;
+  const uint32_t* colors;  // colour pool (null: no colour)
+  uint8_t* rgb;            // capacity * 9 (r,g,b per vertex), may be null
 };
 constexpr int kScanChunk = 1024;
 

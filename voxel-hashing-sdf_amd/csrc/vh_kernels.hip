@@ -282,12 +282,27 @@ __device__ __forceinline__ void mean_update(float& s, float& w, float dist, floa
   w = (cap > 0.0f && w_new > cap) ? cap : w_new;
 }
 
+// Colour update of the reference (src/tsdf.cu:743-745): per channel (c * w_old + rgb) / w_new
+// in FP32 (the product and sum are exact integers below 2^24; the quotient correctly
+// rounded by div_rn), truncated to 8 bits. w_old is the stored weight before this update.
+__device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, const uint8_t* px) {
+  const float w_new = w_old + 1.0f;
+  uint32_t out = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float v = div_rn(__builtin_fmaf((float)((c >> (8 * k)) & 255u), w_old, (float)px[k]), w_new);
+    out |= (uint32_t)(uint8_t)v << (8 * k);
+  }
+  return out;
+}
+
 // One wave per touched block: lane l owns voxels l + 64k (x = k, y = l >> 3, z = l & 7), so
 // each of its 8 loads is one coalesced 512-byte wave access and all 8 are in flight at once.
 // Worker w of `workers` (one wave each) takes an equal share of frame `frame`'s touched list
 // (input slot fs), its 64 stripes seen as one index space; workers synchronise only within
 // their own wave.
-template <int kStorage, bool kU16>
+// kColor: the colour pool is updated from the frame's RGB image at the same pixels.
+template <int kStorage, bool kU16, bool kColor>
 __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t frame, uint32_t fs,
                                  unsigned long long* s_val, unsigned long long* s_key) {
   using IO = VoxelIO<kStorage>;
@@ -313,6 +328,7 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
   const size_t elem = kU16 ? 2 : 4;
   const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
   auto dep = [&](int pix) { return depth_at<kU16>(base, a.depth_scale, pix); };
+  const uint8_t* rgb = kColor ? a.rgb + (size_t)fs * a.W * a.H * 3 : nullptr;
   BlockFrame bf;
   block_frame(T, a.vox, bf);
 
@@ -358,6 +374,12 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
       Word old[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) old[k] = fresh ? IO::zero() : vp[64 * k];
+      uint32_t* cp = kColor ? a.colors + (size_t)idx * kB3 + lane : nullptr;
+      uint32_t cold[8];
+      if constexpr (kColor) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cold[k] = fresh ? 0u : cp[64 * k];
+      }
       float cb[3], cl[3];
       block_origin_cam(T, a.vox, bx, by, bz, cb);
       column_cam(bf, cb, lane >> 3, lane & 7, cl);
@@ -381,6 +403,10 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         const bool ok = truncate_obs(pix[k], dv[k], zc[k], a.max_depth, a.mu, a.inv_mu, dist);
         float sv, wv;
         IO::unpack(old[k], sv, wv);
+        if constexpr (kColor) {
+          if (ok) cp[64 * k] = color_update(cold[k], wv, rgb + 3 * (size_t)pix[k]);
+          else if (fresh) cp[64 * k] = 0u;
+        }
         mean_update(sv, wv, dist, a.cap);
         if (ok) vp[64 * k] = IO::pack(sv, wv);
         else if (fresh) vp[64 * k] = IO::zero();
@@ -432,12 +458,12 @@ __device__ void frame_done(const FrameArgs& a, uint32_t groups, uint32_t frame) 
 
 constexpr int kIntegrateWaves = 4;  // independent workers per workgroup
 
-template <int kStorage, bool kU16>
+template <int kStorage, bool kU16, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
   const int wave = threadIdx.x >> 6;
   const uint32_t frame = a.ctl->frame_id;
-  integrate_worker<kStorage, kU16>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, frame,
+  integrate_worker<kStorage, kU16, kColor>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, frame,
                                    frame_slot(a), s_val[wave], s_key[wave]);
   frame_done(a, gridDim.x, frame);
 }
@@ -447,7 +473,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a)
 // (the allocation is latency-bound, the integration throughput-bound); the rest take the
 // remaining role. Every workgroup counts towards the close of frame f, so the frame id and
 // cursor the allocation workgroups read at their start cannot advance under them.
-template <int kStorage, bool kU16>
+template <int kStorage, bool kU16, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate_alloc(FrameArgs a, uint32_t int_groups,
                                                                          uint32_t alloc_groups) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
@@ -467,7 +493,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate_alloc(FrameA
     alloc_tile<kU16>(a, idx, frame + 1u, (cur + 1u) % a.n_seq);
   } else {
     const int wave = threadIdx.x >> 6;
-    integrate_worker<kStorage, kU16>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, frame,
+    integrate_worker<kStorage, kU16, kColor>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, frame,
                                      cur % a.n_seq, s_val[wave], s_key[wave]);
   }
   frame_done(a, gridDim.x, frame);
@@ -496,9 +522,11 @@ __device__ __forceinline__ void mc_load_counts(McShared& S) {
 // of its 7 neighbours (one hash lookup each) are staged as a 9^3 tile in LDS: eight
 // coalesced rows of the block itself, one load per face, one for the three edges and the
 // corner, so no lane divides an index. A corner is observed iff its block exists and its
-// weight is > 0; unobserved corners hold NaN.
-template <int kStorage>
-__device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& S, int& gx0, int& gy0, int& gz0) {
+// weight is > 0; unobserved corners hold NaN. kColor: the colour pool words of the same
+// voxels go to col (9^3, same order) with the same accesses.
+template <int kStorage, bool kColor>
+__device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& S, uint32_t* col, int& gx0, int& gy0,
+                                         int& gz0) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   const int lane = threadIdx.x;
@@ -542,6 +570,22 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
   else if (lane < 24) { enb = S.nb[3]; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; }
   else if (lane == 24) { enb = S.nb[7]; esrc = 0; edst = kHalo3 - 1; }
   const Word we = enb != kNoBlock ? vox[(size_t)enb * kB3 + esrc] : IO::zero();
+  if constexpr (kColor) {
+    const uint32_t* cp = a.colors;
+    uint32_t c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = cp[(size_t)b * kB3 + lane + 64 * k];
+    const uint32_t cfx = nfx != kNoBlock ? cp[(size_t)nfx * kB3 + lane] : 0u;
+    const uint32_t cfy = nfy != kNoBlock ? cp[(size_t)nfy * kB3 + hi * 64 + lo] : 0u;
+    const uint32_t cfz = nfz != kNoBlock ? cp[(size_t)nfz * kB3 + hi * 64 + lo * 8] : 0u;
+    const uint32_t ce = enb != kNoBlock ? cp[(size_t)enb * kB3 + esrc] : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) col[k * 81 + hi * 9 + lo] = c[k];
+    col[8 * 81 + hi * 9 + lo] = cfx;
+    col[hi * 81 + 8 * 9 + lo] = cfy;
+    col[hi * 81 + lo * 9 + 8] = cfz;
+    if (lane < 25) col[edst] = ce;
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) S.val[k * 81 + hi * 9 + lo] = value(w[k]);
   S.val[8 * 81 + hi * 9 + lo] = nfx != kNoBlock ? value(wfx) : nan;
@@ -597,11 +641,13 @@ __device__ __forceinline__ uint32_t mc_classify(McShared& S) {
 
 // The block's (cube, table triangle) items in order lane, x, t, in rounds of kItemCap: each
 // lane lists its own items of the round in LDS, then the wave takes them one per lane.
-// Emit receives (valid, cube x/y/z, case, t, P[9]) per item slot of a 64-item step; valid
-// is false for empty slots and for degenerate triangles (two equal vertices; the
-// reference's test at src/tsdf.cu:1055-1058, with its typo corrected).
-template <class Emit>
-__device__ __forceinline__ void mc_items(McShared& S, uint32_t total, int gx0, int gy0, int gz0, Emit emit) {
+// Emit receives (valid, cube x/y/z, case, t, P[9], C[3]) per item slot of a 64-item step;
+// valid is false for empty slots and for degenerate triangles (two equal vertices; the
+// reference's test at src/tsdf.cu:1055-1058, with its typo corrected). kColor: C holds the
+// vertex colours interpolated from the colour tile col; otherwise zeros.
+template <bool kColor, class Emit>
+__device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint32_t total, int gx0, int gy0, int gz0,
+                                         Emit emit) {
   const int lane = threadIdx.x;
   const uint32_t my0 = S.lane_off[lane], my1 = S.lane_off[lane + 1];
   for (uint32_t r0 = 0; r0 < total; r0 += kItemCap) {
@@ -621,6 +667,7 @@ __device__ __forceinline__ void mc_items(McShared& S, uint32_t total, int gx0, i
       bool valid = false;
       int ci = 0, t = 0, cx = 0, cy = 0, cz = 0;
       float P[9];
+      uint32_t C[3] = {0u, 0u, 0u};
       if (item < r1) {
         const int it = S.items[item - r0];
         const int cube = it >> 3;
@@ -634,19 +681,24 @@ __device__ __forceinline__ void mc_items(McShared& S, uint32_t total, int gx0, i
         for (int j = 0; j < 3; ++j) {
           const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
           const int ca = edge_a(e), cb = edge_b(e);
-          const float va = S.val[(cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca))];
-          const float vb = S.val[(cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb))];
-          vertex_interp((float)(gx0 + cx + corner_dx(ca)), (float)(gy0 + cy + corner_dy(ca)),
-                        (float)(gz0 + cz + corner_dz(ca)), (float)(gx0 + cx + corner_dx(cb)),
-                        (float)(gy0 + cy + corner_dy(cb)), (float)(gz0 + cz + corner_dz(cb)), va, vb, P[3 * j],
-                        P[3 * j + 1], P[3 * j + 2]);
+          const int ia = (cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca));
+          const int ib = (cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb));
+          const float va = S.val[ia], vb = S.val[ib];
+          const float pax = (float)(gx0 + cx + corner_dx(ca)), pay = (float)(gy0 + cy + corner_dy(ca)),
+                      paz = (float)(gz0 + cz + corner_dz(ca)), pbx = (float)(gx0 + cx + corner_dx(cb)),
+                      pby = (float)(gy0 + cy + corner_dy(cb)), pbz = (float)(gz0 + cz + corner_dz(cb));
+          if constexpr (kColor)
+            vertex_interp_rgb(pax, pay, paz, pbx, pby, pbz, va, vb, col[ia], col[ib], P[3 * j], P[3 * j + 1],
+                              P[3 * j + 2], C[j]);
+          else
+            vertex_interp(pax, pay, paz, pbx, pby, pbz, va, vb, P[3 * j], P[3 * j + 1], P[3 * j + 2]);
         }
         const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
         const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
         const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
         valid = !(d01 || d12 || d20);
       }
-      emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P);
+      emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P, C);
     }
     __syncthreads();
   }
@@ -659,11 +711,13 @@ __global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
   mc_load_counts(S);
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
-    mc_stage<kStorage>(a, b, S, gx0, gy0, gz0);
+    mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
     const uint32_t total = mc_classify(S);
     uint32_t cnt = 0;
-    mc_items(S, total, gx0, gy0, gz0,
-             [&](bool valid, int, int, int, int, int, const float*) { cnt += (uint32_t)__popcll(__ballot(valid)); });
+    mc_items<false>(S, nullptr, total, gx0, gy0, gz0, [&](bool valid, int, int, int, int, int, const float*,
+                                                          const uint32_t*) {
+      cnt += (uint32_t)__popcll(__ballot(valid));
+    });
     if (lane == 0) a.counts[b] = cnt;
     __syncthreads();
   }
@@ -739,16 +793,18 @@ __global__ __launch_bounds__(256) void k_scan_top(McArgs a, uint32_t n_chunks) {
   if (tid == 0) *a.total = s_carry;
 }
 
-template <int kStorage>
+template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
   __shared__ McShared S;
+  __shared__ uint32_t s_col[kColor ? kHalo3 : 1];
   mc_load_counts(S);
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
-    mc_stage<kStorage>(a, b, S, gx0, gy0, gz0);
+    mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
     const uint32_t total = mc_classify(S);
     uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
-    mc_items(S, total, gx0, gy0, gz0, [&](bool valid, int x, int y, int z, int ci, int t, const float* P) {
+    mc_items<kColor>(S, s_col, total, gx0, gy0, gz0, [&](bool valid, int x, int y, int z, int ci, int t,
+                                                         const float* P, const uint32_t* C) {
       const uint64_t m = __ballot(valid);
       const uint32_t p = pos + lanes_below(m);
       if (valid && p < a.capacity) {
@@ -758,6 +814,11 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
         if (a.cubes) {
           int32_t* q = a.cubes + (size_t)p * 4;
           q[0] = x; q[1] = y; q[2] = z; q[3] = (ci << 8) | t;
+        }
+        if (kColor && a.rgb) {
+          uint8_t* o8 = a.rgb + (size_t)p * 9;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) o8[k] = (uint8_t)(C[k / 3] >> (8 * (k % 3)));
         }
       }
       pos += (uint32_t)__popcll(m);
@@ -1149,11 +1210,26 @@ static void with_storage(int storage, F f) {
   }
 }
 
+// Calls f(std::integral_constant<bool, u16>(), std::integral_constant<bool, colour>()).
+template <class F>
+static void with_input(bool u16, bool color, F f) {
+  using T = std::true_type;
+  using N = std::false_type;
+  if (u16) {
+    if (color) f(T(), T());
+    else f(T(), N());
+  } else {
+    if (color) f(N(), T());
+    else f(N(), N());
+  }
+}
+
 hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
   with_storage(storage, [&](auto S) {
-    const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
-    if (depth_u16) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), g, b, 0, st, a);
+    with_input(depth_u16 != 0, a.colors != nullptr, [&](auto U, auto C) {
+      const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
+      hipLaunchKernelGGL((k_integrate<decltype(S)::value, decltype(U)::value, decltype(C)::value>), g, b, 0, st, a);
+    });
   });
   return hipGetLastError();
 }
@@ -1162,9 +1238,11 @@ hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16
   const uint32_t ig = (uint32_t)(grid / kIntegrateWaves);
   const uint32_t ag = (uint32_t)(((a.W + 15) >> 4) * ((a.H + 15) >> 4));
   with_storage(storage, [&](auto S) {
-    const dim3 g(ig + ag), b(64 * kIntegrateWaves);
-    if (depth_u16) hipLaunchKernelGGL((k_integrate_alloc<decltype(S)::value, true>), g, b, 0, st, a, ig, ag);
-    else hipLaunchKernelGGL((k_integrate_alloc<decltype(S)::value, false>), g, b, 0, st, a, ig, ag);
+    with_input(depth_u16 != 0, a.colors != nullptr, [&](auto U, auto C) {
+      const dim3 g(ig + ag), b(64 * kIntegrateWaves);
+      hipLaunchKernelGGL((k_integrate_alloc<decltype(S)::value, decltype(U)::value, decltype(C)::value>), g, b, 0,
+                         st, a, ig, ag);
+    });
   });
   return hipGetLastError();
 }
@@ -1179,7 +1257,8 @@ hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, a, n_chunks);
   with_storage(storage, [&](auto S) {
-    hipLaunchKernelGGL(k_mc_write<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
+    if (a.colors) hipLaunchKernelGGL((k_mc_write<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL((k_mc_write<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
   });
   return hipGetLastError();
 }

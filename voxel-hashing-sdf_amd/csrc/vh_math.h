@@ -318,6 +318,35 @@ VH_HD void vertex_interp(float p1x, float p1y, float p1z, float p2x, float p2y, 
   z = p1z + mu * (p2z - p1z);
 }
 
+// VertexInterp with the colour lerp of the reference (src/tsdf.cu:1655-1657): a snapped vertex
+// takes its corner's colour; otherwise each channel is c1 + mu*(c2 - c1) (the difference is
+// an exact integer, the rest FP32), truncated to 8 bits.
+VH_HD void vertex_interp_rgb(float p1x, float p1y, float p1z, float p2x, float p2y, float p2z, float v1,
+                             float v2, uint32_t c1, uint32_t c2, float& x, float& y, float& z, uint32_t& c) {
+  if (near_iso(v1)) {
+    x = p1x; y = p1y; z = p1z; c = c1;
+    return;
+  }
+  if (near_iso(v2)) {
+    x = p2x; y = p2y; z = p2z; c = c2;
+    return;
+  }
+  if (near_iso(v1 - v2)) {
+    x = p1x; y = p1y; z = p1z; c = c1;
+    return;
+  }
+  const float mu = (0.0f - v1) / (v2 - v1);
+  x = p1x + mu * (p2x - p1x);
+  y = p1y + mu * (p2y - p1y);
+  z = p1z + mu * (p2z - p1z);
+  c = 0;
+  for (int k = 0; k < 3; ++k) {
+    const int a = (int)((c1 >> (8 * k)) & 255u), b = (int)((c2 >> (8 * k)) & 255u);
+    const float v = (float)a + mu * (float)(b - a);
+    c |= (uint32_t)(uint8_t)v << (8 * k);
+  }
+}
+
 // ---------------------------------------------------------------- trilinear weights
 // Corner k = (dx<<2)|(dy<<1)|dz of the lattice cell containing g; weight products are
 // formed as (wx*wy)*wz and sums run in k order, so CPU and GPU agree bit-for-bit.

@@ -172,14 +172,23 @@ void pose_of(const vh_scene& s, int f, float T[16]) {
   T[15] = 1.0f;
 }
 
-// Nearest positive hit along o + t d (d has unit camera-z, so t is camera depth).
-double cast(const vh_scene& s, const double o[3], const double d[3]) {
+// Nearest positive hit along o + t d (d has unit camera-z, so t is camera depth). *id (may
+// be null) receives the surface hit: 2a + side for the room's faces (axis a, side 1 = far),
+// 6 + k for sphere k, 6 + spheres + k for box k (walls first), -1 for none.
+double cast(const vh_scene& s, const double o[3], const double d[3], int* id = nullptr) {
   double best = 1e30;
+  int hit = -1;
   // room interior: exit distance
   for (int a = 0; a < 3; ++a) {
-    if (d[a] > 0.0) best = std::min(best, (s.ext[a] - o[a]) / d[a]);
-    else if (d[a] < 0.0) best = std::min(best, (0.0 - o[a]) / d[a]);
+    double t = 1e30;
+    if (d[a] > 0.0) t = (s.ext[a] - o[a]) / d[a];
+    else if (d[a] < 0.0) t = (0.0 - o[a]) / d[a];
+    if (t < best) {
+      best = t;
+      hit = 2 * a + (d[a] > 0.0);
+    }
   }
+  int k = 6;
   for (const Sphere& sp : s.spheres) {
     const double oc[3] = {o[0] - sp.c[0], o[1] - sp.c[1], o[2] - sp.c[2]};
     const double A = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -188,7 +197,11 @@ double cast(const vh_scene& s, const double o[3], const double d[3]) {
     const double disc = B * B - A * C;
     if (disc < 0.0) continue;
     const double t = (-B - std::sqrt(disc)) / A;
-    if (t > 1e-9 && t < best) best = t;
+    if (t > 1e-9 && t < best) {
+      best = t;
+      hit = k;
+    }
+    ++k;
   }
   for (const Box& b : s.boxes) {
     double tmin = -1e30, tmax = 1e30;
@@ -203,12 +216,35 @@ double cast(const vh_scene& s, const double o[3], const double d[3]) {
       tmin = std::max(tmin, t1);
       tmax = std::min(tmax, t2);
     }
-    if (miss || tmax < tmin || tmin <= 1e-9) continue;
-    if (tmin < best) best = tmin;
+    if (!(miss || tmax < tmin || tmin <= 1e-9) && tmin < best) {
+      best = tmin;
+      hit = k;
+    }
+    ++k;
   }
+  if (id) *id = best < 1e29 ? hit : -1;
   return best;
 }
 
+// Colour of surface id at world point p: a per-surface base colour (channels 48..239 from
+// the seed) on a 10 cm world checkerboard whose dark cells carry 5/8 of it. The room's faces
+// lie on cell planes (extents are multiples of 10 cm): the 1e-6 cell nudge puts their hits,
+// which land within rounding of the plane on either side, into one cell.
+void surface_rgb(const vh_scene& s, int id, const double p[3], uint8_t* out) {
+  if (id < 0) {
+    out[0] = out[1] = out[2] = 0;
+    return;
+  }
+  const uint64_t h = splitmix64(splitmix64(s.seed + 7ull) + (uint64_t)id);
+  long cell = 0;
+  for (int a = 0; a < 3; ++a) cell += (long)std::floor(p[a] * 10.0 + 1e-6);
+  for (int c = 0; c < 3; ++c) {
+    const int base = 48 + (int)((h >> (8 * c)) % 192u);
+    out[c] = (uint8_t)((cell & 1) ? base * 5 / 8 : base);
+  }
+}
+
+// out(pixel, depth, surface id, world hit point) per pixel of rows [r0, r1).
 template <class Out>
 void render_rows(const vh_scene& s, int f, const float* T, int r0, int r1, Out out) {
   double R[3][3], t[3];
@@ -221,7 +257,10 @@ void render_rows(const vh_scene& s, int f, const float* T, int r0, int r1, Out o
       const double xc = ((double)u - s.cx) / s.fx, yc = ((double)v - s.cy) / s.fy;
       double d[3];
       for (int r = 0; r < 3; ++r) d[r] = R[r][0] * xc + R[r][1] * yc + R[r][2];
-      out(v * s.W + u, cast(s, t, d));
+      int id = -1;
+      const double z = cast(s, t, d, &id);
+      const double p[3] = {t[0] + z * d[0], t[1] + z * d[1], t[2] + z * d[2]};
+      out(v * s.W + u, z, id, p);
     }
   }
 }
@@ -325,7 +364,7 @@ int vh_scene_render_u16(const vh_scene* s, int32_t f, const float* c2w, uint16_t
   float T[16];
   if (c2w) std::memcpy(T, c2w, sizeof T); else pose_of(*s, f, T);
   parallel_rows(s->H, [&](int r0, int r1) {
-    render_rows(*s, f, T, r0, r1, [&](int pix, double z) { depth[pix] = quantise(*s, f, pix, z); });
+    render_rows(*s, f, T, r0, r1, [&](int pix, double z, int, const double*) { depth[pix] = quantise(*s, f, pix, z); });
   });
   return 0;
 }
@@ -335,7 +374,19 @@ int vh_scene_render_clean(const vh_scene* s, int32_t f, const float* c2w, float*
   float T[16];
   if (c2w) std::memcpy(T, c2w, sizeof T); else pose_of(*s, f, T);
   parallel_rows(s->H, [&](int r0, int r1) {
-    render_rows(*s, f, T, r0, r1, [&](int pix, double z) { depth[pix] = z < 1e29 ? (float)z : 0.0f; });
+    render_rows(*s, f, T, r0, r1,
+                [&](int pix, double z, int, const double*) { depth[pix] = z < 1e29 ? (float)z : 0.0f; });
+  });
+  return 0;
+}
+
+int vh_scene_render_rgb(const vh_scene* s, int32_t f, const float* c2w, uint8_t* rgb) {
+  if (!s || !rgb || f < 0) return 1;
+  float T[16];
+  if (c2w) std::memcpy(T, c2w, sizeof T); else pose_of(*s, f, T);
+  parallel_rows(s->H, [&](int r0, int r1) {
+    render_rows(*s, f, T, r0, r1,
+                [&](int pix, double, int id, const double* p) { surface_rgb(*s, id, p, rgb + 3 * (size_t)pix); });
   });
   return 0;
 }

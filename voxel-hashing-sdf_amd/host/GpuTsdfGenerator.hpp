@@ -7,9 +7,12 @@
 // (tsdf.cuh:609-610) with metric float depth and a row-major camera-to-world matrix,
 // SavePLY(filename) writing an ASCII PLY of welded vertices and triangle faces
 // (tsdf2mesh, src/tsdf.cu:1760-1888), render(), Shutdown(), getVertices()/getFaces().
-// Changed: errors throw std::runtime_error (the reference exits the process, tsdf.cuh:46-61);
+// Colour is integrated as the reference does (each frame's rgb, vertex colours in the PLY);
+// a trailing constructor argument color = false turns it off for depth-only callers.
+// Changed: errors throw std::runtime_error (the reference exits the process, tsdf.cuh:46-61),
+// which includes processFrame with a null rgb on a colour volume (the reference reads it);
 // render() draws nothing (no GL in this library); the mesh comes from marching cubes over
-// every allocated block at the time of the call; colour is not integrated.
+// every allocated block at the time of the call.
 #pragma once
 
 #include <cstdint>
@@ -37,7 +40,8 @@ class GpuTsdfGenerator {
  public:
   GpuTsdfGenerator(int width, int height, float fx, float fy, float cx, float cy, float max_depth,
                    float origin_x = -1.5f, float origin_y = -1.5f, float origin_z = 0.5f, float vox_size = 0.006f,
-                   float trunc_m = 0.03f, int vox_dim_x = 500, int vox_dim_y = 500, int vox_dim_z = 500) {
+                   float trunc_m = 0.03f, int vox_dim_x = 500, int vox_dim_y = 500, int vox_dim_z = 500,
+                   bool color = true) {
     vh_config c;
     vh_config_default(&c);
     c.width = width;
@@ -57,6 +61,7 @@ class GpuTsdfGenerator {
     c.vox_dim[2] = vox_dim_z;
     c.log2_buckets = 20;
     c.pool_blocks = 1 << 19;
+    c.color = color ? 1 : 0;
     open(c);
   }
   // Full control over hash and pool sizes and the voxel storage format.
@@ -66,8 +71,8 @@ class GpuTsdfGenerator {
   GpuTsdfGenerator& operator=(const GpuTsdfGenerator&) = delete;
   ~GpuTsdfGenerator() { Shutdown(); }
 
-  // depth: height x width metres (0 = invalid); rgb: may be null (not integrated);
-  // c2w: row-major 4x4 camera-to-world.
+  // depth: height x width metres (0 = invalid); rgb: height x width x 3 bytes (ignored, and
+  // may be null, when colour is off); c2w: row-major 4x4 camera-to-world.
   void processFrame(float* depth, unsigned char* rgb, float* c2w) {
     vh_frame f;
     std::memset(&f, 0, sizeof f);
@@ -80,8 +85,9 @@ class GpuTsdfGenerator {
     check(vh_integrate(vol_, &f), "processFrame");
   }
 
-  // ASCII PLY: "x y z r g b" vertices (metres, colour 0) welded on identical coordinates,
-  // then "3 i j k" faces.
+  // ASCII PLY: "x y z r g b" vertices (metres; colour 0 when colour is off) welded on
+  // identical coordinates (the first triangle's colour is kept, as tsdf2mesh does), then
+  // "3 i j k" faces.
   void SavePLY(std::string filename) {
     extract();
     FILE* fp = std::fopen(filename.c_str(), "w");
@@ -139,7 +145,9 @@ class GpuTsdfGenerator {
         auto it = index.find(k);
         if (it == index.end()) {
           it = index.emplace(k, (int)vertices_.size()).first;
-          vertices_.push_back(Vertex{p[0], p[1], p[2], 0, 0, 0});
+          const uint8_t* c = m.rgb ? m.rgb + 9 * t + 3 * j : nullptr;
+          vertices_.push_back(Vertex{p[0], p[1], p[2], c ? c[0] : (unsigned char)0, c ? c[1] : (unsigned char)0,
+                                     c ? c[2] : (unsigned char)0});
         }
         f.vIdx[j] = it->second;
       }

@@ -3,13 +3,15 @@
 // processFrame, then SavePly; src/PointCloudGenerator.cpp:19-49,93-143).
 //
 // usage: load_frames <out.ply> [n_frames]
-//          frames from the synthetic small-room generator (include/vh_synth.h)
+//          frames (depth and colour) from the synthetic small-room generator
+//          (include/vh_synth.h)
 //        load_frames <out.ply> --dataset <dir> <settings.yaml> [first] [last]
 //          frames <dir>/depth/<id>.png (uint16) and <dir>/tcw/<id>.txt (camera-to-world) for
 //          id = first.. until last or the first missing depth image (SaveFrame::frameLoad,
 //          src/SaveFrame.cpp:154-218); camera and voxel parameters from the settings file
 //          (the keys PointCloudGenerator reads); depth in metres = value / DepthMapFactor
-//          (the reference multiplies by 0.001, its DepthMapFactor of 1000)
+//          (the reference multiplies by 0.001, its DepthMapFactor of 1000); the colour
+//          images (<dir>/RGB/<id>.jpg) are not decoded, so this mode integrates depth only
 // exit status: 0 ok, 2 usage, 3 no GPU, 1 other failure
 #include <cstdio>
 #include <cstdlib>
@@ -45,12 +47,14 @@ int run_synthetic(const char* out, int n_arg) {
                               0.5f, 0.01f, 0.04f, 500, 500, 500);
     std::vector<uint16_t> mm((size_t)info.width * info.height);
     std::vector<float> depth(mm.size());
+    std::vector<unsigned char> rgb(mm.size() * 3);
     for (int f = 0; f < n; ++f) {
       float c2w[16];
       vh_scene_pose(scene, f, c2w);
       vh_scene_render_u16(scene, f, c2w, mm.data());
+      vh_scene_render_rgb(scene, f, c2w, rgb.data());
       for (size_t i = 0; i < mm.size(); ++i) depth[i] = (float)mm[i] * 0.001f;  // as SaveFrame.cpp:179-180
-      gen.processFrame(depth.data(), nullptr, c2w);
+      gen.processFrame(depth.data(), rgb.data(), c2w);
     }
     gen.SavePLY(out);
     std::printf("frames %d vertices %zu faces %zu\n", n, gen.getVertices()->size(), gen.getFaces()->size());
@@ -78,7 +82,8 @@ int run_dataset(const char* out, const char* dir, const char* settings, int firs
   try {
     // the reference's constructor call (PointCloudGenerator.cpp:43-45)
     ark::GpuTsdfGenerator gen(s.width, s.height, s.fx, s.fy, s.cx, s.cy, s.max_depth, s.origin[0], s.origin[1],
-                              s.origin[2], s.vox_size, s.trunc, s.vox_dim[0], s.vox_dim[1], s.vox_dim[2]);
+                              s.origin[2], s.vox_size, s.trunc, s.vox_dim[0], s.vox_dim[1], s.vox_dim[2],
+                              /*color=*/false);
     std::vector<uint16_t> mm((size_t)s.width * s.height);
     std::vector<float> depth(mm.size());
     int n = 0;

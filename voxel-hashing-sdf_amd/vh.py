@@ -40,7 +40,7 @@ DEPTH_U16 = 0
 DEPTH_F32 = 1
 PRIOR_ABSOLUTE = 0
 PRIOR_INCREMENT = 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 # per-evaluation solver stats: n_valid, rms, |delta xi|, lambda, accepted, accepted pose (12)
 GN_STATS = 17
 
@@ -57,7 +57,7 @@ class Config(C.Structure):
                 ("vox_size", C.c_float), ("trunc", C.c_float), ("log2_buckets", C.c_int32),
                 ("pool_blocks", C.c_int32), ("storage", C.c_int32), ("weight_cap", C.c_int32),
                 ("device", C.c_int32), ("origin", C.c_float * 3), ("vox_dim", C.c_int32 * 3),
-                ("reserved", C.c_int32 * 8)]
+                ("color", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class Frame(C.Structure):
@@ -74,7 +74,8 @@ class Stats(C.Structure):
 
 
 class Mesh(C.Structure):
-    _fields_ = [("n_tris", C.c_int64), ("xyz", C.POINTER(C.c_float)), ("cubes", C.POINTER(C.c_int32))]
+    _fields_ = [("n_tris", C.c_int64), ("xyz", C.POINTER(C.c_float)), ("cubes", C.POINTER(C.c_int32)),
+                ("rgb", C.POINTER(C.c_uint8))]
 
 
 class TrackConfig(C.Structure):
@@ -116,6 +117,7 @@ _PROTOS = [
     ("vh_bind_sequence", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     ("vh_bind_tracking", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
                                    C.POINTER(TrackConfig)]),
+    ("vh_bind_sequence_rgb", C.c_int, [C.c_void_p, C.c_void_p]),
     ("vh_integrate_next", C.c_int, [C.c_void_p, C.c_int32]),
     ("vh_sync", C.c_int, [C.c_void_p]),
     ("vh_set_profiling", C.c_int, [C.c_void_p, C.c_int32]),
@@ -124,9 +126,12 @@ _PROTOS = [
     ("vh_extract_mesh", C.c_int, [C.c_void_p, C.POINTER(Mesh)]),
     ("vh_mesh_free", C.c_int, [C.POINTER(Mesh)]),
     ("vh_extract_mesh_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    ("vh_extract_mesh_device_rgb", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                             C.POINTER(C.c_int64)]),
     ("vh_save_ply", C.c_int, [C.c_void_p, C.c_char_p]),
     ("vh_export_blocks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.POINTER(C.c_int64)]),
+    ("vh_export_block_colors", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     ("vh_query_sdf", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("vh_query_sdf_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("vh_query_sdf_backward_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
@@ -146,6 +151,7 @@ _PROTOS = [
     ("vh_scene_pose", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("vh_scene_render_u16", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("vh_scene_render_clean", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("vh_scene_render_rgb", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("vh_scene_render_range", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
 ]
 
@@ -209,7 +215,7 @@ def make_config(**kw) -> Config:
 
 def config_from_workload(w: dict, **over) -> Config:
     keys = ("width", "height", "fx", "fy", "cx", "cy", "max_depth", "min_depth", "vox_size", "trunc",
-            "log2_buckets", "pool_blocks", "storage", "weight_cap")
+            "log2_buckets", "pool_blocks", "storage", "weight_cap", "color")
     kw = {k: w[k] for k in keys if k in w}
     kw.update(over)
     return make_config(**kw)
@@ -249,7 +255,24 @@ class Volume:
         _check(_lib.vh_reset(self._h))
 
     # -- integration
-    def integrate(self, depth: np.ndarray, c2w, scale: float = 0.001):
+    @staticmethod
+    def _rgb_arg(rgb, shape):
+        """Host RGB image (H, W, 3) uint8 as a pointer and the array that owns it."""
+        if rgb is None:
+            return None, None
+        if hasattr(rgb, "data_ptr"):
+            import torch
+
+            if rgb.dtype != torch.uint8 or rgb.is_cuda or not rgb.is_contiguous() or rgb.numel() != shape[0] * shape[1] * 3:
+                raise TypeError("rgb tensor must be a contiguous CPU uint8 tensor of H*W*3 values")
+            return C.c_void_p(rgb.data_ptr()), rgb
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        if rgb.size != shape[0] * shape[1] * 3:
+            raise ValueError("rgb must hold H*W*3 bytes")
+        return rgb.ctypes.data_as(C.c_void_p), rgb
+
+    def integrate(self, depth: np.ndarray, c2w, scale: float = 0.001, rgb=None):
+        """rgb: (H, W, 3) uint8, required by colour volumes."""
         depth = np.ascontiguousarray(depth)
         f = Frame()
         f.depth = depth.ctypes.data_as(C.c_void_p)
@@ -262,16 +285,17 @@ class Volume:
         else:
             raise TypeError("depth must be uint16 or float32")
         f.pitch = depth.strides[0]
-        f.rgb = None
+        f.rgb, rgb_keep = self._rgb_arg(rgb, depth.shape)
         T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32).reshape(16))
         for k in range(16):
             f.c2w[k] = float(T[k])
         _check(_lib.vh_integrate(self._h, C.byref(f)))
 
-    def integrate_async(self, depth, c2w, scale: float = 0.001):
+    def integrate_async(self, depth, c2w, scale: float = 0.001, rgb=None):
         """Queues one host frame (vh_integrate_async). depth: numpy uint16/float32 array or a
         CPU torch tensor (int16 holding uint16 bits, or float32), ideally page-locked
-        (pin_memory=True); it is kept referenced until sync()."""
+        (pin_memory=True); rgb likewise (uint8, H*W*3, colour volumes). Both are kept
+        referenced until sync()."""
         f = Frame()
         if hasattr(depth, "data_ptr"):
             import torch
@@ -290,11 +314,12 @@ class Volume:
             f.pitch = depth.strides[0]
         f.depth_format = DEPTH_U16 if u16 else DEPTH_F32
         f.depth_scale = scale if u16 else 1.0
+        f.rgb, rgb_keep = self._rgb_arg(rgb, (self.cfg.height, self.cfg.width))
         T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32).reshape(16))
         for k in range(16):
             f.c2w[k] = float(T[k])
         _check(_lib.vh_integrate_async(self._h, C.byref(f)))
-        self._pending.append(depth)
+        self._pending.append((depth, rgb_keep))
 
     def bind_sequence(self, d_depth, fmt: int, scale: float, d_c2w, n: int):
         """d_depth / d_c2w: torch device tensors (or integer device addresses)."""
@@ -317,6 +342,10 @@ class Volume:
         tc.gn_iters, tc.point_stride = gn_iters, point_stride
         tc.max_point_depth, tc.prior_mode = max_point_depth, prior_mode
         _check(_lib.vh_bind_tracking(self._h, _tptr(d_depth), fmt, scale, n, C.byref(tc)))
+
+    def bind_sequence_rgb(self, d_rgb):
+        """RGB frames (uint8 device tensor, n * H * W * 3) of the sequence bound last."""
+        _check(_lib.vh_bind_sequence_rgb(self._h, _tptr(d_rgb)))
 
     def integrate_next(self, n: int = 1):
         _check(_lib.vh_integrate_next(self._h, n))
@@ -352,21 +381,37 @@ class Volume:
         _check(_lib.vh_export_blocks(self._h, _ptr(keys), _ptr(tsdf), _ptr(w), nb, C.byref(n)))
         return keys, tsdf, w
 
-    def extract_mesh(self):
-        """(xyz (T,9) float32 metres, cubes (T,4) int32)."""
+    def export_block_colors(self):
+        """(N, 512, 3) uint8 voxel colours in export_blocks' order (colour volumes)."""
+        n = C.c_int64(0)
+        rc = _lib.vh_export_block_colors(self._h, None, 0, C.byref(n))
+        if rc not in (VH_OK, VH_ERR_TRUNCATED):
+            _check(rc)
+        nb = n.value
+        rgb = np.zeros((nb, 512, 3), np.uint8)
+        _check(_lib.vh_export_block_colors(self._h, _ptr(rgb), nb, C.byref(n)))
+        return rgb
+
+    def extract_mesh(self, with_rgb: bool = False):
+        """(xyz (T,9) float32 metres, cubes (T,4) int32) and, with_rgb, rgb (T,9) uint8 (None
+        for a volume without colour)."""
         m = Mesh()
         _check(_lib.vh_extract_mesh(self._h, C.byref(m)))
         try:
             n = m.n_tris
             xyz = np.ctypeslib.as_array(m.xyz, shape=(max(n, 1) * 9,))[: n * 9].reshape(n, 9).copy()
             cubes = np.ctypeslib.as_array(m.cubes, shape=(max(n, 1) * 4,))[: n * 4].reshape(n, 4).copy()
+            rgb = None
+            if m.rgb:
+                rgb = np.ctypeslib.as_array(m.rgb, shape=(max(n, 1) * 9,))[: n * 9].reshape(n, 9).copy()
         finally:
             _lib.vh_mesh_free(C.byref(m))
-        return xyz, cubes
+        return (xyz, cubes, rgb) if with_rgb else (xyz, cubes)
 
-    def extract_mesh_device(self, d_xyz, d_cubes, capacity: int) -> int:
+    def extract_mesh_device(self, d_xyz, d_cubes, capacity: int, d_rgb=None) -> int:
         n = C.c_int64(0)
-        rc = _lib.vh_extract_mesh_device(self._h, _tptr(d_xyz), _tptr(d_cubes), capacity, C.byref(n))
+        rc = _lib.vh_extract_mesh_device_rgb(self._h, _tptr(d_xyz), _tptr(d_cubes), _tptr(d_rgb), capacity,
+                                             C.byref(n))
         if rc not in (VH_OK, VH_ERR_TRUNCATED):
             _check(rc)
         return n.value
@@ -504,6 +549,13 @@ class Scene:
         Tp = None if c2w is None else np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
         _check(_lib.vh_scene_render_clean(self._h, f, _ptr(Tp), _ptr(d)))
         return d
+
+    def rgb(self, f: int, c2w=None) -> np.ndarray:
+        """Colour image (H, W, 3) uint8 of frame f (or of pose c2w)."""
+        out = np.zeros((self.height, self.width, 3), np.uint8)
+        Tp = None if c2w is None else np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        _check(_lib.vh_scene_render_rgb(self._h, f, _ptr(Tp), _ptr(out)))
+        return out
 
     def frames(self, f0: int, n: int):
         """(depth (n,H,W) uint16, c2w (n,4,4) float32)."""
