@@ -202,13 +202,15 @@ def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.00
     return res
 
 
-def run_pass(vol, vh, seq, nf, mc_every, d_xyz, cap, final_mc=True):
+def run_pass(vol, vh, seq, nf, mc_every, d_xyz, cap, final_mc=True, d_rgb=None, d_vrgb=None):
     """One pass over the bound sequence from an empty volume, extracting the mesh every
     mc_every frames and (if final_mc) at the end; returns (last triangle count, extraction
-    wall times)."""
+    wall times). d_rgb: the sequence's RGB frames (colour volumes); d_vrgb: vertex colours."""
     d_depth, d_pose = seq
     vol.reset()
     vol.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, nf)
+    if d_rgb is not None:
+        vol.bind_sequence_rgb(d_rgb)
     mcs = []
     ntri = 0
     f = 0
@@ -218,7 +220,7 @@ def run_pass(vol, vh, seq, nf, mc_every, d_xyz, cap, final_mc=True):
         f += n
         if mc_every or (final_mc and f == nf):
             t0 = time.perf_counter()
-            ntri = vol.extract_mesh_device(d_xyz, None, cap)  # waits for the frames and the extraction
+            ntri = vol.extract_mesh_device(d_xyz, None, cap, d_vrgb)  # waits for the frames and the extraction
             mcs.append((t0, time.perf_counter()))
     vol.sync()
     return ntri, mcs
@@ -234,6 +236,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--no-color", action="store_true", help="skip the colour variants")
     ap.add_argument("--no-gn", action="store_true", help="skip the pose-refinement block")
     ap.add_argument("--no-tracked", action="store_true", help="skip the tracked floor_large_4mm block")
     ap.add_argument("--tracked-frames", type=int, default=0, help="override the tracked sequence length")
@@ -288,14 +291,27 @@ def main():
     variants = [("f32", {})]
     if not args.no_f16:
         variants += [("f16r", dict(storage=2, weight_cap=255)), ("f16", dict(storage=1, weight_cap=255))]
+    d_rgb = d_vrgb = None
+    if not args.no_color:
+        variants += [("f32c", dict(color=1))]
+        if not args.no_f16:
+            variants += [("f16rc", dict(storage=2, weight_cap=255, color=1))]
+        t = time.perf_counter()
+        rgb = np.stack([scene.rgb(k, poses[k]) for k in range(nf)])
+        log(f"bench: rendered {nf} colour frames in {time.perf_counter() - t:.1f} s")
+        d_rgb = torch.from_numpy(rgb).cuda()
+        del rgb
+        d_vrgb = torch.empty(cap * 9, dtype=torch.uint8, device="cuda")
     for name, over in variants:
+        col = "color" in over
+        cr = dict(d_rgb=d_rgb, d_vrgb=d_vrgb) if col else {}
         vol = vh.Volume(vh.config_from_workload(w, **over))
         for _ in range(args.warmup):
-            run_pass(vol, vh, (d_depth, d_pose), nf, mc_every, d_xyz, cap)
+            run_pass(vol, vh, (d_depth, d_pose), nf, mc_every, d_xyz, cap, **cr)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            ntri, mcs = run_pass(vol, vh, (d_depth, d_pose), nf, mc_every, d_xyz, cap)
+            ntri, mcs = run_pass(vol, vh, (d_depth, d_pose), nf, mc_every, d_xyz, cap, **cr)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         st = vol.stats()
@@ -304,7 +320,7 @@ def main():
         # integration only (no extraction), same sequence
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False)
+        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
         torch.cuda.synchronize()
         dt_int = time.perf_counter() - t1
         # final extraction alone, timed with the frames already integrated
@@ -312,7 +328,7 @@ def main():
         t2 = time.perf_counter()
         reps = 3
         for _ in range(reps):
-            ntri = vol.extract_mesh_device(d_xyz, None, cap)
+            ntri = vol.extract_mesh_device(d_xyz, None, cap, d_vrgb if col else None)
         mc_s = (time.perf_counter() - t2) / reps
         if name == "f32":
             # host frames from page-locked memory: copy ring + allocation + integration per frame
@@ -331,7 +347,7 @@ def main():
             del pinned
         # kernel timing with HIP events (eager launches on the volume's stream)
         vol.set_profiling(True)
-        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False)
+        run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
         pst = vol.stats()
         vol.set_profiling(False)
         if name == "f16" and not args.no_gn:  # the volume now holds the whole sequence
@@ -339,7 +355,7 @@ def main():
         results[name] = dict(fps=args.steps * nf / dt, ms_per_step=1e3 * dt / args.steps, fps_integrate=nf / dt_int,
                              blocks=st["blocks_allocated"], tris=ntri, mc_ms=1e3 * mc_s,
                              mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst,
-                             vox_bytes=8 if name == "f32" else 4)
+                             vox_bytes=(8 if over.get("storage", 0) == 0 else 4) + (4 if col else 0))
         log(f"bench: {name}: {results[name]['fps']:.1f} frames/s (with MC), {results[name]['fps_integrate']:.1f} "
             f"integrate-only, {st['blocks_allocated']} blocks, {ntri} triangles, MC {1e3 * mc_s:.2f} ms")
         vol.close()
@@ -399,8 +415,27 @@ def main():
                     "integrate_kernel_gbs": round(ah, 1),
                     "integrate_avg_launch_us": round(1e6 * ph["integrate_ms"] / 1e3 / ph["profiled_frames"], 2)}
 
+    colour_labels = {"f32c": ("color", "f32", "fp32 tsdf/weight (8 B) + colour word (4 B) per voxel"),
+                     "f16rc": ("color_fp16", "f16r", "F16R tsdf/weight (4 B) + colour word (4 B) per voxel")}
+    for name, (key, base, desc) in colour_labels.items():
+        if name not in results:
+            continue
+        h = results[name]
+        ph = h["prof"]
+        # per touched block: voxel and colour words read and written; per frame: depth and RGB
+        bh = ph["profiled_frames"] * W * H * (2 + 3) + ph["touched_total"] * (512 * h["vox_bytes"] * 2 + 16)
+        ah = bh / (ph["integrate_ms"] / 1e3) / 1e9
+        b = results[base]
+        out[key] = {"storage": desc, "value": round(h["fps"], 2), "integrate_only_fps": round(h["fps_integrate"], 2),
+                    "vs_depth_only": round(h["fps"] / b["fps"], 3),
+                    "integrate_vs_depth_only": round(h["fps_integrate"] / b["fps_integrate"], 3),
+                    "mc_ms": round(h["mc_ms"], 3), "mc_vs_depth_only": round(h["mc_ms"] / b["mc_ms"], 3),
+                    "integrate_kernel_gbs": round(ah, 1),
+                    "integrate_avg_launch_us": round(1e6 * ph["integrate_ms"] / 1e3 / ph["profiled_frames"], 2),
+                    "rgb": "synthetic scene colours (vh_scene_render_rgb), device-resident"}
+
     if not args.no_tracked:
-        del d_xyz, d_depth, d_pose
+        del d_xyz, d_depth, d_pose, d_rgb, d_vrgb
         torch.cuda.empty_cache()
         out["tracked"] = tracked_block(vh, wl, args.tracked_frames)
     if not args.no_cpu:

@@ -82,6 +82,7 @@ struct vh_volume {
   unsigned long long* block_keys = nullptr;
   void* voxels = nullptr;
   uint32_t* colors = nullptr;  // colour volumes: one u32 per voxel
+  uint32_t* rgba = nullptr;    // colour volumes: 2 packed frames of W*H words
   uint32_t* touched = nullptr;
   uint32_t stripe_cap = 0;
   FrameCtl* ctl = nullptr;
@@ -196,6 +197,7 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.stripe_cap = v->stripe_cap;
   a.voxels = v->voxels;
   a.colors = v->colors;
+  a.rgba = v->rgba;
   if (!a.colors) a.rgb = nullptr;
   a.ctl = v->ctl;
   return a;
@@ -446,7 +448,7 @@ void free_volume(vh_volume* v) {
   void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
-                  v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb};
+                  v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -579,7 +581,8 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
       (rc = dev_alloc(&v->touched, (size_t)2 * kStripes * v->stripe_cap * 4)) ||
       (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
       (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float))) ||
-      (cfg->color && ((rc = dev_alloc(&v->colors, pool * kB3 * 4)) || (rc = dev_alloc(&v->stage_rgb, frame_px * 3))))) {
+      (cfg->color && ((rc = dev_alloc(&v->colors, pool * kB3 * 4)) || (rc = dev_alloc(&v->stage_rgb, frame_px * 3)) ||
+                      (rc = dev_alloc(&v->rgba, 2 * frame_px * 4))))) {
     std::string msg = g_last_error;
     free_volume(v);
     return fail(rc, msg);

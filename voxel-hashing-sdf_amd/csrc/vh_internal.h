@@ -98,6 +98,9 @@ struct FrameArgs {
   // frame f at rgb + f*W*H*3) and the colour pool, one u32 {r | g << 8 | b << 16} per voxel
   const uint8_t* rgb;
   uint32_t* colors;
+  // colour volumes: the allocation pass packs each pixel of frame f into one word
+  // {r | g << 8 | b << 16} at rgba + (f & 1) * W*H, which the integration of frame f gathers
+  uint32_t* rgba;
 };
 
 struct McArgs {

@@ -188,6 +188,11 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
   BandDDA s;
   bool active = false;
   if (u < a.W && v < a.H) {
+    if (a.rgba) {  // the pixel's colour, packed for the integration's one-word gathers
+      const size_t pix = (size_t)v * a.W + u;
+      const uint8_t* px = a.rgb + ((size_t)fs * a.W * a.H + pix) * 3;
+      a.rgba[(size_t)(frame & 1u) * a.W * a.H + pix] = (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
+    }
     const float d = depth_at<kU16>(base, a.depth_scale, v * a.W + u);
     if (depth_valid(d, a.max_depth)) {
       active = band_setup(T, a.K, a.bsz, a.mu, a.min_depth, u, v, d, s);
@@ -284,14 +289,15 @@ __device__ __forceinline__ void mean_update(float& s, float& w, float dist, floa
 
 // Colour update of the reference (src/tsdf.cu:743-745): per channel (c * w_old + rgb) / w_new
 // in FP32 (the product and sum are exact integers below 2^24; the quotient correctly
-// rounded by div_rn), truncated to 8 bits. w_old is the stored weight before this update.
-__device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, const uint8_t* px) {
+// rounded by div_rn), truncated to 8 bits. w_old is the stored weight before this update;
+// c and px are packed {r | g << 8 | b << 16}.
+__device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, uint32_t px) {
   const float w_new = w_old + 1.0f;
   uint32_t out = 0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const float v = div_rn(__builtin_fmaf((float)((c >> (8 * k)) & 255u), w_old, (float)px[k]), w_new);
-    out |= (uint32_t)(uint8_t)v << (8 * k);
+    const float num = __builtin_fmaf((float)((c >> (8 * k)) & 255u), w_old, (float)((px >> (8 * k)) & 255u));
+    out |= (uint32_t)(uint8_t)div_rn(num, w_new) << (8 * k);
   }
   return out;
 }
@@ -328,7 +334,7 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
   const size_t elem = kU16 ? 2 : 4;
   const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
   auto dep = [&](int pix) { return depth_at<kU16>(base, a.depth_scale, pix); };
-  const uint8_t* rgb = kColor ? a.rgb + (size_t)fs * a.W * a.H * 3 : nullptr;
+  const uint32_t* rgba = kColor ? a.rgba + (size_t)(frame & 1u) * a.W * a.H : nullptr;
   BlockFrame bf;
   block_frame(T, a.vox, bf);
 
@@ -397,6 +403,11 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
       float dv[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) dv[k] = dep(pix[k] < 0 ? 0 : pix[k]);
+      uint32_t px[8];
+      if constexpr (kColor) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) px[k] = rgba[pix[k] < 0 ? 0 : pix[k]];
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         float dist;
@@ -404,7 +415,7 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         float sv, wv;
         IO::unpack(old[k], sv, wv);
         if constexpr (kColor) {
-          if (ok) cp[64 * k] = color_update(cold[k], wv, rgb + 3 * (size_t)pix[k]);
+          if (ok) cp[64 * k] = color_update(cold[k], wv, px[k]);
           else if (fresh) cp[64 * k] = 0u;
         }
         mean_update(sv, wv, dist, a.cap);
