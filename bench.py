@@ -465,6 +465,9 @@ def cpu_baseline(vh, w, depth, poses, budget_s, no_f16):
                       f"(hardware_concurrency {os.cpu_count()}), {dt:.1f} s"}
     errs = {}
     rk, rs, rw, r64, rw64 = ref.export()
+    t = time.perf_counter()
+    m64 = ref.mc64()  # the FP64 reference mesh of the same frames
+    log(f"bench: FP64 reference mesh of {n} frames: {len(m64[0])} triangles in {time.perf_counter() - t:.1f} s")
     storages = [("fp32", {})]
     if not no_f16:
         storages += [("fp16", dict(storage=2, weight_cap=255)), ("fp16_plain", dict(storage=1, weight_cap=255))]
@@ -481,6 +484,18 @@ def cpu_baseline(vh, w, depth, poses, budget_s, no_f16):
                          "err_frames": n, "err_reference": "oracle FP64 shadow, all observed voxels"}
         else:
             errs[key] = {"tsdf_err": "block sets differ from the oracle"}
+        t = time.perf_counter()
+        xyz, cubes = vol.extract_mesh()
+        h = oracle.hausdorff(xyz, cubes, *m64)
+        errs[key].update({"hausdorff_m": h["hausdorff"], "hausdorff_q0.999_m": h["q0.999"],
+                          "hausdorff_q0.99999_m": h["q0.99999"], "mesh_to_fp64_mean_m": h["a_to_b"]["mean"],
+                          "fp64_to_mesh_mean_m": h["b_to_a"]["mean"],
+                          "hausdorff_unmatched_samples": h["a_to_b"]["far"] + h["b_to_a"]["far"],
+                          "hausdorff_reference": "GPU mesh vs marching cubes over the oracle's FP64 shadow; "
+                                                 "samples = vertices + centroids of both meshes, exact "
+                                                 "point-to-triangle distances within the 27 surrounding cubes"})
+        log(f"bench: {key}: Hausdorff to the FP64 mesh {1e3 * h['hausdorff']:.4f} mm "
+            f"({time.perf_counter() - t:.1f} s)")
         vol.close()
     return base, errs
 

@@ -63,6 +63,11 @@ def load():
     lib.vho_mc_rgb.argtypes = [P, P, P, P, C.c_int64]
     lib.vho_vertex_interp_rgb.argtypes = [P, P, C.c_float, C.c_float, P, P, P, P]
     lib.vho_color_update.argtypes = [P, C.c_float, P]
+    lib.vho_mc64.restype = C.c_int64
+    lib.vho_mc64.argtypes = [P, P, P, C.c_int64]
+    lib.vho_point_triangle_dist.restype = C.c_double
+    lib.vho_point_triangle_dist.argtypes = [P, P, P, P]
+    lib.vho_mesh_distance.argtypes = [P, P, C.c_int64, P, P, C.c_int64, P, P]
     _lib = lib
     return lib
 
@@ -144,6 +149,14 @@ class Oracle:
         _lib.vho_mc_rgb(self._h, _p(xyz), _p(cubes), _p(rgb), n)
         return (xyz, cubes, rgb) if with_rgb else (xyz, cubes)
 
+    def mc64(self):
+        """The FP64 reference mesh (marching cubes over the FP64 shadow): (xyz, cubes)."""
+        n = _lib.vho_mc64(self._h, None, None, 0)
+        xyz = np.zeros((n, 9), np.float32)
+        cubes = np.zeros((n, 4), np.int32)
+        _lib.vho_mc64(self._h, _p(xyz), _p(cubes), n)
+        return xyz, cubes
+
     def query(self, pts):
         pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
         n = pts.shape[0]
@@ -220,6 +233,43 @@ def vertex_interp_rgb(a, b, va, vb, ca, cb):
     cout = np.zeros(3, np.uint8)
     _lib.vho_vertex_interp_rgb(_p(a), _p(b), va, vb, _p(ca), _p(cb), _p(out), _p(cout))
     return out, cout
+
+
+def point_triangle_dist(p, a, b, c):
+    load()
+    arr = [np.ascontiguousarray(x, np.float64) for x in (p, a, b, c)]
+    return float(_lib.vho_point_triangle_dist(*[_p(x) for x in arr]))
+
+
+def mesh_distance(a_xyz, a_cubes, b_xyz, b_cubes, quantiles=(0.999, 0.99999)):
+    """One-sided distances from mesh A to mesh B (metres): dict(max, mean, far, samples) and
+    the given quantiles of the per-sample distances (samples with a B triangle nearby)."""
+    load()
+    ax = np.ascontiguousarray(a_xyz, np.float32)
+    ac = np.ascontiguousarray(a_cubes, np.int32)
+    bx = np.ascontiguousarray(b_xyz, np.float32)
+    bc = np.ascontiguousarray(b_cubes, np.int32)
+    out = np.zeros(4, np.float64)
+    dist = np.zeros(4 * len(ax), np.float32) if quantiles else None
+    _lib.vho_mesh_distance(_p(ax), _p(ac), len(ax), _p(bx), _p(bc), len(bx), _p(out), _p(dist))
+    res = {"max": float(out[0]), "mean": float(out[1]), "far": int(out[2]), "samples": int(out[3])}
+    if quantiles:
+        d = dist[dist >= 0]
+        for q in quantiles:
+            res[f"q{q}"] = float(np.quantile(d, q)) if len(d) else 0.0
+    return res
+
+
+def hausdorff(a_xyz, a_cubes, b_xyz, b_cubes, quantiles=(0.999, 0.99999)):
+    """Symmetric distance between two marching-cubes meshes on the same lattice: the larger of
+    the two one-sided maxima (samples: vertices and centroids), with both directions and the
+    larger of their quantiles."""
+    ab = mesh_distance(a_xyz, a_cubes, b_xyz, b_cubes, quantiles)
+    ba = mesh_distance(b_xyz, b_cubes, a_xyz, a_cubes, quantiles)
+    out = {"hausdorff": max(ab["max"], ba["max"]), "a_to_b": ab, "b_to_a": ba}
+    for q in quantiles or ():
+        out[f"q{q}"] = max(ab[f"q{q}"], ba[f"q{q}"])
+    return out
 
 
 def color_update(c, w_old, px):

@@ -19,11 +19,15 @@
 //   vertex interpolation with 1e-5 snaps       src/tsdf.cu:1640-1660 (position and colour)
 //   Bourke triangle table                      include/tsdf.cuh:278-534
 // Functions without a reference counterpart (trilinear query, its gradient, binary16
-// rounding) are restated from their textbook definitions.
+// rounding) are restated from their textbook definitions. Measurement helpers: marching
+// cubes over the FP64 shadow (the FP64 reference mesh) and one-sided mesh-to-mesh distances
+// (point to triangle: the closest-point construction of Ericson, Real-Time Collision
+// Detection, 2005, sec. 5.1.5).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -379,14 +383,33 @@ void interp(const float a[3], const float b[3], float va, float vb, float out[3]
     for (int k = 0; k < 3; ++k) cout[k] = (uint8_t)((float)ca[k] + m * (float)((int)cb[k] - (int)ca[k]));
 }
 
+// VertexInterp in FP64 (same snaps) for the FP64 reference mesh.
+void interp64(const double a[3], const double b[3], double va, double vb, double out[3]) {
+  if (std::fabs(va) < 0.00001) { std::memcpy(out, a, 24); return; }
+  if (std::fabs(vb) < 0.00001) { std::memcpy(out, b, 24); return; }
+  if (std::fabs(va - vb) < 0.00001) { std::memcpy(out, a, 24); return; }
+  const double m = (0.0 - va) / (vb - va);
+  for (int k = 0; k < 3; ++k) out[k] = a[k] + m * (b[k] - a[k]);
+}
+
+// k64 = false: the twin's values, FP32 interpolation (the GPU's mesh); true: the FP64
+// shadow's values and FP64 interpolation, positions rounded to FP32 once at the end.
+template <bool k64>
 void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& out) {
-  float val[8];
+  using R = typename std::conditional<k64, double, float>::type;
+  R val[8];
   uint8_t col[8][3] = {};
   for (int c = 0; c < 8; ++c) {
     const int x = gx + kCorner[c][0], y = gy + kCorner[c][1], z = gz + kCorner[c][2];
     const int64_t b = o.find(key_of(floor_div8(x), floor_div8(y), floor_div8(z)));
     if (b < 0) return;
     const int i = ((x - floor_div8(x) * 8) * 8 + (y - floor_div8(y) * 8)) * 8 + (z - floor_div8(z) * 8);
+    if (k64) {
+      const size_t v = (size_t)b * 512 + i;
+      if (!(o.w64[v] > 0.0)) return;
+      val[c] = (R)o.s64[v];
+      continue;
+    }
     float s, w;
     o.twin(b, i, s, w);
     if (!(w > 0.0f)) return;  // unobserved corner: no surface (corrects the reference, which has no test)
@@ -395,26 +418,78 @@ void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& o
   }
   int ci = 0;
   for (int c = 0; c < 8; ++c)
-    if (val[c] < 0.0f) ci |= 1 << c;
+    if (val[c] < (R)0) ci |= 1 << c;
   const char* tl = kTri[ci];
   const int n = (int)std::strlen(tl) / 3;
   for (int t = 0; t < n; ++t) {
     Tri tr;
+    double q[9];
     for (int j = 0; j < 3; ++j) {
       const int e = hexval(tl[3 * t + j]);
       const int ca = kEdge[e][0], cb = kEdge[e][1];
-      const float pa[3] = {(float)(gx + kCorner[ca][0]), (float)(gy + kCorner[ca][1]), (float)(gz + kCorner[ca][2])};
-      const float pb[3] = {(float)(gx + kCorner[cb][0]), (float)(gy + kCorner[cb][1]), (float)(gz + kCorner[cb][2])};
-      interp(pa, pb, val[ca], val[cb], tr.p + 3 * j, col[ca], col[cb], tr.rgb + 3 * j);
+      if (k64) {
+        const double pa[3] = {(double)(gx + kCorner[ca][0]), (double)(gy + kCorner[ca][1]), (double)(gz + kCorner[ca][2])};
+        const double pb[3] = {(double)(gx + kCorner[cb][0]), (double)(gy + kCorner[cb][1]), (double)(gz + kCorner[cb][2])};
+        interp64(pa, pb, (double)val[ca], (double)val[cb], q + 3 * j);
+      } else {
+        const float pa[3] = {(float)(gx + kCorner[ca][0]), (float)(gy + kCorner[ca][1]), (float)(gz + kCorner[ca][2])};
+        const float pb[3] = {(float)(gx + kCorner[cb][0]), (float)(gy + kCorner[cb][1]), (float)(gz + kCorner[cb][2])};
+        interp(pa, pb, (float)val[ca], (float)val[cb], tr.p + 3 * j, col[ca], col[cb], tr.rgb + 3 * j);
+      }
     }
+    if (k64)
+      for (int k = 0; k < 9; ++k) tr.p[k] = (float)q[k];  // only for the degenerate test below
     auto same = [&](int a, int b) {
+      if (k64) return q[3 * a] == q[3 * b] && q[3 * a + 1] == q[3 * b + 1] && q[3 * a + 2] == q[3 * b + 2];
       return tr.p[3 * a] == tr.p[3 * b] && tr.p[3 * a + 1] == tr.p[3 * b + 1] && tr.p[3 * a + 2] == tr.p[3 * b + 2];
     };
     if (same(0, 1) || same(1, 2) || same(2, 0)) continue;  // degenerate (typo-free version of :1055-1058)
-    for (int k = 0; k < 9; ++k) tr.p[k] *= o.vox;
+    if (k64) {
+      for (int k = 0; k < 9; ++k) tr.p[k] = (float)(q[k] * (double)o.vox);
+      std::memset(tr.rgb, 0, 9);
+    } else {
+      for (int k = 0; k < 9; ++k) tr.p[k] *= o.vox;
+    }
     tr.c[0] = gx; tr.c[1] = gy; tr.c[2] = gz; tr.c[3] = (ci << 8) | t;
     out.push_back(tr);
   }
+}
+
+// ------------------------------------------------------------------ mesh distance
+// Squared distance from p to triangle (a, b, c): the closest point found by the region test
+// of Ericson (2005) sec. 5.1.5 (vertex regions, edge regions, then the face interior).
+double point_triangle_dist2(const double p[3], const double a[3], const double b[3], const double c[3]) {
+  auto sub = [](const double* x, const double* y, double* o) { for (int k = 0; k < 3; ++k) o[k] = x[k] - y[k]; };
+  auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+  double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+  sub(b, a, ab);
+  sub(c, a, ac);
+  sub(p, a, ap);
+  auto at = [&](double v, double w) {
+    for (int k = 0; k < 3; ++k) q[k] = a[k] + v * ab[k] + w * ac[k];
+    double d[3];
+    sub(p, q, d);
+    return dot(d, d);
+  };
+  const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return at(0.0, 0.0);
+  sub(p, b, bp);
+  const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) return at(1.0, 0.0);
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return at(d1 / (d1 - d3), 0.0);
+  sub(p, c, cp);
+  const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return at(0.0, 1.0);
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return at(0.0, d2 / (d2 - d6));
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return at(1.0 - w, w);  // on bc: b + w (c - b)
+  }
+  const double den = 1.0 / (va + vb + vc);
+  return at(vb * den, vc * den);
 }
 
 // ------------------------------------------------------------------ trilinear query
@@ -553,7 +628,8 @@ int64_t vho_export(void* p, int32_t* xyz, float* s, float* w, double* s64, doubl
 }
 
 // Marching cubes over every block; triangles ordered by cube then by table position.
-int64_t vho_mc_rgb(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap) {
+extern "C++" template <bool k64>
+int64_t mc_all(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap) {
   Oracle* o = static_cast<Oracle*>(p);
   std::vector<int64_t> order(o->keys.begin(), o->keys.end());
   std::sort(order.begin(), order.end());
@@ -563,7 +639,7 @@ int64_t vho_mc_rgb(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t ca
     int bx, by, bz;
     key_xyz(order[(size_t)k], bx, by, bz);
     for (int i = 0; i < 512; ++i)
-      cube_triangles(*o, bx * 8 + (i >> 6), by * 8 + ((i >> 3) & 7), bz * 8 + (i & 7), per[(size_t)k]);
+      cube_triangles<k64>(*o, bx * 8 + (i >> 6), by * 8 + ((i >> 3) & 7), bz * 8 + (i & 7), per[(size_t)k]);
   }
   int64_t n = 0;
   for (auto& v : per) n += (int64_t)v.size();
@@ -578,7 +654,78 @@ int64_t vho_mc_rgb(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t ca
     }
   return n;
 }
-int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) { return vho_mc_rgb(p, xyz, cubes, nullptr, cap); }
+int64_t vho_mc_rgb(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap) {
+  return mc_all<false>(p, xyz, cubes, rgb, cap);
+}
+int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) { return mc_all<false>(p, xyz, cubes, nullptr, cap); }
+// The FP64 reference mesh: marching cubes over the FP64 shadow (same cube records).
+int64_t vho_mc64(void* p, float* xyz, int32_t* cubes, int64_t cap) { return mc_all<true>(p, xyz, cubes, nullptr, cap); }
+
+double vho_point_triangle_dist(const double* p, const double* a, const double* b, const double* c) {
+  return std::sqrt(point_triangle_dist2(p, a, b, c));
+}
+
+// One-sided distances from mesh A to mesh B (metres): the samples are every A triangle's
+// three vertices and centroid; each is measured against B's triangles in the 27 lattice
+// cubes around its triangle's cube (cube records as vho_mc writes them). out[0] = max,
+// out[1] = mean over samples that found a B triangle there, out[2] = samples that found none,
+// out[3] = samples. dist (may be null): 4 * na per-sample distances, -1 where none was found.
+void vho_mesh_distance(const float* a_xyz, const int32_t* a_cubes, int64_t na, const float* b_xyz,
+                       const int32_t* b_cubes, int64_t nb, double* out, float* dist) {
+  std::vector<int64_t> order((size_t)nb);
+  for (int64_t i = 0; i < nb; ++i) order[(size_t)i] = i;
+  auto key = [](const int32_t* c) { return key_of(c[0], c[1], c[2]); };
+  std::sort(order.begin(), order.end(),
+            [&](int64_t x, int64_t y) { return key(b_cubes + 4 * x) < key(b_cubes + 4 * y); });
+  std::unordered_map<int64_t, std::pair<int64_t, int64_t>> range;  // cube -> [first, end) in order
+  range.reserve((size_t)nb);
+  for (int64_t i = 0; i < nb;) {
+    const int64_t k = key(b_cubes + 4 * order[(size_t)i]);
+    int64_t j = i + 1;
+    while (j < nb && key(b_cubes + 4 * order[(size_t)j]) == k) ++j;
+    range.emplace(k, std::make_pair(i, j));
+    i = j;
+  }
+  double mx = 0.0, sum = 0.0;
+  int64_t far = 0, found = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : mx) reduction(+ : sum, far, found)
+  for (long t = 0; t < (long)na; ++t) {
+    const float* P = a_xyz + 9 * t;
+    const int32_t* C = a_cubes + 4 * t;
+    double pts[4][3];
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) pts[j][k] = P[3 * j + k];
+    for (int k = 0; k < 3; ++k) pts[3][k] = (pts[0][k] + pts[1][k] + pts[2][k]) / 3.0;
+    double best[4] = {1e300, 1e300, 1e300, 1e300};
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          auto it = range.find(key_of(C[0] + dx, C[1] + dy, C[2] + dz));
+          if (it == range.end()) continue;
+          for (int64_t i = it->second.first; i < it->second.second; ++i) {
+            const float* Q = b_xyz + 9 * order[(size_t)i];
+            const double a[3] = {Q[0], Q[1], Q[2]}, b[3] = {Q[3], Q[4], Q[5]}, c[3] = {Q[6], Q[7], Q[8]};
+            for (int s2 = 0; s2 < 4; ++s2) best[s2] = std::min(best[s2], point_triangle_dist2(pts[s2], a, b, c));
+          }
+        }
+    for (int s2 = 0; s2 < 4; ++s2) {
+      if (best[s2] >= 1e300) {
+        ++far;
+        if (dist) dist[4 * t + s2] = -1.0f;
+        continue;
+      }
+      const double d = std::sqrt(best[s2]);
+      if (dist) dist[4 * t + s2] = (float)d;
+      mx = std::max(mx, d);
+      sum += d;
+      ++found;
+    }
+  }
+  out[0] = mx;
+  out[1] = found ? sum / (double)found : 0.0;
+  out[2] = (double)far;
+  out[3] = (double)(4 * na);
+}
 
 // Triangles of one cube from 8 given corner values (all observed), lattice origin (0,0,0),
 // vox = 1: for exhaustive checks of the 256 cases.
