@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 3
+#define VH_ABI_VERSION 4
 
 typedef enum vh_status {
   VH_OK = 0,
@@ -81,7 +81,11 @@ typedef struct vh_config {
   int32_t color;         /* 1: integrate colour (Voxel::sdf_color, include/tsdf.cuh:122): every
                             frame then needs its RGB image, and meshes carry vertex colours;
                             0: depth only (4 B per voxel less traffic each way) */
-  int32_t reserved[7];   /* zero */
+  int32_t seq_batch;     /* frames per integration launch on bound sequences (1..8; 0 = 8):
+                            each block's voxels are read and written once per batch and
+                            updated by every frame of the batch that touched it, in frame
+                            order (the same result as frame by frame) */
+  int32_t reserved[6];   /* zero */
 } vh_config;
 
 /* One input frame in host memory (replaces processFrame's float* depth, uchar* rgb,
@@ -111,6 +115,9 @@ typedef struct vh_stats {
   double alloc_ms;            /* profiling mode: summed allocation-kernel time */
   int64_t profiled_frames;
   double track_ms;            /* profiling mode: summed tracking time (back-projection + solver) */
+  int64_t profiled_launches;  /* profiling mode: integration launches timed (one per batch) */
+  int64_t blocks_integrated;  /* sum over integration launches of the blocks read and written
+                                 (a block touched by several frames of a batch counts once) */
 } vh_stats;
 
 typedef enum vh_prior_mode {

@@ -289,3 +289,58 @@ def test_reset_empties_the_volume(vh, oracle, wl, small_frames):
     ref.integrate(depth[5], poses[5])
     assert_same_volume(oracle, vol, ref)
     vol.close()
+
+
+@pytest.mark.parametrize("batch", [1, 3, 8])
+def test_batched_sequences_equal_frame_by_frame(vh, oracle, wl, small_frames, batch):
+    """Bound sequences integrate `batch` frames per launch (each block read and written once
+    per batch, updated by every frame of the batch that touched it, in frame order): the
+    volume, the per-frame statistics and the mesh equal the oracle's frame-by-frame result,
+    with calls of 5 and 7 frames (partial batches) and a profiled call; blocks_integrated is
+    the sum over batches of the union of the frames' touched sets."""
+    import torch
+
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    n = 16
+    print(f"room_small_1cm: {n} sequence frames, batch {batch}, calls 5 + 7 + 4 (profiled)", flush=True)
+    d_depth = torch.from_numpy(depth[:n].view(np.int16).copy()).cuda()
+    d_pose = torch.from_numpy(poses[:n].copy()).cuda()
+    vol = vh.Volume(vh.config_from_workload(w, seq_batch=batch))
+    vol.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, n)
+    ref = oracle.Oracle(w)
+    touched = []
+    sets = []
+
+    def ref_frames(k0, k1):
+        for k in range(k0, k1):
+            touched.append(ref.integrate(depth[k], poses[k]))
+            sets.append({tuple(x) for x in ref.touched()})
+
+    vol.integrate_next(5)
+    vol.integrate_next(7)
+    vol.sync()
+    ref_frames(0, 12)
+    st = vol.stats()
+    assert st["frames"] == 12 and st["touched_last"] == touched[-1] and st["touched_total"] == sum(touched)
+    assert_same_volume(oracle, vol, ref)
+    vol.set_profiling(True)
+    vol.integrate_next(4)
+    vol.sync()
+    ref_frames(12, 16)
+    st = vol.stats()
+    vol.set_profiling(False)
+    assert st["profiled_frames"] == 4 and st["profiled_launches"] == -(-4 // batch)
+    assert st["touched_total"] == sum(touched) and st["touched_last"] == touched[-1]
+    # batches: calls of 5, 7 and 4 frames, each cut into runs of `batch`
+    expect = 0
+    f = 0
+    for call in (5, 7, 4):
+        for b0 in range(0, call, batch):
+            u = set().union(*sets[f + b0:f + min(call, b0 + batch)])
+            expect += len(u)
+        f += call
+    assert st["blocks_integrated"] == expect, (st["blocks_integrated"], expect)
+    assert_same_volume(oracle, vol, ref)
+    assert_same_mesh(vol, ref)
+    vol.close()

@@ -13,9 +13,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "vh.h"
@@ -78,7 +80,7 @@ struct vh_volume {
   // device tables
   unsigned long long* keys = nullptr;
   unsigned long long* vals = nullptr;
-  uint32_t* stamps = nullptr;
+  uint32_t* marks = nullptr;  // 2 x n_slots
   unsigned long long* block_keys = nullptr;
   void* voxels = nullptr;
   uint32_t* colors = nullptr;  // colour volumes: one u32 per voxel
@@ -115,18 +117,21 @@ struct vh_volume {
   float* ring_pose_host = nullptr;  // kRing x 16 (page-locked)
   hipEvent_t ring_copied[kRing] = {}, ring_done[kRing] = {};
   uint64_t ring_next = 0;           // frames queued since create/reset (= device cursor 2)
-  // graphs: [0] staged u16, [1] staged f32, [2] bound sequence step (integrate frame f +
-  // allocate frame f+1), [3] tracked sequence, [4] bound sequence first allocation, [5] bound
-  // sequence last integration, [6] ring u16, [7] ring f32
+  // graphs by (kind, frames integrated, frames allocated): kinds [0] staged u16, [1] staged
+  // f32, [2] bound sequence step (integrate batch b + allocate batch b+1), [3] tracked
+  // sequence, [4] bound sequence first allocation, [5] bound sequence last integration,
+  // [6] ring u16, [7] ring f32
   static constexpr int kGraphs = 8;
-  hipGraphExec_t graph[kGraphs] = {};
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+  int batch = kMaxBatch;  // frames per launch on bound sequences
   float ring_scale = -1.0f;  // depth scale baked into the ring uint16 graph
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<hipEvent_t> ev_pending;  // groups of 4: start, after tracking, after alloc, after integrate
+  std::vector<int> ev_frames;          // frames of each group
   double alloc_ms = 0.0, integrate_ms = 0.0, track_ms = 0.0;
-  int64_t profiled_frames = 0;
+  int64_t profiled_frames = 0, profiled_launches = 0;
   // marching-cubes output buffer (grown on demand) and scan scratch
   float* mc_tris = nullptr;
   int32_t* mc_cubes = nullptr;
@@ -190,7 +195,8 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.inv_mu = 1.0f / c.trunc;
   a.cap = (float)c.weight_cap;
   a.hash = HashView{v->keys, v->vals, c.log2_buckets, ((uint64_t)1 << c.log2_buckets) - 1};
-  a.stamps = v->stamps;
+  a.marks = v->marks;
+  a.n_slots = v->n_slots;
   a.block_keys = v->block_keys;
   a.pool_cap = (uint32_t)c.pool_blocks;
   a.touched = v->touched;
@@ -264,24 +270,25 @@ int frame_u16(vh_volume* v, int which) {
   return which >= 2 ? v->seq_format == VH_DEPTH_U16 : which == 0;
 }
 
-// Kinds 0, 1, 3: allocation then integration of one frame. Bound sequences run as kind 4
-// (allocate the first frame), kind 2 per further frame (integrate the previous frame and
-// allocate this one in one launch) and kind 5 (integrate the last frame).
-int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale) {
+// Kinds 0, 1, 3, 6, 7: allocation then integration of one frame. Bound sequences run in
+// batches of up to `batch` frames: kind 4 (allocate the first batch, ka frames), kind 2 per
+// further batch (integrate the previous batch, ki frames, and allocate this one, ka frames,
+// in one launch) and kind 5 (integrate the last batch, ki frames).
+int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale, int ki, int ka) {
   const int fa = which == 4 || which == 5 ? 2 : which;
   FrameArgs a = frame_args(v, fa);
   const int u16 = frame_u16(v, fa);
   if (fa < 2 || fa >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
   if (which == 2) {
-    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, integrate_grid(), v->stream));
+    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, integrate_grid(), ki, ka, v->stream));
     return VH_OK;
   }
   if (which == 3) {
     int rc = enqueue_track_kernels(v);
     if (rc) return rc;
   }
-  if (which != 5) VH_HIP(launch_alloc(a, u16, v->stream));
-  if (which != 4) VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
+  if (which != 5) VH_HIP(launch_alloc(a, u16, which == 4 ? ka : 1, v->stream));
+  if (which != 4) VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), which == 5 ? ki : 1, v->stream));
   return VH_OK;
 }
 
@@ -296,31 +303,36 @@ hipEvent_t take_event(vh_volume* v) {
   return e;
 }
 
-int launch_frame(vh_volume* v, int which, float staged_scale) {
+// One launch of kind `which` (ki frames integrated, ka allocated; single frames: 1, 1). In
+// profiling mode a batch of k = max(ki, ka) frames runs eagerly as (tracking,) allocation of k
+// frames and integration of k frames, with events around each.
+int launch_frame(vh_volume* v, int which, float staged_scale, int ki = 1, int ka = 1) {
   if (v->profiling) {
     hipEvent_t e[4] = {take_event(v), take_event(v), take_event(v), take_event(v)};
     if (!e[0] || !e[1] || !e[2] || !e[3]) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
     FrameArgs a = frame_args(v, which);
     const int u16 = frame_u16(v, which);
     if (which < 2 || which >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
+    const int k = std::max(ki, ka);
     VH_HIP(hipEventRecord(e[0], v->stream));
     if (which == 3) {
       int rc = enqueue_track_kernels(v);
       if (rc) return rc;
     }
     VH_HIP(hipEventRecord(e[1], v->stream));
-    VH_HIP(launch_alloc(a, u16, v->stream));
+    VH_HIP(launch_alloc(a, u16, k, v->stream));
     VH_HIP(hipEventRecord(e[2], v->stream));
-    VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), v->stream));
+    VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), k, v->stream));
     VH_HIP(hipEventRecord(e[3], v->stream));
     for (hipEvent_t x : e) v->ev_pending.push_back(x);
+    v->ev_frames.push_back(k);
     return VH_OK;
   }
-  hipGraphExec_t& g = v->graph[which];
+  hipGraphExec_t& g = v->graphs[std::make_tuple(which, ki, ka)];
   if (!g) {
     hipGraph_t graph = nullptr;
     VH_HIP(hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_frame_kernels(v, which, staged_scale);
+    int rc = enqueue_frame_kernels(v, which, staged_scale, ki, ka);
     hipError_t e = hipStreamEndCapture(v->stream, &graph);
     if (rc != VH_OK) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -336,11 +348,14 @@ int launch_frame(vh_volume* v, int which, float staged_scale) {
 }
 
 void drop_graphs(vh_volume* v, int which_mask) {
-  for (int k = 0; k < vh_volume::kGraphs; ++k)
-    if ((which_mask >> k) & 1 && v->graph[k]) {
-      (void)hipGraphExecDestroy(v->graph[k]);
-      v->graph[k] = nullptr;
+  for (auto it = v->graphs.begin(); it != v->graphs.end();) {
+    if ((which_mask >> std::get<0>(it->first)) & 1) {
+      if (it->second) (void)hipGraphExecDestroy(it->second);
+      it = v->graphs.erase(it);
+    } else {
+      ++it;
     }
+  }
 }
 
 int collect_profile(vh_volume* v) {
@@ -352,10 +367,12 @@ int collect_profile(vh_volume* v) {
     v->track_ms += t;
     v->alloc_ms += a;
     v->integrate_ms += b;
-    v->profiled_frames += 1;
+    v->profiled_frames += v->ev_frames[k / 4];
+    v->profiled_launches += 1;
   }
   for (hipEvent_t e : v->ev_pending) v->ev_pool.push_back(e);
   v->ev_pending.clear();
+  v->ev_frames.clear();
   return VH_OK;
 }
 
@@ -387,6 +404,7 @@ int sync_and_check(vh_volume* v) {
 int reset_tables(vh_volume* v) {
   v->ring_next = 0;  // the device cursor of the ring restarts with the control block
   VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
+  VH_HIP(hipMemsetAsync(v->marks, 0, 2 * v->n_slots * 4, v->stream));
   FrameCtl c{};
   VH_HIP(hipMemcpyAsync(&c, v->ctl, sizeof c, hipMemcpyDeviceToHost, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
@@ -414,6 +432,8 @@ int validate(const vh_config* c) {
   if (c->storage == VH_STORAGE_F16R && (c->weight_cap < 1 || c->weight_cap > 255))
     return fail(VH_ERR_INVALID_ARG, "F16R storage needs weight_cap in 1..255");
   if (c->color != 0 && c->color != 1) return fail(VH_ERR_INVALID_ARG, "color must be 0 or 1");
+  if (c->seq_batch < 0 || c->seq_batch > kMaxBatch)
+    return fail(VH_ERR_INVALID_ARG, "seq_batch must be in 0..8");
   for (int32_t r : c->reserved)
     if (r) return fail(VH_ERR_INVALID_ARG, "reserved fields must be zero");
   return VH_OK;
@@ -445,7 +465,7 @@ void free_volume(vh_volume* v) {
   if (v->ring_pose_host) (void)hipHostFree(v->ring_pose_host);
   if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
-  void* ptrs[] = {v->keys, v->vals, v->stamps, v->block_keys, v->voxels, v->touched, v->ctl,
+  void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
                   v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba};
@@ -564,6 +584,7 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
   vh_volume* v = new (std::nothrow) vh_volume();
   if (!v) return fail(VH_ERR_OOM, "host allocation failed");
   v->cfg = *cfg;
+  v->batch = cfg->seq_batch ? cfg->seq_batch : kMaxBatch;
   v->device = cfg->device;
   if ((rc = set_device(v))) { delete v; return rc; }
   if (hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -576,18 +597,18 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
   const size_t pool = (size_t)cfg->pool_blocks;
   const size_t frame_px = (size_t)cfg->width * cfg->height;
   if ((rc = dev_alloc(&v->keys, v->n_slots * 8)) || (rc = dev_alloc(&v->vals, v->n_slots * 8)) ||
-      (rc = dev_alloc(&v->stamps, v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
+      (rc = dev_alloc(&v->marks, 2 * v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
       (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||
       (rc = dev_alloc(&v->touched, (size_t)2 * kStripes * v->stripe_cap * 4)) ||
       (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
       (rc = dev_alloc(&v->stage_depth, frame_px * 4)) || (rc = dev_alloc(&v->stage_pose, 16 * sizeof(float))) ||
       (cfg->color && ((rc = dev_alloc(&v->colors, pool * kB3 * 4)) || (rc = dev_alloc(&v->stage_rgb, frame_px * 3)) ||
-                      (rc = dev_alloc(&v->rgba, 2 * frame_px * 4))))) {
+                      (rc = dev_alloc(&v->rgba, 2 * kMaxBatch * frame_px * 4))))) {
     std::string msg = g_last_error;
     free_volume(v);
     return fail(rc, msg);
   }
-  hipError_t e = hipMemsetAsync(v->stamps, 0, v->n_slots * 4, v->stream);
+  hipError_t e = hipMemsetAsync(v->marks, 0, 2 * v->n_slots * 4, v->stream);
   if (e == hipSuccess) e = hipMemsetAsync(v->ctl, 0, sizeof(FrameCtl), v->stream);
   if (e != hipSuccess) {
     free_volume(v);
@@ -812,21 +833,32 @@ int vh_integrate_next(vh_volume* v, int32_t n) {
   std::lock_guard<std::mutex> lk(v->mu);
   int rc = set_device(v);
   if (rc) return rc;
-  if (v->track || v->profiling) {
-    // per frame: (tracking,) allocation, integration
+  if (v->track) {
+    // per frame: tracking, allocation, integration (each pose depends on the volume so far)
     for (int k = 0; k < n; ++k) {
-      rc = launch_frame(v, v->track ? 3 : 2, v->seq_scale);
+      rc = launch_frame(v, 3, v->seq_scale);
       if (rc) return rc;
     }
     return VH_OK;
   }
-  // allocation of frame k+1 overlaps the integration of frame k; nothing stays allocated
+  // batches of up to `batch` frames; the last may be shorter
+  const int K = v->batch;
+  if (v->profiling) {
+    for (int f = 0; f < n; f += K)
+      if ((rc = launch_frame(v, 2, v->seq_scale, std::min(K, n - f), std::min(K, n - f)))) return rc;
+    return VH_OK;
+  }
+  // allocation of batch b+1 overlaps the integration of batch b; nothing stays allocated
   // but unintegrated when the call returns
   if (n == 0) return VH_OK;
-  if ((rc = launch_frame(v, 4, v->seq_scale))) return rc;
-  for (int k = 1; k < n; ++k)
-    if ((rc = launch_frame(v, 2, v->seq_scale))) return rc;
-  return launch_frame(v, 5, v->seq_scale);
+  int prev = std::min(K, n);
+  if ((rc = launch_frame(v, 4, v->seq_scale, 0, prev))) return rc;
+  for (int f = prev; f < n; f += K) {
+    const int cur = std::min(K, n - f);
+    if ((rc = launch_frame(v, 2, v->seq_scale, prev, cur))) return rc;
+    prev = cur;
+  }
+  return launch_frame(v, 5, v->seq_scale, prev, 0);
 }
 
 int vh_sync(vh_volume* v) {
@@ -847,7 +879,7 @@ int vh_set_profiling(vh_volume* v, int32_t on) {
   if (rc) return rc;
   v->profiling = on != 0;
   v->alloc_ms = v->integrate_ms = v->track_ms = 0.0;
-  v->profiled_frames = 0;
+  v->profiled_frames = v->profiled_launches = 0;
   return VH_OK;
 }
 
@@ -873,6 +905,8 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->integrate_ms = v->integrate_ms;
   s->alloc_ms = v->alloc_ms;
   s->profiled_frames = v->profiled_frames;
+  s->profiled_launches = v->profiled_launches;
+  s->blocks_integrated = (int64_t)c.integrated_total;
   s->track_ms = v->track_ms;
   return VH_OK;
 }

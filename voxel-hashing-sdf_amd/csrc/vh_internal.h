@@ -5,13 +5,14 @@
 //                                   a bucket is 4 consecutive slots (32 B), probing walks
 //                                   consecutive buckets;
 //   vals[4 * 2^log2_buckets]        u64 per slot: pool index (low 32) | frame of birth (high 32);
-//   stamps[4 * 2^log2_buckets]      u32 per slot: last frame that touched the block;
+//   marks[2][4 * 2^log2_buckets]    u32 per slot and batch parity: bit j set iff frame j of the
+//                                   batch touched the block (cleared when the batch is integrated);
 //   block_keys[pool_blocks]         u64 key of each pool block (pool order);
 //   voxels[pool_blocks * 512]       FP32: float2 {tsdf, weight}; FP16: u32 {f16 tsdf | u16 w << 16};
 //   colors[pool_blocks * 512]       (colour volumes only) u32 {r | g << 8 | b << 16};
-//   touched[2][kStripes * stripe_cap] u32 slot ids of the blocks crossed by a frame's bands,
-//                                   in kStripes sub-lists with separate counters, one list
-//                                   per frame parity;
+//   touched[2][kStripes * stripe_cap] u32 slot ids of the blocks crossed by a batch's bands
+//                                   (each once), in kStripes sub-lists with separate
+//                                   counters, one list per batch parity;
 //   ctl                             counters of the frame pipeline (FrameCtl).
 //
 // Device-scope atomics on one address serialise (about 11 ns each on MI355X), so every
@@ -39,20 +40,25 @@ __device__ __forceinline__ float recip_rn(float z) {
 // RN(a/b) unless the quotient is subnormal). 6 instructions instead of 11; for 0 < |a| <
 // 2^-76 (where a/b could be subnormal) the IEEE division is taken instead. The check over
 // all a for b = 1..256 is tests/device_checks/recip_exact.hip.
+// The fallback branch is taken by the wave when any active lane needs it (a uniform branch
+// on the ballot of the active lanes), so the common path carries no per-lane branch.
 __device__ __forceinline__ float div_rn(float a, float b) {
-  if (__builtin_expect(__builtin_fabsf(a) < 0x1p-76f && a != 0.0f, 0)) return a / b;
   const float y = recip_rn(b);
   const float q0 = a * y;
-  return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+  float q = __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+  const bool tiny = __builtin_fabsf(a) < 0x1p-76f && a != 0.0f;
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(tiny) != 0, 0)) q = tiny ? a / b : q;
+  return q;
 }
 
 enum : uint32_t { kFlagPoolFull = 1u, kFlagHashFull = 2u, kFlagRange = 4u, kFlagTouchedFull = 8u };
 constexpr int kMaxProbe = 64;  // buckets examined before a key is declared unplaceable
 constexpr int kStripes = 64;   // striped counters / touched sub-lists
 constexpr int kLine = 32;      // u32 per 128-byte line
+constexpr int kMaxBatch = 8;   // frames integrated per launch at most
 
 struct FrameCtl {
-  uint32_t frame_id;       // stamp of the frame being integrated (starts at 1)
+  uint32_t frame_id;       // id of the first frame of the batch being integrated (starts at 1)
   uint32_t pool_count;     // keys inserted (pool indices handed out, may pass capacity)
   uint32_t flags;          // kFlag* raised since the host last read them
   uint32_t touched_last;
@@ -62,8 +68,12 @@ struct FrameCtl {
   unsigned long long touched_total;
   unsigned long long frames;
   unsigned long long new_total;
-  uint32_t pad0[18];
-  uint32_t tcount[2 * kStripes * kLine];  // touched sub-list lengths (one line each), per frame parity
+  unsigned long long integrated_total;  // sum over batches of the blocks listed (read + written once)
+  uint32_t batch_id;       // batches integrated; its parity selects the lists and marks
+  uint32_t pad0[15];
+  // per batch parity, one line per stripe: word 0 = listed blocks, word 1 + j = blocks frame
+  // j of the batch touched first in the stripe
+  uint32_t tcount[2 * kStripes * kLine];
   uint32_t done[kStripes * kLine];        // workgroups of the frame launch finished, per stripe
 };
 
@@ -87,7 +97,8 @@ struct FrameArgs {
   float max_depth, min_depth, vox, bsz, mu, inv_mu, cap;
   // tables
   HashView hash;
-  uint32_t* stamps;
+  uint32_t* marks;         // 2 x n_slots
+  uint64_t n_slots;
   unsigned long long* block_keys;
   uint32_t pool_cap;
   uint32_t* touched;
@@ -98,8 +109,9 @@ struct FrameArgs {
   // frame f at rgb + f*W*H*3) and the colour pool, one u32 {r | g << 8 | b << 16} per voxel
   const uint8_t* rgb;
   uint32_t* colors;
-  // colour volumes: the allocation pass packs each pixel of frame f into one word
-  // {r | g << 8 | b << 16} at rgba + (f & 1) * W*H, which the integration of frame f gathers
+  // colour volumes: the allocation pass packs each pixel of frame j of a batch into one word
+  // {r | g << 8 | b << 16} at rgba + (parity * kMaxBatch + j) * W*H, which the integration
+  // of the batch gathers
   uint32_t* rgba;
 };
 
@@ -177,13 +189,14 @@ struct TrackArgs {
 };
 
 // launchers (vh_kernels.hip); all asynchronous on `st`
-hipError_t launch_alloc(const FrameArgs& a, int depth_u16, hipStream_t st);
-hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid,
+// Allocation of the next k frames (one batch, 1 <= k <= kMaxBatch), then their integration.
+hipError_t launch_alloc(const FrameArgs& a, int depth_u16, int k, hipStream_t st);
+hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, int k,
                             hipStream_t st);
-// Integration of the current frame and allocation of the next one in one launch (bound
-// sequences; the next frame's allocation must not have run).
+// Integration of the allocated batch (k_int frames) and allocation of the next one (k_alloc
+// frames) in one launch (bound sequences; the next batch's allocation must not have run).
 hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16, int grid,
-                                  hipStream_t st);
+                                  int k_int, int k_alloc, hipStream_t st);
 // count pass, two-level exclusive scan, write pass (all on `st`)
 hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,

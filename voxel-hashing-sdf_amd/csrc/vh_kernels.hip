@@ -1,8 +1,11 @@
 // vh_kernels.hip - gfx950 kernels of libvh.
 //
-// Per frame (one captured graph): k_alloc (one lane per pixel: truncation-band DDA, hash
-// insert, first-touch dedup, compaction of the touched list) -> k_integrate (one 256-lane
-// workgroup per touched block, grid-stride; the last workgroup to finish closes the frame).
+// Per batch of k consecutive frames (one captured graph per launch): k_alloc (one lane per
+// pixel of each frame: truncation-band DDA, hash insert, per-frame marks, compaction of the
+// batch's block list) -> k_integrate (one wave per listed block, grid-stride: the block's
+// voxels are read once, updated by each frame that touched it, written once; the last
+// workgroup to finish closes the batch). Bound sequences run k_integrate_alloc: batch b is
+// integrated while batch b+1 is allocated.
 // Extraction: k_mc (one workgroup per pool block; 9^3 halo in LDS; wave prefix sums).
 // Queries: k_query / k_backward / k_gn_accum + k_gn_solve (FP64 reductions).
 //
@@ -28,6 +31,8 @@ __device__ __forceinline__ uint32_t read_lane_u32(uint32_t v, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 __device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+// A value every lane holds identically, moved to a scalar register.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 __device__ __forceinline__ void raise_flag(FrameCtl* ctl, uint32_t f) { atomicOr(&ctl->flags, f); }
 
@@ -86,17 +91,14 @@ __device__ __forceinline__ float depth_at(const void* base, float scale, int pix
   return reinterpret_cast<const float*>(base)[(uint32_t)pix];
 }
 
-__device__ __forceinline__ uint32_t frame_slot(const FrameArgs& a) {
-  return a.ctl->cursor[a.cursor_id] % a.n_seq;
-}
-
 // ============================================================================ allocation
 // Workgroup = 16x16 pixels. Phase 1: every lane walks its pixel's band and inserts the block
 // keys into an LDS hash set, so each distinct block of the tile is kept once (a tile's ~200
 // pixel-block visits typically reduce to ~20 keys). Phase 2: the distinct keys go to the
-// global hash (find or insert), take a pool index if new, claim the block for this frame
-// with an exchange on its stamp, and the claimers append the slot to the workgroup's stripe
-// of the touched list.
+// global hash (find or insert), take a pool index if new, and mark the block for this frame
+// of the batch (an atomic OR of the frame's bit into the slot's mark); the lanes that found
+// the mark empty append the slot to the workgroup's stripe of the batch's block list, and
+// the lanes that found their frame's bit clear count the block for that frame.
 constexpr int kSetSlots = 1024;  // LDS key set (8 KB)
 constexpr int kUniqCap = 1024;   // distinct keys kept per tile (8 KB)
 
@@ -104,19 +106,32 @@ __device__ __forceinline__ uint32_t set_hash(uint64_t key) {
   return (uint32_t)((key * 0x9e3779b97f4a7c15ull) >> 54) & (kSetSlots - 1);
 }
 
-// The touched list and its stripe lengths are double-buffered by frame parity, so the
-// allocation of frame f+1 can run while frame f is integrated.
-__device__ __forceinline__ uint32_t* tcount_of(FrameCtl* ctl, uint32_t frame) {
-  return ctl->tcount + (frame & 1u) * (kStripes * kLine);
+// The block list, its stripe lengths and the slot marks are double-buffered by batch
+// parity, so the allocation of batch b+1 can run while batch b is integrated. Line s of a
+// parity's counters: word 0 = listed blocks of stripe s, word 1 + j = blocks that frame j of
+// the batch touched first in that stripe.
+__device__ __forceinline__ uint32_t* list_count_of(FrameCtl* ctl, uint32_t parity) {
+  return ctl->tcount + parity * (kStripes * kLine);
 }
-__device__ __forceinline__ uint32_t* touched_of(const FrameArgs& a, uint32_t frame) {
-  return a.touched + (size_t)(frame & 1u) * kStripes * a.stripe_cap;
+__device__ __forceinline__ uint32_t* list_of(const FrameArgs& a, uint32_t parity) {
+  return a.touched + (size_t)parity * kStripes * a.stripe_cap;
+}
+__device__ __forceinline__ uint32_t* marks_of(const FrameArgs& a, uint32_t parity) {
+  return a.marks + (size_t)parity * a.n_slots;
+}
+// Input slot of frame j of a batch whose first frame is input `cur` (consecutive, wrapping).
+__device__ __forceinline__ uint32_t input_slot(uint32_t cur, uint32_t j, uint32_t n_seq) {
+  uint32_t fs = cur % n_seq + j;
+  while (fs >= n_seq) fs -= n_seq;
+  return fs;
 }
 
 // Global part for the active lanes' keys: find-or-insert, pool index for new keys (one
-// atomic per wave), first touch of the frame, append to the stripe (one atomic per wave).
-__device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t frame, int lane,
-                           uint32_t stripe) {
+// atomic per wave), the frame's mark, the list append and the frame's count (one atomic per
+// wave each). `frame` is the frame's id (stored as the birth frame of new blocks), jb its
+// index in the batch.
+__device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t frame, uint32_t jb,
+                           uint32_t parity, int lane, uint32_t stripe) {
   FrameCtl* ctl = a.ctl;
   bool inserted = false;
   int64_t slot = -1;
@@ -124,15 +139,16 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
     slot = find_or_insert(a.hash, key, inserted);
     if (slot < 0) raise_flag(ctl, kFlagHashFull);
   }
-  // The pool reservation and the first-touch exchange depend only on the slot: both are in
-  // flight together, and the slot-value stores go out just before the stripe atomic, so
-  // the chain is bucket load -> (CAS) -> pool + stamp -> stripe.
+  // The pool reservation and the mark depend only on the slot: both are in flight together,
+  // and the slot-value stores go out just before the stripe atomics, so the chain is bucket
+  // load -> (CAS) -> pool + mark -> stripe.
   const uint64_t im = __ballot(inserted);
   const int leader = im ? first_lane(im) : 0;
   uint32_t pbase = 0;
   if (im && lane == leader) pbase = atomicAdd(&ctl->pool_count, (uint32_t)__popcll(im));
-  uint32_t old_stamp = frame;
-  if (slot >= 0) old_stamp = atomicExch(&a.stamps[slot], frame);
+  const uint32_t bit = 1u << jb;
+  uint32_t old_mark = bit;
+  if (slot >= 0) old_mark = atomicOr(&marks_of(a, parity)[slot], bit);
   if (im) {
     pbase = read_lane_u32(pbase, leader);
     if (inserted) {
@@ -147,24 +163,29 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
       a.hash.vals[slot] = (unsigned long long)stored | ((unsigned long long)frame << 32);
     }
   }
-  const bool first = old_stamp != frame;
+  uint32_t* line = list_count_of(ctl, parity) + stripe * kLine;
+  const uint64_t cm = __ballot((old_mark & bit) == 0u);  // first touch of the block by this frame
+  if (cm && lane == first_lane(cm)) atomicAdd(&line[1 + jb], (uint32_t)__popcll(cm));
+  const bool first = old_mark == 0u;  // first touch by the batch: list the block
   const uint64_t fm = __ballot(first);
   if (fm) {
-    const int leader = first_lane(fm);
+    const int fl = first_lane(fm);
     uint32_t tbase = 0;
-    if (lane == leader) tbase = atomicAdd(&tcount_of(ctl, frame)[stripe * kLine], (uint32_t)__popcll(fm));
-    tbase = read_lane_u32(tbase, leader);
+    if (lane == fl) tbase = atomicAdd(&line[0], (uint32_t)__popcll(fm));
+    tbase = read_lane_u32(tbase, fl);
     if (first) {
       const uint32_t pos = tbase + lanes_below(fm);
-      if (pos < a.stripe_cap) touched_of(a, frame)[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
+      if (pos < a.stripe_cap) list_of(a, parity)[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
       else raise_flag(ctl, kFlagTouchedFull);
     }
   }
 }
 
-// Allocation of one 16x16 tile of frame `frame` (input slot fs); 256 lanes.
+// Allocation of one 16x16 tile of frame `frame` (index jb of its batch, input slot fs);
+// 256 lanes.
 template <bool kU16>
-__device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, uint32_t fs) {
+__device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, uint32_t jb, uint32_t parity,
+                           uint32_t fs) {
   __shared__ unsigned long long s_set[kSetSlots];
   __shared__ unsigned long long s_uniq[kUniqCap];
   __shared__ uint32_t s_nuniq;
@@ -191,7 +212,8 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
     if (a.rgba) {  // the pixel's colour, packed for the integration's one-word gathers
       const size_t pix = (size_t)v * a.W + u;
       const uint8_t* px = a.rgb + ((size_t)fs * a.W * a.H + pix) * 3;
-      a.rgba[(size_t)(frame & 1u) * a.W * a.H + pix] = (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
+      a.rgba[((size_t)parity * kMaxBatch + jb) * a.W * a.H + pix] =
+          (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
     }
     const float d = depth_at<kU16>(base, a.depth_scale, v * a.W + u);
     if (depth_valid(d, a.max_depth)) {
@@ -218,7 +240,7 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
         h = (h + 1) & (kSetSlots - 1);
       }
     }
-    if (__ballot(spill)) claim_keys(a, key, spill, frame, lane, stripe);
+    if (__ballot(spill)) claim_keys(a, key, spill, frame, jb, parity, lane, stripe);
     if (active) active = band_next(s);
   }
   __syncthreads();
@@ -229,13 +251,17 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
     const bool act = i < n;
     const uint32_t wave_first = b0 + (uint32_t)(wave << 6);
     if (wave_first >= n) break;  // whole wave idle (uniform per wave)
-    claim_keys(a, act ? s_uniq[i] : kEmptyKey, act, frame, lane, stripe);
+    claim_keys(a, act ? s_uniq[i] : kEmptyKey, act, frame, jb, parity, lane, stripe);
   }
 }
 
+// Allocation of the batch about to be integrated: `tiles` workgroups per frame.
 template <bool kU16>
-__global__ __launch_bounds__(256) void k_alloc(FrameArgs a) {
-  alloc_tile<kU16>(a, blockIdx.x, a.ctl->frame_id, frame_slot(a));
+__global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
+  const uint32_t jb = blockIdx.x / tiles;
+  const FrameCtl* ctl = a.ctl;
+  alloc_tile<kU16>(a, blockIdx.x - jb * tiles, ctl->frame_id + jb, jb, ctl->batch_id & 1u,
+                   input_slot(ctl->cursor[a.cursor_id], jb, a.n_seq));
 }
 
 // ============================================================================ integration
@@ -302,23 +328,30 @@ __device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, uint32
   return out;
 }
 
-// One wave per touched block: lane l owns voxels l + 64k (x = k, y = l >> 3, z = l & 7), so
+// One wave per listed block: lane l owns voxels l + 64k (x = k, y = l >> 3, z = l & 7), so
 // each of its 8 loads is one coalesced 512-byte wave access and all 8 are in flight at once.
-// Worker w of `workers` (one wave each) takes an equal share of frame `frame`'s touched list
-// (input slot fs), its 64 stripes seen as one index space; workers synchronise only within
-// their own wave.
-// kColor: the colour pool is updated from the frame's RGB image at the same pixels.
+// A launch integrates a batch of k frames (ids f0 .. f0+k-1, inputs cur .. cur+k-1): the list
+// holds every block that any of them touched, once, with the mask of those frames. The
+// block's voxels are read once, updated by each marked frame in frame order (each update
+// unpacks and packs the storage word, exactly as that frame's own load and store would), and
+// written once; a voxel gets the same sequence of updates as frame-by-frame integration.
+// Worker w of `workers` (one wave each) takes an equal share of the list, its 64 stripes
+// seen as one index space; workers synchronise only within their own wave.
+// kColor: the colour pool is updated from each frame's packed RGB at the same pixels.
+// s_pose: the batch's k poses (row-major 3x4 each), staged in LDS by stage_poses.
 template <int kStorage, bool kU16, bool kColor>
-__device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t frame, uint32_t fs,
-                                 unsigned long long* s_val, unsigned long long* s_key) {
+__device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t f0, uint32_t k,
+                                 uint32_t cur, uint32_t parity, unsigned long long* s_val, unsigned long long* s_key,
+                                 uint32_t* s_mark, const float (*s_pose)[12]) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
   const int lane = threadIdx.x & 63;
-  const uint32_t* tcount = tcount_of(ctl, frame);
-  const uint32_t* touched = touched_of(a, frame);
+  const uint32_t* lcount = list_count_of(ctl, parity);
+  const uint32_t* list = list_of(a, parity);
+  uint32_t* marks = marks_of(a, parity);
   // lane s holds stripe s's length and exclusive offset
-  const uint32_t my_len = min(tcount[lane * kLine], a.stripe_cap);
+  const uint32_t my_len = min(lcount[lane * kLine], a.stripe_cap);
   uint32_t incl = my_len;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -327,22 +360,16 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
   }
   const uint32_t my_off = incl - my_len;
   const uint32_t n = read_lane_u32(incl, 63);
-  const float* Tp = a.poses + 16 * (size_t)fs;
-  float T[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) T[k] = Tp[k];
   const size_t elem = kU16 ? 2 : 4;
-  const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
-  auto dep = [&](int pix) { return depth_at<kU16>(base, a.depth_scale, pix); };
-  const uint32_t* rgba = kColor ? a.rgba + (size_t)(frame & 1u) * a.W * a.H : nullptr;
-  BlockFrame bf;
-  block_frame(T, a.vox, bf);
+  const size_t frame_px = (size_t)a.W * a.H;
+  const uint32_t fs0 = uniform(cur % a.n_seq);  // the division runs on the vector unit
 
-  // Lane j fetches the list entry, slot value and key of this worker's j-th block and parks
-  // them in LDS, so the per-block chain is a single round trip (voxels + depth). The block
-  // loop reads them back from LDS: a use of the loaded registers inside the loop would make
-  // the compiler wait for every outstanding vector memory operation, the previous block's
-  // stores included (vmcnt counts stores on this architecture).
+  // Lane j fetches the list entry, slot value, key and mark of this worker's j-th block
+  // (clearing the mark for the next batch of this parity) and parks them in LDS, so the
+  // per-block chain is a single round trip (voxels + depth). The block loop reads them back
+  // from LDS: a use of the loaded registers inside the loop would make the compiler wait for
+  // every outstanding vector memory operation, the previous block's stores included (vmcnt
+  // counts stores on this architecture).
   for (uint32_t t0 = worker; t0 < n; t0 += 64u * workers) {
     const uint32_t tj = t0 + (uint32_t)lane * workers;
     const uint32_t tq = min(tj, n - 1u);
@@ -356,84 +383,111 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
     }
     const uint32_t stripe_off = (uint32_t)__shfl((int)my_off, lo);
     unsigned long long jval = ~0ull, jkey = 0ull;
+    uint32_t jmark = 0u;
     if (tj < n) {
-      const uint32_t sl = touched[(size_t)lo * a.stripe_cap + (tj - stripe_off)];
+      const uint32_t sl = list[(size_t)lo * a.stripe_cap + (tj - stripe_off)];
       jval = a.hash.vals[sl];
       jkey = a.hash.keys[sl];
+      jmark = marks[sl];
+      marks[sl] = 0u;
     }
     s_val[lane] = jval;
     s_key[lane] = jkey;
+    s_mark[lane] = jmark;
     __builtin_amdgcn_wave_barrier();  // one wave owns this slice: it only needs its own writes
     const uint32_t nj = min(64u, (n - t0 + workers - 1) / workers);
     for (uint32_t j = 0; j < nj; ++j) {
       const unsigned long long val = s_val[j];
       const unsigned long long keyj = s_key[j];
+      const uint32_t mark = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_mark[j]) & ((1u << k) - 1u);
       const uint32_t idx = (uint32_t)val;
       if (idx >= a.pool_cap) {
         if (lane == 0) raise_flag(ctl, kFlagPoolFull);
         continue;
       }
-      const bool fresh = (uint32_t)(val >> 32) == frame;
+      const bool fresh = (uint32_t)(val >> 32) - f0 < k;  // born in this batch: starts at zero
       int bx, by, bz;
       unpack_key(keyj, bx, by, bz);
       Word* vp = reinterpret_cast<Word*>(a.voxels) + (size_t)idx * kB3 + lane;
       Word old[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) old[k] = fresh ? IO::zero() : vp[64 * k];
+      for (int q = 0; q < 8; ++q) old[q] = fresh ? IO::zero() : vp[64 * q];
       uint32_t* cp = kColor ? a.colors + (size_t)idx * kB3 + lane : nullptr;
       uint32_t cold[8];
       if constexpr (kColor) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) cold[k] = fresh ? 0u : cp[64 * k];
+        for (int q = 0; q < 8; ++q) cold[q] = fresh ? 0u : cp[64 * q];
       }
-      float cb[3], cl[3];
-      block_origin_cam(T, a.vox, bx, by, bz, cb);
-      column_cam(bf, cb, lane >> 3, lane & 7, cl);
-      // all projections, then all depth fetches, then the updates: the 8 fetches are in
-      // flight together instead of one round trip per voxel
-      float zc[8];
-      int pix[8];
+      uint32_t upd = 0u;  // bit q: voxel q observed by a frame of the batch
+      for (uint32_t m = mark; m != 0u; m &= m - 1u) {
+        const uint32_t jf = (uint32_t)__builtin_ctz(m);
+        uint32_t fs = fs0 + jf;
+        while (fs >= a.n_seq) fs -= a.n_seq;
+        float T[12];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float fk = (float)k;
-        zc[k] = __builtin_fmaf(fk, bf.ax[2], cl[2]);
-        pix[k] = project_cam(__builtin_fmaf(fk, bf.ax[0], cl[0]), __builtin_fmaf(fk, bf.ax[1], cl[1]), zc[k],
-                             recip_rn(zc[k]), a.K, a.W, a.H);
-      }
-      float dv[8];
+        for (int q = 0; q < 12; ++q) T[q] = s_pose[jf][q];
+        BlockFrame bf;
+        block_frame(T, a.vox, bf);
+        const void* base = reinterpret_cast<const char*>(a.depth) + fs * frame_px * elem;
+        float cb[3], cl[3];
+        block_origin_cam(T, a.vox, bx, by, bz, cb);
+        column_cam(bf, cb, lane >> 3, lane & 7, cl);
+        // all projections, then all depth fetches, then the updates: the 8 fetches are in
+        // flight together instead of one round trip per voxel
+        float zc[8];
+        int pix[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) dv[k] = dep(pix[k] < 0 ? 0 : pix[k]);
-      uint32_t px[8];
-      if constexpr (kColor) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) px[k] = rgba[pix[k] < 0 ? 0 : pix[k]];
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        float dist;
-        const bool ok = truncate_obs(pix[k], dv[k], zc[k], a.max_depth, a.mu, a.inv_mu, dist);
-        float sv, wv;
-        IO::unpack(old[k], sv, wv);
-        if constexpr (kColor) {
-          if (ok) cp[64 * k] = color_update(cold[k], wv, px[k]);
-          else if (fresh) cp[64 * k] = 0u;
+        for (int q = 0; q < 8; ++q) {
+          const float fq = (float)q;
+          zc[q] = __builtin_fmaf(fq, bf.ax[2], cl[2]);
+          pix[q] = project_cam(__builtin_fmaf(fq, bf.ax[0], cl[0]), __builtin_fmaf(fq, bf.ax[1], cl[1]), zc[q],
+                               recip_rn(zc[q]), a.K, a.W, a.H);
         }
-        mean_update(sv, wv, dist, a.cap);
-        if (ok) vp[64 * k] = IO::pack(sv, wv);
-        else if (fresh) vp[64 * k] = IO::zero();
+        float dv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) dv[q] = depth_at<kU16>(base, a.depth_scale, pix[q] < 0 ? 0 : pix[q]);
+        uint32_t px[8];
+        if constexpr (kColor) {
+          const uint32_t* rgba = a.rgba + ((size_t)parity * kMaxBatch + jf) * frame_px;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) px[q] = rgba[pix[q] < 0 ? 0 : pix[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // branch-free: every lane computes, the flag selects
+          float dist;
+          const bool ok = truncate_obs(pix[q], dv[q], zc[q], a.max_depth, a.mu, a.inv_mu, dist);
+          float sv, wv;
+          IO::unpack(old[q], sv, wv);
+          if constexpr (kColor) {
+            const uint32_t c = color_update(cold[q], wv, px[q]);
+            cold[q] = ok ? c : cold[q];
+          }
+          mean_update(sv, wv, dist, a.cap);
+          const Word nw = IO::pack(sv, wv);
+          old[q] = ok ? nw : old[q];
+          upd |= (uint32_t)ok << q;
+        }
       }
+      // fresh blocks are written whole (never-observed voxels as zero), others where observed
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (fresh || ((upd >> q) & 1u)) {
+          vp[64 * q] = old[q];
+          if constexpr (kColor) cp[64 * q] = cold[q];
+        }
     }
-    __builtin_amdgcn_wave_barrier();  // the next batch overwrites s_val / s_key
+    __builtin_amdgcn_wave_barrier();  // the next batch of entries overwrites s_val / s_key / s_mark
   }
 }
 
-// End of a workgroup of a frame launch (all 256 lanes call it): the last of `groups`
-// workgroups to finish (two-level striped counter) closes frame `frame`: its stripe lengths
-// are summed into the statistics and cleared, the frame id and the input cursor advance.
-// No fence before the counters: the closing workgroup reads no voxel data, and the voxel
-// writes are published by the kernel boundary (a device-scope fence per workgroup would
-// write back the L2 thousands of times per frame).
-__device__ void frame_done(const FrameArgs& a, uint32_t groups, uint32_t frame) {
+// End of a workgroup of a batch launch (all 256 lanes call it): the last of `groups`
+// workgroups to finish (two-level striped counter) closes the batch (first frame f0, k
+// frames, list parity): its per-frame counts are summed into the statistics and cleared with
+// the list lengths; the frame id, input cursor and batch id advance. No fence before the
+// counters: the closing workgroup reads no voxel data, and the voxel writes are published by
+// the kernel boundary (a device-scope fence per workgroup would write back the L2 thousands
+// of times per batch).
+__device__ void batch_done(const FrameArgs& a, uint32_t groups, uint32_t f0, uint32_t k, uint32_t parity) {
   __shared__ uint32_t s_closes;
   FrameCtl* ctl = a.ctl;
   __syncthreads();
@@ -447,67 +501,102 @@ __device__ void frame_done(const FrameArgs& a, uint32_t groups, uint32_t frame) 
   }
   __syncthreads();
   if (threadIdx.x >= 64 || !s_closes) return;
-  // The stripe lengths were final before this launch started (the allocation of this frame
-  // ran earlier): each lane reads and clears one stripe, one round trip in all.
+  // The counts were final before this launch started (the batch's allocation ran earlier):
+  // each lane reads and clears one stripe's line, one round trip in all.
   const int lane = threadIdx.x;
-  uint32_t* tcount = tcount_of(ctl, frame);
-  const uint32_t c = min(tcount[lane * kLine], a.stripe_cap);
-  tcount[lane * kLine] = 0u;
-  ctl->done[lane * kLine] = 0u;
-  uint32_t total = c;
+  uint32_t* line = list_count_of(ctl, parity) + lane * kLine;
+  uint32_t listed = min(line[0], a.stripe_cap);
+  uint32_t per[kMaxBatch];
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) total += (uint32_t)__shfl_xor((int)total, m);
+  for (int j = 0; j < kMaxBatch; ++j) per[j] = (uint32_t)j < k ? line[1 + j] : 0u;
+#pragma unroll
+  for (int j = 0; j <= kMaxBatch; ++j)
+    if ((uint32_t)j <= k) line[j] = 0u;
+  ctl->done[lane * kLine] = 0u;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    listed += (uint32_t)__shfl_xor((int)listed, m);
+#pragma unroll
+    for (int j = 0; j < kMaxBatch; ++j) per[j] += (uint32_t)__shfl_xor((int)per[j], m);
+  }
   if (lane == 0) {
-    ctl->touched_last = total;
-    ctl->touched_total += total;
-    ctl->frames += 1ull;
-    ctl->cursor[a.cursor_id] += 1u;
-    ctl->frame_id = frame + 1u;
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxBatch; ++j) sum += per[j];
+    uint32_t last = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxBatch; ++j)
+      if ((uint32_t)j + 1u == k) last = per[j];
+    ctl->touched_last = last;
+    ctl->touched_total += sum;
+    ctl->integrated_total += listed;
+    ctl->frames += k;
+    ctl->cursor[a.cursor_id] += k;
+    ctl->frame_id = f0 + k;
+    ctl->batch_id += 1u;
     ctl->done_top = 0u;
   }
 }
 
 constexpr int kIntegrateWaves = 4;  // independent workers per workgroup
 
-template <int kStorage, bool kU16, bool kColor>
-__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a) {
-  __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
-  const int wave = threadIdx.x >> 6;
-  const uint32_t frame = a.ctl->frame_id;
-  integrate_worker<kStorage, kU16, kColor>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, frame,
-                                   frame_slot(a), s_val[wave], s_key[wave]);
-  frame_done(a, gridDim.x, frame);
+// The k poses of a batch (inputs cur .. cur+k-1) into LDS, read by every block of the
+// workgroup's workers (a global load per block and frame would put a memory round trip in
+// front of each frame's projections). All 256 lanes call it.
+__device__ __forceinline__ void stage_poses(const FrameArgs& a, uint32_t cur, uint32_t k, float (*s_pose)[12]) {
+  const uint32_t t = threadIdx.x;
+  if (t < 12u * k) {
+    const uint32_t j = t / 12u, q = t - 12u * j;
+    s_pose[j][q] = a.poses[16 * (size_t)input_slot(cur, j, a.n_seq) + q];
+  }
+  __syncthreads();
 }
 
-// Integration of frame f and allocation of frame f+1 in one launch (bound sequences): the
-// first 2 * min(I, A) workgroups alternate between the two roles so that both start at once
-// (the allocation is latency-bound, the integration throughput-bound); the rest take the
-// remaining role. Every workgroup counts towards the close of frame f, so the frame id and
-// cursor the allocation workgroups read at their start cannot advance under them.
+// Integration of the batch of k frames that the last allocation launch prepared.
+template <int kStorage, bool kU16, bool kColor>
+__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a, uint32_t k) {
+  __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
+  __shared__ uint32_t s_mark[kIntegrateWaves][64];
+  __shared__ float s_pose[kMaxBatch][12];
+  const int wave = threadIdx.x >> 6;
+  const FrameCtl* ctl = a.ctl;
+  // control values are uniform: keep them (and what derives from them) in scalar registers
+  const uint32_t f0 = uniform(ctl->frame_id), parity = uniform(ctl->batch_id) & 1u,
+                 cur = uniform(ctl->cursor[a.cursor_id]);
+  stage_poses(a, cur, k, s_pose);
+  integrate_worker<kStorage, kU16, kColor>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, f0,
+                                           k, cur, parity, s_val[wave], s_key[wave], s_mark[wave], s_pose);
+  batch_done(a, gridDim.x, f0, k, parity);
+}
+
+// Integration of batch b (k_int frames) and allocation of batch b+1 (k_alloc frames) in one
+// launch (bound sequences): the integration workgroups come first in the grid, so they take
+// every CU slot at the start, and the allocation workgroups (short, latency-bound) fill the
+// slots that the integration's tail frees. Every workgroup counts towards the close of batch
+// b, so the frame id, cursor and batch id the allocation workgroups read at their start
+// cannot advance under them.
 template <int kStorage, bool kU16, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate_alloc(FrameArgs a, uint32_t int_groups,
-                                                                         uint32_t alloc_groups) {
+                                                                         uint32_t alloc_groups, uint32_t tiles,
+                                                                         uint32_t k_int) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
-  const uint32_t b = blockIdx.x, m = min(int_groups, alloc_groups);
-  bool alloc;
-  uint32_t idx;
-  if (b < 2 * m) {
-    alloc = b & 1u;
-    idx = b >> 1;
-  } else {
-    alloc = alloc_groups > int_groups;
-    idx = b - m;
-  }
-  const uint32_t frame = a.ctl->frame_id;
-  const uint32_t cur = a.ctl->cursor[a.cursor_id];
+  __shared__ uint32_t s_mark[kIntegrateWaves][64];
+  __shared__ float s_pose[kMaxBatch][12];
+  const uint32_t b = blockIdx.x;
+  const bool alloc = b >= int_groups;
+  const uint32_t idx = alloc ? b - int_groups : b;
+  const FrameCtl* ctl = a.ctl;
+  const uint32_t f0 = uniform(ctl->frame_id), batch = uniform(ctl->batch_id), cur = uniform(ctl->cursor[a.cursor_id]);
   if (alloc) {
-    alloc_tile<kU16>(a, idx, frame + 1u, (cur + 1u) % a.n_seq);
+    const uint32_t jb = idx / tiles;
+    alloc_tile<kU16>(a, idx - jb * tiles, f0 + k_int + jb, jb, (batch + 1u) & 1u, input_slot(cur, k_int + jb, a.n_seq));
   } else {
     const int wave = threadIdx.x >> 6;
-    integrate_worker<kStorage, kU16, kColor>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, frame,
-                                     cur % a.n_seq, s_val[wave], s_key[wave]);
+    stage_poses(a, cur, k_int, s_pose);  // uniform branch: the whole workgroup integrates
+    integrate_worker<kStorage, kU16, kColor>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, f0, k_int,
+                                             cur, batch & 1u, s_val[wave], s_key[wave], s_mark[wave], s_pose);
   }
-  frame_done(a, gridDim.x, frame);
+  batch_done(a, gridDim.x, f0, k_int, batch & 1u);
 }
 
 // ============================================================================ marching cubes
@@ -1202,12 +1291,12 @@ __global__ __launch_bounds__(64) void k_track_end(TrackArgs t) {
 }  // namespace
 
 // ============================================================================ launchers
-hipError_t launch_alloc(const FrameArgs& a, int depth_u16, hipStream_t st) {
-  const int tiles = ((a.W + 15) >> 4) * ((a.H + 15) >> 4);
+hipError_t launch_alloc(const FrameArgs& a, int depth_u16, int k, hipStream_t st) {
+  const uint32_t tiles = (uint32_t)(((a.W + 15) >> 4) * ((a.H + 15) >> 4));
   if (depth_u16)
-    hipLaunchKernelGGL(k_alloc<true>, dim3(tiles), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_alloc<true>, dim3(tiles * k), dim3(256), 0, st, a, tiles);
   else
-    hipLaunchKernelGGL(k_alloc<false>, dim3(tiles), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_alloc<false>, dim3(tiles * k), dim3(256), 0, st, a, tiles);
   return hipGetLastError();
 }
 
@@ -1235,24 +1324,27 @@ static void with_input(bool u16, bool color, F f) {
   }
 }
 
-hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
+hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, int k, hipStream_t st) {
   with_storage(storage, [&](auto S) {
     with_input(depth_u16 != 0, a.colors != nullptr, [&](auto U, auto C) {
       const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
-      hipLaunchKernelGGL((k_integrate<decltype(S)::value, decltype(U)::value, decltype(C)::value>), g, b, 0, st, a);
+      hipLaunchKernelGGL((k_integrate<decltype(S)::value, decltype(U)::value, decltype(C)::value>), g, b, 0, st, a,
+                         (uint32_t)k);
     });
   });
   return hipGetLastError();
 }
 
-hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16, int grid, hipStream_t st) {
+hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16, int grid, int k_int, int k_alloc,
+                                  hipStream_t st) {
   const uint32_t ig = (uint32_t)(grid / kIntegrateWaves);
-  const uint32_t ag = (uint32_t)(((a.W + 15) >> 4) * ((a.H + 15) >> 4));
+  const uint32_t tiles = (uint32_t)(((a.W + 15) >> 4) * ((a.H + 15) >> 4));
+  const uint32_t ag = tiles * (uint32_t)k_alloc;
   with_storage(storage, [&](auto S) {
     with_input(depth_u16 != 0, a.colors != nullptr, [&](auto U, auto C) {
       const dim3 g(ig + ag), b(64 * kIntegrateWaves);
       hipLaunchKernelGGL((k_integrate_alloc<decltype(S)::value, decltype(U)::value, decltype(C)::value>), g, b, 0,
-                         st, a, ig, ag);
+                         st, a, ig, ag, tiles, (uint32_t)k_int);
     });
   });
   return hipGetLastError();

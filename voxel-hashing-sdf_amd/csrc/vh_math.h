@@ -137,7 +137,9 @@ VH_HD int project_cam(float x, float y, float z, float iz, const Intr& K, int W,
   const float uf = rintf(fmaf(K.fx, x * iz, K.cx));
   const float vf = rintf(fmaf(K.fy, y * iz, K.cy));
   const bool inside = (z >= kMinZ) & (z <= kMaxZ) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
-  return inside ? (int)((uint32_t)vf * (uint32_t)W + (uint32_t)uf) : -1;
+  // both factors are below 2^24 (W, H <= 16384): the masks let the compiler use a 24-bit
+  // multiply-add instead of a 64-bit one
+  return inside ? (int)(((uint32_t)vf & 0xffffffu) * ((uint32_t)W & 0xffffffu) + (uint32_t)uf) : -1;
 }
 VH_HD bool truncate_obs(int pix, float d, float z, float max_depth, float mu, float inv_mu, float& dist) {
   const float diff = d - z;

@@ -40,7 +40,7 @@ DEPTH_U16 = 0
 DEPTH_F32 = 1
 PRIOR_ABSOLUTE = 0
 PRIOR_INCREMENT = 1
-ABI_VERSION = 3
+ABI_VERSION = 4
 # per-evaluation solver stats: n_valid, rms, |delta xi|, lambda, accepted, accepted pose (12)
 GN_STATS = 17
 
@@ -57,7 +57,7 @@ class Config(C.Structure):
                 ("vox_size", C.c_float), ("trunc", C.c_float), ("log2_buckets", C.c_int32),
                 ("pool_blocks", C.c_int32), ("storage", C.c_int32), ("weight_cap", C.c_int32),
                 ("device", C.c_int32), ("origin", C.c_float * 3), ("vox_dim", C.c_int32 * 3),
-                ("color", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("color", C.c_int32), ("seq_batch", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class Frame(C.Structure):
@@ -70,7 +70,7 @@ class Stats(C.Structure):
                 ("slots_used", C.c_int64), ("touched_last", C.c_int64), ("touched_total", C.c_int64),
                 ("new_total", C.c_int64), ("storage", C.c_int32), ("flags", C.c_int32),
                 ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64),
-                ("track_ms", C.c_double)]
+                ("track_ms", C.c_double), ("profiled_launches", C.c_int64), ("blocks_integrated", C.c_int64)]
 
 
 class Mesh(C.Structure):
@@ -215,7 +215,7 @@ def make_config(**kw) -> Config:
 
 def config_from_workload(w: dict, **over) -> Config:
     keys = ("width", "height", "fx", "fy", "cx", "cy", "max_depth", "min_depth", "vox_size", "trunc",
-            "log2_buckets", "pool_blocks", "storage", "weight_cap", "color")
+            "log2_buckets", "pool_blocks", "storage", "weight_cap", "color", "seq_batch")
     kw = {k: w[k] for k in keys if k in w}
     kw.update(over)
     return make_config(**kw)
