@@ -138,6 +138,7 @@ struct vh_volume {
   uint8_t* mc_rgb = nullptr;       // colour volumes
   int64_t mc_cap = 0;
   uint32_t* mc_counts = nullptr;   // pool_blocks
+  uint32_t* mc_nbrs = nullptr;     // pool_blocks x 8
   uint32_t* mc_chunks = nullptr;   // pool_blocks / kScanChunk + 1
   uint32_t* mc_total = nullptr;
   // scratch for queries / Gauss-Newton
@@ -466,7 +467,7 @@ void free_volume(vh_volume* v) {
   if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->voxels, v->touched, v->ctl,
-                  v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total,
+                  v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total, v->mc_nbrs,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
                   v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba};
   for (void* p : ptrs)
@@ -480,6 +481,7 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t
     int rc;
     if ((rc = dev_alloc(&v->mc_counts, (size_t)v->cfg.pool_blocks * 4)) ||
         (rc = dev_alloc(&v->mc_chunks, ((size_t)v->cfg.pool_blocks / kScanChunk + 1) * 4)) ||
+        (rc = dev_alloc(&v->mc_nbrs, (size_t)v->cfg.pool_blocks * 8 * 4)) ||
         (rc = dev_alloc(&v->mc_total, 4)))
       return rc;
   }
@@ -494,6 +496,7 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t
   a.n_blocks = std::min<uint32_t>(c.pool_count, (uint32_t)v->cfg.pool_blocks);
   a.vox = v->cfg.vox_size;
   a.counts = v->mc_counts;
+  a.nbrs = v->mc_nbrs;
   a.chunk_sums = v->mc_chunks;
   a.total = v->mc_total;
   a.tris = d_xyz;

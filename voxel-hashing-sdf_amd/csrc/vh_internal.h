@@ -122,6 +122,7 @@ struct McArgs {
   uint32_t pool_cap;
   uint32_t n_blocks;   // pool blocks to extract
   float vox;
+  uint32_t* nbrs;      // per pool block: the 8 neighbourhood pool indices (k_mc_neighbours)
   uint32_t* counts;    // per pool block: triangles (count pass), then exclusive offsets
   uint32_t* chunk_sums;// per 1024 pool blocks
   uint32_t* total;     // total triangles

@@ -338,6 +338,89 @@ __device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, uint32
 // Worker w of `workers` (one wave each) takes an equal share of the list, its 64 stripes
 // seen as one index space; workers synchronise only within their own wave.
 // kColor: the colour pool is updated from each frame's packed RGB at the same pixels.
+// True iff every voxel of the block projects inside the image at a depth within the
+// projection's range: then the per-voxel range checks can be skipped (frame_update<..,
+// true>). The lane's column runs from voxel 0 to voxel 7 of cl + q*ax; the block's lattice
+// box is the convex hull of the columns' end points, and for depths > 0 the perspective
+// image of a convex combination is a convex combination of the images, so if every end
+// point has z in [2^-60, 2^60] and fx*x/z + cx in [1, W-2] (likewise v), every voxel's
+// z lies in [2^-64, 2^64] and its rounded pixel inside the image (the arithmetic of the
+// voxel's own projection differs from the end points' by far less than the 1-pixel margin).
+__device__ __forceinline__ bool block_inside(const FrameArgs& a, const BlockFrame& bf, const float cl[3]) {
+  bool in = true;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float fq = e ? 7.0f : 0.0f;
+    const float z = __builtin_fmaf(fq, bf.ax[2], cl[2]);
+    const float iz = recip_rn(z);
+    const float u = __builtin_fmaf(a.K.fx, __builtin_fmaf(fq, bf.ax[0], cl[0]) * iz, a.K.cx);
+    const float v = __builtin_fmaf(a.K.fy, __builtin_fmaf(fq, bf.ax[1], cl[1]) * iz, a.K.cy);
+    in = in & (z >= 0x1p-60f) & (z <= 0x1p60f) & (u >= 1.0f) & (u <= (float)(a.W - 2)) & (v >= 1.0f) &
+         (v <= (float)(a.H - 2));
+  }
+  return __builtin_amdgcn_ballot_w64(!in) == 0;  // wave-uniform
+}
+
+// One frame's update of the lane's 8 voxels of a block (camera column cl, step bf.ax): per
+// half, all projections, then all depth (and colour) fetches, then the updates, so the
+// fetches are in flight together. Branch-free: every lane computes, the observation flag
+// selects.
+// kInside (block_inside holds): the range checks are known to pass and are left out; the
+// values computed are the same.
+template <int kStorage, bool kU16, bool kColor, bool kInside>
+__device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFrame& bf, const float cl[3],
+                                             const void* base, const uint32_t* rgba,
+                                             typename VoxelIO<kStorage>::Word old[8], uint32_t cold[8],
+                                             uint32_t& upd) {
+  using IO = VoxelIO<kStorage>;
+  using Word = typename IO::Word;
+  // two halves of 4 voxels: fewer values live at once (occupancy) for 4 fetches in flight
+#pragma unroll
+  for (int h = 0; h < 8; h += 4) {
+    float zc[4];
+    int pix[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float fq = (float)(h + r);
+      zc[r] = __builtin_fmaf(fq, bf.ax[2], cl[2]);
+      const float x = __builtin_fmaf(fq, bf.ax[0], cl[0]), y = __builtin_fmaf(fq, bf.ax[1], cl[1]);
+      const float iz = recip_rn(zc[r]);
+      if constexpr (kInside) {
+        const float uf = rintf(__builtin_fmaf(a.K.fx, x * iz, a.K.cx));
+        const float vf = rintf(__builtin_fmaf(a.K.fy, y * iz, a.K.cy));
+        pix[r] = (int)(((uint32_t)vf & 0xffffffu) * ((uint32_t)a.W & 0xffffffu) + (uint32_t)uf);
+      } else {
+        pix[r] = project_cam(x, y, zc[r], iz, a.K, a.W, a.H);
+      }
+    }
+    float dv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dv[r] = depth_at<kU16>(base, a.depth_scale, kInside ? pix[r] : max(pix[r], 0));
+    uint32_t px[4];
+    if constexpr (kColor) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) px[r] = rgba[kInside ? pix[r] : max(pix[r], 0)];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = h + r;
+      float dist;
+      const bool ok = truncate_obs(kInside ? 0 : pix[r], dv[r], zc[r], a.max_depth, a.mu, a.inv_mu, dist);
+      float sv, wv;
+      IO::unpack(old[q], sv, wv);
+      if constexpr (kColor) {
+        const uint32_t c = color_update(cold[q], wv, px[r]);
+        cold[q] = ok ? c : cold[q];
+      }
+      mean_update(sv, wv, dist, a.cap);
+      const Word nw = IO::pack(sv, wv);
+      old[q] = ok ? nw : old[q];
+      upd |= (uint32_t)ok << q;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the halves apart: the scheduler would merge them
+  }
+}
+
 // s_pose: the batch's k poses (row-major 3x4 each), staged in LDS by stage_poses.
 template <int kStorage, bool kU16, bool kColor>
 __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t f0, uint32_t k,
@@ -432,41 +515,11 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         float cb[3], cl[3];
         block_origin_cam(T, a.vox, bx, by, bz, cb);
         column_cam(bf, cb, lane >> 3, lane & 7, cl);
-        // all projections, then all depth fetches, then the updates: the 8 fetches are in
-        // flight together instead of one round trip per voxel
-        float zc[8];
-        int pix[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const float fq = (float)q;
-          zc[q] = __builtin_fmaf(fq, bf.ax[2], cl[2]);
-          pix[q] = project_cam(__builtin_fmaf(fq, bf.ax[0], cl[0]), __builtin_fmaf(fq, bf.ax[1], cl[1]), zc[q],
-                               recip_rn(zc[q]), a.K, a.W, a.H);
-        }
-        float dv[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) dv[q] = depth_at<kU16>(base, a.depth_scale, pix[q] < 0 ? 0 : pix[q]);
-        uint32_t px[8];
-        if constexpr (kColor) {
-          const uint32_t* rgba = a.rgba + ((size_t)parity * kMaxBatch + jf) * frame_px;
-#pragma unroll
-          for (int q = 0; q < 8; ++q) px[q] = rgba[pix[q] < 0 ? 0 : pix[q]];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {  // branch-free: every lane computes, the flag selects
-          float dist;
-          const bool ok = truncate_obs(pix[q], dv[q], zc[q], a.max_depth, a.mu, a.inv_mu, dist);
-          float sv, wv;
-          IO::unpack(old[q], sv, wv);
-          if constexpr (kColor) {
-            const uint32_t c = color_update(cold[q], wv, px[q]);
-            cold[q] = ok ? c : cold[q];
-          }
-          mean_update(sv, wv, dist, a.cap);
-          const Word nw = IO::pack(sv, wv);
-          old[q] = ok ? nw : old[q];
-          upd |= (uint32_t)ok << q;
-        }
+        const uint32_t* rgba = kColor ? a.rgba + ((size_t)parity * kMaxBatch + jf) * frame_px : nullptr;
+        if (block_inside(a, bf, cl))
+          frame_update<kStorage, kU16, kColor, true>(a, bf, cl, base, rgba, old, cold, upd);
+        else
+          frame_update<kStorage, kU16, kColor, false>(a, bf, cl, base, rgba, old, cold, upd);
       }
       // fresh blocks are written whole (never-observed voxels as zero), others where observed
 #pragma unroll
@@ -539,6 +592,10 @@ __device__ void batch_done(const FrameArgs& a, uint32_t groups, uint32_t f0, uin
 }
 
 constexpr int kIntegrateWaves = 4;  // independent workers per workgroup
+// Register budget of the integration kernels: 5 waves per SIMD (96 VGPRs). Uncapped, the
+// two frame_update variants take 106 (FP32) and leave 4 waves; capped, a few values spill
+// to scratch and the kernel runs faster for every voxel format (measured on room_medium_5mm).
+#define VH_INTEGRATE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(5)))
 
 // The k poses of a batch (inputs cur .. cur+k-1) into LDS, read by every block of the
 // workgroup's workers (a global load per block and frame would put a memory round trip in
@@ -554,7 +611,7 @@ __device__ __forceinline__ void stage_poses(const FrameArgs& a, uint32_t cur, ui
 
 // Integration of the batch of k frames that the last allocation launch prepared.
 template <int kStorage, bool kU16, bool kColor>
-__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a, uint32_t k) {
+__global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate(FrameArgs a, uint32_t k) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
   __shared__ uint32_t s_mark[kIntegrateWaves][64];
   __shared__ float s_pose[kMaxBatch][12];
@@ -576,7 +633,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate(FrameArgs a,
 // b, so the frame id, cursor and batch id the allocation workgroups read at their start
 // cannot advance under them.
 template <int kStorage, bool kU16, bool kColor>
-__global__ __launch_bounds__(64 * kIntegrateWaves) void k_integrate_alloc(FrameArgs a, uint32_t int_groups,
+__global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate_alloc(FrameArgs a, uint32_t int_groups,
                                                                          uint32_t alloc_groups, uint32_t tiles,
                                                                          uint32_t k_int) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
@@ -632,15 +689,7 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
   const int lane = threadIdx.x;
   int bx, by, bz;
   unpack_key(a.block_keys[b], bx, by, bz);
-  if (lane < 8) {
-    uint32_t nb = b;
-    if (lane > 0) {
-      const int nx = bx + (lane >> 2), ny = by + ((lane >> 1) & 1), nz = bz + (lane & 1);
-      nb = key_in_range(nx, ny, nz) ? find_block(a.hash, pack_key(nx, ny, nz)) : kNoBlock;
-      if (nb >= a.pool_cap) nb = kNoBlock;
-    }
-    S.nb[lane] = nb;
-  }
+  if (lane < 8) S.nb[lane] = a.nbrs[(size_t)b * 8 + lane];  // k_mc_neighbours
   __syncthreads();
   const Word* vox = reinterpret_cast<const Word*>(a.voxels);
   const float nan = __builtin_nanf("");
@@ -701,27 +750,32 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
 // Lane l classifies the 8 cubes of its (y, z) column (x = 0..7), sharing each x-face's
 // corners with the next cube. A cube has triangles iff its 8 corners are observed and its
 // case is not 0 or 255 (bit c set iff corner c < 0, reference corner order). Returns the
-// block's table triangle count; lane_off holds the per-lane exclusive prefix.
-__device__ __forceinline__ uint32_t mc_classify(McShared& S) {
+// block's table triangle count; lane_off holds the per-lane exclusive prefix. near: bit x
+// set iff cube x of the lane's column has triangles and a corner with |value| < tiny (see
+// mc_count_exact).
+__device__ __forceinline__ uint32_t mc_classify(McShared& S, float tiny, uint32_t& near) {
   const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
   float f0[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) f0[k] = S.val[(cy + (k >> 1)) * 9 + cz + (k & 1)];
   uint32_t ntri = 0;
+  near = 0u;
   for (int cx = 0; cx < 8; ++cx) {
     float f1[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) f1[k] = S.val[(cx + 1) * 81 + (cy + (k >> 1)) * 9 + cz + (k & 1)];
     // corner c -> (dx, dy, dz); face slot k = (dy << 1) | dz
     const float val[8] = {f0[0], f0[2], f1[2], f1[0], f0[1], f0[3], f1[3], f1[1]};
-    bool obs = true;
+    bool obs = true, small = false;
     int ci = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       obs = obs && !__builtin_isnan(val[c]);
+      small = small || __builtin_fabsf(val[c]) < tiny;
       if (val[c] < 0.0f) ci |= 1 << c;
     }
     if (!obs || ci == 255) ci = 0;
+    if (ci != 0 && small) near |= 1u << cx;
     S.cases[cx * 64 + lane] = (uint8_t)ci;
     ntri += S.cnt[ci];
 #pragma unroll
@@ -737,6 +791,63 @@ __device__ __forceinline__ uint32_t mc_classify(McShared& S) {
   if (lane == 63) S.lane_off[64] = x;
   __syncthreads();
   return S.lane_off[64];
+}
+
+// Vertices (voxel units) of table triangle t of cube (cx, cy, cz) with case ci, and with
+// kColor their colours; false for a degenerate triangle (two equal vertices; the
+// reference's test at src/tsdf.cu:1055-1058, with its typo corrected).
+template <bool kColor>
+__device__ __forceinline__ bool mc_triangle(const McShared& S, const uint32_t* col, int ci, int t, int cx, int cy,
+                                            int cz, int gx0, int gy0, int gz0, float* P, uint32_t* C) {
+  const uint64_t pk = kTriTable[ci];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
+    const int ca = edge_a(e), cb = edge_b(e);
+    const int ia = (cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca));
+    const int ib = (cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb));
+    const float va = S.val[ia], vb = S.val[ib];
+    const float pax = (float)(gx0 + cx + corner_dx(ca)), pay = (float)(gy0 + cy + corner_dy(ca)),
+                paz = (float)(gz0 + cz + corner_dz(ca)), pbx = (float)(gx0 + cx + corner_dx(cb)),
+                pby = (float)(gy0 + cy + corner_dy(cb)), pbz = (float)(gz0 + cz + corner_dz(cb));
+    if constexpr (kColor)
+      vertex_interp_rgb(pax, pay, paz, pbx, pby, pbz, va, vb, col[ia], col[ib], P[3 * j], P[3 * j + 1], P[3 * j + 2],
+                        C[j]);
+    else
+      vertex_interp(pax, pay, paz, pbx, pby, pbz, va, vb, P[3 * j], P[3 * j + 1], P[3 * j + 2]);
+  }
+  const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
+  const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
+  const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
+  return !(d01 || d12 || d20);
+}
+
+// Triangles that k_mc_write will emit for the lane's column, without computing vertices
+// where none can coincide. Vertices of one triangle lie on three distinct cube edges, in
+// the closed edge segments (the other two coordinates are exact lattice values, the
+// interpolation parameter lies in [0, 1]); two such points can only be equal at a corner
+// the two edges share, and a vertex lands exactly on corner a of its edge only by a snap
+// (|v_a| <= 1e-5, or |v_a - v_b| <= 1e-5 with opposite signs) or by rounding a + mu*(b - a)
+// to a, which needs |mu| <= ulp(a)/2 and so |v_a| <= ulp(a) (|v_b - v_a| <= 2); likewise for
+// b. A cube whose corners all have |value| >= tiny (tiny above both bounds) therefore emits
+// exactly its table count; the others ("near" bits) are counted triangle by triangle.
+__device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t near, int gx0, int gy0, int gz0) {
+  const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
+  uint32_t n = 0;
+  for (int cx = 0; cx < 8; ++cx) {
+    const int ci = S.cases[cx * 64 + lane];
+    const int nt = S.cnt[ci];
+    if (!((near >> cx) & 1u)) {
+      n += (uint32_t)nt;
+      continue;
+    }
+    for (int t = 0; t < nt; ++t) {
+      float P[9];
+      uint32_t C[3];
+      n += mc_triangle<false>(S, nullptr, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C) ? 1u : 0u;
+    }
+  }
+  return n;
 }
 
 // The block's (cube, table triangle) items in order lane, x, t, in rounds of kItemCap: each
@@ -776,32 +887,38 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
         cy = (cube >> 3) & 7;
         cz = cube & 7;
         ci = S.cases[cube];
-        const uint64_t pk = kTriTable[ci];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
-          const int ca = edge_a(e), cb = edge_b(e);
-          const int ia = (cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca));
-          const int ib = (cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb));
-          const float va = S.val[ia], vb = S.val[ib];
-          const float pax = (float)(gx0 + cx + corner_dx(ca)), pay = (float)(gy0 + cy + corner_dy(ca)),
-                      paz = (float)(gz0 + cz + corner_dz(ca)), pbx = (float)(gx0 + cx + corner_dx(cb)),
-                      pby = (float)(gy0 + cy + corner_dy(cb)), pbz = (float)(gz0 + cz + corner_dz(cb));
-          if constexpr (kColor)
-            vertex_interp_rgb(pax, pay, paz, pbx, pby, pbz, va, vb, col[ia], col[ib], P[3 * j], P[3 * j + 1],
-                              P[3 * j + 2], C[j]);
-          else
-            vertex_interp(pax, pay, paz, pbx, pby, pbz, va, vb, P[3 * j], P[3 * j + 1], P[3 * j + 2]);
-        }
-        const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
-        const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
-        const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
-        valid = !(d01 || d12 || d20);
+        valid = mc_triangle<kColor>(S, col, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C);
       }
       emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P, C);
     }
     __syncthreads();
   }
+}
+
+// Threshold of mc_count_exact for the block at lattice origin (gx0, gy0, gz0): above the
+// snap bound 1e-5 and above ulp(c) for every lattice coordinate c of the block and its
+// +1 layer (ulp(c) <= |c| * 2^-23).
+__device__ __forceinline__ float mc_tiny(int gx0, int gy0, int gz0) {
+  const int m = max(max(max(abs(gx0), abs(gx0 + 8)), max(abs(gy0), abs(gy0 + 8))), max(abs(gz0), abs(gz0 + 8)));
+  return 2e-5f + (float)(m + 1) * 0x1p-22f;
+}
+
+// The pool indices of each block's 8 neighbourhood blocks (+x/+y/+z and their combinations,
+// index (dx << 2) | (dy << 1) | dz; itself at 0; kNoBlock where absent), one lane per entry,
+// so that the extraction passes read one word instead of walking the hash per neighbour.
+__global__ __launch_bounds__(256) void k_mc_neighbours(McArgs a) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)a.n_blocks * 8) return;
+  const uint32_t b = (uint32_t)(i >> 3), d = (uint32_t)(i & 7);
+  uint32_t nb = b;
+  if (d > 0) {
+    int bx, by, bz;
+    unpack_key(a.block_keys[b], bx, by, bz);
+    const int nx = bx + (int)(d >> 2), ny = by + (int)((d >> 1) & 1), nz = bz + (int)(d & 1);
+    nb = key_in_range(nx, ny, nz) ? find_block(a.hash, pack_key(nx, ny, nz)) : kNoBlock;
+    if (nb >= a.pool_cap) nb = kNoBlock;
+  }
+  a.nbrs[i] = nb;
 }
 
 template <int kStorage>
@@ -812,12 +929,11 @@ __global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
     mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
-    const uint32_t total = mc_classify(S);
-    uint32_t cnt = 0;
-    mc_items<false>(S, nullptr, total, gx0, gy0, gz0, [&](bool valid, int, int, int, int, int, const float*,
-                                                          const uint32_t*) {
-      cnt += (uint32_t)__popcll(__ballot(valid));
-    });
+    uint32_t near;
+    mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
+    uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, m);
     if (lane == 0) a.counts[b] = cnt;
     __syncthreads();
   }
@@ -901,7 +1017,8 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
-    const uint32_t total = mc_classify(S);
+    uint32_t near;
+    const uint32_t total = mc_classify(S, 0.0f, near);
     uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
     mc_items<kColor>(S, s_col, total, gx0, gy0, gz0, [&](bool valid, int x, int y, int z, int ci, int t,
                                                          const float* P, const uint32_t* C) {
@@ -1354,6 +1471,7 @@ hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
   if (a.n_blocks == 0) return hipSuccess;
   const uint32_t grid = min(a.n_blocks, 8192u);
   const uint32_t n_chunks = (a.n_blocks + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_mc_neighbours, dim3((uint32_t)(((size_t)a.n_blocks * 8 + 255) / 256)), dim3(256), 0, st, a);
   with_storage(storage, [&](auto S) {
     hipLaunchKernelGGL(k_mc_count<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
   });

@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvh.so")
+# VH_LIBRARY selects another build of the library (for comparing builds); default: in-tree
+LIB_PATH = os.environ.get("VH_LIBRARY") or os.path.join(_HERE, "libvh.so")
 
 VH_OK = 0
 VH_ERR_INVALID_ARG = 1
