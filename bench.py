@@ -293,6 +293,25 @@ def main():
         return
 
     results = {}
+    # host frames from page-locked memory: copy ring + allocation + integration per frame.
+    # Measured first: a large host allocation made just before (the colour frames below) has
+    # been seen to halve this rate for the next seconds.
+    vol = vh.Volume(vh.config_from_workload(w))
+    pinned = torch.from_numpy(depth.view(np.int16)).pin_memory()
+    for k in range(min(20, nf)):
+        vol.integrate_async(pinned[k], poses[k])
+    vol.sync()
+    vol.reset()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for k in range(nf):
+        vol.integrate_async(pinned[k], poses[k])
+    vol.sync()
+    results["pcie"] = nf / (time.perf_counter() - t3)
+    vol.close()
+    del pinned
+    log(f"bench: host input: {results['pcie']:.1f} frames/s")
+
     variants = [("f32", {})]
     if not args.no_f16:
         variants += [("f16r", dict(storage=2, weight_cap=255)), ("f16", dict(storage=1, weight_cap=255))]
@@ -335,21 +354,6 @@ def main():
         for _ in range(reps):
             ntri = vol.extract_mesh_device(d_xyz, None, cap, d_vrgb if col else None)
         mc_s = (time.perf_counter() - t2) / reps
-        if name == "f32":
-            # host frames from page-locked memory: copy ring + allocation + integration per frame
-            pinned = torch.from_numpy(depth.view(np.int16)).pin_memory()
-            vol.reset()
-            for k in range(min(20, nf)):
-                vol.integrate_async(pinned[k], poses[k])
-            vol.sync()
-            vol.reset()
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-            for k in range(nf):
-                vol.integrate_async(pinned[k], poses[k])
-            vol.sync()
-            results["pcie"] = nf / (time.perf_counter() - t3)
-            del pinned
         # kernel timing with HIP events (eager launches on the volume's stream)
         vol.set_profiling(True)
         run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
@@ -377,7 +381,7 @@ def main():
     traffic, traffic_src = None, None
     prof = os.path.join(ROOT, "profiles", "k_integrate_room_medium_5mm.json")
     pj = json.load(open(prof)) if os.path.exists(prof) else {}
-    # the summary must describe the same launches (batch size and workload) to be used
+    # the summary must describe the same launches (workload, frames per launch) to be used
     if (args.workload == "room_medium_5mm" and "blocks_per_dispatch" in pj
             and abs(pj["frames_per_dispatch"] - p["profiled_frames"] / launches) < 1e-6):
         traffic = int(pj["traffic_bytes_per_dispatch"])
