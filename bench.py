@@ -388,8 +388,13 @@ def main():
         traffic_src = (f"rocprofv3 FETCH_SIZE x{pj['fetch_scale']:.3f} + WRITE_SIZE x{pj['write_scale']:.3f} "
                        f"(1 GiB copy calibration), {pj['workload']}, "
                        f"{pj['blocks_per_dispatch']:.0f} blocks and {pj['frames_per_dispatch']:.0f} frames per launch")
+    valu = None
+    if traffic is not None and "valu_busy" in pj:
+        valu = {"busy": round(pj["valu_busy"], 3),
+                "source": "rocprofv3 SQ_ACTIVE_INST_VALU / GRBM_GUI_ACTIVE of the same launches: the batched "
+                          "kernel is bound by vector-instruction issue, not by HBM"}
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src, valu=valu,
                     kernel="k_integrate<F32,u16>", avg_launch_us=round(1e6 * t_int / launches, 2),
                     bytes_per_launch=int(bytes_total / launches),
                     frames_per_launch=round(p["profiled_frames"] / launches, 3),
