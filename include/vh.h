@@ -233,8 +233,36 @@ int vh_extract_mesh_device(vh_volume* vol, float* d_xyz, int32_t* d_cubes, int64
  * triangle. A non-NULL d_rgb needs a colour volume. */
 int vh_extract_mesh_device_rgb(vh_volume* vol, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb,
                                int64_t capacity, int64_t* n_tris);
-/* Writes the mesh as binary little-endian PLY (vertices of each triangle listed in order;
- * colour volumes add uchar red, green, blue per vertex). */
+/* Indexed mesh: one vertex per lattice edge crossed by the surface, shared by every face
+ * that uses the edge and interpolated along it from its lower lattice point to its upper one
+ * (VertexInterp with that orientation), and per face the indices of its three vertices. The
+ * faces are the triangles of vh_extract_mesh in its order and winding, plus the zero-area
+ * triangles vh_extract_mesh drops (two corners coincide where the surface passes exactly
+ * through a lattice point): with them the mesh of a closed surface is closed, each edge in
+ * two faces. Device buffers: d_vertices (vertex_capacity x 3 floats, metres), d_faces
+ * (face_capacity x 3 int32), d_vrgb (vertex_capacity x 3 bytes, colour volumes, may be
+ * NULL). *n_vertices and *n_faces receive the counts; VH_ERR_TRUNCATED if either exceeds its
+ * capacity (d_faces is then not written; d_vertices / d_vrgb may hold the vertices below
+ * vertex_capacity). Synchronous. Replaces the welding step of tsdf2mesh
+ * (src/tsdf.cu:1772-1850), which merges vertices with identical float coordinates and so
+ * leaves the two interpolations of one edge by neighbouring cubes (in opposite directions)
+ * apart when they differ in the last bit. */
+int vh_extract_mesh_indexed_device(vh_volume* vol, float* d_vertices, int64_t vertex_capacity,
+                                   int32_t* d_faces, int64_t face_capacity, uint8_t* d_vrgb,
+                                   int64_t* n_vertices, int64_t* n_faces);
+/* Host variant: arrays owned by the library (freed by vh_indexed_mesh_free). */
+typedef struct vh_indexed_mesh {
+  int64_t n_vertices;
+  int64_t n_faces;
+  float* vertices;   /* n_vertices x 3 floats, metres */
+  int32_t* faces;    /* n_faces x 3 vertex indices */
+  uint8_t* rgb;      /* colour volumes: n_vertices x 3 bytes; else NULL */
+} vh_indexed_mesh;
+int vh_extract_mesh_indexed(vh_volume* vol, vh_indexed_mesh* out);
+int vh_indexed_mesh_free(vh_indexed_mesh* mesh);
+/* Writes the indexed mesh (vh_extract_mesh_indexed) as binary little-endian PLY: float x, y,
+ * z per vertex (colour volumes add uchar red, green, blue), then per face the count 3 and
+ * three int32 vertex indices. */
 int vh_save_ply(vh_volume* vol, const char* path);
 
 /* ------------------------------------------------------------------ block export */

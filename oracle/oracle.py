@@ -68,6 +68,8 @@ def load():
     lib.vho_point_triangle_dist.restype = C.c_double
     lib.vho_point_triangle_dist.argtypes = [P, P, P, P]
     lib.vho_mesh_distance.argtypes = [P, P, C.c_int64, P, P, C.c_int64, P, P]
+    lib.vho_mesh_indexed.restype = C.c_int64
+    lib.vho_mesh_indexed.argtypes = [P, P, P, C.c_int64, P, P, C.c_int64, P]
     _lib = lib
     return lib
 
@@ -156,6 +158,21 @@ class Oracle:
         cubes = np.zeros((n, 4), np.int32)
         _lib.vho_mc64(self._h, _p(xyz), _p(cubes), n)
         return xyz, cubes
+
+    def mesh_indexed(self, with_rgb=False, with_cubes=False):
+        """(vertices (V,3) float32 metres, faces (F,3) int32[, rgb (V,3) uint8][, cubes (F,4)
+        int32]): every table triangle of the twin's observed cubes (zero-area ones included),
+        one vertex per lattice edge they use, interpolated lower to upper end point."""
+        nf = C.c_int64(0)
+        nv = _lib.vho_mesh_indexed(self._h, None, None, 0, None, None, 0, C.cast(C.pointer(nf), C.c_void_p))
+        nf = nf.value
+        v = np.zeros((nv, 3), np.float32)
+        f = np.zeros((nf, 3), np.int32)
+        rgb = np.zeros((nv, 3), np.uint8) if with_rgb else None
+        cubes = np.zeros((nf, 4), np.int32) if with_cubes else None
+        _lib.vho_mesh_indexed(self._h, _p(v), _p(rgb), nv, _p(f), _p(cubes), nf, None)
+        out = (v, f) + ((rgb,) if with_rgb else ()) + ((cubes,) if with_cubes else ())
+        return out
 
     def query(self, pts):
         pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)

@@ -26,7 +26,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <array>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -394,8 +397,9 @@ void interp64(const double a[3], const double b[3], double va, double vb, double
 
 // k64 = false: the twin's values, FP32 interpolation (the GPU's mesh); true: the FP64
 // shadow's values and FP64 interpolation, positions rounded to FP32 once at the end.
+// keep: also the zero-area triangles (the indexed mesh's faces).
 template <bool k64>
-void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& out) {
+void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& out, bool keep = false) {
   using R = typename std::conditional<k64, double, float>::type;
   R val[8];
   uint8_t col[8][3] = {};
@@ -443,7 +447,7 @@ void cube_triangles(const Oracle& o, int gx, int gy, int gz, std::vector<Tri>& o
       if (k64) return q[3 * a] == q[3 * b] && q[3 * a + 1] == q[3 * b + 1] && q[3 * a + 2] == q[3 * b + 2];
       return tr.p[3 * a] == tr.p[3 * b] && tr.p[3 * a + 1] == tr.p[3 * b + 1] && tr.p[3 * a + 2] == tr.p[3 * b + 2];
     };
-    if (same(0, 1) || same(1, 2) || same(2, 0)) continue;  // degenerate (typo-free version of :1055-1058)
+    if (!keep && (same(0, 1) || same(1, 2) || same(2, 0))) continue;  // degenerate (typo-free :1055-1058)
     if (k64) {
       for (int k = 0; k < 9; ++k) tr.p[k] = (float)(q[k] * (double)o.vox);
       std::memset(tr.rgb, 0, 9);
@@ -629,7 +633,7 @@ int64_t vho_export(void* p, int32_t* xyz, float* s, float* w, double* s64, doubl
 
 // Marching cubes over every block; triangles ordered by cube then by table position.
 extern "C++" template <bool k64>
-int64_t mc_all(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap) {
+int64_t mc_all(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap, bool keep = false) {
   Oracle* o = static_cast<Oracle*>(p);
   std::vector<int64_t> order(o->keys.begin(), o->keys.end());
   std::sort(order.begin(), order.end());
@@ -639,7 +643,7 @@ int64_t mc_all(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t cap) {
     int bx, by, bz;
     key_xyz(order[(size_t)k], bx, by, bz);
     for (int i = 0; i < 512; ++i)
-      cube_triangles<k64>(*o, bx * 8 + (i >> 6), by * 8 + ((i >> 3) & 7), bz * 8 + (i & 7), per[(size_t)k]);
+      cube_triangles<k64>(*o, bx * 8 + (i >> 6), by * 8 + ((i >> 3) & 7), bz * 8 + (i & 7), per[(size_t)k], keep);
   }
   int64_t n = 0;
   for (auto& v : per) n += (int64_t)v.size();
@@ -660,6 +664,68 @@ int64_t vho_mc_rgb(void* p, float* xyz, int32_t* cubes, uint8_t* rgb, int64_t ca
 int64_t vho_mc(void* p, float* xyz, int32_t* cubes, int64_t cap) { return mc_all<false>(p, xyz, cubes, nullptr, cap); }
 // The FP64 reference mesh: marching cubes over the FP64 shadow (same cube records).
 int64_t vho_mc64(void* p, float* xyz, int32_t* cubes, int64_t cap) { return mc_all<true>(p, xyz, cubes, nullptr, cap); }
+
+// Indexed mesh of the twin's surface: the faces are every table triangle of every cube whose
+// eight corners are observed, in the order of vho_mc, the zero-area ones included (the soup
+// of vho_mc drops those; where the surface passes exactly through a lattice point, dropping
+// them would open the edge-shared mesh); one vertex per lattice edge used by a face, numbered
+// in order of first use, interpolated from the edge's lower lattice point to its upper one
+// with the twin's values (VertexInterp with that orientation). Returns the vertex count and
+// the face count in *n_faces; arrays are written when their capacities suffice (vertices:
+// vcap x 3 floats in metres, rgb: vcap x 3 bytes, faces: fcap x 3 ints, face_cubes: fcap x 4
+// cube records as vho_mc writes them).
+int64_t vho_mesh_indexed(void* p, float* vertices, uint8_t* rgb, int64_t vcap, int32_t* faces, int32_t* face_cubes,
+                         int64_t fcap, int64_t* n_faces) {
+  Oracle* o = static_cast<Oracle*>(p);
+  const int64_t nt = mc_all<false>(p, nullptr, nullptr, nullptr, 0, true);
+  std::vector<float> xyz((size_t)nt * 9);
+  std::vector<int32_t> cubes((size_t)nt * 4);
+  mc_all<false>(p, xyz.data(), cubes.data(), nullptr, nt, true);
+  if (n_faces) *n_faces = nt;
+  std::map<std::tuple<int, int, int, int>, int32_t> id;
+  std::vector<std::array<int, 4>> edges;  // lower lattice point and axis, by vertex id
+  std::vector<int32_t> f((size_t)nt * 3);
+  for (int64_t t = 0; t < nt; ++t) {
+    const int32_t* c = &cubes[(size_t)t * 4];
+    const int ci = c[3] >> 8, tt = c[3] & 255;
+    for (int j = 0; j < 3; ++j) {
+      const int e = hexval(kTri[ci][3 * tt + j]);
+      const int* a = kCorner[kEdge[e][0]];
+      const int* b = kCorner[kEdge[e][1]];
+      const int lx = c[0] + std::min(a[0], b[0]), ly = c[1] + std::min(a[1], b[1]), lz = c[2] + std::min(a[2], b[2]);
+      const int axis = a[0] != b[0] ? 0 : (a[1] != b[1] ? 1 : 2);
+      auto ins = id.emplace(std::make_tuple(lx, ly, lz, axis), (int32_t)edges.size());
+      if (ins.second) edges.push_back({lx, ly, lz, axis});
+      f[(size_t)t * 3 + j] = ins.first->second;
+    }
+  }
+  const int64_t nv = (int64_t)edges.size();
+  auto value = [&](int x, int y, int z, float& s, const uint8_t*& col) {
+    const int64_t b = o->find(key_of(floor_div8(x), floor_div8(y), floor_div8(z)));
+    const int i = ((x - floor_div8(x) * 8) * 8 + (y - floor_div8(y) * 8)) * 8 + (z - floor_div8(z) * 8);
+    float w;
+    o->twin(b, i, s, w);
+    col = o->color ? &o->rgb8[3 * ((size_t)b * 512 + i)] : nullptr;
+  };
+  if (vertices && vcap >= nv)
+    for (int64_t k = 0; k < nv; ++k) {
+      const auto& E = edges[(size_t)k];
+      const float pa[3] = {(float)E[0], (float)E[1], (float)E[2]};
+      const float pb[3] = {(float)(E[0] + (E[3] == 0)), (float)(E[1] + (E[3] == 1)), (float)(E[2] + (E[3] == 2))};
+      float va, vb;
+      const uint8_t *ca, *cb;
+      value(E[0], E[1], E[2], va, ca);
+      value((int)pb[0], (int)pb[1], (int)pb[2], vb, cb);
+      float q[3];
+      uint8_t col[3] = {0, 0, 0};
+      interp(pa, pb, va, vb, q, ca, cb, (rgb && ca) ? col : nullptr);
+      for (int r = 0; r < 3; ++r) vertices[3 * k + r] = q[r] * o->vox;
+      if (rgb) std::memcpy(rgb + 3 * k, col, 3);
+    }
+  if (faces && fcap >= nt) std::memcpy(faces, f.data(), f.size() * sizeof(int32_t));
+  if (face_cubes && fcap >= nt) std::memcpy(face_cubes, cubes.data(), cubes.size() * sizeof(int32_t));
+  return nv;
+}
 
 double vho_point_triangle_dist(const double* p, const double* a, const double* b, const double* c) {
   return std::sqrt(point_triangle_dist2(p, a, b, c));

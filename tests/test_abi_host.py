@@ -118,6 +118,7 @@ def test_struct_layouts_match_the_header(vh, tmp_path):
     if not shutil.which("gcc"):
         pytest.skip("gcc not available")
     structs = {"vh_config": vh.Config, "vh_frame": vh.Frame, "vh_stats": vh.Stats, "vh_mesh": vh.Mesh,
+               "vh_indexed_mesh": vh.IndexedMesh,
                "vh_gn_stats": vh.GnStats, "vh_track_config": vh.TrackConfig, "vh_settings": vh.Settings}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vh.h"', '#include "vh_io.h"', "int main(void) {"]
     for cname, py in structs.items():

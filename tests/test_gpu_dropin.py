@@ -1,6 +1,6 @@
 """The C++ drop-in (ark::GpuTsdfGenerator over the C ABI) on the GPU: the example caller
-integrates frames and writes the reference's ASCII PLY layout; welding and the binary PLY
-writer agree with the library's triangle soup."""
+integrates frames and writes the reference's ASCII PLY layout with shared vertices; both PLY
+writers carry the library's indexed mesh."""
 import os
 import subprocess
 
@@ -27,7 +27,14 @@ def read_ascii_ply(path, with_rgb=False):
     return v, f
 
 
-def test_load_frames_example_writes_welded_ply(vh, wl, small_frames, tmp_path):
+def _rows(*cols):
+    """Rows of the given per-item arrays as uint32 bit patterns (floats by their bits), sorted."""
+    r = np.concatenate([np.ascontiguousarray(c).astype(np.float32).view(np.uint32) if c.dtype.kind == "f"
+                        else c.astype(np.uint32) for c in cols], 1)
+    return r[np.lexsort(r.T[::-1])]
+
+
+def test_load_frames_example_writes_indexed_ply(vh, wl, small_frames, tmp_path):
     exe = os.path.join(ROOT, "voxel-hashing-sdf_amd", "load_frames")
     out = tmp_path / "room.ply"
     print("drop-in: load_frames, 10 frames with colour", flush=True)
@@ -36,41 +43,41 @@ def test_load_frames_example_writes_welded_ply(vh, wl, small_frames, tmp_path):
     v, f, vrgb = read_ascii_ply(out, with_rgb=True)
     assert len(v) > 1000 and len(f) > 1000
     assert f.min() >= 0 and f.max() < len(v)
-    # the same frames through the Python binding: as many faces as triangles, and the welded
-    # vertices are the distinct triangle corners, each with one of the colours the library
-    # gives that position
+    # the same frames through the Python binding: the PLY holds the library's indexed mesh
+    # (vertex order follows pool indices, which differ between runs: compared as sorted rows
+    # of vertex bits and colours, and of face corner bits)
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
     scene = vh.Scene(w["scene"], wl.SEED)
     vol = vh.Volume(vh.config_from_workload(w, log2_buckets=20, pool_blocks=1 << 19, color=1))
     for k in range(10):
         vol.integrate(depth[k].astype(np.float32) * np.float32(0.001), poses[k], rgb=scene.rgb(k, poses[k]))
-    xyz, _, trgb = vol.extract_mesh(with_rgb=True)
-    assert len(xyz) == len(f)
-    corners = np.unique(xyz.reshape(-1, 3), axis=0)
-    assert len(corners) == len(v)
-    colours = {}
-    for p, c in zip(xyz.reshape(-1, 3).astype(np.float32).view(np.uint32), trgb.reshape(-1, 3)):
-        colours.setdefault(p.tobytes(), set()).add(tuple(int(x) for x in c))
-    vkeys = v.astype(np.float32).view(np.uint32)
-    bad = sum(tuple(vrgb[i]) not in colours[vkeys[i].tobytes()] for i in range(len(v)))
-    assert bad == 0, f"{bad} PLY vertex colours differ from the library's"
+    iv, fi, irgb = vol.extract_mesh_indexed(with_rgb=True)
+    assert len(fi) == len(f) and len(iv) == len(v)
+    assert np.array_equal(_rows(v, vrgb), _rows(iv, irgb))
+    assert np.array_equal(_rows(v.astype(np.float32)[f].reshape(-1, 9)), _rows(iv[fi].reshape(-1, 9)))
     assert len({tuple(c) for c in vrgb}) > 100  # textured, not constant
-    # each edge of a closed patch is shared by two faces at most (welded, manifold soup)
+    # every vertex is used, and each edge lies in two faces at most (a manifold mesh)
+    assert len(np.unique(f)) == len(v)
     e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), 1)
     _, counts = np.unique(e, axis=0, return_counts=True)
     assert counts.max() <= 2
-    # binary PLY of the library
+    # the soup's triangles are the faces of positive area
+    xyz, _ = vol.extract_mesh()
+    assert len(fi) - len(xyz) < 0.01 * len(xyz)
+    # binary PLY of the library: the same volume, so the same arrays
     bpath = tmp_path / "room_bin.ply"
     vol.save_ply(str(bpath))
     raw = bpath.read_bytes()
     hdr_end = raw.index(b"end_header\n") + len(b"end_header\n")
-    assert b"property uchar red\nproperty uchar green\nproperty uchar blue\n" in raw[:hdr_end]
-    nv = len(xyz) * 3
-    rec = np.frombuffer(raw[hdr_end:hdr_end + nv * 15], np.dtype([("p", "<f4", 3), ("c", "u1", 3)]))
-    verts = rec["p"].reshape(-1, 9)
-    assert np.array_equal(verts.view(np.uint32), xyz.view(np.uint32))
-    assert np.array_equal(rec["c"].reshape(-1, 9), trgb)
+    hdr = raw[:hdr_end].decode()
+    assert "property uchar red\nproperty uchar green\nproperty uchar blue\n" in hdr
+    assert f"element vertex {len(iv)}\n" in hdr and f"element face {len(fi)}\n" in hdr
+    rec = np.frombuffer(raw[hdr_end:hdr_end + len(iv) * 15], np.dtype([("p", "<f4", 3), ("c", "u1", 3)]))
+    assert np.array_equal(rec["p"].view(np.uint32), iv.view(np.uint32))
+    assert np.array_equal(rec["c"], irgb)
+    frec = np.frombuffer(raw[hdr_end + len(iv) * 15:], np.dtype([("n", "u1"), ("i", "<i4", 3)]))
+    assert len(frec) == len(fi) and (frec["n"] == 3).all() and np.array_equal(frec["i"], fi)
     vol.close()
 
 

@@ -136,6 +136,16 @@ struct vh_volume {
   float* mc_tris = nullptr;
   int32_t* mc_cubes = nullptr;
   uint8_t* mc_rgb = nullptr;       // colour volumes
+  // indexed-mesh scratch for ix_cap triangles (3 corners each)
+  unsigned long long* ix_keys = nullptr;  // 2 x 3 ix_cap: soup keys, sorted keys
+  uint32_t* ix_u32 = nullptr;             // 4 x 3 ix_cap: idx, idx_sorted, flag, vid
+  int32_t* ix_faces = nullptr;            // 3 ix_cap
+  float* ix_verts = nullptr;              // 3 ix_cap x 3 (host variant)
+  uint8_t* ix_vrgb = nullptr;             // 3 ix_cap x 3 (host variant, colour volumes)
+  void* ix_temp = nullptr;
+  size_t ix_temp_bytes = 0;
+  uint32_t* ix_count = nullptr;
+  int64_t ix_cap = 0;
   int64_t mc_cap = 0;
   uint32_t* mc_counts = nullptr;   // pool_blocks
   uint32_t* mc_nbrs = nullptr;     // pool_blocks x 8
@@ -452,6 +462,8 @@ int dev_alloc(T** p, size_t bytes) {
   return VH_OK;
 }
 
+void free_index_buffers(vh_volume* v);
+
 void free_volume(vh_volume* v) {
   if (!v) return;
   (void)hipSetDevice(v->device);
@@ -466,6 +478,7 @@ void free_volume(vh_volume* v) {
   if (v->ring_pose_host) (void)hipHostFree(v->ring_pose_host);
   if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
+  free_index_buffers(v);
   void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total, v->mc_nbrs,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
@@ -476,7 +489,8 @@ void free_volume(vh_volume* v) {
   delete v;
 }
 
-int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t capacity, int64_t* n_tris) {
+int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t capacity, int64_t* n_tris,
+           unsigned long long* d_vkeys = nullptr, bool keep_degenerate = false) {
   if (!v->mc_counts) {
     int rc;
     if ((rc = dev_alloc(&v->mc_counts, (size_t)v->cfg.pool_blocks * 4)) ||
@@ -503,6 +517,8 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t
   a.cubes = d_cubes;
   a.colors = v->colors;
   a.rgb = d_rgb;
+  a.vkeys = d_vkeys;
+  a.keep_degenerate = keep_degenerate ? 1 : 0;
   a.capacity = (uint32_t)std::min<int64_t>(capacity, 0xffffffffll);
   uint32_t cnt = 0;
   if (a.n_blocks) {
@@ -530,6 +546,89 @@ int ensure_mc_buffer(vh_volume* v, int64_t cap) {
   if (rc) return rc;
   if (v->colors && (rc = dev_alloc(&v->mc_rgb, (size_t)cap * 9))) return rc;
   v->mc_cap = cap;
+  return VH_OK;
+}
+
+void free_index_buffers(vh_volume* v) {
+  void* ptrs[] = {v->ix_keys, v->ix_u32, v->ix_faces, v->ix_verts, v->ix_vrgb, v->ix_temp, v->ix_count};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  v->ix_keys = nullptr;
+  v->ix_u32 = nullptr;
+  v->ix_faces = nullptr;
+  v->ix_verts = nullptr;
+  v->ix_vrgb = nullptr;
+  v->ix_temp = nullptr;
+  v->ix_count = nullptr;
+  v->ix_cap = 0;
+}
+
+int ensure_index_buffers(vh_volume* v, int64_t cap) {
+  if (cap <= v->ix_cap) return VH_OK;
+  if (cap * 3 > 0x7fffffffll) return fail(VH_ERR_INVALID_ARG, "mesh too large to index (2^31 corners)");
+  free_index_buffers(v);
+  const size_t c = (size_t)cap * 3;
+  v->ix_temp_bytes = mesh_index_temp_bytes((uint32_t)c);
+  int rc;
+  if ((rc = dev_alloc(&v->ix_keys, 2 * c * 8)) || (rc = dev_alloc(&v->ix_u32, 4 * c * 4)) ||
+      (rc = dev_alloc(&v->ix_faces, c * 4)) || (rc = dev_alloc(&v->ix_verts, c * 3 * sizeof(float))) ||
+      (v->colors && (rc = dev_alloc(&v->ix_vrgb, c * 3))) || (rc = dev_alloc(&v->ix_temp, v->ix_temp_bytes)) ||
+      (rc = dev_alloc(&v->ix_count, 4))) {
+    free_index_buffers(v);
+    return rc;
+  }
+  v->ix_cap = cap;
+  return VH_OK;
+}
+
+// Marching cubes keeping zero-area triangles, with edge keys only (no soup), then the
+// indexed mesh: faces in ix_faces, vertices (up to vertex_cap) in d_vertices / d_vrgb, or
+// with internal set in ix_verts / ix_vrgb (one vertex per corner fits, after any growth of
+// the index buffers). *n_tris and *n_vertices receive the counts.
+int index_mesh(vh_volume* v, bool internal, float* d_vertices, uint8_t* d_vrgb, int64_t vertex_cap, int64_t* n_tris,
+               int64_t* n_vertices) {
+  int rc = ensure_index_buffers(v, std::max<int64_t>(v->ix_cap, 1 << 20));
+  if (rc) return rc;
+  int64_t n = 0;
+  rc = run_mc(v, nullptr, nullptr, nullptr, v->ix_cap, &n, v->ix_keys, true);
+  if (rc == VH_ERR_TRUNCATED) {
+    if ((rc = ensure_index_buffers(v, n + (n >> 3) + 1024))) return rc;
+    rc = run_mc(v, nullptr, nullptr, nullptr, v->ix_cap, &n, v->ix_keys, true);
+  }
+  if (rc) return rc;
+  if (internal) {
+    d_vertices = v->ix_verts;
+    d_vrgb = v->ix_vrgb;
+    vertex_cap = v->ix_cap * 3;
+  }
+  const size_t c = (size_t)n * 3;
+  IndexArgs a{};
+  a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
+  a.block_keys = v->block_keys;
+  a.voxels = v->voxels;
+  a.colors = v->colors;
+  a.pool_cap = (uint32_t)v->cfg.pool_blocks;
+  a.vox = v->cfg.vox_size;
+  a.n = (uint32_t)c;
+  a.keys = v->ix_keys;
+  a.keys_sorted = v->ix_keys + (size_t)v->ix_cap * 3;
+  a.idx = v->ix_u32;
+  a.idx_sorted = v->ix_u32 + (size_t)v->ix_cap * 3;
+  a.flag = v->ix_u32 + 2 * (size_t)v->ix_cap * 3;
+  a.vid = v->ix_u32 + 3 * (size_t)v->ix_cap * 3;
+  a.temp = v->ix_temp;
+  a.temp_bytes = v->ix_temp_bytes;
+  a.vertices = d_vertices;
+  a.vrgb = d_vrgb;
+  a.faces = v->ix_faces;
+  a.vertex_cap = (uint32_t)std::min<int64_t>(vertex_cap, 0xffffffffll);
+  a.n_vertices = v->ix_count;
+  VH_HIP(launch_mesh_index(a, v->cfg.storage, v->stream));
+  uint32_t nv = 0;
+  VH_HIP(hipMemcpyAsync(&nv, v->ix_count, sizeof nv, hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  *n_tris = n;
+  *n_vertices = nv;
   return VH_OK;
 }
 
@@ -961,6 +1060,60 @@ int vh_extract_mesh(vh_volume* v, vh_mesh* out) {
   return VH_OK;
 }
 
+int vh_extract_mesh_indexed_device(vh_volume* v, float* d_vertices, int64_t vertex_capacity, int32_t* d_faces,
+                                   int64_t face_capacity, uint8_t* d_vrgb, int64_t* n_vertices, int64_t* n_faces) {
+  if (!v || !n_vertices || !n_faces || vertex_capacity < 0 || face_capacity < 0 ||
+      (vertex_capacity > 0 && !d_vertices) || (face_capacity > 0 && !d_faces))
+    return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (d_vrgb && !v->colors) return fail(VH_ERR_INVALID_ARG, "the volume has no colour");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  int64_t n = 0, nv = 0;
+  if ((rc = index_mesh(v, false, vertex_capacity ? d_vertices : nullptr, d_vrgb, vertex_capacity, &n, &nv)))
+    return rc;
+  *n_faces = n;
+  *n_vertices = nv;
+  if (nv > vertex_capacity || n > face_capacity)
+    return fail(VH_ERR_TRUNCATED, "indexed mesh larger than the output buffers");
+  if (n) VH_HIP(hipMemcpy(d_faces, v->ix_faces, (size_t)n * 3 * sizeof(int32_t), hipMemcpyDeviceToDevice));
+  return VH_OK;
+}
+
+int vh_extract_mesh_indexed(vh_volume* v, vh_indexed_mesh* out) {
+  if (!v || !out) return fail(VH_ERR_INVALID_ARG, "null argument");
+  std::memset(out, 0, sizeof *out);
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  int64_t n = 0, nv = 0;
+  if ((rc = index_mesh(v, true, nullptr, nullptr, 0, &n, &nv))) return rc;
+  if (nv > v->ix_cap * 3) return fail(VH_ERR_DEVICE, "vertex count exceeds the corner count");
+  out->n_vertices = nv;
+  out->n_faces = n;
+  const size_t cv = (size_t)std::max<int64_t>(nv, 1) * 3, cf = (size_t)std::max<int64_t>(n, 1) * 3;
+  out->vertices = static_cast<float*>(std::malloc(cv * sizeof(float)));
+  out->faces = static_cast<int32_t*>(std::malloc(cf * sizeof(int32_t)));
+  if (v->colors) out->rgb = static_cast<uint8_t*>(std::malloc(cv));
+  if (!out->vertices || !out->faces || (v->colors && !out->rgb)) {
+    vh_indexed_mesh_free(out);
+    return fail(VH_ERR_OOM, "host allocation failed");
+  }
+  if (nv) VH_HIP(hipMemcpy(out->vertices, v->ix_verts, (size_t)nv * 3 * sizeof(float), hipMemcpyDeviceToHost));
+  if (nv && out->rgb) VH_HIP(hipMemcpy(out->rgb, v->ix_vrgb, (size_t)nv * 3, hipMemcpyDeviceToHost));
+  if (n) VH_HIP(hipMemcpy(out->faces, v->ix_faces, (size_t)n * 3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return VH_OK;
+}
+
+int vh_indexed_mesh_free(vh_indexed_mesh* m) {
+  if (!m) return fail(VH_ERR_INVALID_ARG, "null mesh");
+  std::free(m->vertices);
+  std::free(m->faces);
+  std::free(m->rgb);
+  std::memset(m, 0, sizeof *m);
+  return VH_OK;
+}
+
 int vh_mesh_free(vh_mesh* m) {
   if (!m) return fail(VH_ERR_INVALID_ARG, "null mesh");
   std::free(m->xyz);
@@ -972,38 +1125,40 @@ int vh_mesh_free(vh_mesh* m) {
 
 int vh_save_ply(vh_volume* v, const char* path) {
   if (!v || !path) return fail(VH_ERR_INVALID_ARG, "null argument");
-  vh_mesh m{};
-  int rc = vh_extract_mesh(v, &m);
+  vh_indexed_mesh m{};
+  int rc = vh_extract_mesh_indexed(v, &m);
   if (rc) return rc;
   FILE* f = std::fopen(path, "wb");
   if (!f) {
-    vh_mesh_free(&m);
+    vh_indexed_mesh_free(&m);
     return fail(VH_ERR_IO, std::string("cannot open ") + path);
   }
   std::fprintf(f,
                "ply\nformat binary_little_endian 1.0\nelement vertex %lld\nproperty float x\nproperty float y\n"
                "property float z\n%selement face %lld\nproperty list uchar int vertex_indices\nend_header\n",
-               (long long)(m.n_tris * 3), m.rgb ? "property uchar red\nproperty uchar green\nproperty uchar blue\n" : "",
-               (long long)m.n_tris);
+               (long long)m.n_vertices, m.rgb ? "property uchar red\nproperty uchar green\nproperty uchar blue\n" : "",
+               (long long)m.n_faces);
   bool ok = true;
   if (m.rgb) {
     // vertex records of 15 bytes: x, y, z, r, g, b
-    std::vector<unsigned char> rec((size_t)m.n_tris * 3 * 15);
-    for (size_t i = 0; i < (size_t)m.n_tris * 3; ++i) {
-      std::memcpy(&rec[15 * i], m.xyz + 3 * i, 12);
+    std::vector<unsigned char> rec((size_t)m.n_vertices * 15);
+    for (size_t i = 0; i < (size_t)m.n_vertices; ++i) {
+      std::memcpy(&rec[15 * i], m.vertices + 3 * i, 12);
       std::memcpy(&rec[15 * i + 12], m.rgb + 3 * i, 3);
     }
     ok = std::fwrite(rec.data(), 1, rec.size(), f) == rec.size();
   } else {
-    ok = std::fwrite(m.xyz, sizeof(float), (size_t)m.n_tris * 9, f) == (size_t)m.n_tris * 9;
+    ok = std::fwrite(m.vertices, sizeof(float), (size_t)m.n_vertices * 3, f) == (size_t)m.n_vertices * 3;
   }
-  for (int64_t t = 0; ok && t < m.n_tris; ++t) {
-    unsigned char c = 3;
-    int32_t idx[3] = {(int32_t)(3 * t), (int32_t)(3 * t + 1), (int32_t)(3 * t + 2)};
-    ok = std::fwrite(&c, 1, 1, f) == 1 && std::fwrite(idx, 4, 3, f) == 3;
+  // face records of 13 bytes: the count 3, then three int32 indices
+  std::vector<unsigned char> frec((size_t)m.n_faces * 13);
+  for (size_t t = 0; t < (size_t)m.n_faces; ++t) {
+    frec[13 * t] = 3;
+    std::memcpy(&frec[13 * t + 1], m.faces + 3 * t, 12);
   }
+  ok = ok && std::fwrite(frec.data(), 1, frec.size(), f) == frec.size();
   ok = (std::fclose(f) == 0) && ok;
-  vh_mesh_free(&m);
+  vh_indexed_mesh_free(&m);
   return ok ? VH_OK : fail(VH_ERR_IO, "write failed");
 }
 

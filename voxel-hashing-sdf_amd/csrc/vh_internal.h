@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "vh.h"
 #include "vh_math.h"
 
 namespace vh {
@@ -126,11 +127,106 @@ struct McArgs {
   uint32_t* counts;    // per pool block: triangles (count pass), then exclusive offsets
   uint32_t* chunk_sums;// per 1024 pool blocks
   uint32_t* total;     // total triangles
-  float* tris;         // capacity * 9
+  float* tris;         // capacity * 9, may be null
   int32_t* cubes;      // capacity * 4, may be null
   uint32_t capacity;
   const uint32_t* colors;  // colour pool (null: no colour)
   uint8_t* rgb;            // capacity * 9 (r,g,b per vertex), may be null
+  unsigned long long* vkeys;  // capacity * 3 edge keys (mc_edge_key), may be null
+  int32_t keep_degenerate;    // nonzero: emit every table triangle, zero-area ones included
+};
+
+// Read-only lookup (no concurrent inserts): pool index of `key`, or kNoBlock.
+// Slots of a bucket fill in order, so the first empty slot ends the search.
+__device__ __forceinline__ uint32_t find_block(const HashView& h, uint64_t key) {
+  uint64_t b = bucket_of(key, h.log2_buckets);
+  for (int p = 0; p < kMaxProbe; ++p) {
+    const ulonglong2* bp = reinterpret_cast<const ulonglong2*>(h.keys + b * kSlotsPerBucket);
+    const ulonglong2 k01 = bp[0], k23 = bp[1];
+    const uint64_t ks[4] = {k01.x, k01.y, k23.x, k23.y};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (ks[s] == key) return (uint32_t)h.vals[b * kSlotsPerBucket + s];
+      if (ks[s] == kEmptyKey) return kNoBlock;
+    }
+    b = (b + 1) & h.bucket_mask;
+  }
+  return kNoBlock;
+}
+
+// Storage formats: unpack a voxel word to (tsdf, weight) in FP32, pack it back, the empty word.
+template <int kStorage>
+struct VoxelIO;
+
+template <>
+struct VoxelIO<VH_STORAGE_F32> {
+  using Word = float2;
+  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) { s = w.x; wt = w.y; }
+  static __device__ __forceinline__ Word pack(float s, float wt) { return make_float2(s, wt); }
+  static __device__ __forceinline__ Word zero() { return make_float2(0.0f, 0.0f); }
+};
+template <>
+struct VoxelIO<VH_STORAGE_F16> {
+  using Word = uint32_t;
+  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
+    s = (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    wt = (float)(w >> 16);
+  }
+  static __device__ __forceinline__ Word pack(float s, float wt) {
+    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
+    return (uint32_t)h | ((uint32_t)wt << 16);
+  }
+  static __device__ __forceinline__ Word zero() { return 0u; }
+};
+
+template <>
+struct VoxelIO<VH_STORAGE_F16R> {
+  using Word = uint32_t;
+  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
+    const uint16_t h = (uint16_t)(w & 0xffffu);
+    s = (float)__builtin_bit_cast(_Float16, h) + (float)(int)(int8_t)(w >> 24) * f16r_scale_from_bits(h);
+    wt = (float)((w >> 16) & 0xffu);
+  }
+  static __device__ __forceinline__ Word pack(float s, float wt) {
+    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
+    const float d = (s - (float)__builtin_bit_cast(_Float16, h)) * f16r_inv_scale_from_bits(h);
+    const float r = fminf(127.0f, fmaxf(-128.0f, roundf(d)));
+    return (uint32_t)h | ((uint32_t)wt << 16) | ((uint32_t)(uint8_t)(int8_t)(int)r << 24);
+  }
+  static __device__ __forceinline__ Word zero() { return 0u; }
+};
+
+// Edge key of a triangle corner: the lattice edge it lies on, named by its lower end point
+// (pool block, in-block index (x*8 + y)*8 + z) and its axis (0 x, 1 y, 2 z):
+// (pool << 11) | (in_block << 2) | axis; 38 bits for pools below 2^27 blocks.
+constexpr int kEdgeKeyBits = 38;
+__host__ __device__ inline unsigned long long mc_edge_key(uint32_t pool, uint32_t in_block, uint32_t axis) {
+  return ((unsigned long long)pool << 11) | ((unsigned long long)in_block << 2) | axis;
+}
+
+// Indexed mesh from a triangle soup with edge keys (vh_mesh.hip): one vertex per distinct
+// key, interpolated from the edge's lower end point to its upper one.
+struct IndexArgs {
+  HashView hash;
+  const unsigned long long* block_keys;
+  const void* voxels;
+  const uint32_t* colors;  // may be null
+  uint32_t pool_cap;
+  float vox;
+  uint32_t n;                       // triangle corners (3 per triangle)
+  const unsigned long long* keys;   // n edge keys (soup order)
+  unsigned long long* keys_sorted;  // n
+  uint32_t* idx;                    // n: corner index (iota, then sorted)
+  uint32_t* idx_sorted;             // n
+  uint32_t* flag;                   // n: first of its key (sorted order)
+  uint32_t* vid;                    // n: inclusive scan of flag (distinct keys up to each corner)
+  void* temp;                       // sort / scan scratch
+  size_t temp_bytes;
+  float* vertices;                  // vertex capacity x 3 (metres)
+  uint8_t* vrgb;                    // vertex capacity x 3, may be null
+  int32_t* faces;                   // n: vertex index per corner
+  uint32_t vertex_cap;
+  uint32_t* n_vertices;             // device counter
 };
 constexpr int kScanChunk = 1024;
 
@@ -200,6 +296,10 @@ hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16
                                   int k_int, int k_alloc, hipStream_t st);
 // count pass, two-level exclusive scan, write pass (all on `st`)
 hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
+// Scratch bytes of the sort and scan of launch_mesh_index for n corners.
+size_t mesh_index_temp_bytes(uint32_t n);
+// keys -> sorted keys -> vertex ids -> faces and canonical vertices (all on `st`).
+hipError_t launch_mesh_index(const IndexArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,
                         uint8_t* valid, hipStream_t st);
 hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, int64_t n,

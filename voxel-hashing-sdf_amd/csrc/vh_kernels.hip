@@ -37,24 +37,6 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__bui
 __device__ __forceinline__ void raise_flag(FrameCtl* ctl, uint32_t f) { atomicOr(&ctl->flags, f); }
 
 
-// Read-only lookup (no concurrent inserts): pool index of `key`, or kNoBlock.
-// Slots of a bucket fill in order, so the first empty slot ends the search.
-__device__ __forceinline__ uint32_t find_block(const HashView& h, uint64_t key) {
-  uint64_t b = bucket_of(key, h.log2_buckets);
-  for (int p = 0; p < kMaxProbe; ++p) {
-    const ulonglong2* bp = reinterpret_cast<const ulonglong2*>(h.keys + b * kSlotsPerBucket);
-    const ulonglong2 k01 = bp[0], k23 = bp[1];
-    const uint64_t ks[4] = {k01.x, k01.y, k23.x, k23.y};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      if (ks[s] == key) return (uint32_t)h.vals[b * kSlotsPerBucket + s];
-      if (ks[s] == kEmptyKey) return kNoBlock;
-    }
-    b = (b + 1) & h.bucket_mask;
-  }
-  return kNoBlock;
-}
-
 // Lock-free find-or-insert. A key is written once into the first empty slot of its probe
 // sequence by a compare-and-swap; a stale cached read can only show EMPTY where a key now
 // lives, and the CAS then returns the real occupant, so no lane ever spins on another.
@@ -265,47 +247,6 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
 }
 
 // ============================================================================ integration
-template <int kStorage>
-struct VoxelIO;
-
-template <>
-struct VoxelIO<VH_STORAGE_F32> {
-  using Word = float2;
-  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) { s = w.x; wt = w.y; }
-  static __device__ __forceinline__ Word pack(float s, float wt) { return make_float2(s, wt); }
-  static __device__ __forceinline__ Word zero() { return make_float2(0.0f, 0.0f); }
-};
-template <>
-struct VoxelIO<VH_STORAGE_F16> {
-  using Word = uint32_t;
-  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
-    s = (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
-    wt = (float)(w >> 16);
-  }
-  static __device__ __forceinline__ Word pack(float s, float wt) {
-    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
-    return (uint32_t)h | ((uint32_t)wt << 16);
-  }
-  static __device__ __forceinline__ Word zero() { return 0u; }
-};
-
-template <>
-struct VoxelIO<VH_STORAGE_F16R> {
-  using Word = uint32_t;
-  static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
-    const uint16_t h = (uint16_t)(w & 0xffffu);
-    s = (float)__builtin_bit_cast(_Float16, h) + (float)(int)(int8_t)(w >> 24) * f16r_scale_from_bits(h);
-    wt = (float)((w >> 16) & 0xffu);
-  }
-  static __device__ __forceinline__ Word pack(float s, float wt) {
-    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
-    const float d = (s - (float)__builtin_bit_cast(_Float16, h)) * f16r_inv_scale_from_bits(h);
-    const float r = fminf(127.0f, fmaxf(-128.0f, roundf(d)));
-    return (uint32_t)h | ((uint32_t)wt << 16) | ((uint32_t)(uint8_t)(int8_t)(int)r << 24);
-  }
-  static __device__ __forceinline__ Word zero() { return 0u; }
-};
-
 // tsdf_update (vh_math.h) with the quotient by div_rn: the same correctly rounded result.
 __device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap) {
   const float w_new = w + 1.0f;
@@ -853,12 +794,12 @@ __device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t n
 // The block's (cube, table triangle) items in order lane, x, t, in rounds of kItemCap: each
 // lane lists its own items of the round in LDS, then the wave takes them one per lane.
 // Emit receives (valid, cube x/y/z, case, t, P[9], C[3]) per item slot of a 64-item step;
-// valid is false for empty slots and for degenerate triangles (two equal vertices; the
-// reference's test at src/tsdf.cu:1055-1058, with its typo corrected). kColor: C holds the
-// vertex colours interpolated from the colour tile col; otherwise zeros.
+// valid is false for empty slots and, unless keep, for degenerate triangles (two equal
+// vertices; the reference's test at src/tsdf.cu:1055-1058, with its typo corrected). kColor:
+// C holds the vertex colours interpolated from the colour tile col; otherwise zeros.
 template <bool kColor, class Emit>
 __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint32_t total, int gx0, int gy0, int gz0,
-                                         Emit emit) {
+                                         bool keep, Emit emit) {
   const int lane = threadIdx.x;
   const uint32_t my0 = S.lane_off[lane], my1 = S.lane_off[lane + 1];
   for (uint32_t r0 = 0; r0 < total; r0 += kItemCap) {
@@ -887,7 +828,7 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
         cy = (cube >> 3) & 7;
         cz = cube & 7;
         ci = S.cases[cube];
-        valid = mc_triangle<kColor>(S, col, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C);
+        valid = mc_triangle<kColor>(S, col, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C) || keep;
       }
       emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P, C);
     }
@@ -929,8 +870,8 @@ __global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
     mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
-    uint32_t near;
-    mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
+    uint32_t near;  // a threshold of 0 marks no cube as near: the table counts
+    mc_classify(S, a.keep_degenerate ? 0.0f : mc_tiny(gx0, gy0, gz0), near);
     uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, m);
@@ -1020,14 +961,16 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);
     uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
-    mc_items<kColor>(S, s_col, total, gx0, gy0, gz0, [&](bool valid, int x, int y, int z, int ci, int t,
-                                                         const float* P, const uint32_t* C) {
+    mc_items<kColor>(S, s_col, total, gx0, gy0, gz0, a.keep_degenerate != 0,
+                     [&](bool valid, int x, int y, int z, int ci, int t, const float* P, const uint32_t* C) {
       const uint64_t m = __ballot(valid);
       const uint32_t p = pos + lanes_below(m);
       if (valid && p < a.capacity) {
-        float* o = a.tris + (size_t)p * 9;
+        if (a.tris) {
+          float* o = a.tris + (size_t)p * 9;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
+          for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
+        }
         if (a.cubes) {
           int32_t* q = a.cubes + (size_t)p * 4;
           q[0] = x; q[1] = y; q[2] = z; q[3] = (ci << 8) | t;
@@ -1036,6 +979,21 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
           uint8_t* o8 = a.rgb + (size_t)p * 9;
 #pragma unroll
           for (int k = 0; k < 9; ++k) o8[k] = (uint8_t)(C[k / 3] >> (8 * (k % 3)));
+        }
+        if (a.vkeys) {  // the lattice edge of each corner, named by its lower end point
+          const uint64_t pk = kTriTable[ci];
+          const int lx = x - gx0, ly = y - gy0, lz = z - gz0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
+            const int ca = edge_a(e), cb = edge_b(e);
+            const int ax = lx + min(corner_dx(ca), corner_dx(cb)), ay = ly + min(corner_dy(ca), corner_dy(cb)),
+                      az = lz + min(corner_dz(ca), corner_dz(cb));
+            const int axis = corner_dx(ca) != corner_dx(cb) ? 0 : (corner_dy(ca) != corner_dy(cb) ? 1 : 2);
+            const uint32_t pool = S.nb[((ax >> 3) << 2) | ((ay >> 3) << 1) | (az >> 3)];
+            a.vkeys[(size_t)p * 3 + j] =
+                mc_edge_key(pool, (uint32_t)((((ax & 7) << 3) | (ay & 7)) << 3 | (az & 7)), (uint32_t)axis);
+          }
         }
       }
       pos += (uint32_t)__popcll(m);

@@ -5,14 +5,17 @@
 // its defaults, which the reference only gives on the out-of-line definition at
 // src/tsdf.cu:1249-1252, are on the declaration here), processFrame(depth, rgb, c2w)
 // (tsdf.cuh:609-610) with metric float depth and a row-major camera-to-world matrix,
-// SavePLY(filename) writing an ASCII PLY of welded vertices and triangle faces
+// SavePLY(filename) writing an ASCII PLY of shared vertices and triangle faces
 // (tsdf2mesh, src/tsdf.cu:1760-1888), render(), Shutdown(), getVertices()/getFaces().
 // Colour is integrated as the reference does (each frame's rgb, vertex colours in the PLY);
 // a trailing constructor argument color = false turns it off for depth-only callers.
 // Changed: errors throw std::runtime_error (the reference exits the process, tsdf.cuh:46-61),
 // which includes processFrame with a null rgb on a colour volume (the reference reads it);
 // render() draws nothing (no GL in this library); the mesh comes from marching cubes over
-// every allocated block at the time of the call.
+// every allocated block at the time of the call, indexed on the GPU: one vertex per crossed
+// lattice edge, shared by every face that uses the edge (the reference welds float-identical
+// coordinates on the host instead), and the faces keep the zero-area triangles that appear
+// where the surface passes exactly through a lattice point, so closed surfaces stay closed.
 #pragma once
 
 #include <cstdint>
@@ -20,7 +23,6 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "vh.h"
@@ -85,8 +87,7 @@ class GpuTsdfGenerator {
     check(vh_integrate(vol_, &f), "processFrame");
   }
 
-  // ASCII PLY: "x y z r g b" vertices (metres; colour 0 when colour is off) welded on
-  // identical coordinates (the first triangle's colour is kept, as tsdf2mesh does), then
+  // ASCII PLY: "x y z r g b" vertices (metres; colour 0 when colour is off), then
   // "3 i j k" faces.
   void SavePLY(std::string filename) {
     extract();
@@ -112,48 +113,25 @@ class GpuTsdfGenerator {
     }
   }
 
-  // The welded mesh of the last SavePLY/extract call.
+  // The mesh of the last SavePLY/extract call.
   std::vector<Vertex>* getVertices() { return &vertices_; }
   std::vector<Face>* getFaces() { return &faces_; }
 
-  // Recomputes the welded mesh from the current volume.
+  // Recomputes the indexed mesh from the current volume.
   void extract() {
-    vh_mesh m;
-    check(vh_extract_mesh(vol_, &m), "extract mesh");
-    vertices_.clear();
-    faces_.clear();
-    struct Key {
-      uint32_t b[3];
-      bool operator==(const Key& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
-    };
-    struct KeyHash {
-      size_t operator()(const Key& k) const {
-        uint64_t h = (uint64_t)k.b[0] * 0x9e3779b97f4a7c15ull ^ (uint64_t)k.b[1] * 0xc2b2ae3d27d4eb4full ^
-                     (uint64_t)k.b[2] * 0x165667b19e3779f9ull;
-        return (size_t)(h ^ (h >> 29));
-      }
-    };
-    std::unordered_map<Key, int, KeyHash> index;
-    index.reserve((size_t)m.n_tris * 2);
-    faces_.reserve((size_t)m.n_tris);
-    for (int64_t t = 0; t < m.n_tris; ++t) {
-      Face f;
-      for (int j = 0; j < 3; ++j) {
-        const float* p = m.xyz + 9 * t + 3 * j;
-        Key k;
-        std::memcpy(k.b, p, 12);
-        auto it = index.find(k);
-        if (it == index.end()) {
-          it = index.emplace(k, (int)vertices_.size()).first;
-          const uint8_t* c = m.rgb ? m.rgb + 9 * t + 3 * j : nullptr;
-          vertices_.push_back(Vertex{p[0], p[1], p[2], c ? c[0] : (unsigned char)0, c ? c[1] : (unsigned char)0,
-                                     c ? c[2] : (unsigned char)0});
-        }
-        f.vIdx[j] = it->second;
-      }
-      faces_.push_back(f);
+    vh_indexed_mesh m;
+    check(vh_extract_mesh_indexed(vol_, &m), "extract mesh");
+    vertices_.resize((size_t)m.n_vertices);
+    faces_.resize((size_t)m.n_faces);
+    for (int64_t i = 0; i < m.n_vertices; ++i) {
+      const float* p = m.vertices + 3 * i;
+      const uint8_t* c = m.rgb ? m.rgb + 3 * i : nullptr;
+      vertices_[(size_t)i] = Vertex{p[0], p[1], p[2], c ? c[0] : (unsigned char)0, c ? c[1] : (unsigned char)0,
+                                    c ? c[2] : (unsigned char)0};
     }
-    vh_mesh_free(&m);
+    for (int64_t t = 0; t < m.n_faces; ++t)
+      for (int j = 0; j < 3; ++j) faces_[(size_t)t].vIdx[j] = m.faces[3 * t + j];
+    vh_indexed_mesh_free(&m);
   }
 
   vh_volume* volume() { return vol_; }

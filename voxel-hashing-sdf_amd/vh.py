@@ -79,6 +79,11 @@ class Mesh(C.Structure):
                 ("rgb", C.POINTER(C.c_uint8))]
 
 
+class IndexedMesh(C.Structure):
+    _fields_ = [("n_vertices", C.c_int64), ("n_faces", C.c_int64), ("vertices", C.POINTER(C.c_float)),
+                ("faces", C.POINTER(C.c_int32)), ("rgb", C.POINTER(C.c_uint8))]
+
+
 class TrackConfig(C.Structure):
     _fields_ = [("d_prior_c2w", C.c_void_p), ("d_out_c2w", C.c_void_p), ("d_stats", C.c_void_p),
                 ("gn_iters", C.c_int32), ("point_stride", C.c_int32), ("max_point_depth", C.c_float),
@@ -129,6 +134,10 @@ _PROTOS = [
     ("vh_extract_mesh_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     ("vh_extract_mesh_device_rgb", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                              C.POINTER(C.c_int64)]),
+    ("vh_extract_mesh_indexed_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                                 C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("vh_extract_mesh_indexed", C.c_int, [C.c_void_p, C.POINTER(IndexedMesh)]),
+    ("vh_indexed_mesh_free", C.c_int, [C.POINTER(IndexedMesh)]),
     ("vh_save_ply", C.c_int, [C.c_void_p, C.c_char_p]),
     ("vh_export_blocks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.POINTER(C.c_int64)]),
@@ -416,6 +425,33 @@ class Volume:
         if rc not in (VH_OK, VH_ERR_TRUNCATED):
             _check(rc)
         return n.value
+
+    def extract_mesh_indexed(self, with_rgb: bool = False):
+        """(vertices (V,3) float32 metres, faces (F,3) int32) and, with_rgb, rgb (V,3) uint8 (None
+        without colour): one vertex per crossed lattice edge, shared by its faces; the faces are
+        every table triangle, zero-area ones included, so a closed surface gives a closed mesh."""
+        m = IndexedMesh()
+        _check(_lib.vh_extract_mesh_indexed(self._h, C.byref(m)))
+        try:
+            nv, nf = m.n_vertices, m.n_faces
+            v = np.ctypeslib.as_array(m.vertices, shape=(max(nv, 1) * 3,))[: nv * 3].reshape(nv, 3).copy()
+            f = np.ctypeslib.as_array(m.faces, shape=(max(nf, 1) * 3,))[: nf * 3].reshape(nf, 3).copy()
+            rgb = None
+            if m.rgb:
+                rgb = np.ctypeslib.as_array(m.rgb, shape=(max(nv, 1) * 3,))[: nv * 3].reshape(nv, 3).copy()
+        finally:
+            _lib.vh_indexed_mesh_free(C.byref(m))
+        return (v, f, rgb) if with_rgb else (v, f)
+
+    def extract_mesh_indexed_device(self, d_vertices, vertex_capacity: int, d_faces, face_capacity: int,
+                                    d_vrgb=None):
+        """Device variant; returns (n_vertices, n_faces, status)."""
+        nv, nf = C.c_int64(0), C.c_int64(0)
+        rc = _lib.vh_extract_mesh_indexed_device(self._h, _tptr(d_vertices), vertex_capacity, _tptr(d_faces),
+                                                 face_capacity, _tptr(d_vrgb), C.byref(nv), C.byref(nf))
+        if rc not in (VH_OK, VH_ERR_TRUNCATED):
+            _check(rc)
+        return nv.value, nf.value, rc
 
     def save_ply(self, path: str):
         _check(_lib.vh_save_ply(self._h, path.encode()))
