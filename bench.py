@@ -354,6 +354,17 @@ def main():
         for _ in range(reps):
             ntri = vol.extract_mesh_device(d_xyz, None, cap, d_vrgb if col else None)
         mc_s = (time.perf_counter() - t2) / reps
+        ix = None
+        if name == "f32":  # the indexed mesh of the same volume (device arrays sized by a first call)
+            nv, nfc, _ = vol.extract_mesh_indexed_device(None, 0, None, 0)
+            d_iv = torch.empty((max(nv, 1), 3), dtype=torch.float32, device="cuda")
+            d_if = torch.empty((max(nfc, 1), 3), dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            for _ in range(reps):
+                nv, nfc, rc = vol.extract_mesh_indexed_device(d_iv, nv, d_if, nfc)
+            ix = dict(ms=1e3 * (time.perf_counter() - t2) / reps, vertices=nv, faces=nfc, ok=rc == vh.VH_OK)
+            del d_iv, d_if
         # kernel timing with HIP events (eager launches on the volume's stream)
         vol.set_profiling(True)
         run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
@@ -363,7 +374,7 @@ def main():
             results["gn"] = gn_block(vh, wl, w, vol, nf)
         results[name] = dict(fps=args.steps * nf / dt, ms_per_step=1e3 * dt / args.steps, fps_integrate=nf / dt_int,
                              blocks=st["blocks_allocated"], tris=ntri, mc_ms=1e3 * mc_s,
-                             mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst,
+                             mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst, indexed=ix,
                              vox_bytes=(8 if over.get("storage", 0) == 0 else 4) + (4 if col else 0))
         log(f"bench: {name}: {results[name]['fps']:.1f} frames/s (with MC), {results[name]['fps_integrate']:.1f} "
             f"integrate-only, {st['blocks_allocated']} blocks, {ntri} triangles, MC {1e3 * mc_s:.2f} ms")
@@ -416,6 +427,14 @@ def main():
            "integrate_only_fps": round(r["fps_integrate"], 2), "mc_blocks": r["blocks"],
            "mc_triangles": r["tris"], "mc_ms": round(r["mc_ms"], 3), "mc_blocks_per_s": round(r["mc_blocks_per_s"], 1),
            "roofline": roofline}
+    if r.get("indexed"):
+        ix = r["indexed"]
+        out["mesh_indexed"] = {"ms": round(ix["ms"], 3), "vertices": ix["vertices"], "faces": ix["faces"],
+                               "faces_per_s": round(ix["faces"] / (ix["ms"] / 1e3), 1), "complete": ix["ok"],
+                               "vs_soup_ms": round(ix["ms"] / r["mc_ms"], 3),
+                               "note": "vh_extract_mesh_indexed_device on the final volume: used-edge bits per "
+                                       "block (mark pass), vertex ids from per-block popcount offsets, faces and "
+                                       "vertices in one write pass; one vertex per crossed lattice edge"}
     labels = {"f16r": ("fp16", "binary16 tsdf + uint8 weight (cap 255) + int8 rounding residual, 4 B/voxel"),
               "f16": ("fp16_plain", "binary16 tsdf + uint16 weight (cap 255), 4 B/voxel")}
     if "pcie" in results:

@@ -242,11 +242,11 @@ int vh_extract_mesh_device_rgb(vh_volume* vol, float* d_xyz, int32_t* d_cubes, u
  * two faces. Device buffers: d_vertices (vertex_capacity x 3 floats, metres), d_faces
  * (face_capacity x 3 int32), d_vrgb (vertex_capacity x 3 bytes, colour volumes, may be
  * NULL). *n_vertices and *n_faces receive the counts; VH_ERR_TRUNCATED if either exceeds its
- * capacity (d_faces is then not written; d_vertices / d_vrgb may hold the vertices below
- * vertex_capacity). Synchronous. Replaces the welding step of tsdf2mesh
- * (src/tsdf.cu:1772-1850), which merges vertices with identical float coordinates and so
- * leaves the two interpolations of one edge by neighbouring cubes (in opposite directions)
- * apart when they differ in the last bit. */
+ * capacity (nothing is written then). Vertices are numbered by owning block (the block of
+ * the edge's lower end point, in pool order), then by axis and in-block index.
+ * Synchronous. Replaces the welding step of tsdf2mesh (src/tsdf.cu:1772-1850), which merges
+ * vertices with identical float coordinates and so leaves the two interpolations of one edge
+ * by neighbouring cubes (in opposite directions) apart when they differ in the last bit. */
 int vh_extract_mesh_indexed_device(vh_volume* vol, float* d_vertices, int64_t vertex_capacity,
                                    int32_t* d_faces, int64_t face_capacity, uint8_t* d_vrgb,
                                    int64_t* n_vertices, int64_t* n_faces);

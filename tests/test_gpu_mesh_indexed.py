@@ -3,9 +3,9 @@
 Bar: the same vertices (float bits and colours) and the same faces (corner bits, in winding
 order) as the oracle's indexed mesh, bit for bit, compared as sorted rows (vertex ids follow
 pool indices, which concurrent allocation hands out in a different order on every run); a
-closed sphere gives a closed genus-0 mesh; a mesh of several million faces (which makes the
-library grow its index buffers) equals the oracle's too; the device variant reports counts,
-truncates, and writes the arrays of the host variant.
+closed sphere gives a closed genus-0 mesh; a mesh of several million faces equals the
+oracle's too; the device variant reports counts, writes nothing when a buffer is too small,
+and writes the arrays of the host variant.
 """
 import numpy as np
 import pytest
@@ -80,10 +80,10 @@ def test_room_colour_indexed_bit_exact(vh, oracle, wl, small_frames, storage, ca
 
 
 def test_large_mesh_and_device_variant(vh, oracle, wl):
-    """room_medium_5mm, frames 0, 10, 20 (about 4.3 million faces, beyond the library's first
-    index buffer of 2^20 faces): the host variant equals the oracle; the device variant
-    reports the counts with zero capacities, truncates with one too few, and with exact
-    capacities writes the host variant's arrays (same volume, same order)."""
+    """room_medium_5mm, frames 0, 10, 20 (about 4.3 million faces): the host variant equals
+    the oracle; the device variant reports the counts with zero capacities, writes nothing
+    with one face or one vertex too few, and with exact capacities writes the host variant's
+    arrays (same volume, same order)."""
     import torch
 
     w = wl.WORKLOADS["room_medium_5mm"]
@@ -105,7 +105,9 @@ def test_large_mesh_and_device_variant(vh, oracle, wl):
     d_f = torch.full((nf, 3), -1, dtype=torch.int32, device="cuda")
     _, _, rc = vol.extract_mesh_indexed_device(d_v, nv, d_f, nf - 1)
     assert rc == vh.VH_ERR_TRUNCATED
-    assert (d_f == -1).all().item()  # faces are not written when they do not fit
+    _, _, rc = vol.extract_mesh_indexed_device(d_v, nv - 1, d_f, nf)
+    assert rc == vh.VH_ERR_TRUNCATED
+    assert (d_f == -1).all().item() and (d_v == -1).all().item()  # nothing written
     _, _, rc = vol.extract_mesh_indexed_device(d_v, nv, d_f, nf)
     assert rc == vh.VH_OK
     assert np.array_equal(d_v.cpu().numpy().view(np.uint32), v.view(np.uint32))

@@ -136,16 +136,17 @@ struct vh_volume {
   float* mc_tris = nullptr;
   int32_t* mc_cubes = nullptr;
   uint8_t* mc_rgb = nullptr;       // colour volumes
-  // indexed-mesh scratch for ix_cap triangles (3 corners each)
-  unsigned long long* ix_keys = nullptr;  // 2 x 3 ix_cap: soup keys, sorted keys
-  uint32_t* ix_u32 = nullptr;             // 4 x 3 ix_cap: idx, idx_sorted, flag, vid
-  int32_t* ix_faces = nullptr;            // 3 ix_cap
-  float* ix_verts = nullptr;              // 3 ix_cap x 3 (host variant)
-  uint8_t* ix_vrgb = nullptr;             // 3 ix_cap x 3 (host variant, colour volumes)
-  void* ix_temp = nullptr;
-  size_t ix_temp_bytes = 0;
-  uint32_t* ix_count = nullptr;
-  int64_t ix_cap = 0;
+  // indexed mesh: per pool block used-edge words, their prefixes and vertex counts; the
+  // host variant's device output (ix_fcap faces, ix_vcap vertices)
+  uint32_t* ix_bits = nullptr;     // pool_blocks x kEdgeWords
+  uint16_t* ix_pre = nullptr;      // pool_blocks x kEdgeWords
+  uint32_t* ix_vcounts = nullptr;  // pool_blocks
+  uint32_t* ix_vchunks = nullptr;  // pool_blocks / kScanChunk + 1
+  uint32_t* ix_vtotal = nullptr;
+  int32_t* ix_faces = nullptr;     // ix_fcap x 3
+  float* ix_verts = nullptr;       // ix_vcap x 3
+  uint8_t* ix_vrgb = nullptr;      // ix_vcap x 3 (colour volumes)
+  int64_t ix_fcap = 0, ix_vcap = 0;
   int64_t mc_cap = 0;
   uint32_t* mc_counts = nullptr;   // pool_blocks
   uint32_t* mc_nbrs = nullptr;     // pool_blocks x 8
@@ -489,20 +490,22 @@ void free_volume(vh_volume* v) {
   delete v;
 }
 
-int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t capacity, int64_t* n_tris,
-           unsigned long long* d_vkeys = nullptr, bool keep_degenerate = false) {
-  if (!v->mc_counts) {
-    int rc;
-    if ((rc = dev_alloc(&v->mc_counts, (size_t)v->cfg.pool_blocks * 4)) ||
-        (rc = dev_alloc(&v->mc_chunks, ((size_t)v->cfg.pool_blocks / kScanChunk + 1) * 4)) ||
-        (rc = dev_alloc(&v->mc_nbrs, (size_t)v->cfg.pool_blocks * 8 * 4)) ||
-        (rc = dev_alloc(&v->mc_total, 4)))
-      return rc;
-  }
+int ensure_mc_scratch(vh_volume* v) {
+  if (v->mc_counts) return VH_OK;
+  int rc;
+  if ((rc = dev_alloc(&v->mc_counts, (size_t)v->cfg.pool_blocks * 4)) ||
+      (rc = dev_alloc(&v->mc_chunks, ((size_t)v->cfg.pool_blocks / kScanChunk + 1) * 4)) ||
+      (rc = dev_alloc(&v->mc_nbrs, (size_t)v->cfg.pool_blocks * 8 * 4)) || (rc = dev_alloc(&v->mc_total, 4)))
+    return rc;
+  return VH_OK;
+}
+
+// Marching-cubes arguments over the volume's current blocks (reads the frame control block).
+int mc_args(vh_volume* v, McArgs& a) {
   FrameCtl c{};
-  int rc = read_ctl(v, &c);
-  if (rc) return rc;
-  McArgs a{};
+  int rc = ensure_mc_scratch(v);
+  if (rc || (rc = read_ctl(v, &c))) return rc;
+  a = McArgs{};
   a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
   a.block_keys = v->block_keys;
   a.voxels = v->voxels;
@@ -513,12 +516,17 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t
   a.nbrs = v->mc_nbrs;
   a.chunk_sums = v->mc_chunks;
   a.total = v->mc_total;
+  a.colors = v->colors;
+  return VH_OK;
+}
+
+int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t capacity, int64_t* n_tris) {
+  McArgs a;
+  int rc = mc_args(v, a);
+  if (rc) return rc;
   a.tris = d_xyz;
   a.cubes = d_cubes;
-  a.colors = v->colors;
   a.rgb = d_rgb;
-  a.vkeys = d_vkeys;
-  a.keep_degenerate = keep_degenerate ? 1 : 0;
   a.capacity = (uint32_t)std::min<int64_t>(capacity, 0xffffffffll);
   uint32_t cnt = 0;
   if (a.n_blocks) {
@@ -530,7 +538,6 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t
   if ((int64_t)cnt > capacity) return fail(VH_ERR_TRUNCATED, "mesh larger than the output buffer");
   return VH_OK;
 }
-
 int ensure_mc_buffer(vh_volume* v, int64_t cap) {
   if (cap <= v->mc_cap) return VH_OK;
   if (v->mc_tris) (void)hipFree(v->mc_tris);
@@ -550,85 +557,61 @@ int ensure_mc_buffer(vh_volume* v, int64_t cap) {
 }
 
 void free_index_buffers(vh_volume* v) {
-  void* ptrs[] = {v->ix_keys, v->ix_u32, v->ix_faces, v->ix_verts, v->ix_vrgb, v->ix_temp, v->ix_count};
+  void* ptrs[] = {v->ix_bits, v->ix_pre, v->ix_vcounts, v->ix_vchunks, v->ix_vtotal, v->ix_faces, v->ix_verts, v->ix_vrgb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  v->ix_keys = nullptr;
-  v->ix_u32 = nullptr;
+  v->ix_bits = nullptr;
+  v->ix_pre = nullptr;
+  v->ix_vcounts = nullptr;
+  v->ix_vchunks = nullptr;
+  v->ix_vtotal = nullptr;
   v->ix_faces = nullptr;
   v->ix_verts = nullptr;
   v->ix_vrgb = nullptr;
-  v->ix_temp = nullptr;
-  v->ix_count = nullptr;
-  v->ix_cap = 0;
+  v->ix_fcap = v->ix_vcap = 0;
 }
 
-int ensure_index_buffers(vh_volume* v, int64_t cap) {
-  if (cap <= v->ix_cap) return VH_OK;
-  if (cap * 3 > 0x7fffffffll) return fail(VH_ERR_INVALID_ARG, "mesh too large to index (2^31 corners)");
-  free_index_buffers(v);
-  const size_t c = (size_t)cap * 3;
-  v->ix_temp_bytes = mesh_index_temp_bytes((uint32_t)c);
-  int rc;
-  if ((rc = dev_alloc(&v->ix_keys, 2 * c * 8)) || (rc = dev_alloc(&v->ix_u32, 4 * c * 4)) ||
-      (rc = dev_alloc(&v->ix_faces, c * 4)) || (rc = dev_alloc(&v->ix_verts, c * 3 * sizeof(float))) ||
-      (v->colors && (rc = dev_alloc(&v->ix_vrgb, c * 3))) || (rc = dev_alloc(&v->ix_temp, v->ix_temp_bytes)) ||
-      (rc = dev_alloc(&v->ix_count, 4))) {
-    free_index_buffers(v);
-    return rc;
+// Indexed mesh, first half: used edges, face and vertex counts and offsets. *a receives the
+// arguments for index_write; *n_faces and *n_vertices the totals.
+int index_count(vh_volume* v, McArgs& a, int64_t* n_faces, int64_t* n_vertices) {
+  int rc = mc_args(v, a);
+  if (rc) return rc;
+  if (!v->ix_bits) {
+    const size_t pb = (size_t)v->cfg.pool_blocks;
+    if ((rc = dev_alloc(&v->ix_bits, pb * kEdgeWords * 4)) || (rc = dev_alloc(&v->ix_pre, pb * kEdgeWords * 2)) ||
+        (rc = dev_alloc(&v->ix_vcounts, pb * 4)) || (rc = dev_alloc(&v->ix_vchunks, (pb / kScanChunk + 1) * 4)) ||
+        (rc = dev_alloc(&v->ix_vtotal, 4)))
+      return rc;
   }
-  v->ix_cap = cap;
+  a.ebits = v->ix_bits;
+  a.epre = v->ix_pre;
+  a.vcounts = v->ix_vcounts;
+  a.vchunks = v->ix_vchunks;
+  a.vtotal = v->ix_vtotal;
+  uint32_t tot[2] = {0u, 0u};
+  if (a.n_blocks) {
+    VH_HIP(launch_mesh_index_count(a, v->cfg.storage, v->stream));
+    VH_HIP(hipMemcpyAsync(&tot[0], v->mc_total, 4, hipMemcpyDeviceToHost, v->stream));
+    VH_HIP(hipMemcpyAsync(&tot[1], v->ix_vtotal, 4, hipMemcpyDeviceToHost, v->stream));
+  }
+  VH_HIP(hipStreamSynchronize(v->stream));
+  *n_faces = tot[0];
+  *n_vertices = tot[1];
+  if (tot[1] > 0x7fffffffu) return fail(VH_ERR_INVALID_ARG, "more than 2^31 vertices");
   return VH_OK;
 }
 
-// Marching cubes keeping zero-area triangles, with edge keys only (no soup), then the
-// indexed mesh: faces in ix_faces, vertices (up to vertex_cap) in d_vertices / d_vrgb, or
-// with internal set in ix_verts / ix_vrgb (one vertex per corner fits, after any growth of
-// the index buffers). *n_tris and *n_vertices receive the counts.
-int index_mesh(vh_volume* v, bool internal, float* d_vertices, uint8_t* d_vrgb, int64_t vertex_cap, int64_t* n_tris,
-               int64_t* n_vertices) {
-  int rc = ensure_index_buffers(v, std::max<int64_t>(v->ix_cap, 1 << 20));
-  if (rc) return rc;
-  int64_t n = 0;
-  rc = run_mc(v, nullptr, nullptr, nullptr, v->ix_cap, &n, v->ix_keys, true);
-  if (rc == VH_ERR_TRUNCATED) {
-    if ((rc = ensure_index_buffers(v, n + (n >> 3) + 1024))) return rc;
-    rc = run_mc(v, nullptr, nullptr, nullptr, v->ix_cap, &n, v->ix_keys, true);
-  }
-  if (rc) return rc;
-  if (internal) {
-    d_vertices = v->ix_verts;
-    d_vrgb = v->ix_vrgb;
-    vertex_cap = v->ix_cap * 3;
-  }
-  const size_t c = (size_t)n * 3;
-  IndexArgs a{};
-  a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
-  a.block_keys = v->block_keys;
-  a.voxels = v->voxels;
-  a.colors = v->colors;
-  a.pool_cap = (uint32_t)v->cfg.pool_blocks;
-  a.vox = v->cfg.vox_size;
-  a.n = (uint32_t)c;
-  a.keys = v->ix_keys;
-  a.keys_sorted = v->ix_keys + (size_t)v->ix_cap * 3;
-  a.idx = v->ix_u32;
-  a.idx_sorted = v->ix_u32 + (size_t)v->ix_cap * 3;
-  a.flag = v->ix_u32 + 2 * (size_t)v->ix_cap * 3;
-  a.vid = v->ix_u32 + 3 * (size_t)v->ix_cap * 3;
-  a.temp = v->ix_temp;
-  a.temp_bytes = v->ix_temp_bytes;
+// Second half: faces and vertices into buffers that hold them all.
+int index_write(vh_volume* v, McArgs& a, int32_t* d_faces, int64_t n_faces, float* d_vertices, uint8_t* d_vrgb,
+                int64_t n_vertices) {
+  if (n_faces == 0) return VH_OK;
+  a.faces = d_faces;
+  a.capacity = (uint32_t)n_faces;
   a.vertices = d_vertices;
   a.vrgb = d_vrgb;
-  a.faces = v->ix_faces;
-  a.vertex_cap = (uint32_t)std::min<int64_t>(vertex_cap, 0xffffffffll);
-  a.n_vertices = v->ix_count;
-  VH_HIP(launch_mesh_index(a, v->cfg.storage, v->stream));
-  uint32_t nv = 0;
-  VH_HIP(hipMemcpyAsync(&nv, v->ix_count, sizeof nv, hipMemcpyDeviceToHost, v->stream));
+  a.vertex_cap = (uint32_t)n_vertices;
+  VH_HIP(launch_mesh_index_write(a, v->cfg.storage, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
-  *n_tris = n;
-  *n_vertices = nv;
   return VH_OK;
 }
 
@@ -1069,15 +1052,14 @@ int vh_extract_mesh_indexed_device(vh_volume* v, float* d_vertices, int64_t vert
   std::lock_guard<std::mutex> lk(v->mu);
   int rc = set_device(v);
   if (rc) return rc;
-  int64_t n = 0, nv = 0;
-  if ((rc = index_mesh(v, false, vertex_capacity ? d_vertices : nullptr, d_vrgb, vertex_capacity, &n, &nv)))
-    return rc;
-  *n_faces = n;
+  McArgs a;
+  int64_t nf = 0, nv = 0;
+  if ((rc = index_count(v, a, &nf, &nv))) return rc;
+  *n_faces = nf;
   *n_vertices = nv;
-  if (nv > vertex_capacity || n > face_capacity)
+  if (nv > vertex_capacity || nf > face_capacity)
     return fail(VH_ERR_TRUNCATED, "indexed mesh larger than the output buffers");
-  if (n) VH_HIP(hipMemcpy(d_faces, v->ix_faces, (size_t)n * 3 * sizeof(int32_t), hipMemcpyDeviceToDevice));
-  return VH_OK;
+  return index_write(v, a, d_faces, nf, d_vertices, d_vrgb, nv);
 }
 
 int vh_extract_mesh_indexed(vh_volume* v, vh_indexed_mesh* out) {
@@ -1086,9 +1068,26 @@ int vh_extract_mesh_indexed(vh_volume* v, vh_indexed_mesh* out) {
   std::lock_guard<std::mutex> lk(v->mu);
   int rc = set_device(v);
   if (rc) return rc;
+  McArgs a;
   int64_t n = 0, nv = 0;
-  if ((rc = index_mesh(v, true, nullptr, nullptr, 0, &n, &nv))) return rc;
-  if (nv > v->ix_cap * 3) return fail(VH_ERR_DEVICE, "vertex count exceeds the corner count");
+  if ((rc = index_count(v, a, &n, &nv))) return rc;
+  // device output of the host variant, grown with 1/8 slack
+  if (n > v->ix_fcap || nv > v->ix_vcap) {
+    const int64_t fcap = std::max(n + (n >> 3), v->ix_fcap), vcap = std::max(nv + (nv >> 3), v->ix_vcap);
+    void* old[] = {v->ix_faces, v->ix_verts, v->ix_vrgb};
+    for (void* p : old)
+      if (p) (void)hipFree(p);
+    v->ix_faces = nullptr;
+    v->ix_verts = nullptr;
+    v->ix_vrgb = nullptr;
+    v->ix_fcap = v->ix_vcap = 0;
+    if ((rc = dev_alloc(&v->ix_faces, (size_t)fcap * 3 * 4)) || (rc = dev_alloc(&v->ix_verts, (size_t)vcap * 3 * 4)) ||
+        (v->colors && (rc = dev_alloc(&v->ix_vrgb, (size_t)vcap * 3))))
+      return rc;
+    v->ix_fcap = fcap;
+    v->ix_vcap = vcap;
+  }
+  if ((rc = index_write(v, a, v->ix_faces, n, v->ix_verts, v->ix_vrgb, nv))) return rc;
   out->n_vertices = nv;
   out->n_faces = n;
   const size_t cv = (size_t)std::max<int64_t>(nv, 1) * 3, cf = (size_t)std::max<int64_t>(n, 1) * 3;

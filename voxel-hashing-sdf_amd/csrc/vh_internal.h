@@ -132,8 +132,17 @@ struct McArgs {
   uint32_t capacity;
   const uint32_t* colors;  // colour pool (null: no colour)
   uint8_t* rgb;            // capacity * 9 (r,g,b per vertex), may be null
-  unsigned long long* vkeys;  // capacity * 3 edge keys (mc_edge_key), may be null
-  int32_t keep_degenerate;    // nonzero: emit every table triangle, zero-area ones included
+  // indexed mesh (launch_mesh_index_*): lattice edges are owned by the block holding their
+  // lower end point; bit axis * 512 + in-block index of a block's 48 words marks a used edge
+  uint32_t* ebits;     // per pool block: 48 words of used-edge bits
+  uint16_t* epre;      // per pool block: 48 exclusive popcount prefixes of ebits
+  uint32_t* vcounts;   // per pool block: owned vertices, then exclusive offsets
+  uint32_t* vchunks;   // per kScanChunk pool blocks
+  uint32_t* vtotal;    // total vertices
+  int32_t* faces;      // capacity * 3 vertex indices
+  float* vertices;     // vertex_cap * 3 (metres)
+  uint8_t* vrgb;       // vertex_cap * 3, may be null
+  uint32_t vertex_cap;
 };
 
 // Read-only lookup (no concurrent inserts): pool index of `key`, or kNoBlock.
@@ -196,39 +205,8 @@ struct VoxelIO<VH_STORAGE_F16R> {
   static __device__ __forceinline__ Word zero() { return 0u; }
 };
 
-// Edge key of a triangle corner: the lattice edge it lies on, named by its lower end point
-// (pool block, in-block index (x*8 + y)*8 + z) and its axis (0 x, 1 y, 2 z):
-// (pool << 11) | (in_block << 2) | axis; 38 bits for pools below 2^27 blocks.
-constexpr int kEdgeKeyBits = 38;
-__host__ __device__ inline unsigned long long mc_edge_key(uint32_t pool, uint32_t in_block, uint32_t axis) {
-  return ((unsigned long long)pool << 11) | ((unsigned long long)in_block << 2) | axis;
-}
-
-// Indexed mesh from a triangle soup with edge keys (vh_mesh.hip): one vertex per distinct
-// key, interpolated from the edge's lower end point to its upper one.
-struct IndexArgs {
-  HashView hash;
-  const unsigned long long* block_keys;
-  const void* voxels;
-  const uint32_t* colors;  // may be null
-  uint32_t pool_cap;
-  float vox;
-  uint32_t n;                       // triangle corners (3 per triangle)
-  const unsigned long long* keys;   // n edge keys (soup order)
-  unsigned long long* keys_sorted;  // n
-  uint32_t* idx;                    // n: corner index (iota, then sorted)
-  uint32_t* idx_sorted;             // n
-  uint32_t* flag;                   // n: first of its key (sorted order)
-  uint32_t* vid;                    // n: inclusive scan of flag (distinct keys up to each corner)
-  void* temp;                       // sort / scan scratch
-  size_t temp_bytes;
-  float* vertices;                  // vertex capacity x 3 (metres)
-  uint8_t* vrgb;                    // vertex capacity x 3, may be null
-  int32_t* faces;                   // n: vertex index per corner
-  uint32_t vertex_cap;
-  uint32_t* n_vertices;             // device counter
-};
 constexpr int kScanChunk = 1024;
+constexpr int kEdgeWords = 3 * kB3 / 32;  // used-edge words per block (indexed mesh)
 
 struct QueryArgs {
   HashView hash;
@@ -296,10 +274,11 @@ hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16
                                   int k_int, int k_alloc, hipStream_t st);
 // count pass, two-level exclusive scan, write pass (all on `st`)
 hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
-// Scratch bytes of the sort and scan of launch_mesh_index for n corners.
-size_t mesh_index_temp_bytes(uint32_t n);
-// keys -> sorted keys -> vertex ids -> faces and canonical vertices (all on `st`).
-hipError_t launch_mesh_index(const IndexArgs& a, int storage, hipStream_t st);
+// Indexed mesh, first half: neighbour table, used-edge bits and face counts, vertex counts,
+// and the exclusive scans of both (face total in *a.total, vertex total in *a.vtotal).
+hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st);
+// Second half (after the count on the same arguments): faces and vertices.
+hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,
                         uint8_t* valid, hipStream_t st);
 hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, int64_t n,

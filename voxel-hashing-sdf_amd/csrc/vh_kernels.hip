@@ -794,12 +794,13 @@ __device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t n
 // The block's (cube, table triangle) items in order lane, x, t, in rounds of kItemCap: each
 // lane lists its own items of the round in LDS, then the wave takes them one per lane.
 // Emit receives (valid, cube x/y/z, case, t, P[9], C[3]) per item slot of a 64-item step;
-// valid is false for empty slots and, unless keep, for degenerate triangles (two equal
-// vertices; the reference's test at src/tsdf.cu:1055-1058, with its typo corrected). kColor:
-// C holds the vertex colours interpolated from the colour tile col; otherwise zeros.
-template <bool kColor, class Emit>
+// With kPos, valid is false for empty slots and for degenerate triangles (two equal
+// vertices; the reference's test at src/tsdf.cu:1055-1058, with its typo corrected), and
+// kColor makes C hold the vertex colours interpolated from the colour tile col (otherwise
+// zeros). Without kPos, P and C are not computed and every item is valid.
+template <bool kColor, bool kPos, class Emit>
 __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint32_t total, int gx0, int gy0, int gz0,
-                                         bool keep, Emit emit) {
+                                         Emit emit) {
   const int lane = threadIdx.x;
   const uint32_t my0 = S.lane_off[lane], my1 = S.lane_off[lane + 1];
   for (uint32_t r0 = 0; r0 < total; r0 += kItemCap) {
@@ -828,7 +829,8 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
         cy = (cube >> 3) & 7;
         cz = cube & 7;
         ci = S.cases[cube];
-        valid = mc_triangle<kColor>(S, col, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C) || keep;
+        if constexpr (kPos) valid = mc_triangle<kColor>(S, col, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C);
+        else valid = true;
       }
       emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P, C);
     }
@@ -847,10 +849,15 @@ __device__ __forceinline__ float mc_tiny(int gx0, int gy0, int gz0) {
 // The pool indices of each block's 8 neighbourhood blocks (+x/+y/+z and their combinations,
 // index (dx << 2) | (dy << 1) | dz; itself at 0; kNoBlock where absent), one lane per entry,
 // so that the extraction passes read one word instead of walking the hash per neighbour.
+// With ebits (indexed mesh), each lane also clears 6 of its block's 48 used-edge words.
 __global__ __launch_bounds__(256) void k_mc_neighbours(McArgs a) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= (size_t)a.n_blocks * 8) return;
   const uint32_t b = (uint32_t)(i >> 3), d = (uint32_t)(i & 7);
+  if (a.ebits) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.ebits[(size_t)b * kEdgeWords + d * 6 + k] = 0u;
+  }
   uint32_t nb = b;
   if (d > 0) {
     int bx, by, bz;
@@ -870,8 +877,8 @@ __global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     int gx0, gy0, gz0;
     mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
-    uint32_t near;  // a threshold of 0 marks no cube as near: the table counts
-    mc_classify(S, a.keep_degenerate ? 0.0f : mc_tiny(gx0, gy0, gz0), near);
+    uint32_t near;
+    mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
     uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, m);
@@ -961,16 +968,14 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);
     uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
-    mc_items<kColor>(S, s_col, total, gx0, gy0, gz0, a.keep_degenerate != 0,
-                     [&](bool valid, int x, int y, int z, int ci, int t, const float* P, const uint32_t* C) {
+    mc_items<kColor, true>(S, s_col, total, gx0, gy0, gz0,
+                           [&](bool valid, int x, int y, int z, int ci, int t, const float* P, const uint32_t* C) {
       const uint64_t m = __ballot(valid);
       const uint32_t p = pos + lanes_below(m);
       if (valid && p < a.capacity) {
-        if (a.tris) {
-          float* o = a.tris + (size_t)p * 9;
+        float* o = a.tris + (size_t)p * 9;
 #pragma unroll
-          for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
-        }
+        for (int k = 0; k < 9; ++k) o[k] = P[k] * a.vox;
         if (a.cubes) {
           int32_t* q = a.cubes + (size_t)p * 4;
           q[0] = x; q[1] = y; q[2] = z; q[3] = (ci << 8) | t;
@@ -980,24 +985,188 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
 #pragma unroll
           for (int k = 0; k < 9; ++k) o8[k] = (uint8_t)(C[k / 3] >> (8 * (k % 3)));
         }
-        if (a.vkeys) {  // the lattice edge of each corner, named by its lower end point
-          const uint64_t pk = kTriTable[ci];
-          const int lx = x - gx0, ly = y - gy0, lz = z - gz0;
+      }
+      pos += (uint32_t)__popcll(m);
+    });
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- indexed mesh
+// One vertex per used lattice edge (an edge some face lies on), owned by the block that holds
+// the edge's lower end point; its id is the owner's vertex offset plus the edge's rank among
+// the owner's used edges (bit order axis * 512 + in-block index). Faces are every table
+// triangle of every cube with 8 observed corners (zero-area ones included: dropping them
+// would open the mesh where the surface passes exactly through a lattice point), in the
+// order of k_mc_write; the table's triangles of a case use exactly the edges whose corners
+// differ in sign, so those are the edges marked. Vertices are interpolated from the lower
+// end point to the upper one (VertexInterp, src/tsdf.cu:1640-1660, with that orientation):
+// every face that uses an edge gets the same vertex, and the mesh has no cracks where
+// neighbouring cubes would interpolate an edge in opposite directions (the reference welds
+// float-identical coordinates instead, src/tsdf.cu:1772-1850).
+
+// Per cube edge e, 5 bits at 5e: the offset of its lower end point in the cube (dx | dy << 1
+// | dz << 2) and its axis << 3 (0 x, 1 y, 2 z).
+constexpr uint64_t edge_lo_table() {
+  uint64_t t = 0;
+  for (int e = 0; e < 12; ++e) {
+    const int a = edge_a(e), b = edge_b(e);
+    const int dx = corner_dx(a) < corner_dx(b) ? corner_dx(a) : corner_dx(b);
+    const int dy = corner_dy(a) < corner_dy(b) ? corner_dy(a) : corner_dy(b);
+    const int dz = corner_dz(a) < corner_dz(b) ? corner_dz(a) : corner_dz(b);
+    const int axis = corner_dx(a) != corner_dx(b) ? 0 : (corner_dy(a) != corner_dy(b) ? 1 : 2);
+    t |= (uint64_t)(dx | dy << 1 | dz << 2 | axis << 3) << (5 * e);
+  }
+  return t;
+}
+constexpr uint64_t kEdgeLo = edge_lo_table();
+
+// Edges of case ci whose two corners lie on opposite sides (bit e): corners 0-3 and 4-7 form
+// the z = 0 and z = 1 rings (edges 0-3 and 4-7 join ring neighbours i, i + 1 mod 4), edges
+// 8-11 join corner i to corner i + 4.
+__host__ __device__ constexpr uint32_t crossed_edges(int ci) {
+  const uint32_t lo = (uint32_t)ci & 15u, hi = ((uint32_t)ci >> 4) & 15u;
+  const uint32_t rlo = ((lo >> 1) | (lo << 3)) & 15u, rhi = ((hi >> 1) | (hi << 3)) & 15u;
+  return (lo ^ rlo) | ((hi ^ rhi) << 4) | ((lo ^ hi) << 8);
+}
+constexpr bool crossed_edges_ok() {
+  for (int ci = 0; ci < 256; ++ci)
+    for (int e = 0; e < 12; ++e)
+      if ((((ci >> edge_a(e)) ^ (ci >> edge_b(e))) & 1) != (int)((crossed_edges(ci) >> e) & 1u)) return false;
+  return true;
+}
+static_assert(crossed_edges_ok(), "crossed_edges disagrees with the edge table");
+
+// Owner slot (index into S.nb) of edge e of cube (cx, cy, cz) and the edge's bit in its
+// owner's used-edge words: the lower end point (block-local, 0..8 per axis) decides both.
+__device__ __forceinline__ void edge_owner(int e, int cx, int cy, int cz, int& o, int& bit) {
+  const uint32_t info = (uint32_t)(kEdgeLo >> (5 * e)) & 31u;
+  const int lx = cx + (int)(info & 1u), ly = cy + (int)((info >> 1) & 1u), lz = cz + (int)((info >> 2) & 1u);
+  o = ((lx >> 3) << 2) | ((ly >> 3) << 1) | (lz >> 3);
+  bit = (int)(info >> 3) * kB3 + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+}
+
+// Marks the used edges and counts the faces (table counts) of each block. Edges the block
+// owns are gathered in LDS and merged with one atomic OR per word; edges owned by a +x/+y/+z
+// neighbour go to its words directly.
+template <int kStorage>
+__global__ __launch_bounds__(64) void k_ix_mark(McArgs a) {
+  __shared__ McShared S;
+  __shared__ uint32_t s_bits[kEdgeWords];
+  const int lane = threadIdx.x;
+  mc_load_counts(S);
+  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    if (lane < kEdgeWords) s_bits[lane] = 0u;
+    int gx0, gy0, gz0;
+    mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
+    uint32_t near;
+    const uint32_t total = mc_classify(S, 0.0f, near);
+    const int cy = lane >> 3, cz = lane & 7;
+    for (int cx = 0; cx < 8; ++cx) {
+      const int ci = S.cases[cx * 64 + lane];
+      for (uint32_t m = ci ? crossed_edges(ci) : 0u; m; m &= m - 1u) {
+        const int e = __builtin_ctz(m);
+        int o, bit;
+        edge_owner(e, cx, cy, cz, o, bit);
+        if (o == 0) atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
+        else atomicOr(&a.ebits[(size_t)S.nb[o] * kEdgeWords + (bit >> 5)], 1u << (bit & 31));
+      }
+    }
+    if (lane == 0) a.counts[b] = total;
+    __syncthreads();
+    if (lane < kEdgeWords && s_bits[lane]) atomicOr(&a.ebits[(size_t)b * kEdgeWords + lane], s_bits[lane]);
+    __syncthreads();
+  }
+}
+
+// Per block: popcount prefixes of the 48 used-edge words and the block's vertex count (one
+// wave per block, lanes 0..47 one word each).
+__global__ __launch_bounds__(256) void k_ix_count(McArgs a) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6); b < a.n_blocks; b += gridDim.x * 4u) {
+    const uint32_t w = lane < kEdgeWords ? a.ebits[(size_t)b * kEdgeWords + lane] : 0u;
+    const uint32_t c = (uint32_t)__popc(w);
+    uint32_t x = c;
 #pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
-            const int ca = edge_a(e), cb = edge_b(e);
-            const int ax = lx + min(corner_dx(ca), corner_dx(cb)), ay = ly + min(corner_dy(ca), corner_dy(cb)),
-                      az = lz + min(corner_dz(ca), corner_dz(cb));
-            const int axis = corner_dx(ca) != corner_dx(cb) ? 0 : (corner_dy(ca) != corner_dy(cb) ? 1 : 2);
-            const uint32_t pool = S.nb[((ax >> 3) << 2) | ((ay >> 3) << 1) | (az >> 3)];
-            a.vkeys[(size_t)p * 3 + j] =
-                mc_edge_key(pool, (uint32_t)((((ax & 7) << 3) | (ay & 7)) << 3 | (az & 7)), (uint32_t)axis);
-          }
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane < kEdgeWords) a.epre[(size_t)b * kEdgeWords + lane] = (uint16_t)(x - c);
+    if (lane == kEdgeWords - 1) a.vcounts[b] = x;
+  }
+}
+
+// Faces (vertex ids of each table triangle's corners) and the vertices each block owns.
+template <int kStorage, bool kColor>
+__global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
+  __shared__ McShared S;
+  __shared__ uint32_t s_col[kColor ? kHalo3 : 1];
+  __shared__ uint32_t s_vbase[8];
+  const int lane = threadIdx.x;
+  mc_load_counts(S);
+  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    int gx0, gy0, gz0;
+    mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
+    if (lane < 8) {
+      const uint32_t nb = S.nb[lane];
+      s_vbase[lane] = nb != kNoBlock ? a.vchunks[nb / kScanChunk] + a.vcounts[nb] : 0u;
+    }
+    uint32_t near;
+    const uint32_t total = mc_classify(S, 0.0f, near);  // ends with a barrier
+    uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
+    mc_items<false, false>(S, nullptr, total, gx0, gy0, gz0,
+                           [&](bool valid, int x, int y, int z, int ci, int t, const float*, const uint32_t*) {
+      const uint64_t m = __ballot(valid);
+      const uint32_t p = pos + lanes_below(m);
+      if (valid && p < a.capacity) {
+        const uint64_t pk = kTriTable[ci];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          int o, bit;
+          edge_owner((int)((pk >> (4 * (3 * t + j))) & 15u), x - gx0, y - gy0, z - gz0, o, bit);
+          const size_t wi = (size_t)S.nb[o] * kEdgeWords + (bit >> 5);
+          const uint32_t below = a.ebits[wi] & ((1u << (bit & 31)) - 1u);
+          a.faces[(size_t)p * 3 + j] = (int32_t)(s_vbase[o] + a.epre[wi] + (uint32_t)__popc(below));
         }
       }
       pos += (uint32_t)__popcll(m);
     });
+    // the block's own vertices: lane l < 48 walks the set bits of word l
+    if (lane < kEdgeWords) {
+      uint32_t w = a.ebits[(size_t)b * kEdgeWords + lane];
+      uint32_t id = s_vbase[0] + a.epre[(size_t)b * kEdgeWords + lane];
+      while (w) {
+        const int bit = lane * 32 + __builtin_ctz(w);
+        w &= w - 1u;
+        const int axis = bit / kB3, vi = bit % kB3;
+        const int lx = vi >> 6, ly = (vi >> 3) & 7, lz = vi & 7;
+        const int ux = lx + (axis == 0), uy = ly + (axis == 1), uz = lz + (axis == 2);
+        const int i0 = lx * 81 + ly * 9 + lz, i1 = ux * 81 + uy * 9 + uz;
+        float px, py, pz;
+        uint32_t c = 0u;
+        if constexpr (kColor)
+          vertex_interp_rgb((float)(gx0 + lx), (float)(gy0 + ly), (float)(gz0 + lz), (float)(gx0 + ux),
+                            (float)(gy0 + uy), (float)(gz0 + uz), S.val[i0], S.val[i1], s_col[i0], s_col[i1], px,
+                            py, pz, c);
+        else
+          vertex_interp((float)(gx0 + lx), (float)(gy0 + ly), (float)(gz0 + lz), (float)(gx0 + ux),
+                        (float)(gy0 + uy), (float)(gz0 + uz), S.val[i0], S.val[i1], px, py, pz);
+        if (id < a.vertex_cap) {
+          float* q = a.vertices + (size_t)id * 3;
+          q[0] = px * a.vox;
+          q[1] = py * a.vox;
+          q[2] = pz * a.vox;
+          if (kColor) {
+            uint8_t* q8 = a.vrgb + (size_t)id * 3;
+            q8[0] = (uint8_t)c;
+            q8[1] = (uint8_t)(c >> 8);
+            q8[2] = (uint8_t)(c >> 16);
+          }
+        }
+        ++id;
+      }
+    }
     __syncthreads();
   }
 }
@@ -1438,6 +1607,36 @@ hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
   with_storage(storage, [&](auto S) {
     if (a.colors) hipLaunchKernelGGL((k_mc_write<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
     else hipLaunchKernelGGL((k_mc_write<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st) {
+  if (a.n_blocks == 0) return hipSuccess;
+  const uint32_t grid = min(a.n_blocks, 8192u);
+  const uint32_t n_chunks = (a.n_blocks + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_mc_neighbours, dim3((uint32_t)(((size_t)a.n_blocks * 8 + 255) / 256)), dim3(256), 0, st, a);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_ix_mark<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
+  });
+  hipLaunchKernelGGL(k_ix_count, dim3(min((a.n_blocks + 3u) / 4u, 16384u)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, a, n_chunks);
+  McArgs v = a;  // the same scans over the vertex counts
+  v.counts = a.vcounts;
+  v.chunk_sums = a.vchunks;
+  v.total = a.vtotal;
+  hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, v);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, v, n_chunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st) {
+  if (a.n_blocks == 0) return hipSuccess;
+  const uint32_t grid = min(a.n_blocks, 8192u);
+  with_storage(storage, [&](auto S) {
+    if (a.colors && a.vrgb) hipLaunchKernelGGL((k_ix_write<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL((k_ix_write<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
   });
   return hipGetLastError();
 }

@@ -288,11 +288,11 @@ VH_HD bool depth_valid(float d, float max_depth) { return d > 0.0f && d <= max_d
 // Corner order of the reference (include/tsdf.cuh:234-241): x/y swapped against Bourke's
 // numbering, which fixes the triangle winding. Edge e joins corners kEdgeA[e], kEdgeB[e]
 // (reference VertexInterp calls, src/tsdf.cu:1001-1037).
-VH_HD int corner_dx(int c) { return (c == 2 || c == 3 || c == 6 || c == 7) ? 1 : 0; }
-VH_HD int corner_dy(int c) { return (c == 1 || c == 2 || c == 5 || c == 6) ? 1 : 0; }
-VH_HD int corner_dz(int c) { return c >= 4 ? 1 : 0; }
-VH_HD int edge_a(int e) { return e < 8 ? e : e - 8; }
-VH_HD int edge_b(int e) { return e < 8 ? ((e & 3) == 3 ? e - 3 : e + 1) : e - 4; }
+VH_HD constexpr int corner_dx(int c) { return (c == 2 || c == 3 || c == 6 || c == 7) ? 1 : 0; }
+VH_HD constexpr int corner_dy(int c) { return (c == 1 || c == 2 || c == 5 || c == 6) ? 1 : 0; }
+VH_HD constexpr int corner_dz(int c) { return c >= 4 ? 1 : 0; }
+VH_HD constexpr int edge_a(int e) { return e < 8 ? e : e - 8; }
+VH_HD constexpr int edge_b(int e) { return e < 8 ? ((e & 3) == 3 ? e - 3 : e + 1) : e - 4; }
 
 // |v| < 1e-5 evaluated as the reference does it (float |v| promoted to double against the
 // double literal, src/tsdf.cu:1645-1650): equivalent to |v| <= 1e-5f in float.
