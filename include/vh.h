@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 4
+#define VH_ABI_VERSION 5
 
 typedef enum vh_status {
   VH_OK = 0,
@@ -118,6 +118,10 @@ typedef struct vh_stats {
   int64_t profiled_launches;  /* profiling mode: integration launches timed (one per batch) */
   int64_t blocks_integrated;  /* sum over integration launches of the blocks read and written
                                  (a block touched by several frames of a batch counts once) */
+  int64_t host_blocks;        /* blocks in the host store (vh_stream_out) */
+  int64_t streamed_out_total; /* blocks moved to the host store since create/reset */
+  int64_t streamed_in_total;  /* blocks moved back into the pool since create/reset */
+  int64_t stream_merged_total;/* of those, blocks merged into a resident block of the same key */
 } vh_stats;
 
 typedef enum vh_prior_mode {
@@ -216,6 +220,37 @@ int vh_set_profiling(vh_volume* vol, int32_t on);
 int vh_get_stats(vh_volume* vol, vh_stats* out);
 /* The volume's HIP stream (hipStream_t), for callers that order their own work on it. */
 void* vh_stream(vh_volume* vol);
+
+/* ------------------------------------------------------------------ out-of-core streaming */
+/* streamOutGPU2CPU (src/tsdf.cu:469-596): every resident block whose centre (the mean of its
+ * 8^3 lattice points) lies farther than radius metres from center moves from the GPU pool
+ * into the volume's host store, its voxel words (and colours) byte for byte. The pool is then
+ * compacted (blocks above the new count take the vacated indices below it) and the hash
+ * table rebuilt. *n_blocks receives the blocks moved. Waits for queued work first; meshes,
+ * exports and queries see resident blocks only. */
+int vh_stream_out(vh_volume* vol, const float center[3], float radius, int64_t* n_blocks);
+/* streamInCPU2GPU (src/tsdf.cu:277-457): every host-store block whose centre lies within
+ * radius metres of center moves back into the pool (radius = INFINITY: all of them). A block
+ * whose key was allocated again while it was out (a frame touched its region) is merged into
+ * the resident block voxel by voxel: weights add (capped by weight_cap), the TSDF value and
+ * each colour channel take the weighted mean; a side with weight 0 leaves the other as it
+ * is. VH_ERR_POOL_FULL, with nothing moved, when the pool cannot hold all selected blocks
+ * without merging. A frame only changes blocks that are resident: to reproduce a volume
+ * that never streamed, stream in every block within reach of the cameras of the frames to
+ * come: (max_depth + trunc) times the longest pixel ray per unit depth (the image corner's,
+ * sqrt(1 + ((W-1-cx)/fx)^2 + ((H-1-cy)/fy)^2) at the farther corner), plus one block
+ * diagonal. */
+int vh_stream_in(vh_volume* vol, const float center[3], float radius, int64_t* n_blocks);
+/* Checkpoint (no reference counterpart: the reference's volume lives only in its host
+ * chunks): the configuration, the frame counters and every block, resident and in the host
+ * store, voxel words and colours byte for byte. */
+int vh_save_volume(vh_volume* vol, const char* path);
+/* Creates a volume from a checkpoint on device `device` (-1: the saved one). All blocks go
+ * into the pool if they fit, otherwise they stay in the host store (vh_stream_in brings them
+ * in by region). Integrating after a load gives what integrating without the save would. */
+int vh_load_volume(const char* path, int32_t device, vh_volume** out);
+/* The volume's configuration (as created or loaded). No reference counterpart. */
+int vh_get_config(vh_volume* vol, vh_config* out);
 
 /* ------------------------------------------------------------------ mesh */
 /* GpuTsdfGenerator::SavePLY -> tsdf2mesh (src/tsdf.cu:1696-1708,1760-1888), but computed

@@ -39,7 +39,7 @@ def test_every_declared_function_is_exported(vh):
 
 def test_abi_version_and_defaults(vh):
     lib = vh.load()
-    assert lib.vh_abi_version() == vh.ABI_VERSION == 4
+    assert lib.vh_abi_version() == vh.ABI_VERSION == 5
     c = vh.make_config()
     assert (c.width, c.height, c.log2_buckets, c.storage, c.color) == (640, 480, 18, 0, 0)
     assert C.sizeof(vh.Config) == 4 * (2 + 8 + 5 + 3 + 3 + 1 + 1 + 6)
@@ -139,3 +139,21 @@ def test_struct_layouts_match_the_header(vh, tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_load_volume_rejects_files_that_are_not_checkpoints(vh, tmp_path):
+    """Checked before any device work: a missing file and a file without the checkpoint
+    header are VH_ERR_IO, and no volume is returned."""
+    lib = vh.load()
+    h = C.c_void_p()
+    assert lib.vh_load_volume(str(tmp_path / "missing.vhv").encode(), -1, C.byref(h)) == vh.VH_ERR_IO
+    bad = tmp_path / "bad.vhv"
+    bad.write_bytes(b"not a checkpoint" * 64)
+    assert lib.vh_load_volume(str(bad).encode(), -1, C.byref(h)) == vh.VH_ERR_IO
+    assert "not a volume checkpoint" in lib.vh_last_error().decode()
+    assert not h.value
+    n = C.c_int64(0)
+    c = (C.c_float * 3)(0, 0, 0)
+    assert lib.vh_stream_out(None, c, 1.0, C.byref(n)) == vh.VH_ERR_INVALID_ARG
+    assert lib.vh_stream_in(None, c, 1.0, C.byref(n)) == vh.VH_ERR_INVALID_ARG
+    assert lib.vh_get_config(None, None) == vh.VH_ERR_INVALID_ARG

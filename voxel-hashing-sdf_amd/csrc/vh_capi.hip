@@ -18,6 +18,7 @@
 #include <new>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "vh.h"
@@ -147,6 +148,18 @@ struct vh_volume {
   float* ix_verts = nullptr;       // ix_vcap x 3
   uint8_t* ix_vrgb = nullptr;      // ix_vcap x 3 (colour volumes)
   int64_t ix_fcap = 0, ix_vcap = 0;
+  // out-of-core streaming: the host store holds blocks moved off the GPU (voxel words and
+  // colours byte for byte, slot order), device staging holds up to st_cap blocks
+  std::unordered_map<unsigned long long, uint32_t> store_index;  // key -> store slot
+  std::vector<unsigned long long> store_keys;
+  std::vector<uint32_t> store_words;   // slot x wpb
+  std::vector<uint32_t> store_colors;  // slot x 512 (colour volumes)
+  uint32_t* st_stage = nullptr;        // st_cap x wpb
+  uint32_t* st_col = nullptr;          // st_cap x 512 (colour volumes)
+  uint32_t* st_idx = nullptr;          // 2 x st_cap
+  unsigned long long* st_keys = nullptr;
+  uint32_t st_cap = 0;
+  int64_t streamed_out = 0, streamed_in = 0, stream_merged = 0;
   int64_t mc_cap = 0;
   uint32_t* mc_counts = nullptr;   // pool_blocks
   uint32_t* mc_nbrs = nullptr;     // pool_blocks x 8
@@ -415,6 +428,11 @@ int sync_and_check(vh_volume* v) {
 
 int reset_tables(vh_volume* v) {
   v->ring_next = 0;  // the device cursor of the ring restarts with the control block
+  v->store_index.clear();
+  v->store_keys.clear();
+  v->store_words.clear();
+  v->store_colors.clear();
+  v->streamed_out = v->streamed_in = v->stream_merged = 0;
   VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
   VH_HIP(hipMemsetAsync(v->marks, 0, 2 * v->n_slots * 4, v->stream));
   FrameCtl c{};
@@ -483,7 +501,8 @@ void free_volume(vh_volume* v) {
   void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total, v->mc_nbrs,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
-                  v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba};
+                  v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba,
+                  v->st_stage, v->st_col, v->st_idx, v->st_keys};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -613,6 +632,186 @@ int index_write(vh_volume* v, McArgs& a, int32_t* d_faces, int64_t n_faces, floa
   VH_HIP(launch_mesh_index_write(a, v->cfg.storage, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
   return VH_OK;
+}
+
+// ---------------------------------------------------------------- streaming internals
+constexpr uint32_t kStreamChunk = 65536;  // blocks per staging round (256 MB of FP32 words)
+
+uint32_t words_per_block(const vh_volume* v) { return (uint32_t)(voxel_bytes(v->cfg.storage) * kB3 / 4); }
+
+int ensure_stream_staging(vh_volume* v, uint32_t n) {
+  n = std::min(n, kStreamChunk);
+  if (n <= v->st_cap) return VH_OK;
+  void* old[] = {v->st_stage, v->st_col, v->st_idx, v->st_keys};
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  v->st_stage = v->st_col = v->st_idx = nullptr;
+  v->st_keys = nullptr;
+  v->st_cap = 0;
+  int rc;
+  if ((rc = dev_alloc(&v->st_stage, (size_t)n * words_per_block(v) * 4)) ||
+      (v->colors && (rc = dev_alloc(&v->st_col, (size_t)n * kB3 * 4))) || (rc = dev_alloc(&v->st_idx, (size_t)n * 2 * 4)) ||
+      (rc = dev_alloc(&v->st_keys, (size_t)n * 8)))
+    return rc;
+  v->st_cap = n;
+  return VH_OK;
+}
+
+StreamArgs stream_args(vh_volume* v, uint32_t birth) {
+  StreamArgs a{};
+  a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
+  a.block_keys = v->block_keys;
+  a.pool = static_cast<uint32_t*>(v->voxels);
+  a.colors = v->colors;
+  a.wpb = words_per_block(v);
+  a.stage = v->st_stage;
+  a.stage_col = v->st_col;
+  a.ctl = v->ctl;
+  a.pool_cap = (uint32_t)v->cfg.pool_blocks;
+  a.birth = birth;
+  a.cap = (float)v->cfg.weight_cap;
+  return a;
+}
+
+// Squared distance (metres) from the centre of block key k (the mean of its 8^3 lattice
+// points, voxel i at i * vox_size) to c.
+double block_dist2(unsigned long long k, double vox, const double c[3]) {
+  int b[3];
+  unpack_key(k, b[0], b[1], b[2]);
+  double d2 = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double x = ((double)b[i] * kB + 0.5 * (kB - 1)) * vox - c[i];
+    d2 += x * x;
+  }
+  return d2;
+}
+
+// Moves the resident blocks that `out` selects (by pool index, ascending) into the host
+// store, compacts the pool and rebuilds the hash table. Queued work must have finished.
+int stream_out_blocks(vh_volume* v, const std::vector<unsigned long long>& bk, const std::vector<uint32_t>& out,
+                      uint32_t frame_id) {
+  const uint32_t N = (uint32_t)bk.size(), n_out = (uint32_t)out.size(), K = N - n_out;
+  const uint32_t wpb = words_per_block(v);
+  int rc = ensure_stream_staging(v, std::max(n_out, 1u));
+  if (rc) return rc;
+  StreamArgs a = stream_args(v, frame_id - 1u);
+  // 1. the outgoing blocks' words and colours into the host store
+  for (uint32_t c0 = 0; c0 < n_out; c0 += v->st_cap) {
+    const uint32_t n = std::min(v->st_cap, n_out - c0);
+    VH_HIP(hipMemcpyAsync(v->st_idx, out.data() + c0, (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
+    a.n = n;
+    a.src = v->st_idx;
+    VH_HIP(launch_stream_gather(a, v->stream));
+    const size_t s0 = v->store_keys.size();
+    v->store_words.resize((s0 + n) * wpb);
+    VH_HIP(hipMemcpyAsync(v->store_words.data() + s0 * wpb, v->st_stage, (size_t)n * wpb * 4, hipMemcpyDeviceToHost,
+                          v->stream));
+    if (v->colors) {
+      v->store_colors.resize((s0 + n) * kB3);
+      VH_HIP(hipMemcpyAsync(v->store_colors.data() + s0 * kB3, v->st_col, (size_t)n * kB3 * 4, hipMemcpyDeviceToHost,
+                            v->stream));
+    }
+    VH_HIP(hipStreamSynchronize(v->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+      const unsigned long long key = bk[out[c0 + i]];
+      v->store_index[key] = (uint32_t)v->store_keys.size();
+      v->store_keys.push_back(key);
+    }
+  }
+  // 2. kept blocks above K fill the vacated indices below K, both in ascending order
+  std::vector<uint32_t> holes, srcs;
+  for (uint32_t b : out)
+    if (b < K) holes.push_back(b);
+  {
+    size_t j = 0;
+    for (uint32_t b = K; b < N; ++b) {
+      while (j < out.size() && out[j] < b) ++j;
+      if (j < out.size() && out[j] == b) continue;
+      srcs.push_back(b);
+    }
+  }
+  if (holes.size() != srcs.size()) return fail(VH_ERR_DEVICE, "stream-out compaction mismatch");
+  for (size_t c0 = 0; c0 < holes.size(); c0 += v->st_cap) {
+    const uint32_t n = (uint32_t)std::min<size_t>(v->st_cap, holes.size() - c0);
+    VH_HIP(hipMemcpyAsync(v->st_idx, srcs.data() + c0, (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
+    VH_HIP(hipMemcpyAsync(v->st_idx + v->st_cap, holes.data() + c0, (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
+    a.n = n;
+    a.src = v->st_idx;
+    a.dst = v->st_idx + v->st_cap;
+    VH_HIP(launch_stream_move(a, v->stream));
+    VH_HIP(hipStreamSynchronize(v->stream));  // the index buffers are reused by the next round
+  }
+  // 3. the hash table from the K remaining keys; the pool counter
+  VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
+  a.n = K;
+  VH_HIP(launch_stream_rehash(a, v->stream));
+  VH_HIP(hipMemcpyAsync(&v->ctl->pool_count, &K, sizeof K, hipMemcpyHostToDevice, v->stream));
+  v->streamed_out += n_out;
+  return sync_and_check(v);
+}
+
+// Moves the host-store blocks at `slots` into the pool (inserting new keys, merging keys that
+// are resident again) and drops them from the store. Queued work must have finished.
+int stream_in_blocks(vh_volume* v, const std::vector<uint32_t>& slots, uint32_t frame_id, uint32_t pool_count,
+                     int64_t* moved) {
+  const uint32_t n_in = (uint32_t)slots.size(), wpb = words_per_block(v);
+  *moved = 0;
+  if (!n_in) return VH_OK;
+  if ((uint64_t)pool_count + n_in > (uint64_t)v->cfg.pool_blocks)
+    return fail(VH_ERR_POOL_FULL, "the pool cannot take the blocks to stream in (nothing moved)");
+  int rc = ensure_stream_staging(v, n_in);
+  if (rc) return rc;
+  StreamArgs a = stream_args(v, frame_id - 1u);
+  std::vector<uint8_t> done(v->store_keys.size(), 0);
+  std::vector<unsigned long long> keys;
+  std::vector<uint32_t> words, cols, placed;
+  for (uint32_t c0 = 0; c0 < n_in; c0 += v->st_cap) {
+    const uint32_t n = std::min(v->st_cap, n_in - c0);
+    keys.resize(n);
+    words.resize((size_t)n * wpb);
+    if (v->colors) cols.resize((size_t)n * kB3);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t sl = slots[c0 + i];
+      keys[i] = v->store_keys[sl];
+      std::memcpy(&words[(size_t)i * wpb], &v->store_words[(size_t)sl * wpb], (size_t)wpb * 4);
+      if (v->colors) std::memcpy(&cols[(size_t)i * kB3], &v->store_colors[(size_t)sl * kB3], kB3 * 4);
+    }
+    VH_HIP(hipMemcpyAsync(v->st_keys, keys.data(), (size_t)n * 8, hipMemcpyHostToDevice, v->stream));
+    VH_HIP(hipMemcpyAsync(v->st_stage, words.data(), (size_t)n * wpb * 4, hipMemcpyHostToDevice, v->stream));
+    if (v->colors)
+      VH_HIP(hipMemcpyAsync(v->st_col, cols.data(), (size_t)n * kB3 * 4, hipMemcpyHostToDevice, v->stream));
+    a.n = n;
+    a.stage_keys = v->st_keys;
+    a.placed = v->st_idx;
+    VH_HIP(launch_stream_insert(a, v->cfg.storage, v->stream));
+    placed.resize(n);
+    VH_HIP(hipMemcpyAsync(placed.data(), v->st_idx, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+    VH_HIP(hipStreamSynchronize(v->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+      if (placed[i] == kNoBlock) continue;
+      done[slots[c0 + i]] = 1;
+      ++*moved;
+      if (placed[i] & kMergeBit) ++v->stream_merged;
+    }
+  }
+  // drop the moved blocks from the store (order of the rest kept)
+  size_t w = 0;
+  v->store_index.clear();
+  for (size_t sl = 0; sl < v->store_keys.size(); ++sl) {
+    if (done[sl]) continue;
+    if (w != sl) {
+      v->store_keys[w] = v->store_keys[sl];
+      std::memmove(&v->store_words[w * wpb], &v->store_words[sl * wpb], (size_t)wpb * 4);
+      if (v->colors) std::memmove(&v->store_colors[w * kB3], &v->store_colors[sl * kB3], kB3 * 4);
+    }
+    v->store_index[v->store_keys[w]] = (uint32_t)w;
+    ++w;
+  }
+  v->store_keys.resize(w);
+  v->store_words.resize(w * wpb);
+  if (v->colors) v->store_colors.resize(w * kB3);
+  v->streamed_in += *moved;
+  return sync_and_check(v);
 }
 
 int ensure_gn_scratch(vh_volume* v) {
@@ -987,6 +1186,10 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->new_total = s->blocks_allocated;
   s->storage = v->cfg.storage;
   s->flags = (int32_t)c.flags;
+  s->host_blocks = (int64_t)v->store_keys.size();
+  s->streamed_out_total = v->streamed_out;
+  s->streamed_in_total = v->streamed_in;
+  s->stream_merged_total = v->stream_merged;
   s->integrate_ms = v->integrate_ms;
   s->alloc_ms = v->alloc_ms;
   s->profiled_frames = v->profiled_frames;
@@ -1338,6 +1541,186 @@ int vh_pool_view(vh_volume* v, void** d_voxels, void** d_block_keys, int64_t* n_
     if ((rc = read_ctl(v, &c))) return rc;
     *n_blocks = std::min<int64_t>(c.pool_count, v->cfg.pool_blocks);
   }
+  return VH_OK;
+}
+
+// ------------------------------------------------------------------ out-of-core streaming
+namespace {
+bool finite3(const float c[3]) { return std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2]); }
+
+// The control block and the resident blocks' keys in pool order; queued work has finished.
+int resident_blocks(vh_volume* v, FrameCtl& c, std::vector<unsigned long long>& bk) {
+  int rc = read_ctl(v, &c);
+  if (rc) return rc;
+  const uint32_t N = std::min<uint32_t>(c.pool_count, (uint32_t)v->cfg.pool_blocks);
+  bk.resize(N);
+  if (N) VH_HIP(hipMemcpy(bk.data(), v->block_keys, (size_t)N * 8, hipMemcpyDeviceToHost));
+  return VH_OK;
+}
+
+struct CkptHeader {
+  char magic[8];
+  int32_t abi;
+  int32_t words_per_block;
+  vh_config cfg;
+  uint32_t frame_id;
+  uint32_t reserved;
+  uint64_t frames;
+  int64_t n_blocks;
+};
+constexpr char kCkptMagic[8] = {'V', 'H', 'V', 'O', 'L', 'U', 'M', 'E'};
+}  // namespace
+
+int vh_stream_out(vh_volume* v, const float center[3], float radius, int64_t* n_blocks) {
+  if (!v || !center || !n_blocks || !finite3(center) || !(radius >= 0.0f))
+    return fail(VH_ERR_INVALID_ARG, "bad argument");
+  *n_blocks = 0;
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc || (rc = sync_and_check(v))) return rc;
+  FrameCtl c{};
+  std::vector<unsigned long long> bk;
+  if ((rc = resident_blocks(v, c, bk))) return rc;
+  const double cc[3] = {center[0], center[1], center[2]}, r2 = (double)radius * (double)radius;
+  std::vector<uint32_t> out;
+  for (uint32_t b = 0; b < (uint32_t)bk.size(); ++b)
+    if (block_dist2(bk[b], v->cfg.vox_size, cc) > r2) out.push_back(b);
+  if (out.empty()) return VH_OK;
+  *n_blocks = (int64_t)out.size();
+  return stream_out_blocks(v, bk, out, c.frame_id);
+}
+
+int vh_stream_in(vh_volume* v, const float center[3], float radius, int64_t* n_blocks) {
+  if (!v || !center || !n_blocks || !finite3(center) || !(radius >= 0.0f))
+    return fail(VH_ERR_INVALID_ARG, "bad argument");
+  *n_blocks = 0;
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc || (rc = sync_and_check(v))) return rc;
+  const double cc[3] = {center[0], center[1], center[2]}, r2 = (double)radius * (double)radius;
+  std::vector<uint32_t> slots;
+  for (uint32_t sl = 0; sl < (uint32_t)v->store_keys.size(); ++sl)
+    if (block_dist2(v->store_keys[sl], v->cfg.vox_size, cc) <= r2) slots.push_back(sl);
+  FrameCtl c{};
+  if ((rc = read_ctl(v, &c))) return rc;
+  return stream_in_blocks(v, slots, c.frame_id, c.pool_count, n_blocks);
+}
+
+int vh_get_config(vh_volume* v, vh_config* out) {
+  if (!v || !out) return fail(VH_ERR_INVALID_ARG, "null argument");
+  *out = v->cfg;
+  return VH_OK;
+}
+
+int vh_save_volume(vh_volume* v, const char* path) {
+  if (!v || !path) return fail(VH_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc || (rc = sync_and_check(v))) return rc;
+  FrameCtl c{};
+  std::vector<unsigned long long> bk;
+  if ((rc = resident_blocks(v, c, bk))) return rc;
+  const uint32_t wpb = words_per_block(v);
+  const int64_t n_res = (int64_t)bk.size(), n_host = (int64_t)v->store_keys.size();
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.abi = VH_ABI_VERSION;
+  h.words_per_block = (int32_t)wpb;
+  h.cfg = v->cfg;
+  h.frame_id = c.frame_id;
+  h.frames = c.frames;
+  h.n_blocks = n_res + n_host;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(VH_ERR_IO, std::string("cannot open ") + path);
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  ok = ok && std::fwrite(bk.data(), 8, (size_t)n_res, f) == (size_t)n_res;
+  ok = ok && std::fwrite(v->store_keys.data(), 8, (size_t)n_host, f) == (size_t)n_host;
+  // resident voxel words (then colours) in rounds of kStreamChunk blocks, then the store's
+  std::vector<uint32_t> buf;
+  for (int pass = 0; pass < (v->colors ? 2 : 1) && ok; ++pass) {
+    const size_t per = pass == 0 ? wpb : (size_t)kB3;
+    const uint32_t* src = pass == 0 ? static_cast<const uint32_t*>(v->voxels) : v->colors;
+    for (int64_t b0 = 0; b0 < n_res && ok; b0 += kStreamChunk) {
+      const size_t n = (size_t)std::min<int64_t>(kStreamChunk, n_res - b0);
+      buf.resize(n * per);
+      if (hipMemcpy(buf.data(), src + (size_t)b0 * per, n * per * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        std::fclose(f);
+        return fail(VH_ERR_DEVICE, "device read failed");
+      }
+      ok = std::fwrite(buf.data(), 4, n * per, f) == n * per;
+    }
+    const std::vector<uint32_t>& hs = pass == 0 ? v->store_words : v->store_colors;
+    ok = ok && std::fwrite(hs.data(), 4, hs.size(), f) == hs.size();
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? VH_OK : fail(VH_ERR_IO, "write failed");
+}
+
+int vh_load_volume(const char* path, int32_t device, vh_volume** out) {
+  if (!path || !out) return fail(VH_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(VH_ERR_IO, std::string("cannot open ") + path);
+  CkptHeader h{};
+  if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, 8) != 0) {
+    std::fclose(f);
+    return fail(VH_ERR_IO, "not a volume checkpoint");
+  }
+  if (h.abi != VH_ABI_VERSION || h.n_blocks < 0) {
+    std::fclose(f);
+    return fail(VH_ERR_IO, "checkpoint of another library version");
+  }
+  vh_config cfg = h.cfg;
+  if (device >= 0) cfg.device = device;
+  vh_volume* v = nullptr;
+  int rc = vh_create(&cfg, &v);
+  if (rc) {
+    std::fclose(f);
+    return rc;
+  }
+  const uint32_t wpb = words_per_block(v);
+  if ((uint32_t)h.words_per_block != wpb) {
+    std::fclose(f);
+    vh_destroy(v);
+    return fail(VH_ERR_IO, "checkpoint voxel format does not match its configuration");
+  }
+  const size_t n = (size_t)h.n_blocks;
+  v->store_keys.resize(n);
+  v->store_words.resize(n * wpb);
+  if (v->colors) v->store_colors.resize(n * kB3);
+  bool ok = std::fread(v->store_keys.data(), 8, n, f) == n &&
+            std::fread(v->store_words.data(), 4, n * wpb, f) == n * wpb &&
+            (!v->colors || std::fread(v->store_colors.data(), 4, n * kB3, f) == n * kB3);
+  std::fclose(f);
+  if (!ok) {
+    vh_destroy(v);
+    return fail(VH_ERR_IO, "checkpoint truncated");
+  }
+  for (size_t i = 0; i < n; ++i) v->store_index[v->store_keys[i]] = (uint32_t)i;
+  if (v->store_index.size() != n) {
+    vh_destroy(v);
+    return fail(VH_ERR_IO, "checkpoint holds a block twice");
+  }
+  // the frame counters continue from the checkpoint; every block goes to the pool if it fits
+  FrameCtl c{};
+  if ((rc = read_ctl(v, &c)) == VH_OK) {
+    c.frame_id = h.frame_id ? h.frame_id : 1u;
+    c.frames = h.frames;
+    if (hipMemcpy(v->ctl, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess) rc = fail(VH_ERR_DEVICE, "device write failed");
+  }
+  if (rc == VH_OK && n <= (size_t)cfg.pool_blocks) {
+    std::vector<uint32_t> slots(n);
+    for (size_t i = 0; i < n; ++i) slots[i] = (uint32_t)i;
+    int64_t moved = 0;
+    std::lock_guard<std::mutex> lk(v->mu);
+    rc = stream_in_blocks(v, slots, c.frame_id, 0u, &moved);
+  }
+  if (rc) {
+    std::string msg = vh_last_error();
+    vh_destroy(v);
+    return fail(rc, msg);
+  }
+  *out = v;
   return VH_OK;
 }
 

@@ -205,6 +205,29 @@ struct VoxelIO<VH_STORAGE_F16R> {
   static __device__ __forceinline__ Word zero() { return 0u; }
 };
 
+// Out-of-core streaming (launch_stream_*): blocks move between the pool and a staging
+// buffer of n blocks (wpb 32-bit words of voxels each, plus 512 colour words).
+struct StreamArgs {
+  HashView hash;
+  unsigned long long* block_keys;
+  uint32_t* pool;              // voxel pool as 32-bit words (wpb per block)
+  uint32_t* colors;            // colour pool (null: no colour)
+  uint32_t wpb;                // 32-bit words per block: 1024 (FP32) or 512 (16-bit formats)
+  uint32_t n;                  // blocks of this launch
+  const uint32_t* src;         // gather: pool blocks to stage; move: pool blocks to move
+  const uint32_t* dst;         // move: their new pool blocks
+  uint32_t* stage;             // n x wpb words
+  uint32_t* stage_col;         // n x 512 colour words (colour volumes)
+  const unsigned long long* stage_keys;  // insert: the staged blocks' keys
+  uint32_t* placed;            // insert: pool block per staged block (kMergeBit: merged into
+                               // a resident block), kNoBlock where the hash is full
+  FrameCtl* ctl;
+  uint32_t pool_cap;
+  uint32_t birth;              // birth frame written with rebuilt and inserted keys
+  float cap;                   // weight cap (0: none)
+};
+constexpr uint32_t kMergeBit = 0x80000000u;
+
 constexpr int kScanChunk = 1024;
 constexpr int kEdgeWords = 3 * kB3 / 32;  // used-edge words per block (indexed mesh)
 
@@ -279,6 +302,15 @@ hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
 hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st);
 // Second half (after the count on the same arguments): faces and vertices.
 hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st);
+// Streaming: stage[i] = pool[src[i]] (and colours).
+hipError_t launch_stream_gather(const StreamArgs& a, hipStream_t st);
+// pool[dst[i]] = pool[src[i]] with its key and colours (source and destination sets disjoint).
+hipError_t launch_stream_move(const StreamArgs& a, hipStream_t st);
+// Inserts block_keys[0 .. n) into an empty hash table, slot value = index | birth << 32.
+hipError_t launch_stream_rehash(const StreamArgs& a, hipStream_t st);
+// Inserts the staged blocks: new keys take pool blocks from ctl->pool_count and their staged
+// words; keys already resident are merged into the resident block (placed[] says which).
+hipError_t launch_stream_insert(const StreamArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,
                         uint8_t* valid, hipStream_t st);
 hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, int64_t n,

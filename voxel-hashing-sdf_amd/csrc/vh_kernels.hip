@@ -1171,6 +1171,130 @@ __global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
   }
 }
 
+// ============================================================================ streaming
+// Out-of-core streaming (the reference's streamOutGPU2CPU / streamInCPU2GPU, src/tsdf.cu:
+// 277-596, re-designed: whole 8^3 blocks move, not 5^3-block chunks, and the pool stays
+// dense). One wave per block in the copy kernels, 64 lanes striding over its words.
+
+__global__ __launch_bounds__(256) void k_st_gather(StreamArgs a) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < a.n; i += gridDim.x * 4u) {
+    const size_t b = a.src[i];
+    for (uint32_t k = lane; k < a.wpb; k += 64) a.stage[(size_t)i * a.wpb + k] = a.pool[b * a.wpb + k];
+    if (a.colors)
+      for (uint32_t k = lane; k < kB3; k += 64) a.stage_col[(size_t)i * kB3 + k] = a.colors[b * kB3 + k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_st_move(StreamArgs a) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < a.n; i += gridDim.x * 4u) {
+    const size_t s = a.src[i], d = a.dst[i];
+    for (uint32_t k = lane; k < a.wpb; k += 64) a.pool[d * a.wpb + k] = a.pool[s * a.wpb + k];
+    if (a.colors)
+      for (uint32_t k = lane; k < kB3; k += 64) a.colors[d * kB3 + k] = a.colors[s * kB3 + k];
+    if (lane == 0) a.block_keys[d] = a.block_keys[s];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_st_rehash(StreamArgs a) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  if (b >= a.n) return;
+  bool inserted = false;
+  const int64_t slot = find_or_insert(a.hash, a.block_keys[b], inserted);
+  if (slot < 0) raise_flag(a.ctl, kFlagHashFull);
+  else a.hash.vals[slot] = (unsigned long long)b | ((unsigned long long)a.birth << 32);
+}
+
+// One lane per staged block: find or insert its key; new keys take pool blocks (one counter
+// atomic per wave), resident keys are marked for merging.
+__global__ __launch_bounds__(256) void k_st_insert(StreamArgs a) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool act = i < a.n;
+  bool inserted = false;
+  int64_t slot = -1;
+  if (act) {
+    slot = find_or_insert(a.hash, a.stage_keys[i], inserted);
+    if (slot < 0) raise_flag(a.ctl, kFlagHashFull);
+  }
+  const uint64_t im = __ballot(inserted);
+  const int leader = im ? first_lane(im) : 0;
+  uint32_t pbase = 0;
+  if (im && lane == leader) pbase = atomicAdd(&a.ctl->pool_count, (uint32_t)__popcll(im));
+  if (im) pbase = read_lane_u32(pbase, leader);
+  if (!act) return;
+  uint32_t placed = kNoBlock;
+  if (inserted) {
+    const uint32_t idx = pbase + lanes_below(im);
+    uint32_t stored = kNoBlock;
+    if (idx < a.pool_cap) {
+      a.block_keys[idx] = a.stage_keys[i];
+      stored = placed = idx;
+    } else {
+      raise_flag(a.ctl, kFlagPoolFull);
+    }
+    a.hash.vals[slot] = (unsigned long long)stored | ((unsigned long long)a.birth << 32);
+  } else if (slot >= 0) {
+    const uint32_t v = (uint32_t)a.hash.vals[slot];
+    if (v < a.pool_cap) placed = v | kMergeBit;
+  }
+  a.placed[i] = placed;
+}
+
+// Per-voxel merge of a staged observation history into a resident one: weights add (capped),
+// the TSDF value is the weighted mean (s_r w_r + s_s w_s) / (w_r + w_s), colours per channel
+// (c_r w_r + c_s w_s) / (w_r + w_s) truncated to 8 bits; a side with weight 0 contributes
+// nothing (the other side is kept bit for bit).
+template <int kStorage>
+__global__ __launch_bounds__(256) void k_st_place(StreamArgs a) {
+  using IO = VoxelIO<kStorage>;
+  using Word = typename IO::Word;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < a.n; i += gridDim.x * 4u) {
+    const uint32_t p = a.placed[i];
+    if (p == kNoBlock) continue;
+    const size_t d = p & ~kMergeBit;
+    if (!(p & kMergeBit)) {
+      for (uint32_t k = lane; k < a.wpb; k += 64) a.pool[d * a.wpb + k] = a.stage[(size_t)i * a.wpb + k];
+      if (a.colors)
+        for (uint32_t k = lane; k < kB3; k += 64) a.colors[d * kB3 + k] = a.stage_col[(size_t)i * kB3 + k];
+      continue;
+    }
+    Word* rw = reinterpret_cast<Word*>(a.pool) + d * kB3;
+    const Word* sw = reinterpret_cast<const Word*>(a.stage) + (size_t)i * kB3;
+    for (uint32_t k = lane; k < kB3; k += 64) {
+      float sr, wr, ss, ws;
+      const Word r = rw[k], q = sw[k];
+      IO::unpack(r, sr, wr);
+      IO::unpack(q, ss, ws);
+      if (!(ws > 0.0f)) continue;
+      uint32_t* col = a.colors ? a.colors + d * kB3 + k : nullptr;
+      const uint32_t cs = a.colors ? a.stage_col[(size_t)i * kB3 + k] : 0u;
+      if (!(wr > 0.0f)) {
+        rw[k] = q;
+        if (col) *col = cs;
+        continue;
+      }
+      const float wsum = wr + ws;
+      const float pr = sr * wr, ps = ss * ws;
+      const float sv = div_rn(pr + ps, wsum);
+      const float wv = (a.cap > 0.0f && wsum > a.cap) ? a.cap : wsum;
+      rw[k] = IO::pack(sv, wv);
+      if (col) {
+        const uint32_t cr = *col;
+        uint32_t out = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float xr = (float)((cr >> (8 * c)) & 255u) * wr, xs = (float)((cs >> (8 * c)) & 255u) * ws;
+          out |= (uint32_t)(uint8_t)div_rn(xr + xs, wsum) << (8 * c);
+        }
+        *col = out;
+      }
+    }
+  }
+}
+
 // ============================================================================ queries
 // Trilinear sample of the normalised TSDF at world point q; false unless all 8 lattice
 // corners are allocated and observed. The lattice position q/vox and its cell fraction are
@@ -1637,6 +1761,32 @@ hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st)
   with_storage(storage, [&](auto S) {
     if (a.colors && a.vrgb) hipLaunchKernelGGL((k_ix_write<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
     else hipLaunchKernelGGL((k_ix_write<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
+  });
+  return hipGetLastError();
+}
+
+static uint32_t stream_grid(uint32_t n) { return min((n + 3u) / 4u, 16384u); }
+
+hipError_t launch_stream_gather(const StreamArgs& a, hipStream_t st) {
+  if (a.n) hipLaunchKernelGGL(k_st_gather, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_move(const StreamArgs& a, hipStream_t st) {
+  if (a.n) hipLaunchKernelGGL(k_st_move, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_rehash(const StreamArgs& a, hipStream_t st) {
+  if (a.n) hipLaunchKernelGGL(k_st_rehash, dim3((a.n + 255u) / 256u), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_insert(const StreamArgs& a, int storage, hipStream_t st) {
+  if (!a.n) return hipSuccess;
+  hipLaunchKernelGGL(k_st_insert, dim3((a.n + 255u) / 256u), dim3(256), 0, st, a);
+  with_storage(storage, [&](auto S) {
+    hipLaunchKernelGGL(k_st_place<decltype(S)::value>, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
   });
   return hipGetLastError();
 }

@@ -41,7 +41,7 @@ DEPTH_U16 = 0
 DEPTH_F32 = 1
 PRIOR_ABSOLUTE = 0
 PRIOR_INCREMENT = 1
-ABI_VERSION = 4
+ABI_VERSION = 5
 # per-evaluation solver stats: n_valid, rms, |delta xi|, lambda, accepted, accepted pose (12)
 GN_STATS = 17
 
@@ -71,7 +71,9 @@ class Stats(C.Structure):
                 ("slots_used", C.c_int64), ("touched_last", C.c_int64), ("touched_total", C.c_int64),
                 ("new_total", C.c_int64), ("storage", C.c_int32), ("flags", C.c_int32),
                 ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64),
-                ("track_ms", C.c_double), ("profiled_launches", C.c_int64), ("blocks_integrated", C.c_int64)]
+                ("track_ms", C.c_double), ("profiled_launches", C.c_int64), ("blocks_integrated", C.c_int64),
+                ("host_blocks", C.c_int64), ("streamed_out_total", C.c_int64), ("streamed_in_total", C.c_int64),
+                ("stream_merged_total", C.c_int64)]
 
 
 class Mesh(C.Structure):
@@ -139,6 +141,11 @@ _PROTOS = [
     ("vh_extract_mesh_indexed", C.c_int, [C.c_void_p, C.POINTER(IndexedMesh)]),
     ("vh_indexed_mesh_free", C.c_int, [C.POINTER(IndexedMesh)]),
     ("vh_save_ply", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("vh_stream_out", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int64)]),
+    ("vh_stream_in", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int64)]),
+    ("vh_save_volume", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("vh_load_volume", C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("vh_get_config", C.c_int, [C.c_void_p, C.POINTER(Config)]),
     ("vh_export_blocks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.POINTER(C.c_int64)]),
     ("vh_export_block_colors", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
@@ -234,13 +241,44 @@ def config_from_workload(w: dict, **over) -> Config:
 class Volume:
     """One TSDF volume on one GPU (the engine behind ark::GpuTsdfGenerator)."""
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, _handle=None):
         lib = load()
         self.cfg = cfg
-        h = C.c_void_p()
-        _check(lib.vh_create(C.byref(cfg), C.byref(h)))
+        h = _handle
+        if h is None:
+            h = C.c_void_p()
+            _check(lib.vh_create(C.byref(cfg), C.byref(h)))
         self._h = h
         self._pending = []  # host frames queued by integrate_async, referenced until sync()
+
+    @classmethod
+    def load(cls, path: str, device: int = -1) -> "Volume":
+        """A volume from a checkpoint written by save() (vh_load_volume)."""
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.vh_load_volume(path.encode(), device, C.byref(h)))
+        cfg = Config()
+        _check(lib.vh_get_config(h, C.byref(cfg)))
+        return cls(cfg, _handle=h)
+
+    def save(self, path: str):
+        """Checkpoint: configuration, frame counters, every block (vh_save_volume)."""
+        _check(_lib.vh_save_volume(self._h, path.encode()))
+
+    # -- out-of-core streaming
+    def stream_out(self, center, radius: float) -> int:
+        """Moves resident blocks farther than radius (metres) from center to the host store."""
+        c = (C.c_float * 3)(*[float(x) for x in center])
+        n = C.c_int64(0)
+        _check(_lib.vh_stream_out(self._h, c, float(radius), C.byref(n)))
+        return n.value
+
+    def stream_in(self, center=(0.0, 0.0, 0.0), radius: float = float("inf")) -> int:
+        """Moves host-store blocks within radius of center back into the pool."""
+        c = (C.c_float * 3)(*[float(x) for x in center])
+        n = C.c_int64(0)
+        _check(_lib.vh_stream_in(self._h, c, float(radius), C.byref(n)))
+        return n.value
 
     # -- lifetime
     def close(self):
