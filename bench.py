@@ -370,11 +370,20 @@ def main():
         run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
         pst = vol.stats()
         vol.set_profiling(False)
+        stream = None
+        if name == "f32":  # the whole volume to the host store and back (after the profiled pass)
+            t2 = time.perf_counter()
+            n_out = vol.stream_out((0.0, 0.0, 0.0), 0.0)
+            t_out = time.perf_counter() - t2
+            t2 = time.perf_counter()
+            n_in = vol.stream_in()
+            t_in = time.perf_counter() - t2
+            stream = dict(blocks=n_out, back=n_in, out_s=t_out, in_s=t_in)
         if name == "f16" and not args.no_gn:  # the volume now holds the whole sequence
             results["gn"] = gn_block(vh, wl, w, vol, nf)
         results[name] = dict(fps=args.steps * nf / dt, ms_per_step=1e3 * dt / args.steps, fps_integrate=nf / dt_int,
                              blocks=st["blocks_allocated"], tris=ntri, mc_ms=1e3 * mc_s,
-                             mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst, indexed=ix,
+                             mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst, indexed=ix, stream=stream,
                              vox_bytes=(8 if over.get("storage", 0) == 0 else 4) + (4 if col else 0))
         log(f"bench: {name}: {results[name]['fps']:.1f} frames/s (with MC), {results[name]['fps_integrate']:.1f} "
             f"integrate-only, {st['blocks_allocated']} blocks, {ntri} triangles, MC {1e3 * mc_s:.2f} ms")
@@ -435,6 +444,16 @@ def main():
                                "note": "vh_extract_mesh_indexed_device on the final volume: used-edge bits per "
                                        "block (mark pass), vertex ids from per-block popcount offsets, faces and "
                                        "vertices in one write pass; one vertex per crossed lattice edge"}
+    if r.get("stream"):
+        sm = r["stream"]
+        gb = sm["blocks"] * 512 * r["vox_bytes"] / 1e9
+        out["streaming"] = {"blocks": sm["blocks"], "gbytes": round(gb, 3),
+                            "out_ms": round(1e3 * sm["out_s"], 1), "in_ms": round(1e3 * sm["in_s"], 1),
+                            "out_gbs": round(gb / sm["out_s"], 2), "in_gbs": round(gb / sm["in_s"], 2),
+                            "complete": sm["back"] == sm["blocks"],
+                            "note": "vh_stream_out of every block of the final volume to the host store "
+                                    "(radius 0), then vh_stream_in of all of them: voxel words over PCIe, "
+                                    "pool compaction, hash rebuild"}
     labels = {"f16r": ("fp16", "binary16 tsdf + uint8 weight (cap 255) + int8 rounding residual, 4 B/voxel"),
               "f16": ("fp16_plain", "binary16 tsdf + uint16 weight (cap 255), 4 B/voxel")}
     if "pcie" in results:

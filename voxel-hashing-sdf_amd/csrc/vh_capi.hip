@@ -149,15 +149,18 @@ struct vh_volume {
   uint8_t* ix_vrgb = nullptr;      // ix_vcap x 3 (colour volumes)
   int64_t ix_fcap = 0, ix_vcap = 0;
   // out-of-core streaming: the host store holds blocks moved off the GPU (voxel words and
-  // colours byte for byte, slot order), device staging holds up to st_cap blocks
-  std::unordered_map<unsigned long long, uint32_t> store_index;  // key -> store slot
-  std::vector<unsigned long long> store_keys;
-  std::vector<uint32_t> store_words;   // slot x wpb
-  std::vector<uint32_t> store_colors;  // slot x 512 (colour volumes)
-  uint32_t* st_stage = nullptr;        // st_cap x wpb
-  uint32_t* st_col = nullptr;          // st_cap x 512 (colour volumes)
-  uint32_t* st_idx = nullptr;          // 2 x st_cap
-  unsigned long long* st_keys = nullptr;
+  // colours byte for byte) in pinned slabs of kSlabBlocks blocks that the kernels address
+  // directly; slot s is block s % kSlabBlocks of slab s / kSlabBlocks
+  std::vector<uint32_t*> slab_w, slab_c;        // host addresses (colour slabs: colour volumes)
+  uint32_t** d_slab_w = nullptr;                // device addresses of the slabs
+  uint32_t** d_slab_c = nullptr;
+  size_t d_slab_cap = 0;                        // entries of the device address tables
+  bool slab_table_dirty = false;
+  std::vector<unsigned long long> slot_key;     // key per slot, kEmptyKey when free
+  std::vector<uint32_t> free_slots;
+  std::unordered_map<unsigned long long, uint32_t> store_index;  // key -> slot
+  uint32_t* st_idx = nullptr;                   // 2 x st_cap index words
+  unsigned long long* st_keys = nullptr;        // st_cap keys
   uint32_t st_cap = 0;
   int64_t streamed_out = 0, streamed_in = 0, stream_merged = 0;
   int64_t mc_cap = 0;
@@ -428,10 +431,9 @@ int sync_and_check(vh_volume* v) {
 
 int reset_tables(vh_volume* v) {
   v->ring_next = 0;  // the device cursor of the ring restarts with the control block
-  v->store_index.clear();
-  v->store_keys.clear();
-  v->store_words.clear();
-  v->store_colors.clear();
+  v->store_index.clear();  // the slabs stay allocated for reuse
+  v->slot_key.clear();
+  v->free_slots.clear();
   v->streamed_out = v->streamed_in = v->stream_merged = 0;
   VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
   VH_HIP(hipMemsetAsync(v->marks, 0, 2 * v->n_slots * 4, v->stream));
@@ -502,9 +504,11 @@ void free_volume(vh_volume* v) {
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total, v->mc_nbrs,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
                   v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba,
-                  v->st_stage, v->st_col, v->st_idx, v->st_keys};
+                  v->st_idx, v->st_keys, v->d_slab_w, v->d_slab_c};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (uint32_t* p : v->slab_w) (void)hipHostFree(p);
+  for (uint32_t* p : v->slab_c) (void)hipHostFree(p);
   if (v->stream) (void)hipStreamDestroy(v->stream);
   delete v;
 }
@@ -635,25 +639,82 @@ int index_write(vh_volume* v, McArgs& a, int32_t* d_faces, int64_t n_faces, floa
 }
 
 // ---------------------------------------------------------------- streaming internals
-constexpr uint32_t kStreamChunk = 65536;  // blocks per staging round (256 MB of FP32 words)
-
 uint32_t words_per_block(const vh_volume* v) { return (uint32_t)(voxel_bytes(v->cfg.storage) * kB3 / 4); }
 
-int ensure_stream_staging(vh_volume* v, uint32_t n) {
-  n = std::min(n, kStreamChunk);
+int ensure_stream_index(vh_volume* v, size_t n) {
   if (n <= v->st_cap) return VH_OK;
-  void* old[] = {v->st_stage, v->st_col, v->st_idx, v->st_keys};
-  for (void* p : old)
-    if (p) (void)hipFree(p);
-  v->st_stage = v->st_col = v->st_idx = nullptr;
+  if (n > 0xffffffffu / 2) return fail(VH_ERR_INVALID_ARG, "too many blocks to stream at once");
+  if (v->st_idx) (void)hipFree(v->st_idx);
+  if (v->st_keys) (void)hipFree(v->st_keys);
+  v->st_idx = nullptr;
   v->st_keys = nullptr;
   v->st_cap = 0;
   int rc;
-  if ((rc = dev_alloc(&v->st_stage, (size_t)n * words_per_block(v) * 4)) ||
-      (v->colors && (rc = dev_alloc(&v->st_col, (size_t)n * kB3 * 4))) || (rc = dev_alloc(&v->st_idx, (size_t)n * 2 * 4)) ||
-      (rc = dev_alloc(&v->st_keys, (size_t)n * 8)))
-    return rc;
-  v->st_cap = n;
+  if ((rc = dev_alloc(&v->st_idx, n * 2 * 4)) || (rc = dev_alloc(&v->st_keys, n * 8))) return rc;
+  v->st_cap = (uint32_t)n;
+  return VH_OK;
+}
+
+// A free host-store slot (new pinned slabs as needed).
+int store_alloc_slot(vh_volume* v, uint32_t* slot) {
+  if (!v->free_slots.empty()) {
+    *slot = v->free_slots.back();
+    v->free_slots.pop_back();
+    return VH_OK;
+  }
+  const size_t s = v->slot_key.size();
+  if (s >= ((size_t)1 << 31)) return fail(VH_ERR_OOM, "host store full");
+  if ((s >> kSlabShift) >= v->slab_w.size()) {
+    uint32_t *w = nullptr, *c = nullptr;
+    VH_HIP(hipHostMalloc((void**)&w, (size_t)kSlabBlocks * words_per_block(v) * 4, hipHostMallocMapped));
+    v->slab_w.push_back(w);
+    if (v->colors) {
+      VH_HIP(hipHostMalloc((void**)&c, (size_t)kSlabBlocks * kB3 * 4, hipHostMallocMapped));
+      v->slab_c.push_back(c);
+    }
+    v->slab_table_dirty = true;
+  }
+  v->slot_key.push_back(kEmptyKey);
+  *slot = (uint32_t)s;
+  return VH_OK;
+}
+
+void store_free_slot(vh_volume* v, uint32_t slot) {
+  v->store_index.erase(v->slot_key[slot]);
+  v->slot_key[slot] = kEmptyKey;
+  v->free_slots.push_back(slot);
+}
+
+uint32_t* store_words(vh_volume* v, uint32_t slot) {
+  return v->slab_w[slot >> kSlabShift] + (size_t)(slot & (kSlabBlocks - 1u)) * words_per_block(v);
+}
+uint32_t* store_cols(vh_volume* v, uint32_t slot) {
+  return v->slab_c[slot >> kSlabShift] + (size_t)(slot & (kSlabBlocks - 1u)) * kB3;
+}
+
+// The device address tables of the slabs, after slabs were added.
+int upload_slab_table(vh_volume* v) {
+  if (!v->slab_table_dirty) return VH_OK;
+  const size_t n = v->slab_w.size();
+  if (n > v->d_slab_cap) {
+    if (v->d_slab_w) (void)hipFree(v->d_slab_w);
+    if (v->d_slab_c) (void)hipFree(v->d_slab_c);
+    v->d_slab_w = v->d_slab_c = nullptr;
+    v->d_slab_cap = 0;
+    const size_t cap = std::max<size_t>(64, 2 * n);
+    int rc;
+    if ((rc = dev_alloc(&v->d_slab_w, cap * sizeof(uint32_t*))) || (rc = dev_alloc(&v->d_slab_c, cap * sizeof(uint32_t*))))
+      return rc;
+    v->d_slab_cap = cap;
+  }
+  std::vector<uint32_t*> dw(n), dc(n, nullptr);
+  for (size_t i = 0; i < n; ++i) {
+    VH_HIP(hipHostGetDevicePointer((void**)&dw[i], v->slab_w[i], 0));
+    if (v->colors) VH_HIP(hipHostGetDevicePointer((void**)&dc[i], v->slab_c[i], 0));
+  }
+  VH_HIP(hipMemcpy(v->d_slab_w, dw.data(), n * sizeof(uint32_t*), hipMemcpyHostToDevice));
+  VH_HIP(hipMemcpy(v->d_slab_c, dc.data(), n * sizeof(uint32_t*), hipMemcpyHostToDevice));
+  v->slab_table_dirty = false;
   return VH_OK;
 }
 
@@ -664,8 +725,8 @@ StreamArgs stream_args(vh_volume* v, uint32_t birth) {
   a.pool = static_cast<uint32_t*>(v->voxels);
   a.colors = v->colors;
   a.wpb = words_per_block(v);
-  a.stage = v->st_stage;
-  a.stage_col = v->st_col;
+  a.slab_words = v->d_slab_w;
+  a.slab_cols = v->d_slab_c;
   a.ctl = v->ctl;
   a.pool_cap = (uint32_t)v->cfg.pool_blocks;
   a.birth = birth;
@@ -691,34 +752,7 @@ double block_dist2(unsigned long long k, double vox, const double c[3]) {
 int stream_out_blocks(vh_volume* v, const std::vector<unsigned long long>& bk, const std::vector<uint32_t>& out,
                       uint32_t frame_id) {
   const uint32_t N = (uint32_t)bk.size(), n_out = (uint32_t)out.size(), K = N - n_out;
-  const uint32_t wpb = words_per_block(v);
-  int rc = ensure_stream_staging(v, std::max(n_out, 1u));
-  if (rc) return rc;
-  StreamArgs a = stream_args(v, frame_id - 1u);
-  // 1. the outgoing blocks' words and colours into the host store
-  for (uint32_t c0 = 0; c0 < n_out; c0 += v->st_cap) {
-    const uint32_t n = std::min(v->st_cap, n_out - c0);
-    VH_HIP(hipMemcpyAsync(v->st_idx, out.data() + c0, (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
-    a.n = n;
-    a.src = v->st_idx;
-    VH_HIP(launch_stream_gather(a, v->stream));
-    const size_t s0 = v->store_keys.size();
-    v->store_words.resize((s0 + n) * wpb);
-    VH_HIP(hipMemcpyAsync(v->store_words.data() + s0 * wpb, v->st_stage, (size_t)n * wpb * 4, hipMemcpyDeviceToHost,
-                          v->stream));
-    if (v->colors) {
-      v->store_colors.resize((s0 + n) * kB3);
-      VH_HIP(hipMemcpyAsync(v->store_colors.data() + s0 * kB3, v->st_col, (size_t)n * kB3 * 4, hipMemcpyDeviceToHost,
-                            v->stream));
-    }
-    VH_HIP(hipStreamSynchronize(v->stream));
-    for (uint32_t i = 0; i < n; ++i) {
-      const unsigned long long key = bk[out[c0 + i]];
-      v->store_index[key] = (uint32_t)v->store_keys.size();
-      v->store_keys.push_back(key);
-    }
-  }
-  // 2. kept blocks above K fill the vacated indices below K, both in ascending order
+  // kept blocks above K take the vacated indices below K, both in ascending order
   std::vector<uint32_t> holes, srcs;
   for (uint32_t b : out)
     if (b < K) holes.push_back(b);
@@ -731,15 +765,36 @@ int stream_out_blocks(vh_volume* v, const std::vector<unsigned long long>& bk, c
     }
   }
   if (holes.size() != srcs.size()) return fail(VH_ERR_DEVICE, "stream-out compaction mismatch");
-  for (size_t c0 = 0; c0 < holes.size(); c0 += v->st_cap) {
-    const uint32_t n = (uint32_t)std::min<size_t>(v->st_cap, holes.size() - c0);
-    VH_HIP(hipMemcpyAsync(v->st_idx, srcs.data() + c0, (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
-    VH_HIP(hipMemcpyAsync(v->st_idx + v->st_cap, holes.data() + c0, (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
-    a.n = n;
+  std::vector<uint32_t> slots(n_out);
+  uint32_t got = 0;  // slots allocated so far (returned to the free list on failure)
+  int rc = ensure_stream_index(v, std::max<size_t>(std::max<size_t>(n_out, holes.size()), 1));
+  for (; !rc && got < n_out; ++got)
+    if ((rc = store_alloc_slot(v, &slots[got]))) break;
+  if (rc || (rc = upload_slab_table(v))) {
+    for (uint32_t i = 0; i < got; ++i) v->free_slots.push_back(slots[i]);
+    return rc;
+  }
+  StreamArgs a = stream_args(v, frame_id - 1u);
+  // 1. the outgoing blocks' words and colours straight into their host slots
+  VH_HIP(hipMemcpyAsync(v->st_idx, out.data(), (size_t)n_out * 4, hipMemcpyHostToDevice, v->stream));
+  VH_HIP(hipMemcpyAsync(v->st_idx + v->st_cap, slots.data(), (size_t)n_out * 4, hipMemcpyHostToDevice, v->stream));
+  a.n = n_out;
+  a.src = v->st_idx;
+  a.slots = v->st_idx + v->st_cap;
+  VH_HIP(launch_stream_out(a, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  for (uint32_t i = 0; i < n_out; ++i) {
+    v->slot_key[slots[i]] = bk[out[i]];
+    v->store_index[bk[out[i]]] = slots[i];
+  }
+  // 2. compaction
+  if (!holes.empty()) {
+    VH_HIP(hipMemcpyAsync(v->st_idx, srcs.data(), holes.size() * 4, hipMemcpyHostToDevice, v->stream));
+    VH_HIP(hipMemcpyAsync(v->st_idx + v->st_cap, holes.data(), holes.size() * 4, hipMemcpyHostToDevice, v->stream));
+    a.n = (uint32_t)holes.size();
     a.src = v->st_idx;
     a.dst = v->st_idx + v->st_cap;
     VH_HIP(launch_stream_move(a, v->stream));
-    VH_HIP(hipStreamSynchronize(v->stream));  // the index buffers are reused by the next round
   }
   // 3. the hash table from the K remaining keys; the pool counter
   VH_HIP(hipMemsetAsync(v->keys, 0xff, v->n_slots * sizeof(unsigned long long), v->stream));
@@ -751,65 +806,35 @@ int stream_out_blocks(vh_volume* v, const std::vector<unsigned long long>& bk, c
 }
 
 // Moves the host-store blocks at `slots` into the pool (inserting new keys, merging keys that
-// are resident again) and drops them from the store. Queued work must have finished.
+// are resident again) and frees their slots. Queued work must have finished.
 int stream_in_blocks(vh_volume* v, const std::vector<uint32_t>& slots, uint32_t frame_id, uint32_t pool_count,
                      int64_t* moved) {
-  const uint32_t n_in = (uint32_t)slots.size(), wpb = words_per_block(v);
+  const uint32_t n = (uint32_t)slots.size();
   *moved = 0;
-  if (!n_in) return VH_OK;
-  if ((uint64_t)pool_count + n_in > (uint64_t)v->cfg.pool_blocks)
+  if (!n) return VH_OK;
+  if ((uint64_t)pool_count + n > (uint64_t)v->cfg.pool_blocks)
     return fail(VH_ERR_POOL_FULL, "the pool cannot take the blocks to stream in (nothing moved)");
-  int rc = ensure_stream_staging(v, n_in);
-  if (rc) return rc;
+  int rc = ensure_stream_index(v, n);
+  if (rc || (rc = upload_slab_table(v))) return rc;
+  std::vector<unsigned long long> keys(n);
+  for (uint32_t i = 0; i < n; ++i) keys[i] = v->slot_key[slots[i]];
   StreamArgs a = stream_args(v, frame_id - 1u);
-  std::vector<uint8_t> done(v->store_keys.size(), 0);
-  std::vector<unsigned long long> keys;
-  std::vector<uint32_t> words, cols, placed;
-  for (uint32_t c0 = 0; c0 < n_in; c0 += v->st_cap) {
-    const uint32_t n = std::min(v->st_cap, n_in - c0);
-    keys.resize(n);
-    words.resize((size_t)n * wpb);
-    if (v->colors) cols.resize((size_t)n * kB3);
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t sl = slots[c0 + i];
-      keys[i] = v->store_keys[sl];
-      std::memcpy(&words[(size_t)i * wpb], &v->store_words[(size_t)sl * wpb], (size_t)wpb * 4);
-      if (v->colors) std::memcpy(&cols[(size_t)i * kB3], &v->store_colors[(size_t)sl * kB3], kB3 * 4);
-    }
-    VH_HIP(hipMemcpyAsync(v->st_keys, keys.data(), (size_t)n * 8, hipMemcpyHostToDevice, v->stream));
-    VH_HIP(hipMemcpyAsync(v->st_stage, words.data(), (size_t)n * wpb * 4, hipMemcpyHostToDevice, v->stream));
-    if (v->colors)
-      VH_HIP(hipMemcpyAsync(v->st_col, cols.data(), (size_t)n * kB3 * 4, hipMemcpyHostToDevice, v->stream));
-    a.n = n;
-    a.stage_keys = v->st_keys;
-    a.placed = v->st_idx;
-    VH_HIP(launch_stream_insert(a, v->cfg.storage, v->stream));
-    placed.resize(n);
-    VH_HIP(hipMemcpyAsync(placed.data(), v->st_idx, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
-    VH_HIP(hipStreamSynchronize(v->stream));
-    for (uint32_t i = 0; i < n; ++i) {
-      if (placed[i] == kNoBlock) continue;
-      done[slots[c0 + i]] = 1;
-      ++*moved;
-      if (placed[i] & kMergeBit) ++v->stream_merged;
-    }
+  VH_HIP(hipMemcpyAsync(v->st_keys, keys.data(), (size_t)n * 8, hipMemcpyHostToDevice, v->stream));
+  VH_HIP(hipMemcpyAsync(v->st_idx, slots.data(), (size_t)n * 4, hipMemcpyHostToDevice, v->stream));
+  a.n = n;
+  a.keys = v->st_keys;
+  a.slots = v->st_idx;
+  a.placed = v->st_idx + v->st_cap;
+  VH_HIP(launch_stream_insert(a, v->cfg.storage, v->stream));
+  std::vector<uint32_t> placed(n);
+  VH_HIP(hipMemcpyAsync(placed.data(), a.placed, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  for (uint32_t i = 0; i < n; ++i) {
+    if (placed[i] == kNoBlock) continue;
+    store_free_slot(v, slots[i]);
+    ++*moved;
+    if (placed[i] & kMergeBit) ++v->stream_merged;
   }
-  // drop the moved blocks from the store (order of the rest kept)
-  size_t w = 0;
-  v->store_index.clear();
-  for (size_t sl = 0; sl < v->store_keys.size(); ++sl) {
-    if (done[sl]) continue;
-    if (w != sl) {
-      v->store_keys[w] = v->store_keys[sl];
-      std::memmove(&v->store_words[w * wpb], &v->store_words[sl * wpb], (size_t)wpb * 4);
-      if (v->colors) std::memmove(&v->store_colors[w * kB3], &v->store_colors[sl * kB3], kB3 * 4);
-    }
-    v->store_index[v->store_keys[w]] = (uint32_t)w;
-    ++w;
-  }
-  v->store_keys.resize(w);
-  v->store_words.resize(w * wpb);
-  if (v->colors) v->store_colors.resize(w * kB3);
   v->streamed_in += *moved;
   return sync_and_check(v);
 }
@@ -1186,7 +1211,7 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->new_total = s->blocks_allocated;
   s->storage = v->cfg.storage;
   s->flags = (int32_t)c.flags;
-  s->host_blocks = (int64_t)v->store_keys.size();
+  s->host_blocks = (int64_t)v->store_index.size();
   s->streamed_out_total = v->streamed_out;
   s->streamed_in_total = v->streamed_in;
   s->stream_merged_total = v->stream_merged;
@@ -1599,8 +1624,9 @@ int vh_stream_in(vh_volume* v, const float center[3], float radius, int64_t* n_b
   if (rc || (rc = sync_and_check(v))) return rc;
   const double cc[3] = {center[0], center[1], center[2]}, r2 = (double)radius * (double)radius;
   std::vector<uint32_t> slots;
-  for (uint32_t sl = 0; sl < (uint32_t)v->store_keys.size(); ++sl)
-    if (block_dist2(v->store_keys[sl], v->cfg.vox_size, cc) <= r2) slots.push_back(sl);
+  for (const auto& kv : v->store_index)
+    if (block_dist2(kv.first, v->cfg.vox_size, cc) <= r2) slots.push_back(kv.second);
+  std::sort(slots.begin(), slots.end());
   FrameCtl c{};
   if ((rc = read_ctl(v, &c))) return rc;
   return stream_in_blocks(v, slots, c.frame_id, c.pool_count, n_blocks);
@@ -1621,7 +1647,10 @@ int vh_save_volume(vh_volume* v, const char* path) {
   std::vector<unsigned long long> bk;
   if ((rc = resident_blocks(v, c, bk))) return rc;
   const uint32_t wpb = words_per_block(v);
-  const int64_t n_res = (int64_t)bk.size(), n_host = (int64_t)v->store_keys.size();
+  std::vector<uint32_t> host;  // occupied store slots, ascending
+  for (const auto& kv : v->store_index) host.push_back(kv.second);
+  std::sort(host.begin(), host.end());
+  const int64_t n_res = (int64_t)bk.size(), n_host = (int64_t)host.size();
   CkptHeader h{};
   std::memcpy(h.magic, kCkptMagic, 8);
   h.abi = VH_ABI_VERSION;
@@ -1634,14 +1663,14 @@ int vh_save_volume(vh_volume* v, const char* path) {
   if (!f) return fail(VH_ERR_IO, std::string("cannot open ") + path);
   bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
   ok = ok && std::fwrite(bk.data(), 8, (size_t)n_res, f) == (size_t)n_res;
-  ok = ok && std::fwrite(v->store_keys.data(), 8, (size_t)n_host, f) == (size_t)n_host;
-  // resident voxel words (then colours) in rounds of kStreamChunk blocks, then the store's
+  for (size_t i = 0; ok && i < host.size(); ++i) ok = std::fwrite(&v->slot_key[host[i]], 8, 1, f) == 1;
+  // voxel words (then colours): resident blocks in rounds of 65,536, then the store's
   std::vector<uint32_t> buf;
   for (int pass = 0; pass < (v->colors ? 2 : 1) && ok; ++pass) {
     const size_t per = pass == 0 ? wpb : (size_t)kB3;
     const uint32_t* src = pass == 0 ? static_cast<const uint32_t*>(v->voxels) : v->colors;
-    for (int64_t b0 = 0; b0 < n_res && ok; b0 += kStreamChunk) {
-      const size_t n = (size_t)std::min<int64_t>(kStreamChunk, n_res - b0);
+    for (int64_t b0 = 0; b0 < n_res && ok; b0 += 65536) {
+      const size_t n = (size_t)std::min<int64_t>(65536, n_res - b0);
       buf.resize(n * per);
       if (hipMemcpy(buf.data(), src + (size_t)b0 * per, n * per * 4, hipMemcpyDeviceToHost) != hipSuccess) {
         std::fclose(f);
@@ -1649,8 +1678,8 @@ int vh_save_volume(vh_volume* v, const char* path) {
       }
       ok = std::fwrite(buf.data(), 4, n * per, f) == n * per;
     }
-    const std::vector<uint32_t>& hs = pass == 0 ? v->store_words : v->store_colors;
-    ok = ok && std::fwrite(hs.data(), 4, hs.size(), f) == hs.size();
+    for (size_t i = 0; ok && i < host.size(); ++i)
+      ok = std::fwrite(pass == 0 ? store_words(v, host[i]) : store_cols(v, host[i]), 4, per, f) == per;
   }
   ok = (std::fclose(f) == 0) && ok;
   return ok ? VH_OK : fail(VH_ERR_IO, "write failed");
@@ -1666,7 +1695,7 @@ int vh_load_volume(const char* path, int32_t device, vh_volume** out) {
     std::fclose(f);
     return fail(VH_ERR_IO, "not a volume checkpoint");
   }
-  if (h.abi != VH_ABI_VERSION || h.n_blocks < 0) {
+  if (h.abi != VH_ABI_VERSION || h.n_blocks < 0 || h.n_blocks >= ((int64_t)1 << 31)) {
     std::fclose(f);
     return fail(VH_ERR_IO, "checkpoint of another library version");
   }
@@ -1678,29 +1707,33 @@ int vh_load_volume(const char* path, int32_t device, vh_volume** out) {
     std::fclose(f);
     return rc;
   }
-  const uint32_t wpb = words_per_block(v);
-  if ((uint32_t)h.words_per_block != wpb) {
+  auto bail = [&](int code, const char* msg) {
     std::fclose(f);
+    std::string m = msg ? msg : vh_last_error();
     vh_destroy(v);
-    return fail(VH_ERR_IO, "checkpoint voxel format does not match its configuration");
-  }
+    return fail(code, m);
+  };
+  const uint32_t wpb = words_per_block(v);
+  if ((uint32_t)h.words_per_block != wpb) return bail(VH_ERR_IO, "checkpoint voxel format does not match its configuration");
+  // the blocks go to the host store (slots 0 .. n-1, slab by slab), then into the pool
   const size_t n = (size_t)h.n_blocks;
-  v->store_keys.resize(n);
-  v->store_words.resize(n * wpb);
-  if (v->colors) v->store_colors.resize(n * kB3);
-  bool ok = std::fread(v->store_keys.data(), 8, n, f) == n &&
-            std::fread(v->store_words.data(), 4, n * wpb, f) == n * wpb &&
-            (!v->colors || std::fread(v->store_colors.data(), 4, n * kB3, f) == n * kB3);
+  std::vector<unsigned long long> keys(n);
+  if (std::fread(keys.data(), 8, n, f) != n) return bail(VH_ERR_IO, "checkpoint truncated");
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t slot = 0;
+    if ((rc = store_alloc_slot(v, &slot))) return bail(rc, nullptr);
+    v->slot_key[slot] = keys[i];
+    if (!v->store_index.emplace(keys[i], slot).second) return bail(VH_ERR_IO, "checkpoint holds a block twice");
+  }
+  for (int pass = 0; pass < (v->colors ? 2 : 1); ++pass) {
+    const size_t per = pass == 0 ? wpb : (size_t)kB3;
+    for (size_t s0 = 0; s0 < n; s0 += kSlabBlocks) {
+      const size_t m = std::min<size_t>(kSlabBlocks, n - s0);
+      uint32_t* dst = pass == 0 ? v->slab_w[s0 >> kSlabShift] : v->slab_c[s0 >> kSlabShift];
+      if (std::fread(dst, 4, m * per, f) != m * per) return bail(VH_ERR_IO, "checkpoint truncated");
+    }
+  }
   std::fclose(f);
-  if (!ok) {
-    vh_destroy(v);
-    return fail(VH_ERR_IO, "checkpoint truncated");
-  }
-  for (size_t i = 0; i < n; ++i) v->store_index[v->store_keys[i]] = (uint32_t)i;
-  if (v->store_index.size() != n) {
-    vh_destroy(v);
-    return fail(VH_ERR_IO, "checkpoint holds a block twice");
-  }
   // the frame counters continue from the checkpoint; every block goes to the pool if it fits
   FrameCtl c{};
   if ((rc = read_ctl(v, &c)) == VH_OK) {

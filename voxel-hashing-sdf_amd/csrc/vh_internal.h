@@ -205,8 +205,10 @@ struct VoxelIO<VH_STORAGE_F16R> {
   static __device__ __forceinline__ Word zero() { return 0u; }
 };
 
-// Out-of-core streaming (launch_stream_*): blocks move between the pool and a staging
-// buffer of n blocks (wpb 32-bit words of voxels each, plus 512 colour words).
+// Out-of-core streaming (launch_stream_*): blocks move between the pool and the host store,
+// pinned host slabs of kSlabBlocks blocks that kernels read and write directly (wpb 32-bit
+// words of voxels per block, plus 512 colour words); slot s lives in slab s >> kSlabShift.
+constexpr uint32_t kSlabShift = 14, kSlabBlocks = 1u << kSlabShift;
 struct StreamArgs {
   HashView hash;
   unsigned long long* block_keys;
@@ -214,13 +216,14 @@ struct StreamArgs {
   uint32_t* colors;            // colour pool (null: no colour)
   uint32_t wpb;                // 32-bit words per block: 1024 (FP32) or 512 (16-bit formats)
   uint32_t n;                  // blocks of this launch
-  const uint32_t* src;         // gather: pool blocks to stage; move: pool blocks to move
+  const uint32_t* src;         // out / move: pool blocks
   const uint32_t* dst;         // move: their new pool blocks
-  uint32_t* stage;             // n x wpb words
-  uint32_t* stage_col;         // n x 512 colour words (colour volumes)
-  const unsigned long long* stage_keys;  // insert: the staged blocks' keys
-  uint32_t* placed;            // insert: pool block per staged block (kMergeBit: merged into
-                               // a resident block), kNoBlock where the hash is full
+  const uint32_t* slots;       // out / insert: the blocks' host-store slots
+  uint32_t* const* slab_words; // device addresses of the host slabs' voxel words
+  uint32_t* const* slab_cols;  // and colour words (colour volumes)
+  const unsigned long long* keys;  // insert: the blocks' keys
+  uint32_t* placed;            // insert: pool block per block (kMergeBit: merged into a
+                               // resident block), kNoBlock where the hash is full
   FrameCtl* ctl;
   uint32_t pool_cap;
   uint32_t birth;              // birth frame written with rebuilt and inserted keys
@@ -302,14 +305,15 @@ hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st);
 hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st);
 // Second half (after the count on the same arguments): faces and vertices.
 hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st);
-// Streaming: stage[i] = pool[src[i]] (and colours).
-hipError_t launch_stream_gather(const StreamArgs& a, hipStream_t st);
+// Streaming: host slot slots[i] = pool block src[i] (and colours), written over the link.
+hipError_t launch_stream_out(const StreamArgs& a, hipStream_t st);
 // pool[dst[i]] = pool[src[i]] with its key and colours (source and destination sets disjoint).
 hipError_t launch_stream_move(const StreamArgs& a, hipStream_t st);
 // Inserts block_keys[0 .. n) into an empty hash table, slot value = index | birth << 32.
 hipError_t launch_stream_rehash(const StreamArgs& a, hipStream_t st);
-// Inserts the staged blocks: new keys take pool blocks from ctl->pool_count and their staged
-// words; keys already resident are merged into the resident block (placed[] says which).
+// Inserts the host-store blocks at slots[]: new keys take pool blocks from ctl->pool_count
+// and their words; keys already resident are merged into the resident block (placed[] says
+// which).
 hipError_t launch_stream_insert(const StreamArgs& a, int storage, hipStream_t st);
 hipError_t launch_query(const QueryArgs& q, int storage, const float* pts, int64_t n, float* sdf,
                         uint8_t* valid, hipStream_t st);

@@ -1173,16 +1173,28 @@ __global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
 
 // ============================================================================ streaming
 // Out-of-core streaming (the reference's streamOutGPU2CPU / streamInCPU2GPU, src/tsdf.cu:
-// 277-596, re-designed: whole 8^3 blocks move, not 5^3-block chunks, and the pool stays
-// dense). One wave per block in the copy kernels, 64 lanes striding over its words.
+// 277-596, re-designed: whole 8^3 blocks move, not 5^3-block chunks, the pool stays dense,
+// and the kernels read and write the pinned host store directly, so each block crosses the
+// link once with no staging copy). One wave per block in the copy kernels, 64 lanes
+// striding over its words.
 
-__global__ __launch_bounds__(256) void k_st_gather(StreamArgs a) {
+__device__ __forceinline__ uint32_t* slab_words(const StreamArgs& a, uint32_t slot) {
+  return a.slab_words[slot >> kSlabShift] + (size_t)(slot & (kSlabBlocks - 1u)) * a.wpb;
+}
+__device__ __forceinline__ uint32_t* slab_cols(const StreamArgs& a, uint32_t slot) {
+  return a.slab_cols[slot >> kSlabShift] + (size_t)(slot & (kSlabBlocks - 1u)) * kB3;
+}
+
+__global__ __launch_bounds__(256) void k_st_out(StreamArgs a) {
   const int lane = threadIdx.x & 63;
   for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < a.n; i += gridDim.x * 4u) {
     const size_t b = a.src[i];
-    for (uint32_t k = lane; k < a.wpb; k += 64) a.stage[(size_t)i * a.wpb + k] = a.pool[b * a.wpb + k];
-    if (a.colors)
-      for (uint32_t k = lane; k < kB3; k += 64) a.stage_col[(size_t)i * kB3 + k] = a.colors[b * kB3 + k];
+    uint32_t* w = slab_words(a, a.slots[i]);
+    for (uint32_t k = lane; k < a.wpb; k += 64) w[k] = a.pool[b * a.wpb + k];
+    if (a.colors) {
+      uint32_t* c = slab_cols(a, a.slots[i]);
+      for (uint32_t k = lane; k < kB3; k += 64) c[k] = a.colors[b * kB3 + k];
+    }
   }
 }
 
@@ -1215,7 +1227,7 @@ __global__ __launch_bounds__(256) void k_st_insert(StreamArgs a) {
   bool inserted = false;
   int64_t slot = -1;
   if (act) {
-    slot = find_or_insert(a.hash, a.stage_keys[i], inserted);
+    slot = find_or_insert(a.hash, a.keys[i], inserted);
     if (slot < 0) raise_flag(a.ctl, kFlagHashFull);
   }
   const uint64_t im = __ballot(inserted);
@@ -1229,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_st_insert(StreamArgs a) {
     const uint32_t idx = pbase + lanes_below(im);
     uint32_t stored = kNoBlock;
     if (idx < a.pool_cap) {
-      a.block_keys[idx] = a.stage_keys[i];
+      a.block_keys[idx] = a.keys[i];
       stored = placed = idx;
     } else {
       raise_flag(a.ctl, kFlagPoolFull);
@@ -1242,8 +1254,9 @@ __global__ __launch_bounds__(256) void k_st_insert(StreamArgs a) {
   a.placed[i] = placed;
 }
 
-// Per-voxel merge of a staged observation history into a resident one: weights add (capped),
-// the TSDF value is the weighted mean (s_r w_r + s_s w_s) / (w_r + w_s), colours per channel
+// Host-store blocks into the pool: copied, or merged voxel by voxel into a resident block
+// of the same key (the observation histories combine): weights add (capped), the TSDF value
+// is the weighted mean (s_r w_r + s_s w_s) / (w_r + w_s), colours per channel
 // (c_r w_r + c_s w_s) / (w_r + w_s) truncated to 8 bits; a side with weight 0 contributes
 // nothing (the other side is kept bit for bit).
 template <int kStorage>
@@ -1255,14 +1268,16 @@ __global__ __launch_bounds__(256) void k_st_place(StreamArgs a) {
     const uint32_t p = a.placed[i];
     if (p == kNoBlock) continue;
     const size_t d = p & ~kMergeBit;
+    const uint32_t* hw = slab_words(a, a.slots[i]);
+    const uint32_t* hc = a.colors ? slab_cols(a, a.slots[i]) : nullptr;
     if (!(p & kMergeBit)) {
-      for (uint32_t k = lane; k < a.wpb; k += 64) a.pool[d * a.wpb + k] = a.stage[(size_t)i * a.wpb + k];
+      for (uint32_t k = lane; k < a.wpb; k += 64) a.pool[d * a.wpb + k] = hw[k];
       if (a.colors)
-        for (uint32_t k = lane; k < kB3; k += 64) a.colors[d * kB3 + k] = a.stage_col[(size_t)i * kB3 + k];
+        for (uint32_t k = lane; k < kB3; k += 64) a.colors[d * kB3 + k] = hc[k];
       continue;
     }
     Word* rw = reinterpret_cast<Word*>(a.pool) + d * kB3;
-    const Word* sw = reinterpret_cast<const Word*>(a.stage) + (size_t)i * kB3;
+    const Word* sw = reinterpret_cast<const Word*>(hw);
     for (uint32_t k = lane; k < kB3; k += 64) {
       float sr, wr, ss, ws;
       const Word r = rw[k], q = sw[k];
@@ -1270,7 +1285,7 @@ __global__ __launch_bounds__(256) void k_st_place(StreamArgs a) {
       IO::unpack(q, ss, ws);
       if (!(ws > 0.0f)) continue;
       uint32_t* col = a.colors ? a.colors + d * kB3 + k : nullptr;
-      const uint32_t cs = a.colors ? a.stage_col[(size_t)i * kB3 + k] : 0u;
+      const uint32_t cs = a.colors ? hc[k] : 0u;
       if (!(wr > 0.0f)) {
         rw[k] = q;
         if (col) *col = cs;
@@ -1767,8 +1782,8 @@ hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st)
 
 static uint32_t stream_grid(uint32_t n) { return min((n + 3u) / 4u, 16384u); }
 
-hipError_t launch_stream_gather(const StreamArgs& a, hipStream_t st) {
-  if (a.n) hipLaunchKernelGGL(k_st_gather, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
+hipError_t launch_stream_out(const StreamArgs& a, hipStream_t st) {
+  if (a.n) hipLaunchKernelGGL(k_st_out, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
