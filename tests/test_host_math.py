@@ -1,0 +1,17 @@
+"""Host checks of device math in csrc/vh_math.h, compiled with g++ (no GPU): vertex_end
+decides exactly when two vertices of a cube coincide (tests/host_checks/vertex_end.cpp)."""
+import os
+import subprocess
+
+from conftest import ROOT
+
+
+def test_vertex_end_predicts_coincident_vertices(tmp_path):
+    src = os.path.join(ROOT, "tests", "host_checks", "vertex_end.cpp")
+    exe = str(tmp_path / "vertex_end")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "voxel-hashing-sdf_amd", "csrc"),
+                    "-I", os.path.join(ROOT, "include"), "-o", exe, src], check=True, timeout=120)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ok, pairs, coincident = r.stdout.split()
+    assert ok == "ok" and int(pairs) == 400000 * 66 and int(coincident) > 100000

@@ -701,6 +701,7 @@ __device__ __forceinline__ uint32_t mc_classify(McShared& S, float tiny, uint32_
   for (int k = 0; k < 4; ++k) f0[k] = S.val[(cy + (k >> 1)) * 9 + cz + (k & 1)];
   uint32_t ntri = 0;
   near = 0u;
+#pragma unroll 4  // fully unrolled, the 8 cubes' values and cases need 94 VGPRs (occupancy 5)
   for (int cx = 0; cx < 8; ++cx) {
     float f1[4];
 #pragma unroll
@@ -772,21 +773,74 @@ __device__ __forceinline__ bool mc_triangle(const McShared& S, const uint32_t* c
 // to a, which needs |mu| <= ulp(a)/2 and so |v_a| <= ulp(a) (|v_b - v_a| <= 2); likewise for
 // b. A cube whose corners all have |value| >= tiny (tiny above both bounds) therefore emits
 // exactly its table count; the others ("near" bits) are counted triangle by triangle.
+// Per cube edge e, 5 bits at 5e: the offset of its lower end point in the cube (dx | dy << 1
+// | dz << 2) and its axis << 3 (0 x, 1 y, 2 z).
+constexpr uint64_t edge_lo_table() {
+  uint64_t t = 0;
+  for (int e = 0; e < 12; ++e) {
+    const int a = edge_a(e), b = edge_b(e);
+    const int dx = corner_dx(a) < corner_dx(b) ? corner_dx(a) : corner_dx(b);
+    const int dy = corner_dy(a) < corner_dy(b) ? corner_dy(a) : corner_dy(b);
+    const int dz = corner_dz(a) < corner_dz(b) ? corner_dz(a) : corner_dz(b);
+    const int axis = corner_dx(a) != corner_dx(b) ? 0 : (corner_dy(a) != corner_dy(b) ? 1 : 2);
+    t |= (uint64_t)(dx | dy << 1 | dz << 2 | axis << 3) << (5 * e);
+  }
+  return t;
+}
+constexpr uint64_t kEdgeLo = edge_lo_table();
+
+// Edges of case ci whose two corners lie on opposite sides (bit e): corners 0-3 and 4-7 form
+// the z = 0 and z = 1 rings (edges 0-3 and 4-7 join ring neighbours i, i + 1 mod 4), edges
+// 8-11 join corner i to corner i + 4.
+__host__ __device__ constexpr uint32_t crossed_edges(int ci) {
+  const uint32_t lo = (uint32_t)ci & 15u, hi = ((uint32_t)ci >> 4) & 15u;
+  const uint32_t rlo = ((lo >> 1) | (lo << 3)) & 15u, rhi = ((hi >> 1) | (hi << 3)) & 15u;
+  return (lo ^ rlo) | ((hi ^ rhi) << 4) | ((lo ^ hi) << 8);
+}
+constexpr bool crossed_edges_ok() {
+  for (int ci = 0; ci < 256; ++ci)
+    for (int e = 0; e < 12; ++e)
+      if ((((ci >> edge_a(e)) ^ (ci >> edge_b(e))) & 1) != (int)((crossed_edges(ci) >> e) & 1u)) return false;
+  return true;
+}
+static_assert(crossed_edges_ok(), "crossed_edges disagrees with the edge table");
+
+// Non-degenerate table triangles of a near cube (case ci at (cx, cy, cz)): vertex_end tells
+// on which corner, if any, each vertex lands, so no positions are formed; a triangle is
+// degenerate when two of its vertices land on the same corner (the only way two vertices
+// on distinct edges coincide), which is what mc_triangle's position test finds.
+__device__ __forceinline__ uint32_t mc_count_near(const McShared& S, int ci, int cx, int cy, int cz, int gx0,
+                                                  int gy0, int gz0) {
+  const uint64_t pk = kTriTable[ci];
+  const int nt = S.cnt[ci];
+  const int g0[3] = {gx0 + cx, gy0 + cy, gz0 + cz};
+  uint32_t n = 0;
+  for (int t = 0; t < nt; ++t) {
+    int cid[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
+      const int ca = edge_a(e), cb = edge_b(e);
+      const float va = S.val[(cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca))];
+      const float vb = S.val[(cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb))];
+      const int axis = (int)((kEdgeLo >> (5 * e + 3)) & 3u);
+      const int da = axis == 0 ? corner_dx(ca) : (axis == 1 ? corner_dy(ca) : corner_dz(ca));
+      const int db = axis == 0 ? corner_dx(cb) : (axis == 1 ? corner_dy(cb) : corner_dz(cb));
+      const int end = vertex_end((float)(g0[axis] + da), (float)(g0[axis] + db), va, vb);
+      cid[j] = end == 0 ? ca : (end == 1 ? cb : -1 - j);
+    }
+    n += (cid[0] != cid[1] && cid[1] != cid[2] && cid[2] != cid[0]) ? 1u : 0u;
+  }
+  return n;
+}
+
 __device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t near, int gx0, int gy0, int gz0) {
   const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
   uint32_t n = 0;
   for (int cx = 0; cx < 8; ++cx) {
     const int ci = S.cases[cx * 64 + lane];
-    const int nt = S.cnt[ci];
-    if (!((near >> cx) & 1u)) {
-      n += (uint32_t)nt;
-      continue;
-    }
-    for (int t = 0; t < nt; ++t) {
-      float P[9];
-      uint32_t C[3];
-      n += mc_triangle<false>(S, nullptr, ci, t, cx, cy, cz, gx0, gy0, gz0, P, C) ? 1u : 0u;
-    }
+    if (!((near >> cx) & 1u)) n += (uint32_t)S.cnt[ci];
+    else n += mc_count_near(S, ci, cx, cy, cz, gx0, gy0, gz0);
   }
   return n;
 }
@@ -1015,37 +1069,6 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
 // neighbouring cubes would interpolate an edge in opposite directions (the reference welds
 // float-identical coordinates instead, src/tsdf.cu:1772-1850).
 
-// Per cube edge e, 5 bits at 5e: the offset of its lower end point in the cube (dx | dy << 1
-// | dz << 2) and its axis << 3 (0 x, 1 y, 2 z).
-constexpr uint64_t edge_lo_table() {
-  uint64_t t = 0;
-  for (int e = 0; e < 12; ++e) {
-    const int a = edge_a(e), b = edge_b(e);
-    const int dx = corner_dx(a) < corner_dx(b) ? corner_dx(a) : corner_dx(b);
-    const int dy = corner_dy(a) < corner_dy(b) ? corner_dy(a) : corner_dy(b);
-    const int dz = corner_dz(a) < corner_dz(b) ? corner_dz(a) : corner_dz(b);
-    const int axis = corner_dx(a) != corner_dx(b) ? 0 : (corner_dy(a) != corner_dy(b) ? 1 : 2);
-    t |= (uint64_t)(dx | dy << 1 | dz << 2 | axis << 3) << (5 * e);
-  }
-  return t;
-}
-constexpr uint64_t kEdgeLo = edge_lo_table();
-
-// Edges of case ci whose two corners lie on opposite sides (bit e): corners 0-3 and 4-7 form
-// the z = 0 and z = 1 rings (edges 0-3 and 4-7 join ring neighbours i, i + 1 mod 4), edges
-// 8-11 join corner i to corner i + 4.
-__host__ __device__ constexpr uint32_t crossed_edges(int ci) {
-  const uint32_t lo = (uint32_t)ci & 15u, hi = ((uint32_t)ci >> 4) & 15u;
-  const uint32_t rlo = ((lo >> 1) | (lo << 3)) & 15u, rhi = ((hi >> 1) | (hi << 3)) & 15u;
-  return (lo ^ rlo) | ((hi ^ rhi) << 4) | ((lo ^ hi) << 8);
-}
-constexpr bool crossed_edges_ok() {
-  for (int ci = 0; ci < 256; ++ci)
-    for (int e = 0; e < 12; ++e)
-      if ((((ci >> edge_a(e)) ^ (ci >> edge_b(e))) & 1) != (int)((crossed_edges(ci) >> e) & 1u)) return false;
-  return true;
-}
-static_assert(crossed_edges_ok(), "crossed_edges disagrees with the edge table");
 
 // Owner slot (index into S.nb) of edge e of cube (cx, cy, cz) and the edge's bit in its
 // owner's used-edge words: the lower end point (block-local, 0..8 per axis) decides both.

@@ -320,6 +320,20 @@ VH_HD void vertex_interp(float p1x, float p1y, float p1z, float p2x, float p2y, 
   z = p1z + mu * (p2z - p1z);
 }
 
+// Which end point of its edge the vertex_interp vertex lands on: 0 on p1, 1 on p2, 2 on
+// neither. p1a, p2a are the end points' coordinates along the edge's axis, computed as
+// vertex_interp computes that coordinate; the other two coordinates of the vertex equal the
+// end points' exactly (mu is finite after the snaps, and mu * 0 adds zero). Vertices on
+// distinct edges of a cube coincide exactly when both land on the corner the edges share.
+VH_HD int vertex_end(float p1a, float p2a, float v1, float v2) {
+  if (near_iso(v1)) return 0;
+  if (near_iso(v2)) return 1;
+  if (near_iso(v1 - v2)) return 0;
+  const float mu = (0.0f - v1) / (v2 - v1);
+  const float t = p1a + mu * (p2a - p1a);
+  return t == p1a ? 0 : (t == p2a ? 1 : 2);
+}
+
 // VertexInterp with the colour lerp of the reference (src/tsdf.cu:1655-1657): a snapped vertex
 // takes its corner's colour; otherwise each channel is c1 + mu*(c2 - c1) (the difference is
 // an exact integer, the rest FP32), truncated to 8 bits.
