@@ -221,7 +221,7 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.bsz = c.vox_size * (float)kB;
   a.mu = c.trunc;
   a.inv_mu = 1.0f / c.trunc;
-  a.cap = (float)c.weight_cap;
+  a.cap = c.weight_cap > 0 ? (float)c.weight_cap : INFINITY;
   a.hash = HashView{v->keys, v->vals, c.log2_buckets, ((uint64_t)1 << c.log2_buckets) - 1};
   a.marks = v->marks;
   a.n_slots = v->n_slots;

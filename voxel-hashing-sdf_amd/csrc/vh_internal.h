@@ -52,6 +52,18 @@ __device__ __forceinline__ float div_rn(float a, float b) {
   return q;
 }
 
+// div_rn for dividends that are rarely exactly +0 and never -0 (the TSDF mean's s*w + dist:
+// dist is never -0, and x + y is -0 only when both are): a = +0 takes the fallback, whose
+// quotient +0 is the one the correction gives, so the zero test can go.
+__device__ __forceinline__ float div_rn_rare0(float a, float b) {
+  const float y = recip_rn(b);
+  const float q0 = a * y;
+  float q = __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+  const bool tiny = __builtin_fabsf(a) < 0x1p-76f;
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(tiny) != 0, 0)) q = tiny ? a / b : q;
+  return q;
+}
+
 enum : uint32_t { kFlagPoolFull = 1u, kFlagHashFull = 2u, kFlagRange = 4u, kFlagTouchedFull = 8u };
 constexpr int kMaxProbe = 64;  // buckets examined before a key is declared unplaceable
 constexpr int kStripes = 64;   // striped counters / touched sub-lists
@@ -95,7 +107,8 @@ struct FrameArgs {
   // camera and volume parameters
   int32_t W, H;
   Intr K;
-  float max_depth, min_depth, vox, bsz, mu, inv_mu, cap;
+  float max_depth, min_depth, vox, bsz, mu, inv_mu;
+  float cap;  // weight cap, +inf when uncapped
   // tables
   HashView hash;
   uint32_t* marks;         // 2 x n_slots

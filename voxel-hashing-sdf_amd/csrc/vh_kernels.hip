@@ -248,10 +248,11 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
 
 // ============================================================================ integration
 // tsdf_update (vh_math.h) with the quotient by div_rn: the same correctly rounded result.
-__device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap) {
+// cap_or_inf: the weight cap, +inf when uncapped (min is then the identity).
+__device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap_or_inf) {
   const float w_new = w + 1.0f;
-  s = div_rn(s * w + dist, w_new);
-  w = (cap > 0.0f && w_new > cap) ? cap : w_new;
+  s = div_rn_rare0(s * w + dist, w_new);
+  w = fminf(w_new, cap_or_inf);
 }
 
 // Colour update of the reference (src/tsdf.cu:743-745): per channel (c * w_old + rgb) / w_new
@@ -318,15 +319,22 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFram
   // two halves of 4 voxels: fewer values live at once (occupancy) for 4 fetches in flight
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
+    constexpr uint32_t kElemShift = kU16 ? 1u : 2u;  // log2 of the depth element's bytes
     float zc[4];
     int pix[4];
+    uint32_t off[4];  // kInside: byte offset of the pixel's depth (32-bit, off a scalar base)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float fq = (float)(h + r);
       zc[r] = __builtin_fmaf(fq, bf.ax[2], cl[2]);
       const float x = __builtin_fmaf(fq, bf.ax[0], cl[0]), y = __builtin_fmaf(fq, bf.ax[1], cl[1]);
       const float iz = recip_rn(zc[r]);
-      if constexpr (kInside) {
+      if constexpr (kInside && !kColor) {
+        const float uf = rintf(__builtin_fmaf(a.K.fx, x * iz, a.K.cx));
+        const float vf = rintf(__builtin_fmaf(a.K.fy, y * iz, a.K.cy));
+        off[r] = ((uint32_t)vf & 0xffffffu) * ((uint32_t)(a.W << kElemShift) & 0xffffffu) +
+                 ((uint32_t)uf << kElemShift);
+      } else if constexpr (kInside) {  // colour: pixel indices serve the depth and RGB gathers
         const float uf = rintf(__builtin_fmaf(a.K.fx, x * iz, a.K.cx));
         const float vf = rintf(__builtin_fmaf(a.K.fy, y * iz, a.K.cy));
         pix[r] = (int)(((uint32_t)vf & 0xffffffu) * ((uint32_t)a.W & 0xffffffu) + (uint32_t)uf);
@@ -336,7 +344,14 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFram
     }
     float dv[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) dv[r] = depth_at<kU16>(base, a.depth_scale, kInside ? pix[r] : max(pix[r], 0));
+    for (int r = 0; r < 4; ++r) {
+      if constexpr (kInside && !kColor) {
+        const char* p = reinterpret_cast<const char*>(base) + off[r];
+        dv[r] = kU16 ? (float)*reinterpret_cast<const uint16_t*>(p) * a.depth_scale : *reinterpret_cast<const float*>(p);
+      } else {
+        dv[r] = depth_at<kU16>(base, a.depth_scale, kInside ? pix[r] : max(pix[r], 0));
+      }
+    }
     uint32_t px[4];
     if constexpr (kColor) {
 #pragma unroll
@@ -356,7 +371,7 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFram
       mean_update(sv, wv, dist, a.cap);
       const Word nw = IO::pack(sv, wv);
       old[q] = ok ? nw : old[q];
-      upd |= (uint32_t)ok << q;
+      if constexpr (kColor) upd |= (uint32_t)ok << q;
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the halves apart: the scheduler would merge them
   }
@@ -442,7 +457,7 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
 #pragma unroll
         for (int q = 0; q < 8; ++q) cold[q] = fresh ? 0u : cp[64 * q];
       }
-      uint32_t upd = 0u;  // bit q: voxel q observed by a frame of the batch
+      uint32_t upd = 0u;  // colour volumes: bit q set when voxel q was observed in the batch
       for (uint32_t m = mark; m != 0u; m &= m - 1u) {
         const uint32_t jf = (uint32_t)__builtin_ctz(m);
         uint32_t fs = fs0 + jf;
@@ -462,13 +477,21 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         else
           frame_update<kStorage, kU16, kColor, false>(a, bf, cl, base, rgba, old, cold, upd);
       }
-      // fresh blocks are written whole (never-observed voxels as zero), others where observed
+      // Depth-only volumes write the block whole (unobserved voxels keep the words read,
+      // fresh blocks zero there; the wave owns the block in this launch), which saves the
+      // per-voxel mask; colour volumes (12 B per voxel) write fresh blocks whole and other
+      // blocks where observed.
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (fresh || ((upd >> q) & 1u)) {
+      for (int q = 0; q < 8; ++q) {
+        if constexpr (kColor) {
+          if (fresh || ((upd >> q) & 1u)) {
+            vp[64 * q] = old[q];
+            cp[64 * q] = cold[q];
+          }
+        } else {
           vp[64 * q] = old[q];
-          if constexpr (kColor) cp[64 * q] = cold[q];
         }
+      }
     }
     __builtin_amdgcn_wave_barrier();  // the next batch of entries overwrites s_val / s_key / s_mark
   }
@@ -533,10 +556,11 @@ __device__ void batch_done(const FrameArgs& a, uint32_t groups, uint32_t f0, uin
 }
 
 constexpr int kIntegrateWaves = 4;  // independent workers per workgroup
-// Register budget of the integration kernels: 5 waves per SIMD (96 VGPRs). Uncapped, the
-// two frame_update variants take 106 (FP32) and leave 4 waves; capped, a few values spill
-// to scratch and the kernel runs faster for every voxel format (measured on room_medium_5mm).
-#define VH_INTEGRATE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(5)))
+// Register budget of the integration kernels (measured on room_medium_5mm): depth only, 5
+// waves per SIMD (96 VGPRs; uncapped the two frame_update variants take 106 and leave 4
+// waves; capped, at most 7 values spill and every voxel format runs faster); colour, 4 waves
+// (128 VGPRs: at 96 about 50 values spill into the frame loop, 13 % slower).
+#define VH_INTEGRATE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(kColor ? 4 : 5)))
 
 // The k poses of a batch (inputs cur .. cur+k-1) into LDS, read by every block of the
 // workgroup's workers (a global load per block and frame would put a memory round trip in
