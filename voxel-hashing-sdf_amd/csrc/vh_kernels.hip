@@ -1607,66 +1607,88 @@ __global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int gr
   }
   wg_reduce_store<kGnAcc>(acc, tot);
   if (threadIdx.x != 0) return;
+  // One lane from here: the state and the candidate are read once into registers and written
+  // back once (through the pointers each access would be a dependent memory round trip).
+  GnState S = *st;
+  double c[12], row[kGnStats];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) c[k] = cand[k];
   const double cnt = tot[28];
   const double cost = tot[27] + (tot[29] - cnt);
-  stats_row[0] = cnt;
-  stats_row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
-  stats_row[2] = 0.0;
-  const bool accept = !st->has || cost <= st->cost;
+  row[0] = cnt;
+  row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
+  row[2] = 0.0;
+  const bool accept = !S.has || cost <= S.cost;
   if (accept) {
-    for (int k = 0; k < 12; ++k) st->pose[k] = cand[k];
-    for (int k = 0; k < 27; ++k) st->Ab[k] = tot[k];
-    st->cost = cost;
-    st->lambda = st->has ? fmax(st->lambda / 3.0, 1e-9) : st->lambda;
-    st->has = 1;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) S.pose[k] = c[k];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) S.Ab[k] = tot[k];
+    S.cost = cost;
+    S.lambda = S.has ? fmax(S.lambda / 3.0, 1e-9) : S.lambda;
+    S.has = 1;
   } else {
-    st->lambda = fmin(st->lambda * 4.0, 1e9);
+    S.lambda = fmin(S.lambda * 4.0, 1e9);
   }
-  stats_row[3] = st->lambda;
-  stats_row[4] = accept ? 1.0 : 0.0;
-  for (int k = 0; k < 12; ++k) stats_row[5 + k] = st->pose[k];
-  if (final_step) {
-    for (int k = 0; k < 12; ++k) cand[k] = st->pose[k];
-    return;
-  }
-  double A[6][6], b[6];
-  int m = 0;
-  for (int p = 0; p < 6; ++p)
-    for (int c = p; c < 6; ++c) { A[p][c] = st->Ab[m]; A[c][p] = st->Ab[m]; ++m; }
+  row[3] = S.lambda;
+  row[4] = accept ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) row[5 + k] = S.pose[k];
+  // damped normal equations, Cholesky, the two triangular solves (all loops unrolled: the
+  // matrices stay in registers); a matrix that is not positive definite proposes the
+  // accepted pose again
+  double A[6][6], b[6], L[6][6], x[6], y[6];
+#pragma unroll
+  for (int p = 0, m = 0; p < 6; ++p)
+#pragma unroll
+    for (int q = p; q < 6; ++q, ++m) { A[p][q] = S.Ab[m]; A[q][p] = S.Ab[m]; }
+#pragma unroll
   for (int p = 0; p < 6; ++p) {
-    b[p] = -st->Ab[21 + p];
-    A[p][p] += st->lambda * A[p][p] + 1e-12;
+    b[p] = -S.Ab[21 + p];
+    A[p][p] += S.lambda * A[p][p] + 1e-12;
   }
-  double L[6][6];
-  for (int p = 0; p < 6; ++p)
-    for (int c = 0; c < 6; ++c) L[p][c] = 0.0;
-  for (int c = 0; c < 6; ++c) {
-    double d = A[c][c];
-    for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
-    if (!(d > 0.0)) {  // not positive definite: propose the accepted pose again
-      for (int k = 0; k < 12; ++k) cand[k] = st->pose[k];
-      return;
-    }
-    L[c][c] = sqrt(d);
-    for (int p = c + 1; p < 6; ++p) {
-      double acc = A[p][c];
-      for (int k = 0; k < c; ++k) acc -= L[p][k] * L[c][k];
-      L[p][c] = acc / L[c][c];
+  bool pd = true;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double d = A[q][q];
+#pragma unroll
+    for (int k = 0; k < q; ++k) d -= L[q][k] * L[q][k];
+    pd = pd && d > 0.0;
+    L[q][q] = sqrt(d);
+#pragma unroll
+    for (int p = q + 1; p < 6; ++p) {
+      double a = A[p][q];
+#pragma unroll
+      for (int k = 0; k < q; ++k) a -= L[p][k] * L[q][k];
+      L[p][q] = a / L[q][q];
     }
   }
-  double y[6], x[6];
+#pragma unroll
   for (int p = 0; p < 6; ++p) {
-    double acc = b[p];
-    for (int k = 0; k < p; ++k) acc -= L[p][k] * y[k];
-    y[p] = acc / L[p][p];
+    double a = b[p];
+#pragma unroll
+    for (int k = 0; k < p; ++k) a -= L[p][k] * y[k];
+    y[p] = a / L[p][p];
   }
+#pragma unroll
   for (int p = 5; p >= 0; --p) {
-    double acc = y[p];
-    for (int k = p + 1; k < 6; ++k) acc -= L[k][p] * x[k];
-    x[p] = acc / L[p][p];
+    double a = y[p];
+#pragma unroll
+    for (int k = p + 1; k < 6; ++k) a -= L[k][p] * x[k];
+    x[p] = a / L[p][p];
   }
-  se3_left_update(x, st->pose, cand);
-  stats_row[2] = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+  if (!final_step && pd) {
+    se3_left_update(x, S.pose, c);
+    row[2] = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) c[k] = S.pose[k];
+  }
+  *st = S;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) cand[k] = c[k];
+#pragma unroll
+  for (int k = 0; k < kGnStats; ++k) stats_row[k] = row[k];
 }
 
 // ============================================================================ tracking
