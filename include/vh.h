@@ -243,7 +243,11 @@ int vh_stream_out(vh_volume* vol, const float center[3], float radius, int64_t* 
 int vh_stream_in(vh_volume* vol, const float center[3], float radius, int64_t* n_blocks);
 /* Checkpoint (no reference counterpart: the reference's volume lives only in its host
  * chunks): the configuration, the frame counters and every block, resident and in the host
- * store, voxel words and colours byte for byte. */
+ * store, voxel words and colours byte for byte. Layout (native byte order): char[8]
+ * "VHVOLUME", int32 ABI version, int32 32-bit words per block (1024 FP32, 512 16-bit
+ * formats), vh_config, uint32 frame id, uint32 0, uint64 frames, int64 n; then n packed
+ * block keys (uint64), n blocks of voxel words (in-block order), and for colour volumes n
+ * blocks of 512 colour words. */
 int vh_save_volume(vh_volume* vol, const char* path);
 /* Creates a volume from a checkpoint on device `device` (-1: the saved one). All blocks go
  * into the pool if they fit, otherwise they stay in the host store (vh_stream_in brings them

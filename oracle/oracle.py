@@ -68,6 +68,8 @@ def load():
     lib.vho_point_triangle_dist.restype = C.c_double
     lib.vho_point_triangle_dist.argtypes = [P, P, P, P]
     lib.vho_mesh_distance.argtypes = [P, P, C.c_int64, P, P, C.c_int64, P, P]
+    lib.vho_import_f32.restype = C.c_int
+    lib.vho_import_f32.argtypes = [P, P, P, P, C.c_int64]
     lib.vho_mesh_indexed.restype = C.c_int64
     lib.vho_mesh_indexed.argtypes = [P, P, P, C.c_int64, P, P, C.c_int64, P]
     _lib = lib
@@ -158,6 +160,14 @@ class Oracle:
         cubes = np.zeros((n, 4), np.int32)
         _lib.vho_mc64(self._h, _p(xyz), _p(cubes), n)
         return xyz, cubes
+
+    def import_f32(self, keys, tsdf, weight):
+        """Blocks with chosen values (keys (N,3) block coordinates, tsdf / weight (N,512)) into
+        an FP32 twin."""
+        keys = np.ascontiguousarray(keys, np.int32)
+        tsdf = np.ascontiguousarray(tsdf, np.float32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        assert _lib.vho_import_f32(self._h, _p(keys), _p(tsdf), _p(weight), len(keys)) == 0
 
     def mesh_indexed(self, with_rgb=False, with_cubes=False):
         """(vertices (V,3) float32 metres, faces (F,3) int32[, rgb (V,3) uint8][, cubes (F,4)

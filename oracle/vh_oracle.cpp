@@ -795,6 +795,28 @@ void vho_mesh_distance(const float* a_xyz, const int32_t* a_cubes, int64_t na, c
 
 // Triangles of one cube from 8 given corner values (all observed), lattice origin (0,0,0),
 // vox = 1: for exhaustive checks of the 256 cases.
+// Places n blocks (keys: n x 3 block coordinates) with the given FP32 values and weights
+// (n x 512 each, in-block order) into an FP32 twin and its FP64 shadow, replacing blocks
+// already there: a volume built voxel by voxel, for tests that need chosen values. Returns
+// -1 unless the twin stores FP32.
+int vho_import_f32(void* p, const int32_t* keys, const float* tsdf, const float* weight, int64_t n) {
+  Oracle* o = static_cast<Oracle*>(p);
+  if (o->storage != 0) return -1;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t k = key_of(keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]);
+    int64_t b = o->find(k);
+    if (b < 0) b = o->add_block(k);
+    for (int v = 0; v < 512; ++v) {
+      const size_t d = (size_t)b * 512 + v, s = (size_t)i * 512 + v;
+      o->s32[d] = tsdf[s];
+      o->w32[d] = weight[s];
+      o->s64[d] = tsdf[s];
+      o->w64[d] = weight[s];
+    }
+  }
+  return 0;
+}
+
 int vho_mc_cube(const float* val, float* xyz) {
   int ci = 0;
   for (int c = 0; c < 8; ++c)
