@@ -501,13 +501,57 @@ def main():
         torch.cuda.empty_cache()
         out["tracked"] = tracked_block(vh, wl, args.tracked_frames)
     if not args.no_cpu:
-        out["cpu_baseline"], errs = cpu_baseline(vh, w, depth, poses, args.cpu_seconds, args.no_f16)
+        out["cpu_baseline"], errs = cpu_baseline(vh, w, depth, poses, args.cpu_seconds, args.no_f16, wl)
         for key, err in errs.items():
             out.setdefault(key, {}).update(err)
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(vh, w, depth, poses, budget_s, no_f16):
+def jacobian_vs_fd(vh, wl, w, vol, ref, frame, h=1e-4, max_points=20000):
+    """GPU pose Jacobians (left se(3) perturbation, vh_query_sdf_backward_device) at the points
+    of a held-out frame under a perturbed pose, against central differences (step h) of the
+    oracle's FP64 trilinear query on the same voxel values, over points whose stencil stays in
+    one lattice cell; returns the relative error and the points used."""
+    import torch
+
+    scene = vh.Scene(w["scene"], wl.SEED)
+    d, T = scene.frames(frame, 1)
+    rng = np.random.default_rng(wl.SEED + 2)
+    xi = np.concatenate([rng.normal(0, 0.03, 3), rng.normal(0, np.deg2rad(2.0), 3)])
+    T0 = (T[0].astype(np.float64) @ se3_exp(0.25 * xi)).astype(np.float32)
+    pts = backproject(d[0], w, zmax=3.0)
+    n = len(pts)
+    d_pts = torch.from_numpy(np.ascontiguousarray(pts, np.float32)).cuda()
+    d_r = torch.zeros(n, dtype=torch.float32, device="cuda")
+    d_valid = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    d_J = torch.zeros((n, 6), dtype=torch.float32, device="cuda")
+    vol.backward_device(d_pts, n, T0, None, d_r, d_valid, d_J)
+    torch.cuda.synchronize()
+    valid, J = d_valid.cpu().numpy().astype(bool), d_J.cpu().numpy().astype(np.float64)
+    q = pts.astype(np.float64) @ T0[:3, :3].astype(np.float64).T + T0[:3, 3].astype(np.float64)
+    q, J = q[valid][:max_points], J[valid][:max_points]
+    vox = float(np.float32(w["vox_size"]))
+    cell = np.floor(q / vox)
+    ok = np.ones(len(q), bool)
+    fd = np.zeros((len(q), 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        Ep, Em = se3_exp(e), se3_exp(-e)
+        qp, qm = q @ Ep[:3, :3].T + Ep[:3, 3], q @ Em[:3, :3].T + Em[:3, 3]
+        ok &= np.all(np.floor(qp / vox) == cell, 1) & np.all(np.floor(qm / vox) == cell, 1)
+        sp, _, vp = ref.query64(qp)
+        sm, _, vm = ref.query64(qm)
+        ok &= vp & vm
+        fd[:, k] = (sp - sm) / (2 * h)
+    rel = float(np.linalg.norm(J[ok] - fd[ok]) / np.linalg.norm(fd[ok]))
+    return {"rel_err": rel, "points": int(ok.sum()), "frame": frame, "h": h,
+            "reference": "central differences of the oracle's FP64 trilinear query on the same voxel "
+                         "values (left se(3) perturbation), points of a held-out frame at a pose "
+                         "perturbed by 1/4 of the gn error, stencils within one cell"}
+
+
+def cpu_baseline(vh, w, depth, poses, budget_s, no_f16, wl=None):
     """The oracle (same algorithm: FP32 twin + FP64 shadow, OpenMP) on the first frames of the
     same sequence, as many as fit the time budget; then the GPU FP16 volume after the same
     frames against the FP64 shadow."""
@@ -558,6 +602,10 @@ def cpu_baseline(vh, w, depth, poses, budget_s, no_f16):
                                                  "point-to-triangle distances within the 27 surrounding cubes"})
         log(f"bench: {key}: Hausdorff to the FP64 mesh {1e3 * h['hausdorff']:.4f} mm "
             f"({time.perf_counter() - t:.1f} s)")
+        if key == "fp32" and wl is not None and np.array_equal(keys, rk):
+            g = jacobian_vs_fd(vh, wl, w, vol, ref, n + 17)
+            errs[key]["pose_jacobian_vs_fd"] = g
+            log(f"bench: fp32 pose Jacobian vs FD: relative error {g['rel_err']:.2e} over {g['points']} points")
         vol.close()
     return base, errs
 
