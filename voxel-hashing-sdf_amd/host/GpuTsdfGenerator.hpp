@@ -18,6 +18,7 @@
 // where the surface passes exactly through a lattice point, so closed surfaces stay closed.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -73,9 +74,37 @@ class GpuTsdfGenerator {
   GpuTsdfGenerator& operator=(const GpuTsdfGenerator&) = delete;
   ~GpuTsdfGenerator() { Shutdown(); }
 
+  // Out-of-core operation, the role of the reference's per-frame chunk streaming
+  // (streamInCPU2GPU / streamOutGPU2CPU, src/tsdf.cu:277-596), off by default. When on,
+  // every processFrame first brings into the GPU pool every host-store block within reach of
+  // its camera (the farthest point its truncation bands can touch: (max_depth + trunc) times
+  // the image corner's ray length per unit depth, plus a block diagonal), and every
+  // `every_frames` frames moves the blocks farther than reach + margin_m to the host store.
+  // extract() and SavePLY bring everything back first. The mesh is the one the volume would
+  // have without streaming.
+  void setStreaming(bool on, int every_frames = 8, float margin_m = 0.5f) {
+    vh_config c;
+    check(vh_get_config(vol_, &c), "setStreaming");
+    const double dx = std::fmax(c.cx, c.width - 1 - c.cx) / c.fx, dy = std::fmax(c.cy, c.height - 1 - c.cy) / c.fy;
+    reach_ = (float)((c.max_depth + c.trunc) * std::sqrt(1.0 + dx * dx + dy * dy) + 8.0 * c.vox_size * std::sqrt(3.0));
+    stream_ = on;
+    stream_every_ = every_frames > 0 ? every_frames : 1;
+    stream_margin_ = margin_m > 0.0f ? margin_m : 0.0f;
+    frames_since_out_ = 0;
+  }
+
   // depth: height x width metres (0 = invalid); rgb: height x width x 3 bytes (ignored, and
   // may be null, when colour is off); c2w: row-major 4x4 camera-to-world.
   void processFrame(float* depth, unsigned char* rgb, float* c2w) {
+    if (stream_) {
+      const float cam[3] = {c2w[3], c2w[7], c2w[11]};
+      int64_t n = 0;
+      if (++frames_since_out_ >= stream_every_) {
+        check(vh_stream_out(vol_, cam, reach_ + stream_margin_, &n), "processFrame (stream out)");
+        frames_since_out_ = 0;
+      }
+      check(vh_stream_in(vol_, cam, reach_, &n), "processFrame (stream in)");
+    }
     vh_frame f;
     std::memset(&f, 0, sizeof f);
     f.depth = depth;
@@ -117,8 +146,14 @@ class GpuTsdfGenerator {
   std::vector<Vertex>* getVertices() { return &vertices_; }
   std::vector<Face>* getFaces() { return &faces_; }
 
-  // Recomputes the indexed mesh from the current volume.
+  // Recomputes the indexed mesh from the current volume (all of it: streamed-out blocks
+  // come back first).
   void extract() {
+    if (stream_) {
+      const float origin[3] = {0.0f, 0.0f, 0.0f};
+      int64_t n = 0;
+      check(vh_stream_in(vol_, origin, INFINITY, &n), "extract (stream in)");
+    }
     vh_indexed_mesh m;
     check(vh_extract_mesh_indexed(vol_, &m), "extract mesh");
     vertices_.resize((size_t)m.n_vertices);
@@ -143,6 +178,9 @@ class GpuTsdfGenerator {
   }
 
   vh_volume* vol_ = nullptr;
+  bool stream_ = false;
+  int stream_every_ = 8, frames_since_out_ = 0;
+  float reach_ = 0.0f, stream_margin_ = 0.5f;
   std::vector<Vertex> vertices_;
   std::vector<Face> faces_;
 };

@@ -2,9 +2,11 @@
 // reference's application loop (src/main.cpp:236-283: load a frame, PushFrame -> Reproject ->
 // processFrame, then SavePly; src/PointCloudGenerator.cpp:19-49,93-143).
 //
-// usage: load_frames <out.ply> [n_frames]
+// usage: load_frames <out.ply> [n_frames] [--max-depth <m>] [--stream]
 //          frames (depth and colour) from the synthetic small-room generator
-//          (include/vh_synth.h)
+//          (include/vh_synth.h); --max-depth sets the depth range (default 5 m), --stream
+//          turns on the generator's out-of-core streaming (setStreaming, stream-out every
+//          frame with no margin) and prints the blocks streamed
 //        load_frames <out.ply> --dataset <dir> <settings.yaml> [first] [last]
 //          frames <dir>/depth/<id>.png (uint16) and <dir>/tcw/<id>.txt (camera-to-world) for
 //          id = first.. until last or the first missing depth image (SaveFrame::frameLoad,
@@ -28,13 +30,13 @@ namespace {
 
 int usage(const char* argv0) {
   std::fprintf(stderr,
-               "usage: %s <out.ply> [n_frames]\n"
+               "usage: %s <out.ply> [n_frames] [--max-depth <m>] [--stream]\n"
                "       %s <out.ply> --dataset <dir> <settings.yaml> [first] [last]\n",
                argv0, argv0);
   return 2;
 }
 
-int run_synthetic(const char* out, int n_arg) {
+int run_synthetic(const char* out, int n_arg, float max_depth, bool stream) {
   vh_scene* scene = nullptr;
   if (vh_scene_create(VH_SCENE_ROOM_SMALL, 20240611ull, &scene) != 0) return 1;
   vh_scene_info info;
@@ -43,8 +45,9 @@ int run_synthetic(const char* out, int n_arg) {
   int rc = 0;
   try {
     // voxel 1 cm, truncation 4 cm, as the small-room workload
-    ark::GpuTsdfGenerator gen(info.width, info.height, info.fx, info.fy, info.cx, info.cy, 5.0f, -1.5f, -1.5f,
+    ark::GpuTsdfGenerator gen(info.width, info.height, info.fx, info.fy, info.cx, info.cy, max_depth, -1.5f, -1.5f,
                               0.5f, 0.01f, 0.04f, 500, 500, 500);
+    if (stream) gen.setStreaming(true, 1, 0.0f);
     std::vector<uint16_t> mm((size_t)info.width * info.height);
     std::vector<float> depth(mm.size());
     std::vector<unsigned char> rgb(mm.size() * 3);
@@ -56,8 +59,11 @@ int run_synthetic(const char* out, int n_arg) {
       for (size_t i = 0; i < mm.size(); ++i) depth[i] = (float)mm[i] * 0.001f;  // as SaveFrame.cpp:179-180
       gen.processFrame(depth.data(), rgb.data(), c2w);
     }
+    vh_stats st;
+    if (vh_get_stats(gen.volume(), &st) != 0) throw std::runtime_error(vh_last_error());
     gen.SavePLY(out);
-    std::printf("frames %d vertices %zu faces %zu\n", n, gen.getVertices()->size(), gen.getFaces()->size());
+    std::printf("frames %d vertices %zu faces %zu streamed_out %lld\n", n, gen.getVertices()->size(),
+                gen.getFaces()->size(), (long long)st.streamed_out_total);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "load_frames: %s\n", e.what());
     rc = std::strstr(e.what(), "no HIP device") ? 3 : 1;
@@ -115,5 +121,14 @@ int main(int argc, char** argv) {
     const int last = argc > 6 ? std::atoi(argv[6]) : -1;
     return run_dataset(argv[1], argv[3], argv[4], first, last);
   }
-  return run_synthetic(argv[1], argc > 2 ? std::atoi(argv[2]) : 0);
+  int n = 0;
+  float max_depth = 5.0f;
+  bool stream = false;
+  for (int i = 2; i < argc; ++i) {
+    if (std::strcmp(argv[i], "--stream") == 0) stream = true;
+    else if (std::strcmp(argv[i], "--max-depth") == 0 && i + 1 < argc) max_depth = (float)std::atof(argv[++i]);
+    else if (argv[i][0] != '-') n = std::atoi(argv[i]);
+    else return usage(argv[0]);
+  }
+  return run_synthetic(argv[1], n, max_depth, stream);
 }
