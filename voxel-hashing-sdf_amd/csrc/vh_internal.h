@@ -52,6 +52,15 @@ __device__ __forceinline__ float div_rn(float a, float b) {
   return q;
 }
 
+// div_rn for dividends that are exact non-negative integers (0 or >= 1; the colour update's
+// c * w + rgb): the quotient by b <= 2^25 is then 0 or at least 2^-25, never subnormal, so
+// the correction alone is exact and there is no fallback test.
+__device__ __forceinline__ float div_rn_int(float a, float b) {
+  const float y = recip_rn(b);
+  const float q0 = a * y;
+  return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+}
+
 // div_rn for dividends that are rarely exactly +0 and never -0 (the TSDF mean's s*w + dist:
 // dist is never -0, and x + y is -0 only when both are): a = +0 takes the fallback, whose
 // quotient +0 is the one the correction gives, so the zero test can go.

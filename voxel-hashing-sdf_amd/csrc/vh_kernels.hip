@@ -265,7 +265,7 @@ __device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, uint32
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float num = __builtin_fmaf((float)((c >> (8 * k)) & 255u), w_old, (float)((px >> (8 * k)) & 255u));
-    out |= (uint32_t)(uint8_t)div_rn(num, w_new) << (8 * k);
+    out |= (uint32_t)(uint8_t)div_rn_int(num, w_new) << (8 * k);
   }
   return out;
 }
