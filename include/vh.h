@@ -11,7 +11,8 @@
  * Conventions (kept from the reference):
  *   - depth is H x W row-major, metres (float) or uint16 scaled by depth_scale
  *     (SaveFrame.cpp:179-180 uses 0.001); 0 marks an invalid pixel; a pixel is used iff
- *     0 < d <= max_depth (src/tsdf.cu:715);
+ *     0 < d <= max_depth (src/tsdf.cu:713), so NaN is never used (the reference's form
+ *     of the test lets NaN through; its 16-bit depth inputs never hold one);
  *   - c2w is a row-major 4x4 camera-to-world matrix (PointCloudGenerator.cpp:118-125);
  *   - voxel (i,j,k) sits at (i,j,k)*vox_size, blocks are 8^3 voxels, in-block order
  *     (x*8+y)*8+z (src/tsdf.cu:24-64);

@@ -209,6 +209,45 @@ def test_float_depth_input_matches_u16(vh, oracle, wl, small_frames):
     vol.close()
 
 
+def test_non_finite_and_non_positive_float_depth_is_no_measurement(vh, oracle, wl, small_frames):
+    """Float depth pixels that are NaN, +-inf, negative, -0 or beyond max_depth are no
+    measurement, exactly like 0: the volume equals the oracle's given 0 at those pixels, on
+    the host path and through a bound device sequence (batched graph launches). The
+    reference's rejection test `d <= 0 || d > max_depth` (src/tsdf.cu:713) would let NaN
+    through; its inputs (16-bit PNG depth) never hold NaN."""
+    import torch
+
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    n = 6
+    rng = np.random.default_rng(713)
+    specials = np.array([np.nan, np.inf, -np.inf, -0.5, -0.0, w["max_depth"] + 1.0], np.float32)
+    fd = depth[:n].astype(np.float32) * np.float32(0.001)
+    u = depth[:n].copy()
+    pick = rng.random(fd.shape) < 0.12
+    fd[pick] = specials[rng.integers(0, len(specials), int(pick.sum()))]
+    u[pick] = 0
+    print(f"room_small_1cm: {n} float frames, {int(pick.sum())} special pixels", flush=True)
+    ref = oracle.Oracle(w)
+    for k in range(n):
+        ref.integrate(u[k], poses[k])
+    vol = vh.Volume(vh.config_from_workload(w))
+    for k in range(n):
+        vol.integrate(fd[k], poses[k])
+    assert_same_volume(oracle, vol, ref)
+    assert_same_mesh(vol, ref)
+    vol.close()
+    d_depth = torch.from_numpy(fd).cuda()
+    d_pose = torch.from_numpy(poses[:n].copy()).cuda()
+    torch.cuda.synchronize()
+    seq = vh.Volume(vh.config_from_workload(w))
+    seq.bind_sequence(d_depth, vh.DEPTH_F32, 1.0, d_pose, n)
+    seq.integrate_next(n)
+    seq.sync()
+    assert_same_volume(oracle, seq, ref)
+    seq.close()
+
+
 def test_pitched_depth_rows(vh, oracle, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
