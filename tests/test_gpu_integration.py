@@ -248,6 +248,40 @@ def test_non_finite_and_non_positive_float_depth_is_no_measurement(vh, oracle, w
     seq.close()
 
 
+def test_odd_image_size_partial_tiles(vh, oracle, wl, small_frames):
+    """A 203 x 151 window of the 640 x 480 frames (principal point shifted by the window's
+    origin): neither dimension is a multiple of the allocation tiles (16 x 16 for single
+    frames, 16 x 64 in batched launches), so edge tiles are partial in both directions.
+    Single frames and a bound sequence in batches of 8 equal the oracle bit for bit."""
+    import torch
+
+    w = dict(wl.WORKLOADS["room_small_1cm"])
+    depth, poses = small_frames
+    u0, v0, W, H = 211, 167, 203, 151
+    w.update(width=W, height=H, cx=w["cx"] - u0, cy=w["cy"] - v0)
+    n = 12
+    crop = np.ascontiguousarray(depth[:n, v0:v0 + H, u0:u0 + W])
+    print(f"room_small_1cm window {W}x{H} at ({u0}, {v0}): {n} frames", flush=True)
+    ref = oracle.Oracle(w)
+    vol = vh.Volume(vh.config_from_workload(w))
+    for k in range(n):
+        ref.integrate(crop[k], poses[k])
+        vol.integrate(crop[k], poses[k])
+    keys, _, _, _ = assert_same_volume(oracle, vol, ref)
+    assert len(keys) > 100  # the window sees one wall: 113 blocks of 8 cm
+    assert_same_mesh(vol, ref)
+    vol.close()
+    d_depth = torch.from_numpy(crop.view(np.int16).copy()).cuda()
+    d_pose = torch.from_numpy(poses[:n].copy()).cuda()
+    torch.cuda.synchronize()
+    seq = vh.Volume(vh.config_from_workload(w, seq_batch=8))
+    seq.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, n)
+    seq.integrate_next(n)
+    seq.sync()
+    assert_same_volume(oracle, seq, ref)
+    seq.close()
+
+
 def test_pitched_depth_rows(vh, oracle, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
