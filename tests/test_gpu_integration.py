@@ -282,6 +282,28 @@ def test_odd_image_size_partial_tiles(vh, oracle, wl, small_frames):
     seq.close()
 
 
+def test_far_from_origin_near_the_key_range(vh, oracle, wl, small_frames):
+    """Every pose shifted by (83, -83, 41) km: block coordinates near +-1.04e6, close to the
+    allocation's limit of 1,048,000 blocks per axis (21-bit keys), lattice indices near
+    8.3e6. FP32 spacing there is ~8 mm, so the geometry is coarse, but the GPU and the oracle
+    take the same float inputs: block set, voxels and mesh equal bit for bit."""
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    n = 8
+    far = poses[:n].copy()
+    far[:, 0, 3] += np.float32(83000.0)
+    far[:, 1, 3] -= np.float32(83000.0)
+    far[:, 2, 3] += np.float32(41000.0)
+    print("room_small_1cm: 8 frames at (83, -83, 41) km", flush=True)
+    vol, ref, touched = run_pair(vh, oracle, w, depth[:n], far)
+    assert all(a == b for a, b in touched)
+    keys, _, _, _ = assert_same_volume(oracle, vol, ref)
+    assert len(keys) > 500  # 574: the first frames face one wall
+    assert keys[:, 0].min() > 1_030_000 and keys[:, 1].max() < -1_030_000
+    assert assert_same_mesh(vol, ref) > 1000
+    vol.close()
+
+
 def test_pitched_depth_rows(vh, oracle, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
