@@ -123,6 +123,44 @@ def test_async_sequence_and_tracked_colour_paths(vh, oracle, wl, small_frames):
     vol.close()
 
 
+def test_colour_odd_image_size_batched(vh, oracle, wl, small_frames):
+    """A 203 x 151 window (principal point shifted by its origin) of depth and texture: the
+    allocation tiles, which also pack each pixel's RGB for the integration, are partial in
+    both directions. Host frames and a bound sequence in batches of 8 equal the oracle."""
+    import torch
+
+    w = dict(wl.WORKLOADS["room_small_1cm"])
+    depth, poses = small_frames
+    u0, v0, W, H = 211, 167, 203, 151
+    w.update(width=W, height=H, cx=w["cx"] - u0, cy=w["cy"] - v0)
+    n = 10
+    crop = np.ascontiguousarray(depth[:n, v0:v0 + H, u0:u0 + W])
+    rgb = np.ascontiguousarray(np.stack([texture(480, 640, f) for f in range(n)])[:, v0:v0 + H, u0:u0 + W])
+    print(f"colour window {W}x{H}: {n} frames, host and batched sequence", flush=True)
+    ref = oracle.Oracle(w, color=True)
+    for k in range(n):
+        ref.integrate(crop[k], poses[k], rgb=rgb[k])
+    cfg = vh.config_from_workload(w, color=1, seq_batch=8)
+    vol = vh.Volume(cfg)
+    for k in range(n):
+        vol.integrate(crop[k], poses[k], rgb=rgb[k])
+    assert_same_colour_volume(oracle, vol, ref)
+    assert_same_colour_mesh(vol, ref)
+    vol.close()
+    d_depth = torch.from_numpy(crop.view(np.int16).copy()).cuda()
+    d_pose = torch.from_numpy(poses[:n].copy()).cuda()
+    d_rgb = torch.from_numpy(rgb.copy()).cuda()
+    torch.cuda.synchronize()
+    seq = vh.Volume(cfg)
+    seq.bind_sequence(d_depth, vh.DEPTH_U16, 0.001, d_pose, n)
+    seq.bind_sequence_rgb(d_rgb)
+    seq.integrate_next(n)
+    seq.sync()
+    assert_same_colour_volume(oracle, seq, ref)
+    assert_same_colour_mesh(seq, ref)
+    seq.close()
+
+
 def test_colour_argument_errors(vh, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
