@@ -49,6 +49,14 @@ int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
   } while (0)
 
 constexpr int kIntegrateGrid = 8192;  // integration workers (waves, four per workgroup), a multiple of kStripes
+// Integration workers of the fused batch launch (k_integrate_alloc), whose allocation
+// workgroups are dispatched after the integration ones: fewer workers leave the allocation
+// more of the GPU. room_medium_5mm, integration-only frames/s by worker count (3,072 /
+// 4,096 / 8,192): FP32 63.8k / 62.6k / 59.6k, FP16 63.9k / 63.8k / 62.2k, F16R 51.1k /
+// 52.2k / 51.8k, FP32 + colour 40.6k / 39.6k / 38.9k, F16R + colour 35.4k / 36.5k / 36.0k;
+// FP32 with 2,048 workers 56.4k. F16R, whose voxel update costs more, takes 4,096.
+constexpr int kFusedGrid = 3072;
+constexpr int kFusedGridF16R = 4096;
 constexpr int kGnGrid = 512;  // most workgroups of a solver or backward reduction
 
 // Solver grid for n points: about two points per lane (a partial per workgroup is summed
@@ -58,17 +66,23 @@ int gn_grid(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, kGnGrid));
 }
 
-// Launch-shape tuning: VH_INTEGRATE_GRID overrides the integration grid (rounded to a
-// multiple of kStripes).
+// Launch-shape tuning: VH_INTEGRATE_GRID and VH_FUSED_GRID override the integration grids of
+// the integration launch and of the fused batch launch (rounded to a multiple of kStripes).
+int grid_setting(const char* name, long dflt) {
+  const char* e = std::getenv(name);
+  long v = e ? std::strtol(e, nullptr, 10) : dflt;
+  if (v < kStripes) v = kStripes;
+  if (v > (1 << 20)) v = 1 << 20;
+  return (int)(v / kStripes * kStripes);
+}
 int integrate_grid() {
-  static const int g = [] {
-    const char* e = std::getenv("VH_INTEGRATE_GRID");
-    long v = e ? std::strtol(e, nullptr, 10) : kIntegrateGrid;
-    if (v < kStripes) v = kStripes;
-    if (v > (1 << 20)) v = 1 << 20;
-    return (int)(v / kStripes * kStripes);
-  }();
+  static const int g = grid_setting("VH_INTEGRATE_GRID", kIntegrateGrid);
   return g;
+}
+int fused_grid(int storage) {
+  static const int g = grid_setting("VH_FUSED_GRID", kFusedGrid);
+  static const int g16r = grid_setting("VH_FUSED_GRID", kFusedGridF16R);
+  return storage == VH_STORAGE_F16R ? g16r : g;
 }
 
 }  // namespace
@@ -308,7 +322,7 @@ int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale, int ki, i
   const int u16 = frame_u16(v, fa);
   if (fa < 2 || fa >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
   if (which == 2) {
-    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, integrate_grid(), ki, ka, v->stream));
+    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, fused_grid(v->cfg.storage), ki, ka, v->stream));
     return VH_OK;
   }
   if (which == 3) {
