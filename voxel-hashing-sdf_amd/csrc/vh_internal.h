@@ -347,7 +347,8 @@ hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStr
 hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st);
 hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand,
                            int final_step, double* stats_row, hipStream_t st);
-// Gauss-Newton state reset + prior pose of the current frame; depth back-projection.
+// Gauss-Newton state reset + prior pose of the current frame, and the depth back-projection,
+// in one launch.
 hipError_t launch_track_begin(const TrackArgs& t, hipStream_t st);
 // Accepted pose (and the frame's solver stats) to the per-frame outputs.
 hipError_t launch_track_end(const TrackArgs& t, hipStream_t st);

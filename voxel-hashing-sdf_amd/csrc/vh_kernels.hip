@@ -1717,12 +1717,11 @@ __global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int gr
 // ============================================================================ tracking
 __device__ __forceinline__ uint32_t track_slot(const TrackArgs& t) { return t.ctl->cursor[1] % t.n_seq; }
 
-// One workgroup: the frame's starting pose becomes the candidate; the solver state is
-// cleared (a kernel, not a memset node, so the captured graph replays it exactly). With
-// increments, start = out[f-1] * prior[f] in FP64, each entry summed in column order
-// ((a0 b0 + a1 b1) + a2 b2) + a3 b3.
-__global__ __launch_bounds__(64) void k_track_begin(TrackArgs t) {
-  const uint32_t fs = track_slot(t);
+// The frame's starting pose becomes the candidate; the solver state is cleared (a kernel,
+// not a memset node, so the captured graph replays it exactly). With increments, start =
+// out[f-1] * prior[f] in FP64, each entry summed in column order ((a0 b0 + a1 b1) + a2 b2) +
+// a3 b3. Lanes 0..11 of workgroup 0 of k_track_begin.
+__device__ void track_start(const TrackArgs& t, uint32_t fs) {
   const int k = threadIdx.x;
   const float* P = t.prior + 16 * (size_t)fs;
   if (k < 12) {
@@ -1742,13 +1741,16 @@ __global__ __launch_bounds__(64) void k_track_begin(TrackArgs t) {
   }
 }
 
-// Camera-frame point of every stride-th pixel: ((u - cx) d / fx, (v - cy) d / fy, d) in FP32
-// with this operation order; pixels without a valid depth give NaN (skipped by the solver).
+// Start of a tracked frame: workgroup 0 sets the starting pose (track_start), and every lane
+// back-projects one stride-th pixel to its camera-frame point ((u - cx) d / fx,
+// (v - cy) d / fy, d) in FP32 with this operation order; pixels without a valid depth give
+// NaN (skipped by the solver).
 template <bool kU16>
-__global__ __launch_bounds__(256) void k_backproject(TrackArgs t) {
+__global__ __launch_bounds__(256) void k_track_begin(TrackArgs t) {
+  const uint32_t fs = track_slot(t);
+  if (blockIdx.x == 0) track_start(t, fs);
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= t.pw * t.ph) return;
-  const uint32_t fs = track_slot(t);
   const int u = (i % t.pw) * t.stride, v = (i / t.pw) * t.stride;
   const size_t elem = kU16 ? 2 : 4;
   const void* base = reinterpret_cast<const char*>(t.depth) + (size_t)fs * t.W * t.H * elem;
@@ -1966,12 +1968,10 @@ hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, dou
 }
 
 hipError_t launch_track_begin(const TrackArgs& t, hipStream_t st) {
-  hipLaunchKernelGGL(k_track_begin, dim3(1), dim3(64), 0, st, t);
   const int n = t.pw * t.ph;
-  if (t.depth_u16)
-    hipLaunchKernelGGL(k_backproject<true>, dim3((n + 255) / 256), dim3(256), 0, st, t);
-  else
-    hipLaunchKernelGGL(k_backproject<false>, dim3((n + 255) / 256), dim3(256), 0, st, t);
+  const dim3 g((n + 255) / 256), b(256);
+  if (t.depth_u16) hipLaunchKernelGGL(k_track_begin<true>, g, b, 0, st, t);
+  else hipLaunchKernelGGL(k_track_begin<false>, g, b, 0, st, t);
   return hipGetLastError();
 }
 
