@@ -249,8 +249,8 @@ def main():
                     help="with --counters: one marching-cubes extraction after the integration pass")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the volume is not sharded across GPUs in this build "
-              "(it would shard by space, one owner GPU per block)", file=sys.stderr)
+        print("bench.py runs on one GPU only: libvh does not shard a volume across GPUs "
+              "(sharding would be by space, one owner GPU per block)", file=sys.stderr)
         sys.exit(2)
 
     import torch
