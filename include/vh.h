@@ -347,6 +347,16 @@ int vh_query_sdf_backward_device(vh_volume* vol, const float* d_pts, int64_t n,
  * No reference counterpart (the reference takes poses as given). */
 int vh_gn_refine(vh_volume* vol, const float* pts, int64_t n, double c2w[16], int32_t iters,
                  vh_gn_stats* stats);
+/* vh_gn_refine over device-resident points (d_pts, n x 3 floats) that allocates no device
+ * memory: no reference counterpart; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving or tracking loops, harnesses that forbid hipMalloc). The solver's
+ * scratch (partial sums, candidate pose, state, stats rows; 132 KB) is a static device array,
+ * one copy per GPU held until the process ends. Limit: iters <= 63 (VH_ERR_INVALID_ARG
+ * beyond); any n. Same kernels and results as vh_gn_refine on the same points. Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU (from several threads or
+ * volumes) race and must be serialised by the caller. Synchronous. */
+int vh_gn_refine_device_noalloc(vh_volume* vol, const float* d_pts, int64_t n, double c2w[16],
+                                int32_t iters, vh_gn_stats* stats);
 
 /* Pool layout for callers that wrap the voxel memory (e.g. as a tensor): device pointer of
  * the voxel pool (pool_blocks*512 voxels; FP32: float2 {tsdf, weight}; FP16: uint32

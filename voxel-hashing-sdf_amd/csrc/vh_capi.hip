@@ -868,6 +868,86 @@ int ensure_gn_scratch(vh_volume* v) {
   return dev_alloc(&v->gn_stats, (size_t)kGnStats * 65 * sizeof(double));
 }
 
+// Device scratch of one pose refinement: per-workgroup partial sums, the candidate pose, the
+// solver state and one stats row per evaluation.
+struct GnScratch {
+  double* partials;  // kGnGrid * kGnAcc
+  double* cand;      // 16
+  GnState* state;
+  double* stats;     // kGnStats * (iters + 1)
+};
+
+// iters Levenberg-Marquardt steps over n device points starting from c2w, then a last
+// evaluation; the accepted pose is written back to c2w (FP64, bottom row 0 0 0 1) and the
+// per-evaluation figures to *stats (may be NULL). Synchronous; allocates nothing.
+int run_gn_refine(vh_volume* v, const float* d_pts, int64_t n, double c2w[16], int32_t iters, vh_gn_stats* stats,
+                  const GnScratch& s) {
+  GnStart start{};
+  for (int k = 0; k < 12; ++k) start.pose[k] = c2w[k];
+  start.lambda = 1e-4;
+  VH_HIP(launch_gn_start(start, s.state, s.cand, v->stream));
+  GnArgs a{};
+  a.q = query_args(v);
+  a.pts = d_pts;
+  a.n = n;
+  a.pose = s.cand;  // each accumulation evaluates the current candidate
+  a.partials = s.partials;
+  for (int it = 0; it <= iters; ++it) {
+    VH_HIP(launch_gn_accum(a, v->cfg.storage, gn_grid(n), v->stream));
+    VH_HIP(launch_gn_solve(s.partials, gn_grid(n), s.state, s.cand, it == iters ? 1 : 0, s.stats + kGnStats * it,
+                           v->stream));
+  }
+  double host_stats[kGnStats * 64];
+  VH_HIP(hipMemcpyAsync(c2w, s.cand, 12 * sizeof(double), hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipMemcpyAsync(host_stats, s.stats, (size_t)kGnStats * (iters + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                        v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  c2w[12] = c2w[13] = c2w[14] = 0.0;
+  c2w[15] = 1.0;
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->iters = iters;
+    for (int it = 0; it <= iters && it < 64; ++it) {
+      stats->n_valid[it] = (int32_t)host_stats[kGnStats * it];
+      stats->rms[it] = host_stats[kGnStats * it + 1];
+      stats->step_norm[it] = host_stats[kGnStats * it + 2];
+      for (int k = 0; k < 12; ++k) stats->pose[it][k] = host_stats[kGnStats * it + 5 + k];
+    }
+  }
+  return VH_OK;
+}
+
+// Solver scratch of vh_gn_refine_device_noalloc: a file-scope device array, one copy per GPU
+// (each GPU holds it, 132 KB, until the process ends), laid out as GnScratch with room for
+// kNoAllocMaxIters + 1 stats rows. Host code reaches it only through the per-device address
+// that hipGetSymbolAddress returns; the array's name is not a device address on the host.
+constexpr int kNoAllocMaxIters = 63;
+constexpr size_t kScrPartials = 0;
+constexpr size_t kScrCand = kScrPartials + (size_t)kGnGrid * kGnAcc;
+constexpr size_t kScrState = kScrCand + 16;
+constexpr size_t kScrStats = kScrState + 64;
+constexpr size_t kScrDoubles = kScrStats + (size_t)kGnStats * (kNoAllocMaxIters + 1);
+static_assert(sizeof(GnState) <= 64 * sizeof(double), "GnState outgrew its scratch slot");
+static __device__ __align__(256) double scratch[kScrDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved once per device).
+int scratch_for_device(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(VH_ERR_INVALID_ARG, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    VH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(VH_ERR_DEVICE, "hipGetSymbolAddress returned no address for the solver scratch");
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return VH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1533,41 +1613,25 @@ int vh_gn_refine(vh_volume* v, const float* pts, int64_t n, double c2w[16], int3
     ~Guard() { if (p) (void)hipFree(p); }
   } guard{d_pts};
   VH_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 12, hipMemcpyHostToDevice, v->stream));
-  VH_HIP(hipMemcpyAsync(v->gn_pose, c2w, 12 * sizeof(double), hipMemcpyHostToDevice, v->stream));
-  GnArgs a{};
-  a.q = query_args(v);
-  a.pts = d_pts;
-  a.n = n;
-  a.pose = v->gn_pose;  // the candidate pose: each accumulation evaluates it
-  a.partials = v->gn_partials;
-  GnState init{};
-  init.lambda = 1e-4;
-  VH_HIP(hipMemcpyAsync(v->gn_state, &init, sizeof init, hipMemcpyHostToDevice, v->stream));
-  // iters LM steps, then one evaluation of the last candidate; the result is the best
-  // accepted pose
-  for (int it = 0; it <= iters; ++it) {
-    VH_HIP(launch_gn_accum(a, v->cfg.storage, gn_grid(n), v->stream));
-    VH_HIP(launch_gn_solve(v->gn_partials, gn_grid(n), v->gn_state, v->gn_pose, it == iters ? 1 : 0,
-                           v->gn_stats + kGnStats * it, v->stream));
-  }
-  std::vector<double> host_stats((size_t)kGnStats * (iters + 1));
-  VH_HIP(hipMemcpyAsync(c2w, v->gn_pose, 12 * sizeof(double), hipMemcpyDeviceToHost, v->stream));
-  VH_HIP(hipMemcpyAsync(host_stats.data(), v->gn_stats, host_stats.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        v->stream));
-  VH_HIP(hipStreamSynchronize(v->stream));
-  c2w[12] = c2w[13] = c2w[14] = 0.0;
-  c2w[15] = 1.0;
-  if (stats) {
-    std::memset(stats, 0, sizeof *stats);
-    stats->iters = iters;
-    for (int it = 0; it <= iters && it < 64; ++it) {
-      stats->n_valid[it] = (int32_t)host_stats[kGnStats * it];
-      stats->rms[it] = host_stats[kGnStats * it + 1];
-      stats->step_norm[it] = host_stats[kGnStats * it + 2];
-      for (int k = 0; k < 12; ++k) stats->pose[it][k] = host_stats[kGnStats * it + 5 + k];
-    }
-  }
-  return VH_OK;
+  return run_gn_refine(v, d_pts, n, c2w, iters, stats,
+                       GnScratch{v->gn_partials, v->gn_pose, v->gn_state, v->gn_stats});
+}
+
+int vh_gn_refine_device_noalloc(vh_volume* v, const float* d_pts, int64_t n, double c2w[16], int32_t iters,
+                                vh_gn_stats* stats) {
+  if (!v || !d_pts || !c2w || n <= 0 || iters < 0) return fail(VH_ERR_INVALID_ARG, "bad argument");
+  if (iters > kNoAllocMaxIters)
+    return fail(VH_ERR_INVALID_ARG, "iters exceeds 63, the most the static solver scratch holds");
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(c2w[k])) return fail(VH_ERR_INVALID_ARG, "pose has a non-finite entry");
+  std::lock_guard<std::mutex> lk(v->mu);
+  int rc = set_device(v);
+  if (rc) return rc;
+  double* base = nullptr;
+  if ((rc = scratch_for_device(v->device, &base))) return rc;
+  return run_gn_refine(v, d_pts, n, c2w, iters, stats,
+                       GnScratch{base + kScrPartials, base + kScrCand, reinterpret_cast<GnState*>(base + kScrState),
+                                 base + kScrStats});
 }
 
 int vh_pool_view(vh_volume* v, void** d_voxels, void** d_block_keys, int64_t* n_blocks) {

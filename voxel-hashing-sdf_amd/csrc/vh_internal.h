@@ -279,6 +279,12 @@ struct GnState {
 // per-evaluation stats: n_valid, rms, |delta|, lambda, accepted, accepted pose (12)
 constexpr int kGnStats = 17;
 
+// Starting pose (row-major 3x4) and damping of a refinement (k_gn_start).
+struct GnStart {
+  double pose[12];
+  double lambda;
+};
+
 struct GnArgs {
   QueryArgs q;
   const float* pts;   // camera-frame points (a non-finite coordinate marks a point to skip)
@@ -347,6 +353,8 @@ hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStr
 hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st);
 hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand,
                            int final_step, double* stats_row, hipStream_t st);
+// cand = s.pose, solver state cleared with damping s.lambda.
+hipError_t launch_gn_start(const GnStart& s, GnState* state, double* cand, hipStream_t st);
 // Gauss-Newton state reset + prior pose of the current frame, and the depth back-projection,
 // in one launch.
 hipError_t launch_track_begin(const TrackArgs& t, hipStream_t st);

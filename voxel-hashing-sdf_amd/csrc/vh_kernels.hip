@@ -1714,6 +1714,19 @@ __global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int gr
   for (int k = 0; k < kGnStats; ++k) stats_row[k] = row[k];
 }
 
+// Start of a refinement: the candidate is the starting pose and the solver state is cleared
+// with the initial damping. The pose comes in by value (kernel argument), so starting a
+// refinement needs no host-to-device copy.
+__global__ __launch_bounds__(64) void k_gn_start(GnStart s, GnState* st, double* cand) {
+  const int k = threadIdx.x;
+  if (k < 12) cand[k] = s.pose[k];
+  if (k == 0) {
+    GnState z{};
+    z.lambda = s.lambda;
+    *st = z;
+  }
+}
+
 // ============================================================================ tracking
 __device__ __forceinline__ uint32_t track_slot(const TrackArgs& t) { return t.ctl->cursor[1] % t.n_seq; }
 
@@ -1964,6 +1977,11 @@ hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t s
 hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand, int final_step,
                            double* stats_row, hipStream_t st) {
   hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(256), 0, st, partials, grid, state, cand, final_step, stats_row);
+  return hipGetLastError();
+}
+
+hipError_t launch_gn_start(const GnStart& s, GnState* state, double* cand, hipStream_t st) {
+  hipLaunchKernelGGL(k_gn_start, dim3(1), dim3(64), 0, st, s, state, cand);
   return hipGetLastError();
 }
 

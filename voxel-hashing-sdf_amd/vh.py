@@ -154,6 +154,8 @@ _PROTOS = [
     ("vh_query_sdf_backward_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("vh_gn_refine", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.POINTER(GnStats)]),
+    ("vh_gn_refine_device_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                                              C.POINTER(GnStats)]),
     ("vh_pool_view", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     ("vh_settings_load", C.c_int, [C.c_char_p, C.POINTER(Settings)]),
     ("vh_settings_to_config", C.c_int, [C.POINTER(Settings), C.POINTER(Config)]),
@@ -518,11 +520,22 @@ class Volume:
         T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float64).reshape(16)).copy()
         st = GnStats()
         _check(_lib.vh_gn_refine(self._h, _ptr(pts), pts.shape[0], _ptr(T), iters, C.byref(st)))
+        return T.reshape(4, 4), self._gn_stats(st, iters)
+
+    def gn_refine_device_noalloc(self, d_pts, n: int, c2w, iters: int):
+        """vh_gn_refine_device_noalloc: the refinement over device points (float32 tensor of
+        n x 3), with the solver scratch in the library's static device array."""
+        T = np.ascontiguousarray(np.asarray(c2w, dtype=np.float64).reshape(16)).copy()
+        st = GnStats()
+        _check(_lib.vh_gn_refine_device_noalloc(self._h, _tptr(d_pts), n, _ptr(T), iters, C.byref(st)))
+        return T.reshape(4, 4), self._gn_stats(st, iters)
+
+    @staticmethod
+    def _gn_stats(st, iters):
         k = iters + 1
-        stats = {"n_valid": list(st.n_valid[:k]), "rms": list(st.rms[:k]), "step_norm": list(st.step_norm[:k]),
-                 "pose": [np.vstack([np.array(st.pose[i][:], np.float64).reshape(3, 4), [0, 0, 0, 1]])
-                          for i in range(k)]}
-        return T.reshape(4, 4), stats
+        return {"n_valid": list(st.n_valid[:k]), "rms": list(st.rms[:k]), "step_norm": list(st.step_norm[:k]),
+                "pose": [np.vstack([np.array(st.pose[i][:], np.float64).reshape(3, 4), [0, 0, 0, 1]])
+                         for i in range(k)]}
 
     def pool_view(self):
         vox = C.c_void_p()
