@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 5
+#define VH_ABI_VERSION 6
 
 typedef enum vh_status {
   VH_OK = 0,
@@ -86,7 +86,14 @@ typedef struct vh_config {
                             each block's voxels are read and written once per batch and
                             updated by every frame of the batch that touched it, in frame
                             order (the same result as frame by frame) */
-  int32_t reserved[6];   /* zero */
+  int32_t carve;         /* 1: each frame updates every allocated voxel it observes (projects into
+                            the image at a depth 0 < d <= max_depth with d - z > -trunc), so
+                            free space in front of the surface is carved toward +1, as the
+                            reference's whole-ray block selection does (src/tsdf.cu:2128-2177,
+                            isBlockInCameraFrustum :2056-2064); 0: only the blocks crossed by the
+                            frame's truncation bands are updated (faster; other blocks keep
+                            their values) */
+  int32_t reserved[5];   /* zero */
 } vh_config;
 
 /* One input frame in host memory (replaces processFrame's float* depth, uchar* rgb,
@@ -123,6 +130,9 @@ typedef struct vh_stats {
   int64_t streamed_out_total; /* blocks moved to the host store since create/reset */
   int64_t streamed_in_total;  /* blocks moved back into the pool since create/reset */
   int64_t stream_merged_total;/* of those, blocks merged into a resident block of the same key */
+  int64_t carved_total;       /* carving volumes: block-frame updates added by carving (a frame
+                                 observes a block that its truncation bands did not cross) */
+  double carve_ms;            /* profiling mode: summed carve-pass time */
 } vh_stats;
 
 typedef enum vh_prior_mode {

@@ -33,6 +33,7 @@ def load():
     lib.vho_integrate.argtypes = [P, P, C.c_float, P]
     lib.vho_num_blocks.restype = C.c_int64
     lib.vho_num_blocks.argtypes = [P]
+    lib.vho_set_carve.argtypes = [P, C.c_int]
     lib.vho_touched.restype = C.c_int64
     lib.vho_touched.argtypes = [P, P, C.c_int64]
     lib.vho_export.restype = C.c_int64
@@ -87,9 +88,10 @@ def sort_blocks(keys, *arrays):
 
 
 class Oracle:
-    """color: a colour volume (integrate then needs rgb)."""
+    """color: a colour volume (integrate then needs rgb); carve: every allocated voxel a frame
+    observes is updated (default: the workload's "carve" key, else off)."""
 
-    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0, color=False):
+    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0, color=False, carve=None):
         lib = load()
         if threads:
             lib.vho_set_threads(threads)
@@ -101,6 +103,8 @@ class Oracle:
         self.color = bool(color)
         if self.color:
             lib.vho_set_color(self._h, 1)
+        self.carve = int(w.get("carve", 0) if carve is None else carve)  # 2: no culling bound
+        lib.vho_set_carve(self._h, self.carve)
 
     def __del__(self):
         if getattr(self, "_h", None):

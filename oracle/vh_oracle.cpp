@@ -139,6 +139,7 @@ struct Oracle {
   std::vector<int8_t> r8;                      // compensated FP16: rounding residual
   std::vector<double> s64, w64;                // FP64 shadow
   bool color = false;
+  int carve = 0;  // 1: every allocated block may_observe keeps is a candidate each frame; 2: every one
   std::vector<uint8_t> rgb8;                   // colour volumes: 3 channels per voxel
   std::vector<int64_t> touched;                // last frame, sorted keys
   int64_t frames = 0;
@@ -229,6 +230,43 @@ void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit em
 }
 
 // ------------------------------------------------------------------ integration
+// Carving (every allocated voxel the frame observes is updated; the reference integrates every
+// existing block found along each sampled pixel ray from 0.1 m to max_depth inside its frustum
+// test, src/tsdf.cu:2128-2177, :2056-2064, so free space in front of the surface is updated).
+// A cheap bound first: false only if no voxel of the block can pass the observation test of
+// integrate() below (z >= 2^-64 and a rounded pixel inside the image, then d - z > -mu for a
+// depth d <= max_depth). Exact FP64 camera coordinates of the 8 corners of the block's lattice
+// box bound every voxel's (the map is affine, and for a box wholly in front of the camera the
+// image of the box is the hull of its corners' images); the margins are far wider than the
+// rounding of the FP32 voxel arithmetic: 1 mm + 1e-5 of the magnitudes in depth, and in the
+// image 1 pixel plus 4e-6 of the magnitudes over the nearest depth times (f + |u - c|).
+bool may_observe(const Oracle& o, const double* T, int64_t key) {
+  int b[3];
+  key_xyz(key, b[0], b[1], b[2]);
+  double zmin = INFINITY, zmax = -INFINITY, mag = 0.0, P[8][3];
+  for (int c = 0; c < 8; ++c) {
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = ((double)b[k] * 8.0 + ((c >> k) & 1) * 7.0) * (double)o.vox - T[4 * k + 3];
+    for (int r = 0; r < 3; ++r) P[c][r] = T[r] * d[0] + T[4 + r] * d[1] + T[8 + r] * d[2];
+    mag = std::max(mag, std::fabs(d[0]) + std::fabs(d[1]) + std::fabs(d[2]));
+    zmin = std::min(zmin, P[c][2]);
+    zmax = std::max(zmax, P[c][2]);
+  }
+  const double dz = 1e-3 + 1e-5 * mag;
+  if (zmax < -dz || zmin > (double)o.max_depth + (double)o.mu + dz) return false;
+  if (!(zmin > dz)) return true;
+  double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+  for (int c = 0; c < 8; ++c) {
+    const double u = o.fx * P[c][0] / P[c][2] + o.cx, v = o.fy * P[c][1] / P[c][2] + o.cy;
+    umin = std::min(umin, u); umax = std::max(umax, u);
+    vmin = std::min(vmin, v); vmax = std::max(vmax, v);
+  }
+  const double rel = 4e-6 * mag / zmin;
+  const double eu = 1.0 + rel * (o.fx + std::max(std::fabs(umin - o.cx), std::fabs(umax - o.cx)));
+  const double ev = 1.0 + rel * (o.fy + std::max(std::fabs(vmin - o.cy), std::fabs(vmax - o.cy)));
+  return !(umax < -0.5 - eu || umin > o.W - 0.5 + eu || vmax < -0.5 - ev || vmin > o.H - 0.5 + ev);
+}
+
 // Colour of one observation (src/tsdf.cu:743-745): per channel (c * w_old + rgb) / (w_old + 1)
 // in FP32 (product and sum exact below 2^24), truncated to 8 bits; w_old is the stored weight
 // before the update.
@@ -272,15 +310,25 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, cons
     blk[k] = b >= 0 ? b : o.add_block(t[k]);
   }
 
-  // 2. voxels of the touched blocks
+  // 2. voxels of the touched blocks (carving: of every allocated block the frame may observe,
+  //    the ones just allocated included)
   const double T64[12] = {T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11]};
+  if (o.carve) {
+    const long nb = (long)o.keys.size();
+    std::vector<uint8_t> keep((size_t)nb);
+#pragma omp parallel for schedule(static, 4096)
+    for (long b = 0; b < nb; ++b) keep[(size_t)b] = (o.carve == 2 || may_observe(o, T64, o.keys[(size_t)b])) ? 1 : 0;
+    blk.clear();
+    for (long b = 0; b < nb; ++b)
+      if (keep[(size_t)b]) blk.push_back(b);
+  }
   const double vox64 = o.vox, mu64 = o.mu, scale64 = scale;
   const float capf = (float)o.cap;
   const float inv_mu = 1.0f / o.mu;
 #pragma omp parallel for schedule(dynamic, 4)
-  for (long k = 0; k < (long)t.size(); ++k) {
+  for (long k = 0; k < (long)blk.size(); ++k) {
     int bx, by, bz;
-    key_xyz(t[(size_t)k], bx, by, bz);
+    key_xyz(o.keys[(size_t)blk[(size_t)k]], bx, by, bz);
     const size_t base = (size_t)blk[(size_t)k] * 512;
     float cb[3];  // camera coordinates of the block origin, R^T (o - t)
     {
@@ -601,6 +649,10 @@ int64_t vho_export_colors(void* p, uint8_t* rgb, int64_t cap) {
   return n;
 }
 int64_t vho_num_blocks(void* p) { return (int64_t)static_cast<Oracle*>(p)->keys.size(); }
+// Carving on (1) or off (0): with carving every allocated voxel a frame observes is updated.
+// 2: carving with every allocated block tested voxel by voxel (no may_observe bound; the
+// brute-force check of that bound).
+void vho_set_carve(void* p, int mode) { static_cast<Oracle*>(p)->carve = (mode >= 0 && mode <= 2) ? mode : 0; }
 
 // Last frame's touched blocks, sorted by packed key; returns the count.
 int64_t vho_touched(void* p, int32_t* xyz, int64_t cap) {

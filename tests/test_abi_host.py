@@ -39,7 +39,7 @@ def test_every_declared_function_is_exported(vh):
 
 def test_abi_version_and_defaults(vh):
     lib = vh.load()
-    assert lib.vh_abi_version() == vh.ABI_VERSION == 5
+    assert lib.vh_abi_version() == vh.ABI_VERSION == 6
     c = vh.make_config()
     assert (c.width, c.height, c.log2_buckets, c.storage, c.color) == (640, 480, 18, 0, 0)
     assert C.sizeof(vh.Config) == 4 * (2 + 8 + 5 + 3 + 3 + 1 + 1 + 6)
@@ -48,7 +48,7 @@ def test_abi_version_and_defaults(vh):
 @pytest.mark.parametrize("field,value,msg", [
     ("width", 0, "width"), ("fx", 0.0, "fx"), ("vox_size", -1.0, "vox_size"), ("log2_buckets", 29, "log2_buckets"),
     ("pool_blocks", 0, "pool_blocks"), ("storage", 7, "storage"), ("weight_cap", -1, "weight_cap"),
-    ("color", 2, "color"), ("seq_batch", 9, "seq_batch"), ("seq_batch", -1, "seq_batch")])
+    ("color", 2, "color"), ("seq_batch", 9, "seq_batch"), ("seq_batch", -1, "seq_batch"), ("carve", 2, "carve")])
 def test_create_rejects_bad_config(vh, field, value, msg):
     lib = vh.load()
     c = vh.make_config(**{field: value})
@@ -61,7 +61,7 @@ def test_create_rejects_bad_config(vh, field, value, msg):
 def test_create_rejects_nonzero_reserved_words(vh):
     lib = vh.load()
     c = vh.make_config()
-    c.reserved[5] = 1
+    c.reserved[4] = 1
     h = C.c_void_p()
     assert lib.vh_create(C.byref(c), C.byref(h)) == vh.VH_ERR_INVALID_ARG
     assert "reserved" in lib.vh_last_error().decode()

@@ -47,8 +47,9 @@ def _assert_same_as_oracle(oracle, vol, ref, color):
     assert np.array_equal(cubes[o], rcubes[ro]) and np.array_equal(xyz[o].view(np.uint32), rxyz[ro].view(np.uint32))
 
 
-@pytest.mark.parametrize("storage,cap,color", [(0, 0, False), (2, 255, True)], ids=["f32", "f16r_colour"])
-def test_streamed_sequence_equals_oracle(vh, oracle, wl, storage, cap, color):
+@pytest.mark.parametrize("storage,cap,color,carve", [(0, 0, False, 0), (2, 255, True, 0), (0, 0, False, 1)],
+                         ids=["f32", "f16r_colour", "f32_carve"])
+def test_streamed_sequence_equals_oracle(vh, oracle, wl, storage, cap, color, carve):
     """room_medium_5mm, every 5th of its 500 frames (the camera orbits 1.15 m around the room
     centre), depth up to 2 m, in groups of 5 frames: before each group, blocks farther than
     the group's reach + 0.25 m go out and blocks within its reach come in."""
@@ -60,9 +61,9 @@ def test_streamed_sequence_equals_oracle(vh, oracle, wl, storage, cap, color):
     depth, poses = scene.frames(0, 500)
     group = 5
     print(f"streaming: room_medium_5mm, {len(frames)} frames, max depth {MAX_DEPTH} m, storage {storage}, "
-          f"colour {color}", flush=True)
-    vol = vh.Volume(vh.config_from_workload(w, storage=storage, weight_cap=cap, color=int(color)))
-    ref = oracle.Oracle(w, storage=storage, weight_cap=cap, color=color)
+          f"colour {color}, carve {carve}", flush=True)
+    vol = vh.Volume(vh.config_from_workload(w, storage=storage, weight_cap=cap, color=int(color), carve=carve))
+    ref = oracle.Oracle(w, storage=storage, weight_cap=cap, color=color, carve=carve)
     reach = (MAX_DEPTH + w["trunc"]) * _ray_factor(w) + 8 * w["vox_size"] * math.sqrt(3)
     moved_out = moved_in = 0
     peak_host = 0

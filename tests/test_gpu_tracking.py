@@ -41,13 +41,14 @@ def odometry_increments(poses, sigma_t, sigma_deg, seed):
     return out.astype(np.float32)
 
 
-@pytest.mark.parametrize("storage,mode", [(0, 0), (1, 0), (0, 1)])
-def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_frames, storage, mode):
+@pytest.mark.parametrize("storage,mode,carve", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
+def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_frames, storage, mode, carve):
     import torch
 
     w = dict(wl.WORKLOADS["room_small_1cm"])
     if storage:
         w.update(storage=storage, weight_cap=255)
+    w["carve"] = carve
     depth, poses = small_frames
     nf, iters, stride = 6, 4, 4
     depth, poses = depth[:nf], poses[:nf]
@@ -55,8 +56,8 @@ def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_fr
         prior = odometry_increments(poses, 0.005, 0.5, wl.SEED + 2)
     else:
         prior = noisy_priors(poses, 0.01, 1.0, wl.SEED + 2)
-    print(f"tracking: room_small_1cm storage {storage}, prior mode {mode}, {nf} frames, {iters} iterations, "
-          f"stride {stride}", flush=True)
+    print(f"tracking: room_small_1cm storage {storage}, prior mode {mode}, carve {carve}, {nf} frames, "
+          f"{iters} iterations, stride {stride}", flush=True)
     vol = vh.Volume(vh.config_from_workload(w))
     d_depth = torch.from_numpy(np.ascontiguousarray(depth).view(np.int16)).cuda()
     d_prior = torch.from_numpy(prior.reshape(nf, 16)).cuda()

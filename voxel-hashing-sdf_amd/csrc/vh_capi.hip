@@ -99,6 +99,8 @@ struct vh_volume {
   unsigned long long* vals = nullptr;
   uint32_t* marks = nullptr;  // 2 x n_slots
   unsigned long long* block_keys = nullptr;
+  unsigned long long* block_info = nullptr;  // per pool block: hash slot | birth frame << 32
+  float* tilemax = nullptr;    // carving volumes: 2 x kMaxBatch frames of max-depth tiles
   void* voxels = nullptr;
   uint32_t* colors = nullptr;  // colour volumes: one u32 per voxel
   uint32_t* rgba = nullptr;    // colour volumes: 2 packed frames of W*H words
@@ -145,9 +147,9 @@ struct vh_volume {
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<hipEvent_t> ev_pending;  // groups of 4: start, after tracking, after alloc, after integrate
+  std::vector<hipEvent_t> ev_pending;  // groups of 5: start, after tracking, after alloc, after carve, after integrate
   std::vector<int> ev_frames;          // frames of each group
-  double alloc_ms = 0.0, integrate_ms = 0.0, track_ms = 0.0;
+  double alloc_ms = 0.0, integrate_ms = 0.0, track_ms = 0.0, carve_ms = 0.0;
   int64_t profiled_frames = 0, profiled_launches = 0;
   // marching-cubes output buffer (grown on demand) and scan scratch
   float* mc_tris = nullptr;
@@ -242,6 +244,7 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.marks = v->marks;
   a.n_slots = v->n_slots;
   a.block_keys = v->block_keys;
+  a.block_info = v->block_info;
   a.pool_cap = (uint32_t)c.pool_blocks;
   a.touched = v->touched;
   a.stripe_cap = v->stripe_cap;
@@ -250,6 +253,9 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.rgba = v->rgba;
   if (!a.colors) a.rgb = nullptr;
   a.ctl = v->ctl;
+  a.tilemax = v->tilemax;  // null unless the volume carves
+  a.tiles_x = (c.width + kTile - 1) / kTile;
+  a.tiles_y = (c.height + kTile - 1) / kTile;
   return a;
 }
 
@@ -323,15 +329,21 @@ int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale, int ki, i
   FrameArgs a = frame_args(v, fa);
   const int u16 = frame_u16(v, fa);
   if (fa < 2 || fa >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
+  const bool carve = v->cfg.carve != 0;
   if (which == 2) {
     VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, fused_grid(v->cfg.storage), ki, ka, v->stream));
+    if (carve) VH_HIP(launch_carve(a, ka, v->stream));  // the batch just allocated
     return VH_OK;
   }
   if (which == 3) {
     int rc = enqueue_track_kernels(v);
     if (rc) return rc;
   }
-  if (which != 5) VH_HIP(launch_alloc(a, u16, which == 4 ? ka : 1, v->stream));
+  if (which != 5) {
+    const int k = which == 4 ? ka : 1;
+    VH_HIP(launch_alloc(a, u16, k, v->stream));
+    if (carve) VH_HIP(launch_carve(a, k, v->stream));
+  }
   if (which != 4) VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), which == 5 ? ki : 1, v->stream));
   return VH_OK;
 }
@@ -352,8 +364,8 @@ hipEvent_t take_event(vh_volume* v) {
 // frames and integration of k frames, with events around each.
 int launch_frame(vh_volume* v, int which, float staged_scale, int ki = 1, int ka = 1) {
   if (v->profiling) {
-    hipEvent_t e[4] = {take_event(v), take_event(v), take_event(v), take_event(v)};
-    if (!e[0] || !e[1] || !e[2] || !e[3]) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
+    hipEvent_t e[5] = {take_event(v), take_event(v), take_event(v), take_event(v), take_event(v)};
+    if (!e[0] || !e[1] || !e[2] || !e[3] || !e[4]) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
     FrameArgs a = frame_args(v, which);
     const int u16 = frame_u16(v, which);
     if (which < 2 || which >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
@@ -366,8 +378,10 @@ int launch_frame(vh_volume* v, int which, float staged_scale, int ki = 1, int ka
     VH_HIP(hipEventRecord(e[1], v->stream));
     VH_HIP(launch_alloc(a, u16, k, v->stream));
     VH_HIP(hipEventRecord(e[2], v->stream));
-    VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), k, v->stream));
+    if (v->cfg.carve) VH_HIP(launch_carve(a, k, v->stream));
     VH_HIP(hipEventRecord(e[3], v->stream));
+    VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), k, v->stream));
+    VH_HIP(hipEventRecord(e[4], v->stream));
     for (hipEvent_t x : e) v->ev_pending.push_back(x);
     v->ev_frames.push_back(k);
     return VH_OK;
@@ -403,15 +417,17 @@ void drop_graphs(vh_volume* v, int which_mask) {
 }
 
 int collect_profile(vh_volume* v) {
-  for (size_t k = 0; k + 3 < v->ev_pending.size(); k += 4) {
-    float t = 0.0f, a = 0.0f, b = 0.0f;
+  for (size_t k = 0; k + 4 < v->ev_pending.size(); k += 5) {
+    float t = 0.0f, a = 0.0f, c = 0.0f, b = 0.0f;
     VH_HIP(hipEventElapsedTime(&t, v->ev_pending[k], v->ev_pending[k + 1]));
     VH_HIP(hipEventElapsedTime(&a, v->ev_pending[k + 1], v->ev_pending[k + 2]));
-    VH_HIP(hipEventElapsedTime(&b, v->ev_pending[k + 2], v->ev_pending[k + 3]));
+    VH_HIP(hipEventElapsedTime(&c, v->ev_pending[k + 2], v->ev_pending[k + 3]));
+    VH_HIP(hipEventElapsedTime(&b, v->ev_pending[k + 3], v->ev_pending[k + 4]));
     v->track_ms += t;
     v->alloc_ms += a;
+    v->carve_ms += c;
     v->integrate_ms += b;
-    v->profiled_frames += v->ev_frames[k / 4];
+    v->profiled_frames += v->ev_frames[k / 5];
     v->profiled_launches += 1;
   }
   for (hipEvent_t e : v->ev_pending) v->ev_pool.push_back(e);
@@ -482,6 +498,7 @@ int validate(const vh_config* c) {
   if (c->color != 0 && c->color != 1) return fail(VH_ERR_INVALID_ARG, "color must be 0 or 1");
   if (c->seq_batch < 0 || c->seq_batch > kMaxBatch)
     return fail(VH_ERR_INVALID_ARG, "seq_batch must be in 0..8");
+  if (c->carve != 0 && c->carve != 1) return fail(VH_ERR_INVALID_ARG, "carve must be 0 or 1");
   for (int32_t r : c->reserved)
     if (r) return fail(VH_ERR_INVALID_ARG, "reserved fields must be zero");
   return VH_OK;
@@ -516,7 +533,7 @@ void free_volume(vh_volume* v) {
   if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   free_index_buffers(v);
-  void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->voxels, v->touched, v->ctl,
+  void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->block_info, v->tilemax, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total, v->mc_nbrs,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
                   v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba,
@@ -738,6 +755,7 @@ StreamArgs stream_args(vh_volume* v, uint32_t birth) {
   StreamArgs a{};
   a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
   a.block_keys = v->block_keys;
+  a.block_info = v->block_info;
   a.pool = static_cast<uint32_t*>(v->voxels);
   a.colors = v->colors;
   a.wpb = words_per_block(v);
@@ -1003,6 +1021,9 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
   const size_t frame_px = (size_t)cfg->width * cfg->height;
   if ((rc = dev_alloc(&v->keys, v->n_slots * 8)) || (rc = dev_alloc(&v->vals, v->n_slots * 8)) ||
       (rc = dev_alloc(&v->marks, 2 * v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
+      (rc = dev_alloc(&v->block_info, pool * 8)) ||
+      (cfg->carve && (rc = dev_alloc(&v->tilemax, (size_t)2 * kMaxBatch * ((cfg->width + kTile - 1) / kTile) *
+                                                     ((cfg->height + kTile - 1) / kTile) * sizeof(float)))) ||
       (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||
       (rc = dev_alloc(&v->touched, (size_t)2 * kStripes * v->stripe_cap * 4)) ||
       (rc = dev_alloc(&v->ctl, sizeof(FrameCtl))) ||
@@ -1283,7 +1304,7 @@ int vh_set_profiling(vh_volume* v, int32_t on) {
   rc = collect_profile(v);
   if (rc) return rc;
   v->profiling = on != 0;
-  v->alloc_ms = v->integrate_ms = v->track_ms = 0.0;
+  v->alloc_ms = v->integrate_ms = v->track_ms = v->carve_ms = 0.0;
   v->profiled_frames = v->profiled_launches = 0;
   return VH_OK;
 }
@@ -1316,6 +1337,8 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->profiled_frames = v->profiled_frames;
   s->profiled_launches = v->profiled_launches;
   s->blocks_integrated = (int64_t)c.integrated_total;
+  s->carved_total = (int64_t)c.carved_total;
+  s->carve_ms = v->carve_ms;
   s->track_ms = v->track_ms;
   return VH_OK;
 }

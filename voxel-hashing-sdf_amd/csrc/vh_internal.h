@@ -8,11 +8,14 @@
 //   marks[2][4 * 2^log2_buckets]    u32 per slot and batch parity: bit j set iff frame j of the
 //                                   batch touched the block (cleared when the batch is integrated);
 //   block_keys[pool_blocks]         u64 key of each pool block (pool order);
+//   block_info[pool_blocks]         u64 per pool block: its hash slot (low 32) | birth frame (high 32);
 //   voxels[pool_blocks * 512]       FP32: float2 {tsdf, weight}; FP16: u32 {f16 tsdf | u16 w << 16};
 //   colors[pool_blocks * 512]       (colour volumes only) u32 {r | g << 8 | b << 16};
 //   touched[2][kStripes * stripe_cap] u32 slot ids of the blocks crossed by a batch's bands
 //                                   (each once), in kStripes sub-lists with separate
 //                                   counters, one list per batch parity;
+//   tilemax[2][8][tiles]            carving volumes: per batch parity and frame of the batch, the
+//                                   largest valid depth of each 16 x 16 pixel tile;
 //   ctl                             counters of the frame pipeline (FrameCtl).
 //
 // Device-scope atomics on one address serialise (about 11 ns each on MI355X), so every
@@ -78,6 +81,8 @@ constexpr int kMaxProbe = 64;  // buckets examined before a key is declared unpl
 constexpr int kStripes = 64;   // striped counters / touched sub-lists
 constexpr int kLine = 32;      // u32 per 128-byte line
 constexpr int kMaxBatch = 8;   // frames integrated per launch at most
+constexpr int kLineCarve = 1 + kMaxBatch;  // word of a stripe line counting the mark bits the carve pass added
+constexpr int kTile = 16;      // pixels per edge of a max-depth tile (carving)
 
 struct FrameCtl {
   uint32_t frame_id;       // id of the first frame of the batch being integrated (starts at 1)
@@ -92,9 +97,11 @@ struct FrameCtl {
   unsigned long long new_total;
   unsigned long long integrated_total;  // sum over batches of the blocks listed (read + written once)
   uint32_t batch_id;       // batches integrated; its parity selects the lists and marks
-  uint32_t pad0[15];
+  uint32_t pad0;
+  unsigned long long carved_total;  // block-frame updates added by carve passes (mark bits the bands did not set)
+  uint32_t pad1[12];
   // per batch parity, one line per stripe: word 0 = listed blocks, word 1 + j = blocks frame
-  // j of the batch touched first in the stripe
+  // j of the batch touched first in the stripe, word kLineCarve = mark bits the carve pass added
   uint32_t tcount[2 * kStripes * kLine];
   uint32_t done[kStripes * kLine];        // workgroups of the frame launch finished, per stripe
 };
@@ -123,6 +130,7 @@ struct FrameArgs {
   uint32_t* marks;         // 2 x n_slots
   uint64_t n_slots;
   unsigned long long* block_keys;
+  unsigned long long* block_info;  // per pool block: hash slot | birth frame << 32
   uint32_t pool_cap;
   uint32_t* touched;
   uint32_t stripe_cap;
@@ -136,6 +144,11 @@ struct FrameArgs {
   // {r | g << 8 | b << 16} at rgba + (parity * kMaxBatch + j) * W*H, which the integration
   // of the batch gathers
   uint32_t* rgba;
+  // carving volumes (null otherwise): the allocation pass writes the largest valid depth of each
+  // kTile x kTile pixel tile of frame j of a batch to tilemax + (parity * kMaxBatch + j) *
+  // tiles_x * tiles_y, which the carve pass reads
+  float* tilemax;
+  int32_t tiles_x, tiles_y;
 };
 
 struct McArgs {
@@ -234,6 +247,7 @@ constexpr uint32_t kSlabShift = 14, kSlabBlocks = 1u << kSlabShift;
 struct StreamArgs {
   HashView hash;
   unsigned long long* block_keys;
+  unsigned long long* block_info;  // per pool block: hash slot | birth frame << 32
   uint32_t* pool;              // voxel pool as 32-bit words (wpb per block)
   uint32_t* colors;            // colour pool (null: no colour)
   uint32_t wpb;                // 32-bit words per block: 1024 (FP32) or 512 (16-bit formats)
@@ -322,6 +336,9 @@ struct TrackArgs {
 hipError_t launch_alloc(const FrameArgs& a, int depth_u16, int k, hipStream_t st);
 hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, int k,
                             hipStream_t st);
+// Carving: lists every resident block that frame j of the allocated batch (k frames) may update
+// (conservative visibility and depth test per pool block) and sets its mark bit j.
+hipError_t launch_carve(const FrameArgs& a, int k, hipStream_t st);
 // Integration of the allocated batch (k_int frames) and allocation of the next one (k_alloc
 // frames) in one launch (bound sequences; the next batch's allocation must not have run).
 hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16, int grid,

@@ -148,6 +148,7 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
       uint32_t stored = kNoBlock;
       if (idx < a.pool_cap) {
         a.block_keys[idx] = key;
+        a.block_info[idx] = (unsigned long long)slot | ((unsigned long long)frame << 32);
         stored = idx;
       } else {
         raise_flag(ctl, kFlagPoolFull);
@@ -167,8 +168,14 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
     tbase = read_lane_u32(tbase, fl);
     if (first) {
       const uint32_t pos = tbase + lanes_below(fm);
-      if (pos < a.stripe_cap) list_of(a, parity)[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
-      else raise_flag(ctl, kFlagTouchedFull);
+      if (pos < a.stripe_cap) {
+        list_of(a, parity)[(size_t)stripe * a.stripe_cap + pos] = (uint32_t)slot;
+      } else {
+        // the block cannot be listed: its mark goes back to zero, so that a later lane of this
+        // batch that marks it tries to list it again, and no mark outlives the batch unlisted
+        atomicExch(&marks_of(a, parity)[slot], 0u);
+        raise_flag(ctl, kFlagTouchedFull);
+      }
     }
   }
 }
@@ -211,9 +218,26 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
     }
   }
 
+  // carving volumes: the largest valid depth of each 16 x 16 band (a max-depth tile)
+  __shared__ float s_tmax[4][kRows];
+  if (a.tilemax) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      float m = depth_valid(dr[r], a.max_depth) ? dr[r] : 0.0f;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+      if (lane == 0) s_tmax[wave][r] = m;
+    }
+  }
   for (int i = tid; i < kSetSlots; i += 256) s_set[i] = kEmptyKey;
   if (tid == 0) s_nuniq = 0;
   __syncthreads();
+  if (a.tilemax && tid < kRows) {
+    const int ty = (int)(tile / tiles_x) * kRows + tid;
+    if (ty < a.tiles_y)
+      a.tilemax[((size_t)parity * kMaxBatch + jb) * a.tiles_x * a.tiles_y + (size_t)ty * a.tiles_x + tile % tiles_x] =
+          fmaxf(fmaxf(s_tmax[0][tid], s_tmax[1][tid]), fmaxf(s_tmax[2][tid], s_tmax[3][tid]));
+  }
 
 #pragma unroll 1
   for (int r = 0; r < kRows; ++r) {
@@ -545,16 +569,19 @@ __device__ void batch_done(const FrameArgs& a, uint32_t groups, uint32_t f0, uin
   const int lane = threadIdx.x;
   uint32_t* line = list_count_of(ctl, parity) + lane * kLine;
   uint32_t listed = min(line[0], a.stripe_cap);
+  uint32_t carved = line[kLineCarve];
   uint32_t per[kMaxBatch];
 #pragma unroll
   for (int j = 0; j < kMaxBatch; ++j) per[j] = (uint32_t)j < k ? line[1 + j] : 0u;
 #pragma unroll
   for (int j = 0; j <= kMaxBatch; ++j)
     if ((uint32_t)j <= k) line[j] = 0u;
+  line[kLineCarve] = 0u;
   ctl->done[lane * kLine] = 0u;
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     listed += (uint32_t)__shfl_xor((int)listed, m);
+    carved += (uint32_t)__shfl_xor((int)carved, m);
 #pragma unroll
     for (int j = 0; j < kMaxBatch; ++j) per[j] += (uint32_t)__shfl_xor((int)per[j], m);
   }
@@ -569,6 +596,7 @@ __device__ void batch_done(const FrameArgs& a, uint32_t groups, uint32_t f0, uin
     ctl->touched_last = last;
     ctl->touched_total += sum;
     ctl->integrated_total += listed;
+    ctl->carved_total += carved;
     ctl->frames += k;
     ctl->cursor[a.cursor_id] += k;
     ctl->frame_id = f0 + k;
@@ -642,6 +670,133 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
                                              cur, batch & 1u, s_val[wave], s_key[wave], s_mark[wave], s_pose);
   }
   batch_done(a, gridDim.x, f0, k_int, batch & 1u);
+}
+
+// ============================================================================ carving
+// Conservative form of the observation test for a whole block: false only if no voxel of block
+// (bx, by, bz) can pass frame T's test (z >= 2^-64, its pixel inside the image, 0 < d <=
+// max_depth and d - z > -mu there). The voxels lie in the box spanned by the block's lattice
+// points 0..7 along each axis. The box's 8 corners in camera coordinates bound every voxel's z
+// (the camera map is affine), and, when the whole box lies in front of the camera, every
+// voxel's projection (the perspective image of a convex set in z > 0 is the convex hull of its
+// corners' images). Margins cover the rounding of the voxel arithmetic of the integration
+// (frame_update) against these bounds: every camera coordinate it forms is within
+// 4 * 2^-23 * mag of the exact one (mag bounds the magnitudes summed), so dz bounds the depth
+// error and eps the pixel error ((fx + |u - cx|) * error / z for u = fx x / z + cx).
+// tmax: the frame's max-depth tiles; a voxel is only updated if its z is below its pixel's
+// depth + mu, so a block whose nearest corner lies beyond the largest valid depth of the tiles
+// its pixels fall in, plus mu, is not.
+__device__ bool block_may_update(const FrameArgs& a, const float* T, int bx, int by, int bz, const float* tmax) {
+  BlockFrame bf;
+  block_frame(T, a.vox, bf);
+  float cb[3];
+  block_origin_cam(T, a.vox, bx, by, bz, cb);
+  const float mag = fabsf(cb[0]) + fabsf(cb[1]) + fabsf(cb[2]) + 24.0f * a.vox;
+  const float dz = 1e-4f + 4e-6f * mag;
+  const float zfar = a.max_depth + a.mu + dz;
+  // bounding sphere of the box in z first (centre at lattice offset 3.5, radius 3.5 sqrt(3) vox)
+  const float zc = cb[2] + 3.5f * ((bf.ax[2] + bf.ay[2]) + bf.az[2]);
+  const float rad = 6.07f * a.vox;
+  if (zc + rad < -dz || zc - rad > zfar) return false;
+  float P[8][3];
+  float zmin = INFINITY, zmax = -INFINITY;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float ex = (c & 1) ? 7.0f : 0.0f, ey = (c & 2) ? 7.0f : 0.0f, ez = (c & 4) ? 7.0f : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) P[c][r] = cb[r] + ((ex * bf.ax[r] + ey * bf.ay[r]) + ez * bf.az[r]);
+    zmin = fminf(zmin, P[c][2]);
+    zmax = fmaxf(zmax, P[c][2]);
+  }
+  if (zmax < -dz || zmin > zfar) return false;
+  if (!(zmin > dz)) return true;  // the box reaches the camera plane: no projection bound
+  float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float iz = __builtin_amdgcn_rcpf(P[c][2]);
+    const float u = a.K.fx * (P[c][0] * iz) + a.K.cx, v = a.K.fy * (P[c][1] * iz) + a.K.cy;
+    umin = fminf(umin, u);
+    umax = fmaxf(umax, u);
+    vmin = fminf(vmin, v);
+    vmax = fmaxf(vmax, v);
+  }
+  const float rel = 2e-6f * mag / zmin;
+  const float eu = 1.0f + rel * (a.K.fx + fmaxf(fabsf(umin - a.K.cx), fabsf(umax - a.K.cx)));
+  const float ev = 1.0f + rel * (a.K.fy + fmaxf(fabsf(vmin - a.K.cy), fabsf(vmax - a.K.cy)));
+  if (umax < -0.5f - eu || umin > (float)a.W - 0.5f + eu || vmax < -0.5f - ev || vmin > (float)a.H - 0.5f + ev)
+    return false;
+  if (!tmax) return true;
+  // pixels the voxels can round to (fmaxf / fminf also map NaN bounds to the whole image)
+  const int x0 = (int)fmaxf(0.0f, floorf(umin - eu)), x1 = (int)fminf((float)(a.W - 1), ceilf(umax + eu));
+  const int y0 = (int)fmaxf(0.0f, floorf(vmin - ev)), y1 = (int)fminf((float)(a.H - 1), ceilf(vmax + ev));
+  const int tx0 = x0 / kTile, tx1 = x1 / kTile, ty0 = y0 / kTile, ty1 = y1 / kTile;
+  if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > 64) return true;  // a near block: not worth the reads
+  float D = 0.0f;
+  for (int ty = ty0; ty <= ty1; ++ty)
+    for (int tx = tx0; tx <= tx1; ++tx) D = fmaxf(D, tmax[ty * a.tiles_x + tx]);
+  return zmin <= D + a.mu + dz;
+}
+
+// Carve pass of the allocated batch (k frames, first frame id f0, list parity): one lane per
+// resident pool block (the lanes of a fixed grid stride over the pool). Frame j may update a
+// block born at or before frame f0 + j (a block born later in the batch did not exist when
+// frame j was integrated); for each such frame that may update it (block_may_update) the
+// block's mark gets bit j, and a block whose mark was empty (no band of the batch crossed it)
+// is appended to the batch's list, in the workgroup's stripe; the stripe line's carve word
+// counts the mark bits the pass added. Runs after the batch's allocation and before its
+// integration.
+constexpr uint32_t kCarveGroups = 2048;
+__global__ __launch_bounds__(256) void k_carve(FrameArgs a, uint32_t k) {
+  __shared__ float s_pose[kMaxBatch][12];
+  FrameCtl* ctl = a.ctl;
+  const uint32_t f0 = uniform(ctl->frame_id), parity = uniform(ctl->batch_id) & 1u,
+                 cur = uniform(ctl->cursor[a.cursor_id]);
+  const uint32_t n = min(uniform(ctl->pool_count), a.pool_cap);
+  stage_poses(a, cur, k, s_pose);
+  const int lane = threadIdx.x & 63;
+  const uint32_t stripe = blockIdx.x & (kStripes - 1);
+  uint32_t* line = list_count_of(ctl, parity) + stripe * kLine;
+  uint32_t* marks = marks_of(a, parity);
+  const size_t ntile = (size_t)a.tiles_x * a.tiles_y;
+  for (uint32_t b0 = blockIdx.x * 256u; b0 < n; b0 += gridDim.x * 256u) {
+    const uint32_t b = b0 + threadIdx.x;
+    uint32_t mask = 0u, slot = 0u;
+    if (b < n) {
+      const unsigned long long info = a.block_info[b];
+      slot = (uint32_t)info;
+      const int32_t born = (int32_t)((uint32_t)(info >> 32) - f0);  // birth frame relative to f0
+      int bx, by, bz;
+      unpack_key(a.block_keys[b], bx, by, bz);
+      for (uint32_t j = 0; j < k; ++j)
+        if (born <= (int32_t)j &&
+            block_may_update(a, s_pose[j], bx, by, bz, a.tilemax + ((size_t)parity * kMaxBatch + j) * ntile))
+          mask |= 1u << j;
+    }
+    uint32_t old = 0u;
+    if (mask) old = atomicOr(&marks[slot], mask);
+    // block-frame updates the carve pass adds (frames whose bands did not cross the block)
+    uint32_t added = (uint32_t)__popc(mask & ~old);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) added += (uint32_t)__shfl_xor((int)added, o);
+    if (lane == 0 && added) atomicAdd(&line[kLineCarve], added);
+    const bool first = mask != 0u && old == 0u;
+    const uint64_t fm = __ballot(first);
+    if (fm) {
+      const int fl = first_lane(fm);
+      uint32_t tbase = 0;
+      if (lane == fl) tbase = atomicAdd(&line[0], (uint32_t)__popcll(fm));
+      tbase = read_lane_u32(tbase, fl);
+      if (first) {
+        const uint32_t pos = tbase + lanes_below(fm);
+        if (pos < a.stripe_cap) {
+          list_of(a, parity)[(size_t)stripe * a.stripe_cap + pos] = slot;
+        } else {
+          atomicExch(&marks[slot], 0u);
+          raise_flag(ctl, kFlagTouchedFull);
+        }
+      }
+    }
+  }
 }
 
 // ============================================================================ marching cubes
@@ -1297,8 +1452,12 @@ __global__ __launch_bounds__(256) void k_st_rehash(StreamArgs a) {
   if (b >= a.n) return;
   bool inserted = false;
   const int64_t slot = find_or_insert(a.hash, a.block_keys[b], inserted);
-  if (slot < 0) raise_flag(a.ctl, kFlagHashFull);
-  else a.hash.vals[slot] = (unsigned long long)b | ((unsigned long long)a.birth << 32);
+  if (slot < 0) {
+    raise_flag(a.ctl, kFlagHashFull);
+  } else {
+    a.hash.vals[slot] = (unsigned long long)b | ((unsigned long long)a.birth << 32);
+    a.block_info[b] = (unsigned long long)slot | ((unsigned long long)a.birth << 32);
+  }
 }
 
 // One lane per staged block: find or insert its key; new keys take pool blocks (one counter
@@ -1325,6 +1484,7 @@ __global__ __launch_bounds__(256) void k_st_insert(StreamArgs a) {
     uint32_t stored = kNoBlock;
     if (idx < a.pool_cap) {
       a.block_keys[idx] = a.keys[i];
+      a.block_info[idx] = (unsigned long long)slot | ((unsigned long long)a.birth << 32);
       stored = placed = idx;
     } else {
       raise_flag(a.ctl, kFlagPoolFull);
@@ -1837,6 +1997,12 @@ hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int 
                          (uint32_t)k);
     });
   });
+  return hipGetLastError();
+}
+
+hipError_t launch_carve(const FrameArgs& a, int k, hipStream_t st) {
+  const uint32_t g = min(kCarveGroups, (a.pool_cap + 255u) / 256u);
+  hipLaunchKernelGGL(k_carve, dim3(g), dim3(256), 0, st, a, (uint32_t)k);
   return hipGetLastError();
 }
 

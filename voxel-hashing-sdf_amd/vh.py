@@ -41,7 +41,7 @@ DEPTH_U16 = 0
 DEPTH_F32 = 1
 PRIOR_ABSOLUTE = 0
 PRIOR_INCREMENT = 1
-ABI_VERSION = 5
+ABI_VERSION = 6
 # per-evaluation solver stats: n_valid, rms, |delta xi|, lambda, accepted, accepted pose (12)
 GN_STATS = 17
 
@@ -58,7 +58,7 @@ class Config(C.Structure):
                 ("vox_size", C.c_float), ("trunc", C.c_float), ("log2_buckets", C.c_int32),
                 ("pool_blocks", C.c_int32), ("storage", C.c_int32), ("weight_cap", C.c_int32),
                 ("device", C.c_int32), ("origin", C.c_float * 3), ("vox_dim", C.c_int32 * 3),
-                ("color", C.c_int32), ("seq_batch", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("color", C.c_int32), ("seq_batch", C.c_int32), ("carve", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class Frame(C.Structure):
@@ -73,7 +73,7 @@ class Stats(C.Structure):
                 ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64),
                 ("track_ms", C.c_double), ("profiled_launches", C.c_int64), ("blocks_integrated", C.c_int64),
                 ("host_blocks", C.c_int64), ("streamed_out_total", C.c_int64), ("streamed_in_total", C.c_int64),
-                ("stream_merged_total", C.c_int64)]
+                ("stream_merged_total", C.c_int64), ("carved_total", C.c_int64), ("carve_ms", C.c_double)]
 
 
 class Mesh(C.Structure):
@@ -234,7 +234,7 @@ def make_config(**kw) -> Config:
 
 def config_from_workload(w: dict, **over) -> Config:
     keys = ("width", "height", "fx", "fy", "cx", "cy", "max_depth", "min_depth", "vox_size", "trunc",
-            "log2_buckets", "pool_blocks", "storage", "weight_cap", "color", "seq_batch")
+            "log2_buckets", "pool_blocks", "storage", "weight_cap", "color", "seq_batch", "carve")
     kw = {k: w[k] for k in keys if k in w}
     kw.update(over)
     return make_config(**kw)
