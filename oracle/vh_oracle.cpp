@@ -12,7 +12,8 @@
 //   lattice / block / in-block order          src/tsdf.cu:24-64
 //   world<->camera transforms, projection      src/tsdf.cu:66-116 (base2cam, cam2base, cam2frame)
 //   integration skip rules, truncation         src/tsdf.cu:599-751 (IntegrateHashKernel)
-//   running average with the old weight        src/tsdf.cu:853-858 (dense Integrate: the intent)
+//   running average with the old weight        src/tsdf.cu:853-858 (dense Integrate: the intent;
+//                                              evaluated as s + (dist - s) / (w + 1))
 //   colour running average, 8-bit truncation   src/tsdf.cu:743-745, include/tsdf.cuh:122
 //   block selection along the pixel ray        src/tsdf.cu:2088-2238 (re-specified: truncation band)
 //   marching cubes corner order, sign rule     include/tsdf.cuh:234-241, src/tsdf.cu:978-1061
@@ -112,7 +113,8 @@ double comp_scale(uint16_t h) {
 float decode_comp(uint16_t h, int8_t r) { return (float)((double)from_half(h) + (double)r * comp_scale(h)); }
 void encode_comp(float s, uint16_t& h, int8_t& r) {
   h = to_half(s);
-  const double q = std::round(((double)s - (double)from_half(h)) / comp_scale(h));
+  // the nearest integer, ties to even (the quotient is exact: s - h exactly, a power-of-two scale)
+  const double q = std::nearbyint(((double)s - (double)from_half(h)) / comp_scale(h));
   r = (int8_t)std::max(-128.0, std::min(127.0, q));
 }
 
@@ -279,6 +281,11 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, cons
   const int W = o.W, H = o.H;
   std::vector<float> d((size_t)W * H);
   for (size_t i = 0; i < d.size(); ++i) d[i] = (float)dep[i] * scale;
+  // pre-scaled depth d / mu of each pixel (FP32 product by the rounded 1/mu), -inf where the
+  // pixel has no valid measurement: the observation test below is then one comparison
+  const float inv_mu = 1.0f / o.mu;
+  std::vector<float> dm(d.size());
+  for (size_t i = 0; i < d.size(); ++i) dm[i] = (d[i] > 0.0f && d[i] <= o.max_depth) ? d[i] * inv_mu : -INFINITY;
 
   // 1. touched set
   int nth = 1;
@@ -324,69 +331,80 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, cons
   }
   const double vox64 = o.vox, mu64 = o.mu, scale64 = scale;
   const float capf = (float)o.cap;
-  const float inv_mu = 1.0f / o.mu;
+  // homogeneous voxel steps of the frame: the camera step of each lattice axis, R^T e vox (a
+  // row of the pose times vox), with the intrinsics folded in as (fx x + cx z, fy y + cy z, z)
+  auto hom = [&](const float c[3], float out[3]) {
+    out[0] = std::fmaf(o.fx, c[0], o.cx * c[2]);
+    out[1] = std::fmaf(o.fy, c[1], o.cy * c[2]);
+    out[2] = c[2];
+  };
+  float HS[3][3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const float c[3] = {T[4 * ax] * o.vox, T[4 * ax + 1] * o.vox, T[4 * ax + 2] * o.vox};
+    hom(c, HS[ax]);
+  }
 #pragma omp parallel for schedule(dynamic, 4)
   for (long k = 0; k < (long)blk.size(); ++k) {
     int bx, by, bz;
     key_xyz(o.keys[(size_t)blk[(size_t)k]], bx, by, bz);
     const size_t base = (size_t)blk[(size_t)k] * 512;
-    float cb[3];  // camera coordinates of the block origin, R^T (o - t)
+    float cb[3], hb[3];  // camera coordinates of the block origin, R^T (o - t), and homogeneous
     {
       const float d0 = (float)(bx * 8) * o.vox - T[3], d1 = (float)(by * 8) * o.vox - T[7],
                   d2 = (float)(bz * 8) * o.vox - T[11];
       for (int r = 0; r < 3; ++r) cb[r] = T[r] * d0 + T[4 + r] * d1 + T[8 + r] * d2;
     }
+    hom(cb, hb);
     for (int i = 0; i < 512; ++i) {
       const int gx = bx * 8 + (i >> 6), gy = by * 8 + ((i >> 3) & 7), gz = bz * 8 + (i & 7);
-      // FP32 geometry: decides observation, pixel and the FP32 distance. The camera point is
-      // formed per block origin and voxel step, R^T(o - t) + vox R^T (k, j, i): the column
-      // offset as fma(j, b, i*c) added to the origin, then fma(k, a, column) per voxel; one
-      // reciprocal of z feeds both image coordinates, fma(f, x/z, c), which round to the
-      // nearest pixel with ties to even; the truncation multiplies by 1/mu. (The reference
-      // rotates each voxel's p - t, divides x/z, y/z and by mu, and rounds ties away from
-      // zero: src/tsdf.cu:75-93,738, compiled with -use_fast_math.)
+      // FP32 geometry: decides observation, pixel and the FP32 distance. The voxel's
+      // homogeneous pixel point is formed per block origin and voxel step, H(R^T(o - t)) +
+      // H(vox R^T (k, j, i)): the column offset as fma(j, b, i*c) added to the origin, then
+      // fma(k, a, column) per voxel; one reciprocal of z feeds both image coordinates X/z, Y/z,
+      // which round to the nearest pixel with ties to even; the truncated distance is
+      // fma(-z, 1/mu, d/mu), the voxel observed iff it exceeds -1. (The reference rotates each
+      // voxel's p - t, divides x/z, y/z and by mu, and rounds ties away from zero:
+      // src/tsdf.cu:75-93,738, compiled with -use_fast_math.)
       const int vx = i >> 6, vy = (i >> 3) & 7, vz = i & 7;
-      float cam[3];
+      float P[3];
       for (int r = 0; r < 3; ++r) {
-        const float col = cb[r] + std::fmaf((float)vy, T[4 + r] * o.vox, (float)vz * (T[8 + r] * o.vox));
-        cam[r] = std::fmaf((float)vx, T[r] * o.vox, col);
+        const float col = hb[r] + std::fmaf((float)vy, HS[1][r], (float)vz * HS[2][r]);
+        P[r] = std::fmaf((float)vx, HS[0][r], col);
       }
-      const float xc = cam[0], yc = cam[1], zc = cam[2];
+      const float zc = P[2];
       if (!(zc >= 0x1p-64f && zc <= 0x1p64f)) continue;  // depth range of the projection
       const float rz = 1.0f / zc;
-      const float uf = std::nearbyint(std::fmaf(o.fx, xc * rz, o.cx));
-      const float vf = std::nearbyint(std::fmaf(o.fy, yc * rz, o.cy));
+      const float uf = std::nearbyint(P[0] * rz);
+      const float vf = std::nearbyint(P[1] * rz);
       if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
       const size_t pix = (size_t)vf * W + (size_t)uf;
-      const float dv = d[pix];
-      if (!(dv > 0.0f && dv <= o.max_depth)) continue;
-      const float diff = dv - zc;
-      if (!(diff > -o.mu)) continue;
-      const float dist = std::fmin(1.0f, diff * inv_mu);
+      const float t = std::fmaf(-zc, inv_mu, dm[pix]);
+      if (!(t > -1.0f)) continue;
+      const float dist = std::fmin(1.0f, t);
       if (o.color) {
         float s0, w0;
         o.twin(blk[(size_t)k], i, s0, w0);
         color_update(&o.rgb8[3 * (base + i)], w0, rgb + 3 * pix);
       }
-      // twin update: mean with the old weight, then the cap
+      // twin update: mean with the old weight, s + (dist - s) * RN(1 / (w + 1)), then the cap
       if (o.storage == 2) {
         float s = decode_comp(o.s16[base + i], o.r8[base + i]), w = (float)o.w16[base + i];
         const float wn = w + 1.0f;
-        s = (s * w + dist) / wn;
+        s = std::fmaf(dist - s, 1.0f / wn, s);
         w = (capf > 0.0f && wn > capf) ? capf : wn;
         encode_comp(s, o.s16[base + i], o.r8[base + i]);
         o.w16[base + i] = (uint16_t)w;
       } else if (o.storage == 1) {
         float s = from_half(o.s16[base + i]), w = (float)o.w16[base + i];
         const float wn = w + 1.0f;
-        s = (s * w + dist) / wn;
+        s = std::fmaf(dist - s, 1.0f / wn, s);
         w = (capf > 0.0f && wn > capf) ? capf : wn;
         o.s16[base + i] = to_half(s);
         o.w16[base + i] = (uint16_t)w;
       } else {
         float s = o.s32[base + i], w = o.w32[base + i];
         const float wn = w + 1.0f;
-        s = (s * w + dist) / wn;
+        s = std::fmaf(dist - s, 1.0f / wn, s);
         w = (capf > 0.0f && wn > capf) ? capf : wn;
         o.s32[base + i] = s;
         o.w32[base + i] = w;

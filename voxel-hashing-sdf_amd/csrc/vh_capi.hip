@@ -48,7 +48,10 @@ int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
       return fail(VH_ERR_DEVICE, std::string(#call " failed: ") + hipGetErrorString(e_));      \
   } while (0)
 
-constexpr int kIntegrateGrid = 8192;  // integration workers (waves, four per workgroup), a multiple of kStripes
+// Integration workers (waves, four per workgroup), a multiple of kStripes: 5,120 is one
+// resident round at 5 waves per SIMD (256 CUs x 4 SIMDs). room_medium_5mm FP32, batches of 8,
+// average launch: 8,192 workers 91.4 us, 5,120 84.8 us, 4,096 85.2 us.
+constexpr int kIntegrateGrid = 5120;
 // Integration workers of the fused batch launch (k_integrate_alloc), whose allocation
 // workgroups are dispatched after the integration ones: fewer workers leave the allocation
 // more of the GPU. room_medium_5mm, integration-only frames/s by worker count (3,072 /
@@ -101,6 +104,7 @@ struct vh_volume {
   unsigned long long* block_keys = nullptr;
   unsigned long long* block_info = nullptr;  // per pool block: hash slot | birth frame << 32
   float* tilemax = nullptr;    // carving volumes: 2 x kMaxBatch frames of max-depth tiles
+  float* dm = nullptr;         // 2 x kMaxBatch frames of pre-scaled depth (depth / mu)
   void* voxels = nullptr;
   uint32_t* colors = nullptr;  // colour volumes: one u32 per voxel
   uint32_t* rgba = nullptr;    // colour volumes: 2 packed frames of W*H words
@@ -254,6 +258,7 @@ FrameArgs frame_args(vh_volume* v, int which) {
   if (!a.colors) a.rgb = nullptr;
   a.ctl = v->ctl;
   a.tilemax = v->tilemax;  // null unless the volume carves
+  a.dm = v->dm;
   a.tiles_x = (c.width + kTile - 1) / kTile;
   a.tiles_y = (c.height + kTile - 1) / kTile;
   return a;
@@ -344,7 +349,7 @@ int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale, int ki, i
     VH_HIP(launch_alloc(a, u16, k, v->stream));
     if (carve) VH_HIP(launch_carve(a, k, v->stream));
   }
-  if (which != 4) VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), which == 5 ? ki : 1, v->stream));
+  if (which != 4) VH_HIP(launch_integrate(a, v->cfg.storage, integrate_grid(), which == 5 ? ki : 1, v->stream));
   return VH_OK;
 }
 
@@ -380,7 +385,7 @@ int launch_frame(vh_volume* v, int which, float staged_scale, int ki = 1, int ka
     VH_HIP(hipEventRecord(e[2], v->stream));
     if (v->cfg.carve) VH_HIP(launch_carve(a, k, v->stream));
     VH_HIP(hipEventRecord(e[3], v->stream));
-    VH_HIP(launch_integrate(a, v->cfg.storage, u16, integrate_grid(), k, v->stream));
+    VH_HIP(launch_integrate(a, v->cfg.storage, integrate_grid(), k, v->stream));
     VH_HIP(hipEventRecord(e[4], v->stream));
     for (hipEvent_t x : e) v->ev_pending.push_back(x);
     v->ev_frames.push_back(k);
@@ -533,7 +538,7 @@ void free_volume(vh_volume* v) {
   if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   free_index_buffers(v);
-  void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->block_info, v->tilemax, v->voxels, v->touched, v->ctl,
+  void* ptrs[] = {v->keys, v->vals, v->marks, v->block_keys, v->block_info, v->tilemax, v->dm, v->voxels, v->touched, v->ctl,
                   v->stage_depth, v->stage_pose, v->mc_tris, v->mc_cubes, v->mc_counts, v->mc_chunks, v->mc_total, v->mc_nbrs,
                   v->gn_partials, v->gn_pose, v->gn_stats, v->bw_pose, v->gn_state, v->trk_pts,
                   v->ring_depth, v->ring_pose, v->colors, v->stage_rgb, v->ring_rgb, v->mc_rgb, v->rgba,
@@ -1015,13 +1020,14 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
     return fail(VH_ERR_DEVICE, "hipStreamCreate failed");
   }
   v->n_slots = (uint64_t)kSlotsPerBucket << cfg->log2_buckets;
-  // touched blocks per frame, per stripe: room for twice an even share of the whole pool
-  v->stripe_cap = (uint32_t)(cfg->pool_blocks / (kStripes / 2) + 4096);
+  // blocks listed per batch, per stripe (one stripe per image region, so shares are uneven):
+  // room for four times an even share of the whole pool
+  v->stripe_cap = (uint32_t)(cfg->pool_blocks / (kStripes / 4) + 4096);
   const size_t pool = (size_t)cfg->pool_blocks;
   const size_t frame_px = (size_t)cfg->width * cfg->height;
   if ((rc = dev_alloc(&v->keys, v->n_slots * 8)) || (rc = dev_alloc(&v->vals, v->n_slots * 8)) ||
       (rc = dev_alloc(&v->marks, 2 * v->n_slots * 4)) || (rc = dev_alloc(&v->block_keys, pool * 8)) ||
-      (rc = dev_alloc(&v->block_info, pool * 8)) ||
+      (rc = dev_alloc(&v->block_info, pool * 8)) || (rc = dev_alloc(&v->dm, (size_t)2 * kMaxBatch * frame_px * 4)) ||
       (cfg->carve && (rc = dev_alloc(&v->tilemax, (size_t)2 * kMaxBatch * ((cfg->width + kTile - 1) / kTile) *
                                                      ((cfg->height + kTile - 1) / kTile) * sizeof(float)))) ||
       (rc = dev_alloc(&v->voxels, pool * kB3 * voxel_bytes(cfg->storage))) ||

@@ -14,6 +14,8 @@
 //   touched[2][kStripes * stripe_cap] u32 slot ids of the blocks crossed by a batch's bands
 //                                   (each once), in kStripes sub-lists with separate
 //                                   counters, one list per batch parity;
+//   dm[2][8][W*H]                   per batch parity and frame of the batch, each pixel's depth / mu
+//                                   (-inf where not a valid measurement), gathered by the integration;
 //   tilemax[2][8][tiles]            carving volumes: per batch parity and frame of the batch, the
 //                                   largest valid depth of each 16 x 16 pixel tile;
 //   ctl                             counters of the frame pipeline (FrameCtl).
@@ -144,6 +146,9 @@ struct FrameArgs {
   // {r | g << 8 | b << 16} at rgba + (parity * kMaxBatch + j) * W*H, which the integration
   // of the batch gathers
   uint32_t* rgba;
+  // the allocation pass writes each pixel of frame j of a batch as scaled_depth (depth / mu,
+  // -inf where not valid) to dm + (parity * kMaxBatch + j) * W*H; the integration gathers it
+  float* dm;
   // carving volumes (null otherwise): the allocation pass writes the largest valid depth of each
   // kTile x kTile pixel tile of frame j of a batch to tilemax + (parity * kMaxBatch + j) *
   // tiles_x * tiles_y, which the carve pass reads
@@ -198,6 +203,15 @@ __device__ __forceinline__ uint32_t find_block(const HashView& h, uint64_t key) 
   return kNoBlock;
 }
 
+// Rounds to binary16 from a float that has itself been rounded: without the empty register
+// constraint the compiler folds a preceding fused multiply-add and this conversion into one
+// mixed-precision instruction (v_fma_mixlo_f16) that rounds once, straight to binary16, which
+// differs from rounding to FP32 first (double rounding) in about one value in 10^4.
+__device__ __forceinline__ uint16_t f32_to_f16_rn(float s) {
+  __asm__ volatile("" : "+v"(s));
+  return __builtin_bit_cast(uint16_t, (_Float16)s);
+}
+
 // Storage formats: unpack a voxel word to (tsdf, weight) in FP32, pack it back, the empty word.
 template <int kStorage>
 struct VoxelIO;
@@ -217,7 +231,7 @@ struct VoxelIO<VH_STORAGE_F16> {
     wt = (float)(w >> 16);
   }
   static __device__ __forceinline__ Word pack(float s, float wt) {
-    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
+    const uint16_t h = f32_to_f16_rn(s);
     return (uint32_t)h | ((uint32_t)wt << 16);
   }
   static __device__ __forceinline__ Word zero() { return 0u; }
@@ -226,15 +240,16 @@ struct VoxelIO<VH_STORAGE_F16> {
 template <>
 struct VoxelIO<VH_STORAGE_F16R> {
   using Word = uint32_t;
+  // value = h + r * ulp(h)/256: one fused multiply-add, exact (11 + 8 significant bits)
   static __device__ __forceinline__ void unpack(Word w, float& s, float& wt) {
-    const uint16_t h = (uint16_t)(w & 0xffffu);
-    s = (float)__builtin_bit_cast(_Float16, h) + (float)(int)(int8_t)(w >> 24) * f16r_scale_from_bits(h);
+    const float hf = (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    s = __builtin_fmaf((float)(int)(int8_t)(w >> 24), f16r_scale_of(hf), hf);
     wt = (float)((w >> 16) & 0xffu);
   }
   static __device__ __forceinline__ Word pack(float s, float wt) {
-    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)s);
-    const float d = (s - (float)__builtin_bit_cast(_Float16, h)) * f16r_inv_scale_from_bits(h);
-    const float r = fminf(127.0f, fmaxf(-128.0f, roundf(d)));
+    const uint16_t h = f32_to_f16_rn(s);
+    const float hf = (float)__builtin_bit_cast(_Float16, h);
+    const float r = fminf(127.0f, fmaxf(-128.0f, rintf((s - hf) * f16r_inv_scale_of(hf))));
     return (uint32_t)h | ((uint32_t)wt << 16) | ((uint32_t)(uint8_t)(int8_t)(int)r << 24);
   }
   static __device__ __forceinline__ Word zero() { return 0u; }
@@ -334,8 +349,7 @@ struct TrackArgs {
 // launchers (vh_kernels.hip); all asynchronous on `st`
 // Allocation of the next k frames (one batch, 1 <= k <= kMaxBatch), then their integration.
 hipError_t launch_alloc(const FrameArgs& a, int depth_u16, int k, hipStream_t st);
-hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, int k,
-                            hipStream_t st);
+hipError_t launch_integrate(const FrameArgs& a, int storage, int grid, int k, hipStream_t st);
 // Carving: lists every resident block that frame j of the allocated batch (k frames) may update
 // (conservative visibility and depth test per pool block) and sets its mark bit j.
 hipError_t launch_carve(const FrameArgs& a, int k, hipStream_t st);

@@ -215,6 +215,9 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
             (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
       }
       dr[r] = depth_at<kU16>(base, a.depth_scale, v * a.W + u);
+      // the pre-scaled depth the integration of this frame gathers
+      a.dm[((size_t)parity * kMaxBatch + jb) * a.W * a.H + (size_t)v * a.W + u] =
+          scaled_depth(dr[r], a.max_depth, a.inv_mu);
     }
   }
 
@@ -293,11 +296,11 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
 }
 
 // ============================================================================ integration
-// tsdf_update (vh_math.h) with the quotient by div_rn: the same correctly rounded result.
+// tsdf_update (vh_math.h) with RN(1/(w + 1)) by recip_rn (the correctly rounded reciprocal).
 // cap_or_inf: the weight cap, +inf when uncapped (min is then the identity).
 __device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap_or_inf) {
   const float w_new = w + 1.0f;
-  s = div_rn_rare0(s * w + dist, w_new);
+  s = __builtin_fmaf(dist - s, recip_rn(w_new), s);
   w = fminf(w_new, cap_or_inf);
 }
 
@@ -328,36 +331,37 @@ __device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, uint32
 // kColor: the colour pool is updated from each frame's packed RGB at the same pixels.
 // True iff every voxel of the block projects inside the image at a depth within the
 // projection's range: then the per-voxel range checks can be skipped (frame_update<..,
-// true>). The lane's column runs from voxel 0 to voxel 7 of cl + q*ax; the block's lattice
-// box is the convex hull of the columns' end points, and for depths > 0 the perspective
-// image of a convex combination is a convex combination of the images, so if every end
-// point has z in [2^-60, 2^60] and fx*x/z + cx in [1, W-2] (likewise v), every voxel's
-// z lies in [2^-64, 2^64] and its rounded pixel inside the image (the arithmetic of the
-// voxel's own projection differs from the end points' by far less than the 1-pixel margin).
-__device__ __forceinline__ bool block_inside(const FrameArgs& a, const BlockFrame& bf, const float cl[3]) {
+// true>). The lane's column runs from voxel 0 to voxel 7 of cl + q*ax (homogeneous
+// coordinates); the block's lattice box is the convex hull of the columns' end points, and for
+// depths > 0 the perspective image of a convex combination is a convex combination of the
+// images, so if every end point has z in [2^-60, 2^60] and X/z in [1, W-2] (likewise Y/z),
+// every voxel's z lies in [2^-64, 2^64] and its rounded pixel inside the image (the arithmetic
+// of the voxel's own projection differs from the end points' by far less than the 1-pixel
+// margin).
+__device__ __forceinline__ bool block_inside(const FrameArgs& a, const float ax[3], const float cl[3]) {
   bool in = true;
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     const float fq = e ? 7.0f : 0.0f;
-    const float z = __builtin_fmaf(fq, bf.ax[2], cl[2]);
+    const float z = __builtin_fmaf(fq, ax[2], cl[2]);
     const float iz = recip_rn(z);
-    const float u = __builtin_fmaf(a.K.fx, __builtin_fmaf(fq, bf.ax[0], cl[0]) * iz, a.K.cx);
-    const float v = __builtin_fmaf(a.K.fy, __builtin_fmaf(fq, bf.ax[1], cl[1]) * iz, a.K.cy);
+    const float u = __builtin_fmaf(fq, ax[0], cl[0]) * iz;
+    const float v = __builtin_fmaf(fq, ax[1], cl[1]) * iz;
     in = in & (z >= 0x1p-60f) & (z <= 0x1p60f) & (u >= 1.0f) & (u <= (float)(a.W - 2)) & (v >= 1.0f) &
          (v <= (float)(a.H - 2));
   }
   return __builtin_amdgcn_ballot_w64(!in) == 0;  // wave-uniform
 }
 
-// One frame's update of the lane's 8 voxels of a block (camera column cl, step bf.ax): per
-// half, all projections, then all depth (and colour) fetches, then the updates, so the
-// fetches are in flight together. Branch-free: every lane computes, the observation flag
-// selects.
+// One frame's update of the lane's 8 voxels of a block (homogeneous column cl, step ax): per
+// half, all projections, then all pre-scaled depth (and colour) fetches, then the updates, so
+// the fetches are in flight together. Branch-free: every lane computes, the observation flag
+// selects. dmf: the frame's pre-scaled depth image (scaled_depth of each pixel).
 // kInside (block_inside holds): the range checks are known to pass and are left out; the
 // values computed are the same.
-template <int kStorage, bool kU16, bool kColor, bool kInside>
-__device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFrame& bf, const float cl[3],
-                                             const void* base, const uint32_t* rgba,
+template <int kStorage, bool kColor, bool kInside>
+__device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[3], const float cl[3],
+                                             const float* dmf, const uint32_t* rgba,
                                              typename VoxelIO<kStorage>::Word old[8], uint32_t cold[8],
                                              uint32_t& upd) {
   using IO = VoxelIO<kStorage>;
@@ -365,38 +369,29 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFram
   // two halves of 4 voxels: fewer values live at once (occupancy) for 4 fetches in flight
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
-    constexpr uint32_t kElemShift = kU16 ? 1u : 2u;  // log2 of the depth element's bytes
     float zc[4];
     int pix[4];
-    uint32_t off[4];  // kInside: byte offset of the pixel's depth (32-bit, off a scalar base)
+    uint32_t off[4];  // kInside: byte offset of the pixel's scaled depth (32-bit, off a scalar base)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float fq = (float)(h + r);
-      zc[r] = __builtin_fmaf(fq, bf.ax[2], cl[2]);
-      const float x = __builtin_fmaf(fq, bf.ax[0], cl[0]), y = __builtin_fmaf(fq, bf.ax[1], cl[1]);
+      const float X = __builtin_fmaf(fq, ax[0], cl[0]), Y = __builtin_fmaf(fq, ax[1], cl[1]);
+      zc[r] = __builtin_fmaf(fq, ax[2], cl[2]);
       const float iz = recip_rn(zc[r]);
-      if constexpr (kInside && !kColor) {
-        const float uf = rintf(__builtin_fmaf(a.K.fx, x * iz, a.K.cx));
-        const float vf = rintf(__builtin_fmaf(a.K.fy, y * iz, a.K.cy));
-        off[r] = ((uint32_t)vf & 0xffffffu) * ((uint32_t)(a.W << kElemShift) & 0xffffffu) +
-                 ((uint32_t)uf << kElemShift);
-      } else if constexpr (kInside) {  // colour: pixel indices serve the depth and RGB gathers
-        const float uf = rintf(__builtin_fmaf(a.K.fx, x * iz, a.K.cx));
-        const float vf = rintf(__builtin_fmaf(a.K.fy, y * iz, a.K.cy));
-        pix[r] = (int)(((uint32_t)vf & 0xffffffu) * ((uint32_t)a.W & 0xffffffu) + (uint32_t)uf);
+      if constexpr (kInside) {
+        const float uf = rintf(X * iz), vf = rintf(Y * iz);
+        const uint32_t p = ((uint32_t)vf & 0xffffffu) * ((uint32_t)a.W & 0xffffffu) + (uint32_t)uf;
+        if constexpr (kColor) pix[r] = (int)p;  // the pixel index serves the depth and RGB gathers
+        else off[r] = p << 2;
       } else {
-        pix[r] = project_cam(x, y, zc[r], iz, a.K, a.W, a.H);
+        pix[r] = pixel_of(X, Y, zc[r], iz, a.W, a.H);
       }
     }
     float dv[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      if constexpr (kInside && !kColor) {
-        const char* p = reinterpret_cast<const char*>(base) + off[r];
-        dv[r] = kU16 ? (float)*reinterpret_cast<const uint16_t*>(p) * a.depth_scale : *reinterpret_cast<const float*>(p);
-      } else {
-        dv[r] = depth_at<kU16>(base, a.depth_scale, kInside ? pix[r] : max(pix[r], 0));
-      }
+      if constexpr (kInside && !kColor) dv[r] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmf) + off[r]);
+      else dv[r] = dmf[kInside ? pix[r] : max(pix[r], 0)];
     }
     uint32_t px[4];
     if constexpr (kColor) {
@@ -406,8 +401,9 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFram
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int q = h + r;
-      float dist;
-      const bool ok = truncate_obs(kInside ? 0 : pix[r], dv[r], zc[r], a.max_depth, a.mu, a.inv_mu, dist);
+      const float t = __builtin_fmaf(-zc[r], a.inv_mu, dv[r]);
+      const bool ok = (kInside || pix[r] >= 0) & (t > -1.0f);
+      const float dist = fminf(1.0f, t);
       float sv, wv;
       IO::unpack(old[q], sv, wv);
       if constexpr (kColor) {
@@ -423,11 +419,18 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const BlockFram
   }
 }
 
-// s_pose: the batch's k poses (row-major 3x4 each), staged in LDS by stage_poses.
-template <int kStorage, bool kU16, bool kColor>
+// Per frame of a batch, staged in LDS by stage_frames: the pose (row-major 3x4) and the
+// frame's homogeneous voxel steps (block_frame_h).
+struct FrameLds {
+  float T[12];
+  BlockFrame h;
+};
+
+// s_frm: the batch's k frames.
+template <int kStorage, bool kColor>
 __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t f0, uint32_t k,
-                                 uint32_t cur, uint32_t parity, unsigned long long* s_val, unsigned long long* s_key,
-                                 uint32_t* s_mark, const float (*s_pose)[12]) {
+                                 uint32_t parity, unsigned long long* s_val, unsigned long long* s_key,
+                                 uint32_t* s_mark, const FrameLds* s_frm) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
@@ -445,9 +448,7 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
   }
   const uint32_t my_off = incl - my_len;
   const uint32_t n = read_lane_u32(incl, 63);
-  const size_t elem = kU16 ? 2 : 4;
   const size_t frame_px = (size_t)a.W * a.H;
-  const uint32_t fs0 = uniform(cur % a.n_seq);  // the division runs on the vector unit
 
   // Lane j fetches the list entry, slot value, key and mark of this worker's j-th block
   // (clearing the mark for the next batch of this parity) and parks them in LDS, so the
@@ -506,22 +507,21 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
       uint32_t upd = 0u;  // colour volumes: bit q set when voxel q was observed in the batch
       for (uint32_t m = mark; m != 0u; m &= m - 1u) {
         const uint32_t jf = (uint32_t)__builtin_ctz(m);
-        uint32_t fs = fs0 + jf;
-        while (fs >= a.n_seq) fs -= a.n_seq;
+        const FrameLds& F = s_frm[jf];
         float T[12];
 #pragma unroll
-        for (int q = 0; q < 12; ++q) T[q] = s_pose[jf][q];
-        BlockFrame bf;
-        block_frame(T, a.vox, bf);
-        const void* base = reinterpret_cast<const char*>(a.depth) + fs * frame_px * elem;
-        float cb[3], cl[3];
+        for (int q = 0; q < 12; ++q) T[q] = F.T[q];
+        const BlockFrame hf = F.h;
+        float cb[3], hb[3], cl[3];
         block_origin_cam(T, a.vox, bx, by, bz, cb);
-        column_cam(bf, cb, lane >> 3, lane & 7, cl);
-        const uint32_t* rgba = kColor ? a.rgba + ((size_t)parity * kMaxBatch + jf) * frame_px : nullptr;
-        if (block_inside(a, bf, cl))
-          frame_update<kStorage, kU16, kColor, true>(a, bf, cl, base, rgba, old, cold, upd);
+        to_pixel_h(a.K, cb, hb);
+        column_h(hf, hb, lane >> 3, lane & 7, cl);
+        const size_t fo = ((size_t)parity * kMaxBatch + jf) * frame_px;
+        const uint32_t* rgba = kColor ? a.rgba + fo : nullptr;
+        if (block_inside(a, hf.ax, cl))
+          frame_update<kStorage, kColor, true>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd);
         else
-          frame_update<kStorage, kU16, kColor, false>(a, bf, cl, base, rgba, old, cold, upd);
+          frame_update<kStorage, kColor, false>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd);
       }
       // Depth-only volumes write the block whole (unobserved voxels keep the words read,
       // fresh blocks zero there; the wave owns the block in this launch), which saves the
@@ -623,21 +623,37 @@ __device__ __forceinline__ void stage_poses(const FrameArgs& a, uint32_t cur, ui
   }
   __syncthreads();
 }
+// The poses and the homogeneous steps of a batch's frames into LDS. All 256 lanes call it.
+__device__ __forceinline__ void stage_frames(const FrameArgs& a, uint32_t cur, uint32_t k, FrameLds* s_frm) {
+  const uint32_t t = threadIdx.x;
+  if (t < 12u * k) {
+    const uint32_t j = t / 12u, q = t - 12u * j;
+    s_frm[j].T[q] = a.poses[16 * (size_t)input_slot(cur, j, a.n_seq) + q];
+  }
+  __syncthreads();
+  if (t < k) {
+    float T[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) T[q] = s_frm[t].T[q];
+    block_frame_h(T, a.vox, a.K, s_frm[t].h);
+  }
+  __syncthreads();
+}
 
 // Integration of the batch of k frames that the last allocation launch prepared.
-template <int kStorage, bool kU16, bool kColor>
+template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate(FrameArgs a, uint32_t k) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
   __shared__ uint32_t s_mark[kIntegrateWaves][64];
-  __shared__ float s_pose[kMaxBatch][12];
+  __shared__ FrameLds s_frm[kMaxBatch];
   const int wave = threadIdx.x >> 6;
   const FrameCtl* ctl = a.ctl;
   // control values are uniform: keep them (and what derives from them) in scalar registers
   const uint32_t f0 = uniform(ctl->frame_id), parity = uniform(ctl->batch_id) & 1u,
                  cur = uniform(ctl->cursor[a.cursor_id]);
-  stage_poses(a, cur, k, s_pose);
-  integrate_worker<kStorage, kU16, kColor>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, f0,
-                                           k, cur, parity, s_val[wave], s_key[wave], s_mark[wave], s_pose);
+  stage_frames(a, cur, k, s_frm);
+  integrate_worker<kStorage, kColor>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, f0, k,
+                                     parity, s_val[wave], s_key[wave], s_mark[wave], s_frm);
   batch_done(a, gridDim.x, f0, k, parity);
 }
 
@@ -653,7 +669,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
                                                                          uint32_t k_int) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
   __shared__ uint32_t s_mark[kIntegrateWaves][64];
-  __shared__ float s_pose[kMaxBatch][12];
+  __shared__ FrameLds s_frm[kMaxBatch];
   const uint32_t b = blockIdx.x;
   const bool alloc = b >= int_groups;
   const uint32_t idx = alloc ? b - int_groups : b;
@@ -665,9 +681,9 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
                                  input_slot(cur, k_int + jb, a.n_seq));
   } else {
     const int wave = threadIdx.x >> 6;
-    stage_poses(a, cur, k_int, s_pose);  // uniform branch: the whole workgroup integrates
-    integrate_worker<kStorage, kU16, kColor>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, f0, k_int,
-                                             cur, batch & 1u, s_val[wave], s_key[wave], s_mark[wave], s_pose);
+    stage_frames(a, cur, k_int, s_frm);  // uniform branch: the whole workgroup integrates
+    integrate_worker<kStorage, kColor>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, f0, k_int,
+                                       batch & 1u, s_val[wave], s_key[wave], s_mark[wave], s_frm);
   }
   batch_done(a, gridDim.x, f0, k_int, batch & 1u);
 }
@@ -725,10 +741,10 @@ __device__ bool block_may_update(const FrameArgs& a, const float* T, int bx, int
   const float ev = 1.0f + rel * (a.K.fy + fmaxf(fabsf(vmin - a.K.cy), fabsf(vmax - a.K.cy)));
   if (umax < -0.5f - eu || umin > (float)a.W - 0.5f + eu || vmax < -0.5f - ev || vmin > (float)a.H - 0.5f + ev)
     return false;
-  if (!tmax) return true;
   // pixels the voxels can round to (fmaxf / fminf also map NaN bounds to the whole image)
   const int x0 = (int)fmaxf(0.0f, floorf(umin - eu)), x1 = (int)fminf((float)(a.W - 1), ceilf(umax + eu));
   const int y0 = (int)fmaxf(0.0f, floorf(vmin - ev)), y1 = (int)fminf((float)(a.H - 1), ceilf(vmax + ev));
+  if (!tmax) return true;
   const int tx0 = x0 / kTile, tx1 = x1 / kTile, ty0 = y0 / kTile, ty1 = y1 / kTile;
   if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > 64) return true;  // a near block: not worth the reads
   float D = 0.0f;
@@ -742,8 +758,8 @@ __device__ bool block_may_update(const FrameArgs& a, const float* T, int bx, int
 // block born at or before frame f0 + j (a block born later in the batch did not exist when
 // frame j was integrated); for each such frame that may update it (block_may_update) the
 // block's mark gets bit j, and a block whose mark was empty (no band of the batch crossed it)
-// is appended to the batch's list, in the workgroup's stripe; the stripe line's carve word
-// counts the mark bits the pass added. Runs after the batch's allocation and before its
+// is appended to the batch's list, in the workgroup's stripe; the carve word of that stripe's
+// line counts the mark bits the pass added. Runs after the batch's allocation and before its
 // integration.
 constexpr uint32_t kCarveGroups = 2048;
 __global__ __launch_bounds__(256) void k_carve(FrameArgs a, uint32_t k) {
@@ -779,7 +795,7 @@ __global__ __launch_bounds__(256) void k_carve(FrameArgs a, uint32_t k) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) added += (uint32_t)__shfl_xor((int)added, o);
     if (lane == 0 && added) atomicAdd(&line[kLineCarve], added);
-    const bool first = mask != 0u && old == 0u;
+    const bool first = mask != 0u && old == 0u;  // no band of the batch listed the block
     const uint64_t fm = __ballot(first);
     if (fm) {
       const int fl = first_lane(fm);
@@ -1989,13 +2005,11 @@ static void with_input(bool u16, bool color, F f) {
   }
 }
 
-hipError_t launch_integrate(const FrameArgs& a, int storage, int depth_u16, int grid, int k, hipStream_t st) {
+hipError_t launch_integrate(const FrameArgs& a, int storage, int grid, int k, hipStream_t st) {
   with_storage(storage, [&](auto S) {
-    with_input(depth_u16 != 0, a.colors != nullptr, [&](auto U, auto C) {
-      const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
-      hipLaunchKernelGGL((k_integrate<decltype(S)::value, decltype(U)::value, decltype(C)::value>), g, b, 0, st, a,
-                         (uint32_t)k);
-    });
+    const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
+    if (a.colors) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), g, b, 0, st, a, (uint32_t)k);
+    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), g, b, 0, st, a, (uint32_t)k);
   });
   return hipGetLastError();
 }

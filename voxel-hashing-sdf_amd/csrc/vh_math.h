@@ -94,8 +94,7 @@ VH_HD void cam_to_world(const float* T, float x, float y, float z, float& px, fl
 //   c = R^T (o - t),  then voxel (k, j, i) of the block: c + vox*R^T (k, j, i).
 // With a = R^T e_x vox, b = R^T e_y vox, c = R^T e_z vox (per frame) a lane holding one
 // (j, i) column adds its offset once: c_l = c + fma(j, b, i*c), and voxel k along x is
-// fma(k, a, c_l) (one rounding per coordinate). Every voxel therefore costs 3 fused
-// multiply-adds.
+// fma(k, a, c_l) (one rounding per coordinate).
 struct BlockFrame {
   float ax[3], ay[3], az[3];  // per-frame step vectors (camera coordinates per voxel step)
 };
@@ -113,46 +112,65 @@ VH_HD void block_origin_cam(const float* T, float vox, int bx, int by, int bz, f
   const float d2 = (float)(bz * kB) * vox - T[11];
   for (int i = 0; i < 3; ++i) c[i] = T[i] * d0 + T[4 + i] * d1 + T[8 + i] * d2;
 }
-// Camera coordinates of the column (j, i) of a block: c + fma(j, b, i*c).
-VH_HD void column_cam(const BlockFrame& f, const float c[3], int j, int i, float out[3]) {
+
+// Homogeneous pixel coordinates of a camera-frame vector (x, y, z): (fx x + cx z, fy y + cy z,
+// z), each sum one fused multiply-add over the rounded product c z. The intrinsics are folded
+// into the per-frame steps and the block origin this way, so a voxel's pixel is (X/z, Y/z)
+// and its three coordinates cost three fused multiply-adds, as its camera coordinates did.
+VH_HD void to_pixel_h(const Intr& K, const float c[3], float out[3]) {
+  out[0] = fmaf(K.fx, c[0], K.cx * c[2]);
+  out[1] = fmaf(K.fy, c[1], K.cy * c[2]);
+  out[2] = c[2];
+}
+// Homogeneous steps of a frame (to_pixel_h of the camera steps a, b, c).
+VH_HD void block_frame_h(const float* T, float vox, const Intr& K, BlockFrame& f) {
+  BlockFrame c;
+  block_frame(T, vox, c);
+  to_pixel_h(K, c.ax, f.ax);
+  to_pixel_h(K, c.ay, f.ay);
+  to_pixel_h(K, c.az, f.az);
+}
+// Homogeneous coordinates of the column (j, i) of a block whose origin is at cb (homogeneous):
+// cb + fma(j, b, i*c); voxel k of the column is then fma(k, a, column).
+VH_HD void column_h(const BlockFrame& f, const float cb[3], int j, int i, float out[3]) {
   const float fj = (float)j, fi = (float)i;
-  for (int r = 0; r < 3; ++r) out[r] = c[r] + fmaf(fj, f.ay[r], fi * f.az[r]);
+  for (int r = 0; r < 3; ++r) out[r] = cb[r] + fmaf(fj, f.ay[r], fi * f.az[r]);
 }
 
-// Observation test and truncated distance of the voxel at camera point (x, y, z)
-// (reference IntegrateHashKernel skip rules, src/tsdf.cu:706-738), in two steps so that a
-// caller can issue all depth fetches of a block before using any of them:
-//   project_cam: pixel index of the voxel, or -1 when it is outside the depth range
-//     2^-64 <= z <= 2^64 or outside the image. One correctly rounded reciprocal iz = 1/z
-//     (passed in by the caller) serves both image coordinates, fma(fx, x*iz, cx), rounded
-//     to the nearest pixel with ties to even (rintf); the reference divides per coordinate
-//     and rounds ties away from zero (cam2frame, src/tsdf.cu:75-79), compiled with
-//     -use_fast_math (which contracts to the same fused multiply-add). The range bound keeps
-//     z normal and 1/z normal, where the kernels' two-instruction reciprocal equals the IEEE
-//     quotient.
-//   truncate_obs: with the pixel's depth d, true iff 0 < d <= max_depth and d - z > -mu;
-//     dist = min(1, (d - z) * inv_mu) with inv_mu = 1/mu (reference: divides by mu, :738).
+// Observation of the voxel at homogeneous point (X, Y, z) (reference IntegrateHashKernel skip
+// rules, src/tsdf.cu:706-738), in two steps so that a caller can issue all depth fetches of a
+// block before using any of them:
+//   pixel: one correctly rounded reciprocal iz = 1/z (passed in by the caller) serves both
+//     image coordinates, rint(X * iz), rint(Y * iz) (nearest pixel, ties to even; the
+//     reference divides per coordinate and rounds ties away from zero, cam2frame,
+//     src/tsdf.cu:75-79); -1 when z is outside 2^-64 <= z <= 2^64 or the pixel outside the
+//     image. The range bound keeps z and 1/z normal, where the kernels' two-instruction
+//     reciprocal equals the IEEE quotient.
+//   truncation: the pixel's pre-scaled depth m = d * (1/mu) for a depth 0 < d <= max_depth
+//     (-inf otherwise) gives t = fma(-z, 1/mu, m) = (d - z) / mu with one rounding; the voxel
+//     is observed iff t > -1 (the reference's d - z > -mu, with d > 0 and d <= max_depth), and
+//     dist = min(1, t).
 constexpr float kMinZ = 0x1p-64f, kMaxZ = 0x1p64f;
-VH_HD int project_cam(float x, float y, float z, float iz, const Intr& K, int W, int H) {
-  const float uf = rintf(fmaf(K.fx, x * iz, K.cx));
-  const float vf = rintf(fmaf(K.fy, y * iz, K.cy));
+VH_HD int pixel_of(float X, float Y, float z, float iz, int W, int H) {
+  const float uf = rintf(X * iz);
+  const float vf = rintf(Y * iz);
   const bool inside = (z >= kMinZ) & (z <= kMaxZ) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
   // both factors are below 2^24 (W, H <= 16384): the masks let the compiler use a 24-bit
   // multiply-add instead of a 64-bit one
   return inside ? (int)(((uint32_t)vf & 0xffffffu) * ((uint32_t)W & 0xffffffu) + (uint32_t)uf) : -1;
 }
-VH_HD bool truncate_obs(int pix, float d, float z, float max_depth, float mu, float inv_mu, float& dist) {
-  const float diff = d - z;
-  dist = fminf(1.0f, diff * inv_mu);
-  return (pix >= 0) & (d > 0.0f) & (d <= max_depth) & (diff > -mu);
+VH_HD float scaled_depth(float d, float max_depth, float inv_mu) {
+  return (d > 0.0f && d <= max_depth) ? d * inv_mu : -INFINITY;
 }
 
 // Running average with the old weight (the intent of src/tsdf.cu:855-858; the hash kernel
-// at :739-742 assigns the new weight first, which is a defect and is not reproduced).
-// cap > 0 bounds the stored weight, turning the mean into an exponential average.
+// at :739-742 assigns the new weight first, which is a defect and is not reproduced), in the
+// increment form s + (dist - s) * RN(1/(w + 1)): one rounded difference and one fused
+// multiply-add by the correctly rounded reciprocal of the new weight. cap > 0 bounds the
+// stored weight, turning the mean into an exponential average.
 VH_HD void tsdf_update(float& s, float& w, float dist, float cap) {
   const float w_new = w + 1.0f;
-  s = (s * w + dist) / w_new;
+  s = fmaf(dist - s, 1.0f / w_new, s);
   w = (cap > 0.0f && w_new > cap) ? cap : w_new;
 }
 
@@ -218,12 +236,36 @@ VH_HD float f16r_inv_scale_from_bits(uint16_t h) {
 VH_HD float f16r_decode(uint16_t h, int r) {
   return f16_bits_to_f32(h) + (float)r * f16r_scale_from_bits(h);
 }
-// Residual of s against its binary16 rounding h, in units of ulp(h)/256, clamped to int8.
+// Residual of s against its binary16 rounding h, in units of ulp(h)/256 (an exact quotient:
+// s - h is exact by Sterbenz's lemma, the scale a power of two), rounded to the nearest
+// integer with ties to even and clamped to int8.
 VH_HD int f16r_residual(float s, uint16_t h) {
   const float d = (s - f16_bits_to_f32(h)) * f16r_inv_scale_from_bits(h);
-  float r = roundf(d);
+  float r = rintf(d);
   r = r > 127.0f ? 127.0f : (r < -128.0f ? -128.0f : r);
   return (int)r;
+}
+// f16r_scale_from_bits and its inverse from the FP32 value hf of the binary16 number instead of
+// its bits (what a kernel holds after the conversion): a normal h with biased exponent e has
+// FP32 exponent e + 112, so ulp(h)/256 = 2^(e - 33) has the FP32 exponent field of hf minus 18;
+// subnormal halves and zero have an FP32 exponent field below 113, and the maximum with the
+// exponent field 95 (2^-32) covers them. Equal to the bit forms for every finite binary16
+// (tests/host_checks/f16r_scale.cpp).
+VH_HD int32_t f16r_scale_field(float hf) {
+  union { float f; int32_t i; } c;
+  c.f = hf;
+  const int32_t e = (c.i & 0x7f800000) - (18 << 23);
+  return e > (95 << 23) ? e : (95 << 23);
+}
+VH_HD float f16r_scale_of(float hf) {
+  union { int32_t i; float f; } c;
+  c.i = f16r_scale_field(hf);
+  return c.f;
+}
+VH_HD float f16r_inv_scale_of(float hf) {
+  union { int32_t i; float f; } c;
+  c.i = (254 << 23) - f16r_scale_field(hf);
+  return c.f;
 }
 
 // ---------------------------------------------------------------- truncation-band DDA
