@@ -112,13 +112,15 @@ def gn_block(vh, wl, w, vol, nf):
     return out
 
 
-def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.0005, sigma_deg=0.05):
+def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.0005, sigma_deg=0.05,
+                  depth_points=3.0):
     """floor_large_4mm with tracking from odometry: frame 0 at its true pose; frame f > 0
     starts from the refined pose of frame f-1 times a noisy motion increment
     inc_f = (T_{f-1}^-1 T_f) exp(xi_f), xi_f ~ N(0, sigma_t) and N(0, sigma_deg) per axis
     (camera frame, seeded), and is refined by gn_iters solver iterations on every stride-th
-    pixel up to zmax metres before integration. Dead reckoning (chaining the increments
-    without refinement) is reported beside it."""
+    pixel up to zmax metres before integration, with the odometry's noise as the motion prior
+    and the depth term counted as depth_points points. Dead reckoning (chaining the increments
+    without refinement) and tracking without the motion prior are reported beside it."""
     import torch
 
     w = dict(wl.WORKLOADS["floor_large_4mm"])
@@ -146,15 +148,30 @@ def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.00
     del depth
     vol = vh.Volume(vh.config_from_workload(w))
     mode = vh.PRIOR_INCREMENT
+    motion = dict(prior_sigma_t=sigma_t, prior_sigma_rot=float(np.deg2rad(sigma_deg)), depth_points=depth_points)
     res = {"workload": "floor_large_4mm", "frames": nf, "gn_iters_per_frame": gn_iters, "point_stride": stride,
            "max_point_depth_m": zmax, "prior": "refined pose of the previous frame x noisy odometry increment "
-           f"(camera frame, N(0, {sigma_t * 1e3:g} mm) and N(0, {sigma_deg:g} deg) per axis per frame, seed SEED+2)"}
+           f"(camera frame, N(0, {sigma_t * 1e3:g} mm) and N(0, {sigma_deg:g} deg) per axis per frame, seed SEED+2)",
+           "motion_prior": {"sigma_t_m": sigma_t, "sigma_rot_deg": sigma_deg, "depth_points": depth_points}}
+    summ = lambda e: {"mean": round(float(e.mean()), 4), "median": round(float(np.median(e)), 4),
+                      "p95": round(float(np.percentile(e, 95)), 4), "max": round(float(e.max()), 4),
+                      "last": round(float(e[-1]), 4)}
     try:
+        # without the motion prior (errors only)
         vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode)
+        vol.integrate_next(nf)
+        vol.sync()
+        out = d_out.cpu().numpy().reshape(nf, 4, 4)
+        e_np = np.array([pose_err(out[f], P[f]) for f in range(1, nf)])
+        res["without_motion_prior"] = {"rot_err_deg": summ(e_np[:, 0]), "trans_err_mm": summ(e_np[:, 1])}
+        vol.reset()
+        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode,
+                          **motion)
         vol.integrate_next(1)  # captures the per-frame graph
         vol.sync()
         vol.reset()
-        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode)
+        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode,
+                          **motion)
         torch.cuda.synchronize()
         t = time.perf_counter()
         vol.integrate_next(nf)
@@ -171,9 +188,6 @@ def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.00
         out = d_out.cpu().numpy().reshape(nf, 4, 4)
         e_dead = np.array([pose_err(dead[f], P[f]) for f in range(1, nf)])
         e_trk = np.array([pose_err(out[f], P[f]) for f in range(1, nf)])
-        summ = lambda e: {"mean": round(float(e.mean()), 4), "median": round(float(np.median(e)), 4),
-                          "p95": round(float(np.percentile(e, 95)), 4), "max": round(float(e.max()), 4),
-                          "last": round(float(e[-1]), 4)}
         res.update({"fps": round(nf / dt, 2), "ms_per_frame": round(1e3 * dt / nf, 4),
                     "blocks": st["blocks_allocated"], "pool_blocks": w["pool_blocks"],
                     "mc_triangles": ntri, "mc_ms": round(1e3 * mc_s, 3),
@@ -183,7 +197,8 @@ def tracked_block(vh, wl, n_frames, gn_iters=5, stride=4, zmax=3.0, sigma_t=0.00
         # per-stage device time (eager launches with events, same sequence)
         vol.reset()
         vol.set_profiling(True)
-        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode)
+        vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_inc, d_out, d_stats, gn_iters, stride, zmax, mode,
+                          **motion)
         vol.integrate_next(nf)
         vol.sync()
         p = vol.stats()

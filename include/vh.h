@@ -150,7 +150,16 @@ typedef struct vh_track_config {
   int32_t point_stride;     /* 1..64 */
   float max_point_depth;    /* metres; <= 0: the volume's max_depth */
   int32_t prior_mode;       /* vh_prior_mode */
-  int32_t reserved[4];      /* zero */
+  /* Motion prior (0: off; each >= 0): the solver adds 1/2 xi^T diag(I3/st^2, I3/sr^2) xi,
+   * xi = log(T_start^-1 T) (translation first), to the cost, T_start being the frame's starting
+   * pose; st in metres and sr in radians are the per-frame noise of the prior's motion. */
+  float prior_sigma_t;
+  float prior_sigma_rot;
+  /* > 0: the depth term is scaled by depth_points / (points with a depth), so that the depth
+   * image counts as this many independent points against the motion prior (the residuals of
+   * one frame share the map's errors); 0: unscaled. */
+  float depth_points;
+  int32_t reserved[1];      /* zero */
 } vh_track_config;
 
 /* Triangle soup in host memory, owned by the library (freed by vh_mesh_free). */
@@ -360,7 +369,7 @@ int vh_gn_refine(vh_volume* vol, const float* pts, int64_t n, double c2w[16], in
 /* vh_gn_refine over device-resident points (d_pts, n x 3 floats) that allocates no device
  * memory: no reference counterpart; for callers that cannot allocate (code under HIP graph
  * capture, tight serving or tracking loops, harnesses that forbid hipMalloc). The solver's
- * scratch (partial sums, candidate pose, state, stats rows; 132 KB) is a static device array,
+ * scratch (partial sums, candidate pose, state, stats rows; 264 KB) is a static device array,
  * one copy per GPU held until the process ends. Limit: iters <= 63 (VH_ERR_INVALID_ARG
  * beyond); any n. Same kernels and results as vh_gn_refine on the same points. Every call on
  * one GPU shares that GPU's scratch: calls that overlap on one GPU (from several threads or

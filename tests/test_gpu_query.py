@@ -260,7 +260,7 @@ def test_gauss_newton_matches_cpu_reference_and_converges(vh, oracle, wl, room):
     e0 = pose_error(T0, T_true)
     print(f"GN: {len(pts)} points, initial error {e0[0]:.3f} deg {e0[1] * 1000:.2f} mm", flush=True)
     T1, st = vol.gn_refine(pts, T0, 10)
-    Tc, _ = lm_reference(ref, pts, T0, 10)
+    Tc, _ = lm_reference(ref, pts, T0, 10, w["trunc"])
     e1, ec = pose_error(T1, T_true), pose_error(Tc, T_true)
     print(f"GPU: {e1[0]:.4f} deg {e1[1] * 1000:.3f} mm;  CPU reference: {ec[0]:.4f} deg {ec[1] * 1000:.3f} mm")
     print(f"rms per evaluation {np.round(st['rms'], 5)}")

@@ -14,7 +14,7 @@ that slides along the planes than at the true one; the tracked benchmark reports
 import numpy as np
 import pytest
 
-from gn_reference import backproject_grid, lm_cost, lm_reference, pose_error, se3_exp
+from gn_reference import backproject_grid, lm_reference, pose_error, se3_exp, tracked_cost
 
 pytestmark = pytest.mark.gpu
 
@@ -41,8 +41,15 @@ def odometry_increments(poses, sigma_t, sigma_deg, seed):
     return out.astype(np.float32)
 
 
-@pytest.mark.parametrize("storage,mode,carve", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
-def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_frames, storage, mode, carve):
+# (sigma_t, sigma_rot, depth_points) of the motion prior: the odometry's own noise, the depth
+# image counted as 30 points
+ODOMETRY_PRIOR = (0.005, float(np.deg2rad(0.5)), 30.0)
+
+
+@pytest.mark.parametrize("storage,mode,carve,motion", [(0, 0, 0, None), (1, 0, 0, None), (0, 1, 0, None),
+                                                      (0, 0, 1, None), (0, 1, 0, ODOMETRY_PRIOR),
+                                                      (2, 1, 1, ODOMETRY_PRIOR)])
+def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_frames, storage, mode, carve, motion):
     import torch
 
     w = dict(wl.WORKLOADS["room_small_1cm"])
@@ -56,14 +63,16 @@ def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_fr
         prior = odometry_increments(poses, 0.005, 0.5, wl.SEED + 2)
     else:
         prior = noisy_priors(poses, 0.01, 1.0, wl.SEED + 2)
-    print(f"tracking: room_small_1cm storage {storage}, prior mode {mode}, carve {carve}, {nf} frames, "
-          f"{iters} iterations, stride {stride}", flush=True)
+    print(f"tracking: room_small_1cm storage {storage}, prior mode {mode}, carve {carve}, motion prior {motion}, "
+          f"{nf} frames, {iters} iterations, stride {stride}", flush=True)
     vol = vh.Volume(vh.config_from_workload(w))
     d_depth = torch.from_numpy(np.ascontiguousarray(depth).view(np.int16)).cuda()
     d_prior = torch.from_numpy(prior.reshape(nf, 16)).cuda()
     d_out = torch.zeros(nf, 16, dtype=torch.float32, device="cuda")
     d_stats = torch.zeros(nf * (iters + 1) * vh.GN_STATS, dtype=torch.float64, device="cuda")
-    vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_prior, d_out, d_stats, iters, stride, prior_mode=mode)
+    st, sr, npts = motion or (0.0, 0.0, 0.0)
+    vol.bind_tracking(d_depth, vh.DEPTH_U16, 0.001, nf, d_prior, d_out, d_stats, iters, stride, prior_mode=mode,
+                      prior_sigma_t=st, prior_sigma_rot=sr, depth_points=npts)
     vol.integrate_next(nf)
     vol.sync()
     out = d_out.cpu().numpy().reshape(nf, 4, 4)
@@ -77,7 +86,7 @@ def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_fr
         start = prior[f].astype(np.float64)
         if mode == vh.PRIOR_INCREMENT and f > 0:
             start = out[f - 1].astype(np.float64) @ start
-        Tc, trace = lm_reference(ref, pts, start, iters)
+        Tc, trace = lm_reference(ref, pts, start, iters, w["trunc"], prior=motion)
         dev = np.abs(out[f].astype(np.float64) - Tc).max()
         assert dev < 1e-6, (f, dev)
         for it in range(iters + 1):
@@ -87,7 +96,8 @@ def test_tracked_sequence_matches_cpu_solver_and_oracle(vh, oracle, wl, small_fr
             assert stats[0, :, 0].max() == 0, "no point can have a sample in an empty volume"
         errs.append((pose_error(start, poses[f]), pose_error(out[f], poses[f])))
         # Levenberg-Marquardt never accepts a cost increase
-        assert lm_cost(ref, pts, Tc) <= lm_cost(ref, pts, start)
+        assert tracked_cost(ref, pts, Tc, w["trunc"], start, motion) <= tracked_cost(ref, pts, start, w["trunc"],
+                                                                                    start, motion)
         ref.integrate(depth[f], out[f])
     for f, (e0, e1) in enumerate(errs):
         print(f"frame {f}: start {e0[0]:.3f} deg {e0[1] * 1000:.2f} mm -> refined {e1[0]:.3f} deg "
@@ -112,4 +122,8 @@ def test_tracking_rejects_bad_arguments(vh, wl):
         with pytest.raises(vh.VhError) as e:
             vol.bind_tracking(d, vh.DEPTH_U16, 0.001, 2, p, p, None, iters, stride, prior_mode=mode)
         assert e.value.code == vh.VH_ERR_INVALID_ARG
+    for bad in (dict(prior_sigma_t=-1.0), dict(prior_sigma_rot=float("nan")), dict(depth_points=float("inf"))):
+        with pytest.raises(vh.VhError) as e:
+            vol.bind_tracking(d, vh.DEPTH_U16, 0.001, 2, p, p, None, 5, 4, **bad)
+        assert e.value.code == vh.VH_ERR_INVALID_ARG and "depth_points" in str(e.value)
     vol.close()

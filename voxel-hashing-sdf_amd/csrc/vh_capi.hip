@@ -128,6 +128,7 @@ struct vh_volume {
   double* trk_stats = nullptr;
   int32_t trk_iters = 0, trk_stride = 1, trk_increment = 0;
   float trk_max_depth = 0.0f;  // depth limit of the tracking points
+  float trk_sigma_t = 0.0f, trk_sigma_r = 0.0f, trk_points = 0.0f;  // motion prior, depth-term scale
   float* trk_pts = nullptr;
   size_t trk_pts_cap = 0;  // points the buffer holds
   // host frames queued by vh_integrate_async: copied on copy_stream into kRing device slots;
@@ -293,6 +294,9 @@ TrackArgs track_args(vh_volume* v) {
   t.ph = (c.height + v->trk_stride - 1) / v->trk_stride;
   t.K = Intr{c.fx, c.fy, c.cx, c.cy};
   t.max_depth = v->trk_max_depth;
+  t.prior_sigma_t = v->trk_sigma_t;
+  t.prior_sigma_r = v->trk_sigma_r;
+  t.n_eff = v->trk_points;
   t.pts = v->trk_pts;
   t.pose = v->gn_pose;
   t.state = v->gn_state;
@@ -300,22 +304,42 @@ TrackArgs track_args(vh_volume* v) {
   return t;
 }
 
+// Solver launches after the start pose and state are in place (cand, state[0]): iters + 1
+// evaluations, each one launch that first takes the step from the previous evaluation (the
+// partial sums and states alternate between two buffers by evaluation parity), then the final
+// step, which leaves the best accepted pose in cand. partials: 2 x kGnGrid x kGnAcc doubles,
+// state: 2, stats: (iters + 1) rows.
+hipError_t enqueue_gn_evaluations(vh_volume* v, GnArgs g, int iters, double* partials, GnState* state, double* cand,
+                                  double* stats) {
+  const int grid = gn_grid(g.n);
+  const size_t pstride = (size_t)kGnGrid * kGnAcc;
+  g.pose = cand;
+  for (int it = 0; it <= iters; ++it) {
+    g.partials = partials + (it & 1) * pstride;
+    g.prev_grid = it > 0 ? grid : 0;
+    g.prev_partials = partials + ((it + 1) & 1) * pstride;
+    g.st_in = state + ((it + 1) & 1);
+    g.st_out = state + (it & 1);
+    g.cand_out = nullptr;
+    g.stats_row = it > 0 ? stats + kGnStats * (it - 1) : nullptr;
+    hipError_t e = launch_gn_accum(g, v->cfg.storage, grid, v->stream);
+    if (e != hipSuccess) return e;
+  }
+  return launch_gn_solve(partials + (iters & 1) * pstride, grid, state + (iters & 1), state + ((iters + 1) & 1), cand,
+                         1, stats + kGnStats * iters, v->stream);
+}
+
 // Tracking part of a frame of the tracked sequence: back-projection, iters + 1 solver
-// evaluations (the last only picks the best accepted pose), the pose write-out.
+// evaluations and the final step, the pose write-out.
 int enqueue_track_kernels(vh_volume* v) {
   const TrackArgs t = track_args(v);
   VH_HIP(launch_track_begin(t, v->stream));
   GnArgs g{};
   g.q = query_args(v);
+  g.mu = v->cfg.trunc;
   g.pts = t.pts;
   g.n = (int64_t)t.pw * t.ph;
-  g.pose = v->gn_pose;
-  g.partials = v->gn_partials;
-  for (int it = 0; it <= t.iters; ++it) {
-    VH_HIP(launch_gn_accum(g, v->cfg.storage, gn_grid(g.n), v->stream));
-    VH_HIP(launch_gn_solve(v->gn_partials, gn_grid(g.n), v->gn_state, v->gn_pose, it == t.iters ? 1 : 0,
-                           v->gn_stats + kGnStats * it, v->stream));
-  }
+  VH_HIP(enqueue_gn_evaluations(v, g, t.iters, v->gn_partials, v->gn_state, v->gn_pose, v->gn_stats));
   VH_HIP(launch_track_end(t, v->stream));
   return VH_OK;
 }
@@ -880,13 +904,13 @@ int stream_in_blocks(vh_volume* v, const std::vector<uint32_t>& slots, uint32_t 
 
 int ensure_gn_scratch(vh_volume* v) {
   if (v->gn_partials) return VH_OK;
-  int rc = dev_alloc(&v->gn_partials, (size_t)kGnGrid * kGnAcc * sizeof(double));
+  int rc = dev_alloc(&v->gn_partials, (size_t)2 * kGnGrid * kGnAcc * sizeof(double));
   if (rc) return rc;
   rc = dev_alloc(&v->gn_pose, 16 * sizeof(double));
   if (rc) return rc;
   rc = dev_alloc(&v->bw_pose, 16 * sizeof(float));
   if (rc) return rc;
-  rc = dev_alloc(&v->gn_state, sizeof(GnState));
+  rc = dev_alloc(&v->gn_state, 2 * sizeof(GnState));
   if (rc) return rc;
   return dev_alloc(&v->gn_stats, (size_t)kGnStats * 65 * sizeof(double));
 }
@@ -894,9 +918,9 @@ int ensure_gn_scratch(vh_volume* v) {
 // Device scratch of one pose refinement: per-workgroup partial sums, the candidate pose, the
 // solver state and one stats row per evaluation.
 struct GnScratch {
-  double* partials;  // kGnGrid * kGnAcc
+  double* partials;  // 2 x kGnGrid * kGnAcc
   double* cand;      // 16
-  GnState* state;
+  GnState* state;    // 2
   double* stats;     // kGnStats * (iters + 1)
 };
 
@@ -911,15 +935,10 @@ int run_gn_refine(vh_volume* v, const float* d_pts, int64_t n, double c2w[16], i
   VH_HIP(launch_gn_start(start, s.state, s.cand, v->stream));
   GnArgs a{};
   a.q = query_args(v);
+  a.mu = v->cfg.trunc;
   a.pts = d_pts;
   a.n = n;
-  a.pose = s.cand;  // each accumulation evaluates the current candidate
-  a.partials = s.partials;
-  for (int it = 0; it <= iters; ++it) {
-    VH_HIP(launch_gn_accum(a, v->cfg.storage, gn_grid(n), v->stream));
-    VH_HIP(launch_gn_solve(s.partials, gn_grid(n), s.state, s.cand, it == iters ? 1 : 0, s.stats + kGnStats * it,
-                           v->stream));
-  }
+  VH_HIP(enqueue_gn_evaluations(v, a, iters, s.partials, s.state, s.cand, s.stats));
   double host_stats[kGnStats * 64];
   VH_HIP(hipMemcpyAsync(c2w, s.cand, 12 * sizeof(double), hipMemcpyDeviceToHost, v->stream));
   VH_HIP(hipMemcpyAsync(host_stats, s.stats, (size_t)kGnStats * (iters + 1) * sizeof(double), hipMemcpyDeviceToHost,
@@ -941,16 +960,16 @@ int run_gn_refine(vh_volume* v, const float* d_pts, int64_t n, double c2w[16], i
 }
 
 // Solver scratch of vh_gn_refine_device_noalloc: a file-scope device array, one copy per GPU
-// (each GPU holds it, 132 KB, until the process ends), laid out as GnScratch with room for
+// (each GPU holds it, 264 KB, until the process ends), laid out as GnScratch with room for
 // kNoAllocMaxIters + 1 stats rows. Host code reaches it only through the per-device address
 // that hipGetSymbolAddress returns; the array's name is not a device address on the host.
 constexpr int kNoAllocMaxIters = 63;
 constexpr size_t kScrPartials = 0;
-constexpr size_t kScrCand = kScrPartials + (size_t)kGnGrid * kGnAcc;
+constexpr size_t kScrCand = kScrPartials + (size_t)2 * kGnGrid * kGnAcc;
 constexpr size_t kScrState = kScrCand + 16;
-constexpr size_t kScrStats = kScrState + 64;
+constexpr size_t kScrStats = kScrState + 160;
 constexpr size_t kScrDoubles = kScrStats + (size_t)kGnStats * (kNoAllocMaxIters + 1);
-static_assert(sizeof(GnState) <= 64 * sizeof(double), "GnState outgrew its scratch slot");
+static_assert(2 * sizeof(GnState) <= 160 * sizeof(double), "GnState outgrew its scratch slots");
 static __device__ __align__(256) double scratch[kScrDoubles];
 
 constexpr int kMaxDevices = 64;
@@ -1196,6 +1215,9 @@ int vh_bind_tracking(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   if (std::isnan(tc->max_point_depth)) return fail(VH_ERR_INVALID_ARG, "max_point_depth is NaN");
   if (tc->prior_mode != VH_PRIOR_ABSOLUTE && tc->prior_mode != VH_PRIOR_INCREMENT)
     return fail(VH_ERR_INVALID_ARG, "unknown prior_mode");
+  for (float x : {tc->prior_sigma_t, tc->prior_sigma_rot, tc->depth_points})
+    if (!(x >= 0.0f) || !std::isfinite(x))
+      return fail(VH_ERR_INVALID_ARG, "prior_sigma_t, prior_sigma_rot and depth_points must be finite and >= 0");
   for (int32_t r : tc->reserved)
     if (r) return fail(VH_ERR_INVALID_ARG, "reserved fields must be zero");
   if (fmt != VH_DEPTH_U16 && fmt != VH_DEPTH_F32) return fail(VH_ERR_INVALID_ARG, "unknown depth format");
@@ -1220,7 +1242,9 @@ int vh_bind_tracking(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   const bool same = v->track && v->seq_depth == d_depth && v->trk_prior == tc->d_prior_c2w &&
                     v->trk_out == tc->d_out_c2w && v->trk_stats == tc->d_stats && v->seq_n == n &&
                     v->seq_format == fmt && v->seq_scale == scale && v->trk_iters == tc->gn_iters &&
-                    v->trk_stride == stride && v->trk_max_depth == zmax && v->trk_increment == incr;
+                    v->trk_stride == stride && v->trk_max_depth == zmax && v->trk_increment == incr &&
+                    v->trk_sigma_t == tc->prior_sigma_t && v->trk_sigma_r == tc->prior_sigma_rot &&
+                    v->trk_points == tc->depth_points;
   v->seq_depth = d_depth;
   v->seq_pose = tc->d_prior_c2w;
   v->seq_n = n;
@@ -1234,6 +1258,9 @@ int vh_bind_tracking(vh_volume* v, const void* d_depth, int32_t fmt, float scale
   v->trk_stride = stride;
   v->trk_max_depth = zmax;
   v->trk_increment = incr;
+  v->trk_sigma_t = tc->prior_sigma_t;
+  v->trk_sigma_r = tc->prior_sigma_rot;
+  v->trk_points = tc->depth_points;
   if (!same || v->seq_rgb) drop_graphs(v, 4 | 8 | 16 | 32);
   v->seq_rgb = nullptr;
   const uint32_t zero = 0;

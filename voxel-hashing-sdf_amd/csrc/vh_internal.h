@@ -292,18 +292,29 @@ struct QueryArgs {
   float vox;
 };
 
-// Gauss-Newton accumulators: 21 (J^T J upper) + 6 (J^T r) + 1 (r^2) + 1 (valid samples) +
-// 1 (points considered: points with a non-finite coordinate are skipped).
-constexpr int kGnAcc = 30;
+// Gauss-Newton accumulators: 21 (J^T W J upper) + 6 (J^T W r) + 1 (r^2) + 1 (valid samples) +
+// 1 (points considered: points with a non-finite coordinate are skipped) + 1 (robust cost).
+constexpr int kGnAcc = 31;
+// Residual model of the pose solver. A point measured at depth z has the axial noise of
+// structured-light depth sensors, sigma(z) = 0.0012 + 0.0019 (z - 0.4)^2 metres (Nguyen, Izadi
+// and Lovell 2012; the synthetic frames are drawn from it), so its normalised residual r (TSDF
+// units, mu metres per unit) is scaled to e = r mu / sigma(z), and the cost is the Huber loss
+// rho(e) = e^2/2 for |e| <= k, k (|e| - k/2) beyond, with k = kHuber.
+constexpr double kHuber = 2.0;
 
 // Levenberg-Marquardt state kept on the device between iterations.
 struct GnState {
   double pose[12];  // accepted pose (row-major 3x4)
+  double cand[12];  // the pose the next evaluation is made at
   double Ab[27];    // J^T J (upper, 21) and J^T r (6) at the accepted pose
   double cost;      // its cost
   double lambda;
   int has;          // an accepted pose exists
   int pad;
+  // tracking only (zero otherwise): the motion prior 1/2 xi^T diag(lam_t I3, lam_r I3) xi with
+  // xi = log(start^-1 T), and the depth term scaled by n_eff / (points considered)
+  double start[12];
+  double lam_t, lam_r, n_eff;
 };
 // per-evaluation stats: n_valid, rms, |delta|, lambda, accepted, accepted pose (12)
 constexpr int kGnStats = 17;
@@ -316,10 +327,18 @@ struct GnStart {
 
 struct GnArgs {
   QueryArgs q;
+  float mu;           // truncation (metres): residual scale of the noise model
   const float* pts;   // camera-frame points (a non-finite coordinate marks a point to skip)
   int64_t n;
-  const double* pose; // 12 doubles, row-major 3x4 camera-to-world
+  const double* pose; // without a previous evaluation: the pose to evaluate (12 doubles, 3x4)
   double* partials;   // gridDim.x * kGnAcc
+  // the previous evaluation, stepped from at the start of the launch (prev_grid = 0: none)
+  const double* prev_partials;
+  int prev_grid;
+  const GnState* st_in;  // solver state before that step
+  GnState* st_out;       // after it (written by workgroup 0)
+  double* cand_out;      // the new candidate (workgroup 0), or null
+  double* stats_row;     // the previous evaluation's stats row (workgroup 0)
 };
 
 // Tracking over a bound sequence: frame f (the sequence cursor) starts Gauss-Newton from
@@ -340,6 +359,7 @@ struct TrackArgs {
   int32_t W, H, stride, pw, ph;  // image, stride, point grid (ceil(W/stride) x ceil(H/stride))
   Intr K;
   float max_depth;        // points need 0 < d <= max_depth
+  float prior_sigma_t, prior_sigma_r, n_eff;  // motion prior and depth-term scale (vh_track_config)
   float* pts;             // pw * ph * 3
   double* pose;           // Gauss-Newton candidate (12)
   GnState* state;
@@ -382,7 +402,7 @@ hipError_t launch_backward(const QueryArgs& q, int storage, const float* pts, in
                            hipStream_t st);
 hipError_t launch_reduce6(const double* partials, int grid, double* out6, hipStream_t st);
 hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t st);
-hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand,
+hipError_t launch_gn_solve(const double* partials, int grid, const GnState* st_in, GnState* st_out, double* cand,
                            int final_step, double* stats_row, hipStream_t st);
 // cand = s.pose, solver state cleared with damping s.lambda.
 hipError_t launch_gn_start(const GnStart& s, GnState* state, double* cand, hipStream_t st);

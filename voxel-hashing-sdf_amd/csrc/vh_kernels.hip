@@ -1719,8 +1719,44 @@ __global__ __launch_bounds__(64) void k_reduce6(const double* partials, int grid
 }
 
 // ============================================================================ Gauss-Newton
+// Huber loss of a scaled residual e and its iteratively reweighted least-squares weight
+// psi(e)/e (kHuber, vh_internal.h).
+__device__ __forceinline__ double huber_rho(double e) {
+  const double ae = fabs(e);
+  return ae <= kHuber ? 0.5 * e * e : kHuber * (ae - 0.5 * kHuber);
+}
+__device__ __forceinline__ double huber_weight(double e) {
+  const double ae = fabs(e);
+  return ae <= kHuber ? 1.0 : kHuber / ae;
+}
+
+// Per point (camera frame p, depth z = p.z): s = mu / sigma(z); with a valid sample r at
+// q = R p + t and e = s r, the point adds W J J^T and W r J with W = huber_weight(e) s^2 (one
+// Gauss-Newton step of the iteratively reweighted problem), r^2, and rho(e) to the cost; a
+// point without a valid sample adds rho(s), the cost of a saturated residual r = 1.
+__device__ __forceinline__ void gn_solve_wg(const double* partials, int grid, const GnState* st_in, GnState* st_out, double* s_cand,
+                            double* cand_out, int final_step, double* stats_row);
+__device__ __forceinline__ void gn_step(const double* tot, const GnState* st_in, GnState* st_out, double* s_cand, double* cand_out,
+                        int final_step, double* stats_row);
+
+// One evaluation. With a previous evaluation (prev_grid > 0) every workgroup first takes its
+// Levenberg-Marquardt step itself (gn_solve_wg over the previous partials: the same sums in
+// the same order, so every workgroup gets the same candidate; workgroup 0 stores the state,
+// the candidate and the stats row), which saves a launch per evaluation and needs no ordering
+// between workgroups; otherwise the candidate is a.pose.
+// Register budget: the step (one lane, FP64 matrices unrolled) would otherwise take 280 VGPRs
+// and one wave per SIMD; capped at 4 waves per SIMD, only its own values spill.
 template <int kStorage>
-__global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gn_accum(GnArgs a) {
+  __shared__ double s_pose[12];
+  if (a.prev_grid > 0) {
+    const bool w0 = blockIdx.x == 0;
+    gn_solve_wg(a.prev_partials, a.prev_grid, a.st_in, w0 ? a.st_out : nullptr, s_pose, w0 ? a.cand_out : nullptr, 0,
+                w0 ? a.stats_row : nullptr);
+  } else {
+    if (threadIdx.x < 12) s_pose[threadIdx.x] = a.pose[threadIdx.x];
+    __syncthreads();
+  }
   double acc[kGnAcc];
 #pragma unroll
   for (int k = 0; k < kGnAcc; ++k) acc[k] = 0.0;
@@ -1728,36 +1764,49 @@ __global__ __launch_bounds__(256) void k_gn_accum(GnArgs a) {
     const float px = a.pts[3 * i], py = a.pts[3 * i + 1], pz = a.pts[3 * i + 2];
     if (!(isfinite(px) && isfinite(py) && isfinite(pz))) continue;  // no depth at this pixel
     acc[29] += 1.0;
+    const double zt = (double)pz - 0.4;
+    const double sc = (double)a.mu / (0.0012 + 0.0019 * (zt * zt));
     double qx, qy, qz;
-    pose_apply_d(a.pose, px, py, pz, qx, qy, qz);
+    pose_apply_d(s_pose, px, py, pz, qx, qy, qz);
     float r, g[3], w[8];
     uint32_t vid[8];
-    if (!sample_sdf<kStorage>(a.q, qx, qy, qz, r, g, vid, w)) continue;
+    if (!sample_sdf<kStorage>(a.q, qx, qy, qz, r, g, vid, w)) {
+      acc[30] += huber_rho(sc);
+      continue;
+    }
+    const double rd = (double)r, e = sc * rd;
+    const double W = huber_weight(e) * sc * sc;
     const double gx = g[0], gy = g[1], gz = g[2];
     const double j[6] = {gx, gy, gz, qy * gz - qz * gy, qz * gx - qx * gz, qx * gy - qy * gx};
+    double wj[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) wj[p] = W * j[p];
     int m = 0;
 #pragma unroll
     for (int p = 0; p < 6; ++p)
 #pragma unroll
-      for (int c = p; c < 6; ++c) acc[m++] += j[p] * j[c];
+      for (int c = p; c < 6; ++c) acc[m++] += wj[p] * j[c];
 #pragma unroll
-    for (int p = 0; p < 6; ++p) acc[21 + p] += j[p] * (double)r;
-    acc[27] += (double)r * (double)r;
+    for (int p = 0; p < 6; ++p) acc[21 + p] += wj[p] * rd;
+    acc[27] += rd * rd;
     acc[28] += 1.0;
+    acc[30] += huber_rho(e);
   }
   wg_reduce_store<kGnAcc>(acc, a.partials + kGnAcc * (size_t)blockIdx.x);
 }
 
 // One workgroup, FP64: deterministic sum of the partials at the candidate pose, then a
 // Levenberg-Marquardt decision and step.
-//   cost(candidate) = sum r^2 over valid points + 1 per invalid point (a saturated residual),
-//   so costs at different poses compare; skipped points count neither way. The candidate is
-//   accepted if its cost does not exceed the accepted one (lambda /= 3) and rejected
-//   otherwise (lambda *= 4). Unless `final`, the next candidate is exp(delta^) T_acc with
-//   (A + lambda diag(A)) delta = -b, A = J^T J and b = J^T r at the accepted pose,
-//   delta = (rho, phi), T <- exp(delta^) T.
+//   cost(candidate) = the robust cost of k_gn_accum (every point with finite coordinates
+//   counts, an invalid one as a saturated residual), so costs at different poses compare;
+//   skipped points count neither way. A tracked frame scales that term (and its A, b) by
+//   n_eff / (points considered) and adds its motion prior (motion_prior). The candidate is
+//   accepted if its cost does not exceed
+//   the accepted one (lambda /= 3) and rejected otherwise (lambda *= 4). Unless `final`, the
+//   next candidate is exp(delta^) T_acc with (A + lambda diag(A)) delta = -b, A = J^T W J and
+//   b = J^T W r at the accepted pose, delta = (rho, phi), T <- exp(delta^) T.
 //   stats_row: [n_valid, rms, |delta|, lambda, accepted, accepted pose (12)].
-__device__ void se3_left_update(const double* x, const double* pose, double* out) {
+__device__ __forceinline__ void se3_left_update(const double* x, const double* pose, double* out) {
   const double rho[3] = {x[0], x[1], x[2]}, ph[3] = {x[3], x[4], x[5]};
   const double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
   const double th = sqrt(th2);
@@ -1792,10 +1841,92 @@ __device__ void se3_left_update(const double* x, const double* pose, double* out
   }
 }
 
+// Motion prior of a tracked frame at pose T (row-major 3x4): xi = log(S^-1 T) with S the
+// frame's starting pose, translation first; the angle from atan2(|w|/2, (tr - 1)/2) (no loss of
+// precision near zero), phi = theta / (2 sin theta) w, rho = V(phi)^-1 D_t with
+// V^-1 = I - K/2 + (1/theta^2 - (1 + cos theta) / (2 theta sin theta)) K^2 (series below 1e-5).
+// Its Jacobian for T <- exp(delta^) T is taken as M = Ad(T^-1) (the exact one is
+// Jr^-1(xi) M; xi is a few noise standard deviations), so with Lam = diag(lam_t I3, lam_r I3) it
+// adds M^T Lam M to J^T W J, M^T Lam xi to J^T W r and xi^T Lam xi / 2 to the cost.
+__device__ __forceinline__ void motion_prior(const GnState& S, double* Ab, double& cost) {
+  const double* P = S.start;
+  const double* T = S.cand;
+  double D[3][3], dt[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) D[i][j] = P[i] * T[j] + P[4 + i] * T[4 + j] + P[8 + i] * T[8 + j];
+    dt[i] = P[i] * (T[3] - P[3]) + P[4 + i] * (T[7] - P[7]) + P[8 + i] * (T[11] - P[11]);
+  }
+  const double wv[3] = {D[2][1] - D[1][2], D[0][2] - D[2][0], D[1][0] - D[0][1]};
+  const double sn = 0.5 * sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+  const double cs = 0.5 * (D[0][0] + D[1][1] + D[2][2] - 1.0);
+  const double th = atan2(sn, cs);
+  double f, c2;
+  if (th < 1e-5) {
+    f = 0.5 + th * th / 12.0;
+    c2 = 1.0 / 12.0 + th * th / 720.0;
+  } else {
+    f = th / (2.0 * sn);
+    c2 = 1.0 / (th * th) - (1.0 + cs) / (2.0 * th * sn);
+  }
+  const double ph[3] = {f * wv[0], f * wv[1], f * wv[2]};
+  const double K[3][3] = {{0.0, -ph[2], ph[1]}, {ph[2], 0.0, -ph[0]}, {-ph[1], ph[0], 0.0}};
+  double xi[6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double a = dt[i] - 0.5 * (K[i][0] * dt[0] + K[i][1] * dt[1] + K[i][2] * dt[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a += c2 * (K[i][0] * K[0][k] + K[i][1] * K[1][k] + K[i][2] * K[2][k]) * dt[k];
+    xi[i] = a;
+    xi[3 + i] = ph[i];
+  }
+  // M = Ad(T^-1) = [[R^T, [p]x R^T], [0, R^T]], p = -R^T t
+  double Rt[3][3], p[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Rt[i][j] = T[4 * j + i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) p[i] = -(Rt[i][0] * T[3] + Rt[i][1] * T[7] + Rt[i][2] * T[11]);
+  const double Pk[3][3] = {{0.0, -p[2], p[1]}, {p[2], 0.0, -p[0]}, {-p[1], p[0], 0.0}};
+  double M[6][6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      M[i][j] = Rt[i][j];
+      M[i][3 + j] = Pk[i][0] * Rt[0][j] + Pk[i][1] * Rt[1][j] + Pk[i][2] * Rt[2][j];
+      M[3 + i][j] = 0.0;
+      M[3 + i][3 + j] = Rt[i][j];
+    }
+  const double lam[6] = {S.lam_t, S.lam_t, S.lam_t, S.lam_r, S.lam_r, S.lam_r};
+  double q = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) q += lam[k] * xi[k] * xi[k];
+  cost += 0.5 * q;
+#pragma unroll
+  for (int a = 0, m = 0; a < 6; ++a) {
+#pragma unroll
+    for (int b = a; b < 6; ++b, ++m) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc += M[k][a] * lam[k] * M[k][b];
+      Ab[m] += acc;
+    }
+    double g = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g += M[k][a] * lam[k] * xi[k];
+    Ab[21 + a] += g;
+  }
+}
+
 // The partials are summed by all 256 lanes at once (each lane a strided subset of the
-// workgroups, then the fixed workgroup tree), so the loads are in flight together.
-__global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int grid, GnState* st, double* cand,
-                                                  int final_step, double* stats_row) {
+// workgroups, then the fixed workgroup tree), so the loads are in flight together. All 256
+// lanes call it; the new candidate is in s_cand on return. st_out, cand_out and stats_row
+// (global, each may be null) receive the new state, the candidate and the evaluation's row.
+__device__ __forceinline__ void gn_solve_wg(const double* partials, int grid, const GnState* st_in, GnState* st_out, double* s_cand,
+                            double* cand_out, int final_step, double* stats_row) {
   __shared__ double tot[kGnAcc];
   double acc[kGnAcc];
 #pragma unroll
@@ -1805,24 +1936,33 @@ __global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int gr
     for (int k = 0; k < kGnAcc; ++k) acc[k] += partials[kGnAcc * (size_t)g + k];
   }
   wg_reduce_store<kGnAcc>(acc, tot);
-  if (threadIdx.x != 0) return;
-  // One lane from here: the state and the candidate are read once into registers and written
-  // back once (through the pointers each access would be a dependent memory round trip).
-  GnState S = *st;
+  if (threadIdx.x == 0) gn_step(tot, st_in, st_out, s_cand, cand_out, final_step, stats_row);
+  __syncthreads();
+}
+
+// The Levenberg-Marquardt decision and step of one evaluation (one lane): the state is read
+// once into registers and written once (through the pointers each access would be a dependent
+// memory round trip).
+__device__ __forceinline__ void gn_step(const double* tot, const GnState* st_in, GnState* st_out, double* s_cand, double* cand_out,
+                        int final_step, double* stats_row) {
+  GnState S = *st_in;
   double c[12], row[kGnStats];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) c[k] = cand[k];
   const double cnt = tot[28];
-  const double cost = tot[27] + (tot[29] - cnt);
+  // tracking: depth term scaled to n_eff points, plus the motion prior
+  const double al = S.n_eff > 0.0 && tot[29] > 0.0 ? S.n_eff / tot[29] : 1.0;
+  double cost = al * tot[30], Ab[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) Ab[k] = al * tot[k];
+  if (S.lam_t > 0.0 || S.lam_r > 0.0) motion_prior(S, Ab, cost);
   row[0] = cnt;
   row[1] = cnt > 0.0 ? sqrt(tot[27] / cnt) : 0.0;
   row[2] = 0.0;
   const bool accept = !S.has || cost <= S.cost;
   if (accept) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) S.pose[k] = c[k];
+    for (int k = 0; k < 12; ++k) S.pose[k] = S.cand[k];  // the pose this evaluation was made at
 #pragma unroll
-    for (int k = 0; k < 27; ++k) S.Ab[k] = tot[k];
+    for (int k = 0; k < 27; ++k) S.Ab[k] = Ab[k];
     S.cost = cost;
     S.lambda = S.has ? fmax(S.lambda / 3.0, 1e-9) : S.lambda;
     S.has = 1;
@@ -1883,11 +2023,28 @@ __global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int gr
 #pragma unroll
     for (int k = 0; k < 12; ++k) c[k] = S.pose[k];
   }
-  *st = S;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) cand[k] = c[k];
+  for (int k = 0; k < 12; ++k) {
+    S.cand[k] = c[k];
+    s_cand[k] = c[k];
+  }
+  if (st_out) *st_out = S;
+  if (cand_out) {
 #pragma unroll
-  for (int k = 0; k < kGnStats; ++k) stats_row[k] = row[k];
+    for (int k = 0; k < 12; ++k) cand_out[k] = c[k];
+  }
+  if (stats_row) {
+#pragma unroll
+    for (int k = 0; k < kGnStats; ++k) stats_row[k] = row[k];
+  }
+}
+
+// The last evaluation's step alone (one workgroup): with final_step the accepted pose is the
+// candidate written to cand.
+__global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int grid, const GnState* st_in,
+                                                  GnState* st_out, double* cand, int final_step, double* stats_row) {
+  __shared__ double s_cand[12];
+  gn_solve_wg(partials, grid, st_in, st_out, s_cand, cand, final_step, stats_row);
 }
 
 // Start of a refinement: the candidate is the starting pose and the solver state is cleared
@@ -1895,12 +2052,14 @@ __global__ __launch_bounds__(256) void k_gn_solve(const double* partials, int gr
 // refinement needs no host-to-device copy.
 __global__ __launch_bounds__(64) void k_gn_start(GnStart s, GnState* st, double* cand) {
   const int k = threadIdx.x;
-  if (k < 12) cand[k] = s.pose[k];
   if (k == 0) {
     GnState z{};
     z.lambda = s.lambda;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) z.cand[q] = s.pose[q];
     *st = z;
   }
+  if (k < 12) cand[k] = s.pose[k];
 }
 
 // ============================================================================ tracking
@@ -1914,19 +2073,26 @@ __device__ void track_start(const TrackArgs& t, uint32_t fs) {
   const int k = threadIdx.x;
   const float* P = t.prior + 16 * (size_t)fs;
   if (k < 12) {
+    double p;
     if (t.increment && fs > 0) {
       const float* A = t.out + 16 * (size_t)(fs - 1);
       const int r = k >> 2, c = k & 3;
-      t.pose[k] = (((double)A[4 * r] * (double)P[c] + (double)A[4 * r + 1] * (double)P[4 + c]) +
-                   (double)A[4 * r + 2] * (double)P[8 + c]) + (double)A[4 * r + 3] * (double)P[12 + c];
+      p = (((double)A[4 * r] * (double)P[c] + (double)A[4 * r + 1] * (double)P[4 + c]) +
+           (double)A[4 * r + 2] * (double)P[8 + c]) + (double)A[4 * r + 3] * (double)P[12 + c];
     } else {
-      t.pose[k] = (double)P[k];
+      p = (double)P[k];
     }
+    t.pose[k] = p;
+    t.state->cand[k] = p;   // the first evaluation's pose
+    t.state->start[k] = p;  // the motion prior's centre
   }
-  if (k == 0) {
-    GnState z{};
-    z.lambda = 1e-4;
-    *t.state = z;
+  if (k == 0) {  // no accepted pose yet (the other fields are set by the first acceptance)
+    t.state->has = 0;
+    t.state->lambda = 1e-4;
+    const double st = t.prior_sigma_t, sr = t.prior_sigma_r;
+    t.state->lam_t = st > 0.0 ? 1.0 / (st * st) : 0.0;
+    t.state->lam_r = sr > 0.0 ? 1.0 / (sr * sr) : 0.0;
+    t.state->n_eff = t.n_eff;
   }
 }
 
@@ -1955,12 +2121,12 @@ __global__ __launch_bounds__(256) void k_track_begin(TrackArgs t) {
   t.pts[3 * i + 2] = z;
 }
 
-// One workgroup: the accepted pose, rounded to FP32, is the frame's pose; the solver's rows
-// of this frame go to the per-frame stats.
+// One workgroup: the accepted pose (the candidate after the final step), rounded to FP32, is
+// the frame's pose; the solver's rows of this frame go to the per-frame stats.
 __global__ __launch_bounds__(64) void k_track_end(TrackArgs t) {
   const uint32_t fs = track_slot(t);
   const int k = threadIdx.x;
-  if (k < 16) t.out[16 * (size_t)fs + k] = k < 12 ? (float)t.state->pose[k] : (k == 15 ? 1.0f : 0.0f);
+  if (k < 16) t.out[16 * (size_t)fs + k] = k < 12 ? (float)t.pose[k] : (k == 15 ? 1.0f : 0.0f);
   if (t.stats) {
     const int rows = (t.iters + 1) * kGnStats;
     for (int j = k; j < rows; j += 64) t.stats[(size_t)fs * rows + j] = t.eval_stats[j];
@@ -2154,9 +2320,9 @@ hipError_t launch_gn_accum(const GnArgs& a, int storage, int grid, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_gn_solve(const double* partials, int grid, GnState* state, double* cand, int final_step,
-                           double* stats_row, hipStream_t st) {
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(256), 0, st, partials, grid, state, cand, final_step, stats_row);
+hipError_t launch_gn_solve(const double* partials, int grid, const GnState* st_in, GnState* st_out, double* cand,
+                           int final_step, double* stats_row, hipStream_t st) {
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(256), 0, st, partials, grid, st_in, st_out, cand, final_step, stats_row);
   return hipGetLastError();
 }
 

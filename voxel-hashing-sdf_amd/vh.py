@@ -89,7 +89,8 @@ class IndexedMesh(C.Structure):
 class TrackConfig(C.Structure):
     _fields_ = [("d_prior_c2w", C.c_void_p), ("d_out_c2w", C.c_void_p), ("d_stats", C.c_void_p),
                 ("gn_iters", C.c_int32), ("point_stride", C.c_int32), ("max_point_depth", C.c_float),
-                ("prior_mode", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("prior_mode", C.c_int32), ("prior_sigma_t", C.c_float), ("prior_sigma_rot", C.c_float),
+                ("depth_points", C.c_float), ("reserved", C.c_int32 * 1)]
 
 
 class Settings(C.Structure):
@@ -379,18 +380,22 @@ class Volume:
 
     def bind_tracking(self, d_depth, fmt: int, scale: float, n: int, d_prior, d_out, d_stats=None,
                       gn_iters: int = 5, point_stride: int = 4, max_point_depth: float = 0.0,
-                      prior_mode: int = PRIOR_ABSOLUTE):
+                      prior_mode: int = PRIOR_ABSOLUTE, prior_sigma_t: float = 0.0, prior_sigma_rot: float = 0.0,
+                      depth_points: float = 0.0):
         """Tracked sequence: per frame, gn_iters solver steps from its starting pose (d_prior[f],
         or with PRIOR_INCREMENT the refined pose of frame f-1 times d_prior[f]) over the
         frame's back-projected depth (points up to max_point_depth, 0 = the volume's
         max_depth), then integration at the refined pose (written to d_out[f]).
-        d_stats: None or a float64 device tensor of n * (gn_iters + 1) * GN_STATS values."""
+        d_stats: None or a float64 device tensor of n * (gn_iters + 1) * GN_STATS values.
+        prior_sigma_t (metres) / prior_sigma_rot (radians) > 0 add a motion prior centred on
+        the starting pose; depth_points > 0 scales the depth term to that many points."""
         tc = TrackConfig()
         tc.d_prior_c2w = d_prior.data_ptr()
         tc.d_out_c2w = d_out.data_ptr()
         tc.d_stats = None if d_stats is None else d_stats.data_ptr()
         tc.gn_iters, tc.point_stride = gn_iters, point_stride
         tc.max_point_depth, tc.prior_mode = max_point_depth, prior_mode
+        tc.prior_sigma_t, tc.prior_sigma_rot, tc.depth_points = prior_sigma_t, prior_sigma_rot, depth_points
         _check(_lib.vh_bind_tracking(self._h, _tptr(d_depth), fmt, scale, n, C.byref(tc)))
 
     def bind_sequence_rgb(self, d_rgb):
