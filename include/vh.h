@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 6
+#define VH_ABI_VERSION 7
 
 typedef enum vh_status {
   VH_OK = 0,
@@ -335,6 +335,13 @@ int vh_export_blocks(vh_volume* vol, int32_t* keys, float* tsdf, float* weight,
  * per block (zero where the voxel was never observed). Colour volumes only; *n and
  * VH_ERR_TRUNCATED as vh_export_blocks. */
 int vh_export_block_colors(vh_volume* vol, uint8_t* rgb, int64_t capacity, int64_t* n);
+
+/* The marching-cubes tables of MarchingCubeParam (include/tsdf.cuh:243-534): edge_table[c]
+ * (256 ints) is the mask of the edges crossed in case c (bit e: edge e's end corners differ
+ * in sign), tri_table (256 x 16 ints) case c's triangles as edge indices, padded with -1.
+ * Corners and edges are numbered as in include/tsdf.cuh:234-241 and src/tsdf.cu:1001-1037.
+ * Either pointer may be NULL. Needs no device. */
+int vh_mc_tables(int32_t* edge_table, int32_t* tri_table);
 
 /* ------------------------------------------------------------------ query (extension) */
 /* Trilinear SDF at n world points (host arrays, 3 floats each). valid[i] = 1 iff all 8

@@ -34,6 +34,9 @@ def load():
     lib.vho_num_blocks.restype = C.c_int64
     lib.vho_num_blocks.argtypes = [P]
     lib.vho_set_carve.argtypes = [P, C.c_int]
+    lib.vho_set_shadow.argtypes = [P, C.c_int]
+    lib.vho_scatter_grad.restype = C.c_int64
+    lib.vho_scatter_grad.argtypes = [P, P, P, C.c_int64, P, C.c_int]
     lib.vho_touched.restype = C.c_int64
     lib.vho_touched.argtypes = [P, P, C.c_int64]
     lib.vho_export.restype = C.c_int64
@@ -89,9 +92,10 @@ def sort_blocks(keys, *arrays):
 
 class Oracle:
     """color: a colour volume (integrate then needs rgb); carve: every allocated voxel a frame
-    observes is updated (default: the workload's "carve" key, else off)."""
+    observes is updated (default: the workload's "carve" key, else off); shadow=False: the FP64
+    shadow is not kept (the FP32 twin alone, for timing)."""
 
-    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0, color=False, carve=None):
+    def __init__(self, w: dict, storage=None, weight_cap=None, threads=0, color=False, carve=None, shadow=True):
         lib = load()
         if threads:
             lib.vho_set_threads(threads)
@@ -105,6 +109,8 @@ class Oracle:
             lib.vho_set_color(self._h, 1)
         self.carve = int(w.get("carve", 0) if carve is None else carve)  # 2: no culling bound
         lib.vho_set_carve(self._h, self.carve)
+        if not shadow:
+            lib.vho_set_shadow(self._h, 0)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -139,6 +145,20 @@ class Oracle:
         w64 = np.zeros((n, 512), np.float64)
         _lib.vho_export(self._h, _p(keys), _p(s), _p(w), _p(s64), _p(w64), n)
         return sort_blocks(keys, s, w, s64, w64)
+
+    def scatter_grad(self, pts64, up, twin=False):
+        """Voxel gradient of sum_i up_i r_i (CPU scatter, FP64 sums): (keys, grad (N, 512))
+        sorted like export(), and the number of points with a valid sample. twin: FP32
+        weights and products as the kernels form them (FP64 weights otherwise)."""
+        n = self.num_blocks()
+        keys = np.zeros((n, 3), np.int32)
+        _lib.vho_export(self._h, _p(keys), None, None, None, None, n)
+        pts = np.ascontiguousarray(pts64, np.float64)
+        u = np.ascontiguousarray(up, np.float32)
+        g = np.zeros((n, 512), np.float64)
+        used = _lib.vho_scatter_grad(self._h, _p(pts), _p(u), len(pts), _p(g), int(bool(twin)))
+        k, g = sort_blocks(keys, g)
+        return k, g, int(used)
 
     def export_rgb(self):
         """Sorted (keys, rgb (N, 512, 3) uint8) of a colour volume."""

@@ -142,6 +142,7 @@ struct Oracle {
   std::vector<double> s64, w64;                // FP64 shadow
   bool color = false;
   int carve = 0;  // 1: every allocated block may_observe keeps is a candidate each frame; 2: every one
+  int shadow = 1;  // 0: the FP64 shadow is not updated (times the FP32 twin alone)
   std::vector<uint8_t> rgb8;                   // colour volumes: 3 channels per voxel
   std::vector<int64_t> touched;                // last frame, sorted keys
   int64_t frames = 0;
@@ -410,6 +411,7 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, cons
         o.w32[base + i] = w;
       }
       // FP64 shadow: same observation, distance and mean in double
+      if (!o.shadow) continue;
       const double X = gx * vox64 - T64[3], Y = gy * vox64 - T64[7], Z = gz * vox64 - T64[11];
       const double z64 = T64[2] * X + T64[6] * Y + T64[10] * Z;
       const double d64 = (double)dep[pix] * scale64;
@@ -667,6 +669,8 @@ int64_t vho_export_colors(void* p, uint8_t* rgb, int64_t cap) {
   return n;
 }
 int64_t vho_num_blocks(void* p) { return (int64_t)static_cast<Oracle*>(p)->keys.size(); }
+// FP64 shadow kept (1, the default) or not (0: the FP32 twin alone, for timing).
+void vho_set_shadow(void* p, int on) { static_cast<Oracle*>(p)->shadow = on ? 1 : 0; }
 // Carving on (1) or off (0): with carving every allocated voxel a frame observes is updated.
 // 2: carving with every allocated block tested voxel by voxel (no may_observe bound; the
 // brute-force check of that bound).
@@ -948,6 +952,44 @@ void vho_query64(void* p, const double* pts, int64_t n, double* sdf, double* gra
       for (int k = 0; k < 3; ++k) grad[3 * i + k] = ok ? g[k] : 0.0;
     if (valid) valid[i] = ok ? 1 : 0;
   }
+}
+
+// Voxel gradient of sum_i up_i r_i with r_i the trilinear sample at pts[i]: each point with a
+// valid sample adds up_i w_k to its 8 corner voxels, w_k = (ax ay) az from the same cell
+// fraction as sample(): in FP64 (twin = 0), or as the kernels form it (twin = 1: FP32
+// weights and the FP32 product up_i w_k, so that only the device's FP32 accumulation
+// differs). grad: blocks in insertion order (vho_export's), 512 voxels each, accumulated in
+// FP64 point by point in input order (a CPU scatter with no atomics). Returns the number of
+// points with a valid sample.
+extern "C++" template <class R>
+int64_t scatter_grad(Oracle* o, const double* pts, const float* up, int64_t n, double* grad) {
+  int64_t used = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0, g[3];
+    if (!sample<double>(*o, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], s, g)) continue;
+    ++used;
+    const double vd = (double)o->vox;
+    int ix[3];
+    R f[3];
+    for (int k = 0; k < 3; ++k) {
+      const double gq = pts[3 * i + k] / vd, fl = std::floor(gq);
+      ix[k] = (int)fl;
+      f[k] = (R)(gq - fl);
+    }
+    const R ax[2] = {(R)1 - f[0], f[0]}, ay[2] = {(R)1 - f[1], f[1]}, az[2] = {(R)1 - f[2], f[2]};
+    for (int k = 0; k < 8; ++k) {
+      const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+      const int x = ix[0] + dx, y = ix[1] + dy, z = ix[2] + dz;
+      const int64_t b = o->find(key_of(floor_div8(x), floor_div8(y), floor_div8(z)));
+      const int v = ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
+      grad[(size_t)b * 512 + v] += (double)((R)up[i] * ((ax[dx] * ay[dy]) * az[dz]));
+    }
+  }
+  return used;
+}
+int64_t vho_scatter_grad(void* p, const double* pts, const float* up, int64_t n, double* grad, int twin) {
+  Oracle* o = static_cast<Oracle*>(p);
+  return twin ? scatter_grad<float>(o, pts, up, n, grad) : scatter_grad<double>(o, pts, up, n, grad);
 }
 
 // Blocks crossed by one pixel's truncation band, in walk order; returns the count.

@@ -39,7 +39,7 @@ def test_every_declared_function_is_exported(vh):
 
 def test_abi_version_and_defaults(vh):
     lib = vh.load()
-    assert lib.vh_abi_version() == vh.ABI_VERSION == 6
+    assert lib.vh_abi_version() == vh.ABI_VERSION == 7
     c = vh.make_config()
     assert (c.width, c.height, c.log2_buckets, c.storage, c.color) == (640, 480, 18, 0, 0)
     assert C.sizeof(vh.Config) == 4 * (2 + 8 + 5 + 3 + 3 + 1 + 1 + 6)
@@ -157,3 +157,35 @@ def test_load_volume_rejects_files_that_are_not_checkpoints(vh, tmp_path):
     assert lib.vh_stream_out(None, c, 1.0, C.byref(n)) == vh.VH_ERR_INVALID_ARG
     assert lib.vh_stream_in(None, c, 1.0, C.byref(n)) == vh.VH_ERR_INVALID_ARG
     assert lib.vh_get_config(None, None) == vh.VH_ERR_INVALID_ARG
+
+
+def build_dropin_api(tmp_path):
+    """Compiles and links tests/host_checks/dropin_api.cpp, a caller of every public member of
+    the C++ drop-in, against libvh.so."""
+    src = os.path.join(ROOT, "tests", "host_checks", "dropin_api.cpp")
+    exe = str(tmp_path / "dropin_api")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-I",
+                        os.path.join(PKG, "host"), "-o", exe, src, "-L", PKG, "-lvh", "-Wl,-rpath," + PKG],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_dropin_api_caller_compiles_and_checks_what_needs_no_device(tmp_path):
+    exe = build_dropin_api(tmp_path)
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present (the GPU test runs the caller)")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 3 and "no HIP device" in r.stderr, (r.returncode, r.stdout, r.stderr)
+
+
+def test_mc_tables_match_the_oracle(vh, oracle):
+    lib = vh.load()
+    edge = (C.c_int32 * 256)()
+    tri = (C.c_int32 * (256 * 16))()
+    assert lib.vh_mc_tables(edge, tri) == vh.VH_OK
+    tri = np.array(tri).reshape(256, 16)
+    for ci in range(256):
+        want = [int(c, 16) for c in oracle.tri_case(ci)]
+        assert list(tri[ci, :len(want)]) == want and np.all(tri[ci, len(want):] == -1)
+        assert edge[ci] == sum(1 << e for e in set(want))

@@ -49,7 +49,7 @@ def test_load_frames_example_writes_indexed_ply(vh, wl, small_frames, tmp_path):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames
     scene = vh.Scene(w["scene"], wl.SEED)
-    vol = vh.Volume(vh.config_from_workload(w, log2_buckets=20, pool_blocks=1 << 19, color=1))
+    vol = vh.Volume(vh.config_from_workload(w, log2_buckets=20, pool_blocks=1 << 19, color=1, carve=1))
     for k in range(10):
         vol.integrate(depth[k].astype(np.float32) * np.float32(0.001), poses[k], rgb=scene.rgb(k, poses[k]))
     iv, fi, irgb = vol.extract_mesh_indexed(with_rgb=True)
@@ -146,3 +146,56 @@ def test_streaming_generator_writes_the_same_mesh(tmp_path):
 
     ta, tb = triangles(a), triangles(b)
     assert len(ta) > 1000 and np.array_equal(ta, tb)
+
+
+def test_every_dropin_member_on_the_gpu(vh, tmp_path):
+    """The caller of every public member (tests/host_checks/dropin_api.cpp): a plane 1 m in
+    front of the camera, three frames. Its PLY holds the library's indexed mesh; SaveTSDF's
+    dense grid holds the exported voxel values at their lattice positions (1 and colour 0
+    elsewhere); find_vertex finds every vertex inside the hash grid."""
+    from test_abi_host import build_dropin_api
+
+    exe = build_dropin_api(tmp_path)
+    print("drop-in: every public member, plane at 1 m", flush=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    tok = r.stdout.split()
+    assert tok[0] == "ok", r.stdout
+    nv, nf, found = tok[2], tok[4], tok[6]
+    v, f = read_ascii_ply(tmp_path / "plane.ply")
+    assert len(v) == int(nv) > 1000 and len(f) == int(nf)
+    inside = np.all((v >= 0) & (v < 2.56), 1).sum()
+    assert int(found) == inside > 1000
+    # the same frames through the binding
+    cfg = dict(width=640, height=480, fx=525.0, fy=525.0, cx=319.5, cy=239.5, max_depth=3.0, vox_size=0.01,
+               trunc=0.04, log2_buckets=20, pool_blocks=1 << 19, color=1, carve=1)
+    c = vh.make_config(**cfg)
+    c.origin[:] = (-1.5, -1.5, 0.5)
+    c.vox_dim[:] = (256, 256, 256)
+    vol = vh.Volume(c)
+    depth = np.ones((480, 640), np.float32)
+    rgb = (np.arange(480 * 640 * 3) * 7 % 251).astype(np.uint8).reshape(480, 640, 3)
+    T = np.eye(4, dtype=np.float32)
+    T[:2, 3] = 1.28
+    for _ in range(3):
+        vol.integrate(depth, T, rgb=rgb)
+    keys, tsdf, wt = vol.export_blocks()
+    cols = vol.export_block_colors()
+    vol.close()
+    raw = open(tmp_path / "plane.tsdf", "rb").read()
+    head = np.frombuffer(raw[:32], np.float32)
+    assert list(head) == [256, 256, 256, np.float32(-1.5), np.float32(-1.5), np.float32(0.5), np.float32(0.01),
+                          np.float32(0.04)]
+    n = 256 ** 3
+    grid = np.frombuffer(raw[32:32 + 4 * n], np.float32).reshape(256, 256, 256)  # z, y, x
+    cgrid = np.frombuffer(raw[32 + 4 * n:], np.uint8).reshape(256, 256, 256, 3)
+    i = np.arange(512)
+    lat = keys[:, None, :] * 8 + np.stack([i >> 6, (i >> 3) & 7, i & 7], 1)[None] - np.array([-150, -150, 50])
+    ok = np.all((lat >= 0) & (lat < 256), 2) & (wt > 0)
+    x, y, z = lat[ok].T
+    assert ok.sum() > 5000
+    assert np.array_equal(grid[z, y, x].view(np.uint32), tsdf[ok].view(np.uint32))
+    assert np.array_equal(cgrid[z, y, x], cols[ok])
+    mask = np.zeros((256, 256, 256), bool)
+    mask[z, y, x] = True
+    assert np.all(grid[~mask] == 1.0) and not cgrid[~mask].any()

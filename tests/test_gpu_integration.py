@@ -196,6 +196,29 @@ def test_async_host_frames_match_oracle(vh, oracle, wl, small_frames):
     vol.close()
 
 
+def test_async_frames_may_change_depth_format(vh, oracle, wl, small_frames):
+    """Queued frames of both depth formats in any order (the ring's slots have the element
+    size of the frames in flight; a format change waits for them): equal to the oracle."""
+    import torch
+
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    print("room_small_1cm: 16 asynchronous frames alternating uint16 / float32 depth", flush=True)
+    vol = vh.Volume(vh.config_from_workload(w))
+    ref = oracle.Oracle(w)
+    order = [0, 0, 1, 0, 1, 1, 0, 1, 0, 0, 0, 1, 1, 1, 0, 1]
+    pinned = torch.from_numpy(depth[:16].view(np.int16).copy()).pin_memory()
+    as_float = torch.from_numpy(depth[:16].astype(np.float32) * np.float32(0.001)).pin_memory()
+    for k, fmt in enumerate(order):
+        vol.integrate_async(as_float[k] if fmt else pinned[k], poses[k])
+    vol.sync()
+    for k in range(16):
+        ref.integrate(depth[k], poses[k])
+    assert vol.stats()["frames"] == 16
+    assert_same_volume(oracle, vol, ref)
+    vol.close()
+
+
 def test_float_depth_input_matches_u16(vh, oracle, wl, small_frames):
     w = wl.WORKLOADS["room_small_1cm"]
     depth, poses = small_frames

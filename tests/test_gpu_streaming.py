@@ -199,7 +199,20 @@ def test_streaming_errors_leave_the_volume_unchanged(vh, oracle, wl, small_frame
     for k in range(30, 35):
         probe.integrate(depth[k], poses[k])
     n2 = probe.stats()["blocks_allocated"]
+    probe.stream_in()
+    n_union = probe.stats()["blocks_allocated"]
     probe.close()
+    # the pool check counts only the keys that are not resident: a pool of exactly the union
+    # takes the stored blocks (the others merge in place)
+    exact = vh.Volume(vh.config_from_workload(w, pool_blocks=n_union))
+    for k in range(5):
+        exact.integrate(depth[k], poses[k])
+    exact.stream_out((0, 0, 0), 0.0)
+    for k in range(30, 35):
+        exact.integrate(depth[k], poses[k])
+    assert n1 + n2 > n_union and exact.stream_in() == n1
+    assert exact.stats()["blocks_allocated"] == n_union
+    exact.close()
     print(f"streaming errors: {n1} + {n2} blocks, pool {max(n1, n2) + 16}", flush=True)
     vol = vh.Volume(vh.config_from_workload(w, pool_blocks=max(n1, n2) + 16))
     for k in range(5):
@@ -229,3 +242,63 @@ def test_streaming_errors_leave_the_volume_unchanged(vh, oracle, wl, small_frame
         vh.Volume.load(str(path))
     assert e.value.code == vh.VH_ERR_IO
     vol.close()
+
+
+def _same(got, want):
+    keys, s, wt = got
+    assert np.array_equal(keys, want[0]) and np.array_equal(wt, want[2])
+    assert np.array_equal(s.view(np.uint32), want[1].view(np.uint32))
+
+
+def _stream_cycle(vh, w, depth, poses, **over):
+    """Frames 0-4, every block out, frames 5-9 (part of the keys allocated again)."""
+    vol = vh.Volume(vh.config_from_workload(w, **over))
+    for k in range(5):
+        vol.integrate(depth[k], poses[k])
+    n_a = vol.stream_out((0, 0, 0), 0.0)
+    for k in range(5, 10):
+        vol.integrate(depth[k], poses[k])
+    return vol, n_a
+
+
+def test_stream_out_and_save_keep_one_copy_per_key(vh, oracle, wl, small_frames, tmp_path):
+    """A key that was streamed out and allocated again is in the pool and in the host store.
+    Streaming it out again merges the stored copy into the resident block first (the same
+    weighted mean as vh_stream_in), so nothing is lost; a checkpoint taken in that state holds
+    each key once. Both end equal, bit for bit, to the volume that streamed everything back in."""
+    w = wl.WORKLOADS["room_small_1cm"]
+    depth, poses = small_frames
+    print("streaming: stream out and save with keys resident and stored", flush=True)
+    ref, n_a = _stream_cycle(vh, w, depth, poses)
+    n_b = ref.stats()["blocks_allocated"]
+    ref.stream_in()
+    n_union = ref.stats()["blocks_allocated"]
+    n_both = ref.stats()["stream_merged_total"]
+    assert n_both > 300 and n_union == n_a + n_b - n_both
+    want = _sorted_volume(oracle, ref, False)
+
+    # stream out again: one copy per key in the store, then everything back
+    vol, _ = _stream_cycle(vh, w, depth, poses)
+    assert vol.stream_out((0, 0, 0), 0.0) == n_b
+    st = vol.stats()
+    assert st["host_blocks"] == n_union and st["blocks_allocated"] == 0 and st["stream_merged_total"] == n_both
+    path = tmp_path / "out.vhv"
+    vol.save(str(path))
+    assert vol.stream_in() == n_union
+    _same(_sorted_volume(oracle, vol, False), want)
+    vol.close()
+    loaded = vh.Volume.load(str(path))
+    _same(_sorted_volume(oracle, loaded, False), want)
+    loaded.close()
+
+    # save while keys are in both places: the checkpoint loads, with one copy per key
+    vol, _ = _stream_cycle(vh, w, depth, poses)
+    path2 = tmp_path / "both.vhv"
+    vol.save(str(path2))
+    assert vol.stats()["host_blocks"] == n_a - n_both
+    vol.close()
+    loaded = vh.Volume.load(str(path2))
+    assert loaded.stats()["blocks_allocated"] == n_union
+    _same(_sorted_volume(oracle, loaded, False), want)
+    loaded.close()
+    ref.close()

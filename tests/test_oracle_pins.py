@@ -38,6 +38,47 @@ def test_tables_agree(oracle):
     assert packed_table() == oracle_table(oracle)
 
 
+# SHA-256 of the reference's triTable (include/tsdf.cuh:278-534) in the canonical form of
+# canonical_tri (each case's edge list without the -1 padding, cases joined by ';'), and of its
+# edgeTable (include/tsdf.cuh:243-276, decimal values joined by ','), taken from the text.
+TRI_TABLE_SHA256 = "4e5290eaffc91e59f0a81598feeaeddb136fd45b4c5ca76a814adca163998f1a"
+EDGE_TABLE_SHA256 = "07256d860bcfcb9958bf3fd3af654974575fa3a908c1fb1e6bf6ffab34e3f0be"
+REFERENCE_TABLES = "/root/reference/include/tsdf.cuh"
+
+
+def canonical_tri(table):
+    return ";".join(",".join(str(e) for e in case) for case in table)
+
+
+def edge_masks():
+    return [sum(1 << e for e, (a, b) in enumerate(EDGES) if ((ci >> a) & 1) != ((ci >> b) & 1)) for ci in range(256)]
+
+
+def test_tables_match_the_reference_digest(oracle):
+    import hashlib
+
+    for table in (packed_table(), oracle_table(oracle)):
+        assert hashlib.sha256(canonical_tri(table).encode()).hexdigest() == TRI_TABLE_SHA256
+    # the edge table is derived (the crossed edges of each case), so it is pinned by its digest
+    assert hashlib.sha256(",".join(map(str, edge_masks())).encode()).hexdigest() == EDGE_TABLE_SHA256
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_TABLES), reason="the reference sources are not present")
+def test_tables_match_the_reference_text(oracle):
+    """Parses the two tables out of the reference's header text and compares them entry by
+    entry with the device table (vh_mc_table.h) and the oracle's."""
+    txt = open(REFERENCE_TABLES).read()
+    tri = re.search(r"triTable\[256\]\[16\]\s*=\s*\{(.*?)\};", txt, re.S)
+    nums = [int(x) for x in re.findall(r"-?\d+", tri.group(1))]
+    assert len(nums) == 256 * 16
+    ref = [[e for e in nums[16 * i:16 * i + 16] if e >= 0] for i in range(256)]
+    assert all(nums[16 * i + len(ref[i]):16 * i + 16] == [-1] * (16 - len(ref[i])) for i in range(256))
+    assert packed_table() == ref
+    assert oracle_table(oracle) == ref
+    edge = re.search(r"edgeTable\[256\]\s*=\s*\{(.*?)\};", txt, re.S)
+    assert [int(x, 16) for x in re.findall(r"0x[0-9a-fA-F]+", edge.group(1))] == edge_masks()
+
+
 def test_table_uses_exactly_the_crossed_edges(oracle):
     """Every listed edge joins corners of opposite sign and every such edge is listed:
     the reference's edgeTable is therefore derivable and the two tables agree."""
@@ -254,3 +295,18 @@ def test_touched_set_is_the_union_of_pixel_bands(oracle, wl, small_frames):
     for k in sel:
         want |= {tuple(b) for b in o.band(poses[0], int(us[k]), int(vs[k]), float(d[vs[k], us[k]]))}
     assert want <= got
+
+
+def test_fp32_twin_alone_is_the_same_twin(oracle, wl, small_frames):
+    """The CPU baseline times the oracle without its FP64 shadow: the twin (keys, values,
+    weights) is identical with and without it, and the shadow stays empty."""
+    w = dict(wl.WORKLOADS["room_small_1cm"])
+    depth, poses = small_frames
+    a, b = oracle.Oracle(w, shadow=False), oracle.Oracle(w)
+    for k in range(8):
+        assert a.integrate(depth[k], poses[k]) == b.integrate(depth[k], poses[k])
+    ka, sa, wa, s64a, w64a = a.export()
+    kb, sb, wb, s64b, w64b = b.export()
+    assert np.array_equal(ka, kb) and np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+    assert np.array_equal(wa, wb)
+    assert not s64a.any() and not w64a.any() and w64b.max() > 0

@@ -19,6 +19,7 @@
 #include <string>
 #include <tuple>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "vh.h"
@@ -136,6 +137,7 @@ struct vh_volume {
   static constexpr int kRing = 4;
   hipStream_t copy_stream = nullptr;
   void* ring_depth = nullptr;       // kRing frames of W*H elements
+  int32_t ring_format = -1;         // depth format of the queued frames (slot stride W*H*element)
   uint8_t* ring_rgb = nullptr;      // colour volumes: kRing frames of W*H*3 bytes
   float* ring_pose = nullptr;       // kRing x 16 (device)
   float* ring_pose_host = nullptr;  // kRing x 16 (page-locked)
@@ -869,13 +871,14 @@ int stream_out_blocks(vh_volume* v, const std::vector<unsigned long long>& bk, c
 }
 
 // Moves the host-store blocks at `slots` into the pool (inserting new keys, merging keys that
-// are resident again) and frees their slots. Queued work must have finished.
+// are resident again) and frees their slots; n_new of them are not resident (each takes a pool
+// block, the others merge in place). Queued work must have finished.
 int stream_in_blocks(vh_volume* v, const std::vector<uint32_t>& slots, uint32_t frame_id, uint32_t pool_count,
-                     int64_t* moved) {
+                     int64_t* moved, uint32_t n_new) {
   const uint32_t n = (uint32_t)slots.size();
   *moved = 0;
   if (!n) return VH_OK;
-  if ((uint64_t)pool_count + n > (uint64_t)v->cfg.pool_blocks)
+  if ((uint64_t)pool_count + n_new > (uint64_t)v->cfg.pool_blocks)
     return fail(VH_ERR_POOL_FULL, "the pool cannot take the blocks to stream in (nothing moved)");
   int rc = ensure_stream_index(v, n);
   if (rc || (rc = upload_slab_table(v))) return rc;
@@ -900,6 +903,31 @@ int stream_in_blocks(vh_volume* v, const std::vector<uint32_t>& slots, uint32_t 
   }
   v->streamed_in += *moved;
   return sync_and_check(v);
+}
+
+// Of the host-store slots `slots`, how many hold a key that is not resident (resident: bk).
+uint32_t count_new_keys(const vh_volume* v, const std::vector<uint32_t>& slots, const std::vector<unsigned long long>& bk) {
+  std::unordered_set<unsigned long long> res(bk.begin(), bk.end());
+  uint32_t n = 0;
+  for (uint32_t s : slots) n += res.count(v->slot_key[s]) ? 0u : 1u;
+  return n;
+}
+
+// Merges into the resident blocks every host-store block whose key is resident again (the
+// weighted mean of vh_stream_in), so that no key is in both places. Queued work must have
+// finished; bk: the resident keys.
+int merge_resident_duplicates(vh_volume* v, const std::vector<unsigned long long>& bk, uint32_t frame_id,
+                              uint32_t pool_count) {
+  if (v->store_index.empty()) return VH_OK;
+  std::vector<uint32_t> dup;
+  for (unsigned long long k : bk) {
+    auto it = v->store_index.find(k);
+    if (it != v->store_index.end()) dup.push_back(it->second);
+  }
+  if (dup.empty()) return VH_OK;
+  std::sort(dup.begin(), dup.end());
+  int64_t moved = 0;
+  return stream_in_blocks(v, dup, frame_id, pool_count, &moved, 0);
 }
 
 int ensure_gn_scratch(vh_volume* v) {
@@ -1158,6 +1186,10 @@ int vh_integrate_async(vh_volume* v, const vh_frame* f) {
     drop_graphs(v, 1 << 6);
     v->ring_scale = f->depth_scale;
   }
+  // The slots are laid out with the element size of the frames in flight: before a frame of
+  // the other format is copied, the queued frames finish (their slots would overlap).
+  if (v->ring_format >= 0 && v->ring_format != f->depth_format) VH_HIP(hipStreamSynchronize(v->stream));
+  v->ring_format = f->depth_format;
   const int slot = (int)(v->ring_next % R);
   if (v->ring_next >= (uint64_t)R) {
     VH_HIP(hipStreamWaitEvent(v->copy_stream, v->ring_done[slot], 0));  // the frame that read the slot
@@ -1747,6 +1779,9 @@ int vh_stream_out(vh_volume* v, const float center[3], float radius, int64_t* n_
   for (uint32_t b = 0; b < (uint32_t)bk.size(); ++b)
     if (block_dist2(bk[b], v->cfg.vox_size, cc) > r2) out.push_back(b);
   if (out.empty()) return VH_OK;
+  // a key that left before and was observed again: its stored copy merges into the resident
+  // block first, so the block leaves as one copy (the pool and its order do not change)
+  if ((rc = merge_resident_duplicates(v, bk, c.frame_id, c.pool_count))) return rc;
   *n_blocks = (int64_t)out.size();
   return stream_out_blocks(v, bk, out, c.frame_id);
 }
@@ -1763,9 +1798,11 @@ int vh_stream_in(vh_volume* v, const float center[3], float radius, int64_t* n_b
   for (const auto& kv : v->store_index)
     if (block_dist2(kv.first, v->cfg.vox_size, cc) <= r2) slots.push_back(kv.second);
   std::sort(slots.begin(), slots.end());
+  if (slots.empty()) return VH_OK;
   FrameCtl c{};
-  if ((rc = read_ctl(v, &c))) return rc;
-  return stream_in_blocks(v, slots, c.frame_id, c.pool_count, n_blocks);
+  std::vector<unsigned long long> bk;
+  if ((rc = resident_blocks(v, c, bk))) return rc;
+  return stream_in_blocks(v, slots, c.frame_id, c.pool_count, n_blocks, count_new_keys(v, slots, bk));
 }
 
 int vh_get_config(vh_volume* v, vh_config* out) {
@@ -1782,6 +1819,8 @@ int vh_save_volume(vh_volume* v, const char* path) {
   FrameCtl c{};
   std::vector<unsigned long long> bk;
   if ((rc = resident_blocks(v, c, bk))) return rc;
+  // each key once: stored copies of resident keys merge into the resident blocks first
+  if ((rc = merge_resident_duplicates(v, bk, c.frame_id, c.pool_count))) return rc;
   const uint32_t wpb = words_per_block(v);
   std::vector<uint32_t> host;  // occupied store slots, ascending
   for (const auto& kv : v->store_index) host.push_back(kv.second);
@@ -1882,7 +1921,7 @@ int vh_load_volume(const char* path, int32_t device, vh_volume** out) {
     for (size_t i = 0; i < n; ++i) slots[i] = (uint32_t)i;
     int64_t moved = 0;
     std::lock_guard<std::mutex> lk(v->mu);
-    rc = stream_in_blocks(v, slots, c.frame_id, 0u, &moved);
+    rc = stream_in_blocks(v, slots, c.frame_id, 0u, &moved, (uint32_t)n);
   }
   if (rc) {
     std::string msg = vh_last_error();

@@ -41,7 +41,7 @@ DEPTH_U16 = 0
 DEPTH_F32 = 1
 PRIOR_ABSOLUTE = 0
 PRIOR_INCREMENT = 1
-ABI_VERSION = 6
+ABI_VERSION = 7
 # per-evaluation solver stats: n_valid, rms, |delta xi|, lambda, accepted, accepted pose (12)
 GN_STATS = 17
 
@@ -140,6 +140,7 @@ _PROTOS = [
     ("vh_extract_mesh_indexed_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                                  C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("vh_extract_mesh_indexed", C.c_int, [C.c_void_p, C.POINTER(IndexedMesh)]),
+    ("vh_mc_tables", C.c_int, [C.c_void_p, C.c_void_p]),
     ("vh_indexed_mesh_free", C.c_int, [C.POINTER(IndexedMesh)]),
     ("vh_save_ply", C.c_int, [C.c_void_p, C.c_char_p]),
     ("vh_stream_out", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int64)]),
