@@ -362,7 +362,7 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, cons
       // homogeneous pixel point is formed per block origin and voxel step, H(R^T(o - t)) +
       // H(vox R^T (k, j, i)): the column offset as fma(j, b, i*c) added to the origin, then
       // fma(k, a, column) per voxel; one reciprocal of z feeds both image coordinates X/z, Y/z,
-      // which round to the nearest pixel with ties to even; the truncated distance is
+      // each rounded to the nearest pixel (one rounding, ties to even); the truncated distance is
       // fma(-z, 1/mu, d/mu), the voxel observed iff it exceeds -1. (The reference rotates each
       // voxel's p - t, divides x/z, y/z and by mu, and rounds ties away from zero:
       // src/tsdf.cu:75-93,738, compiled with -use_fast_math.)
@@ -374,11 +374,14 @@ void integrate(Oracle& o, const uint16_t* dep, float scale, const float* T, cons
       }
       const float zc = P[2];
       if (!(zc >= 0x1p-64f && zc <= 0x1p64f)) continue;  // depth range of the projection
+      // the image coordinates: X * (1/z) rounded once to the nearest integer (ties to even)
+      // by adding M = 1.5 * 2^23 in the same fused multiply-add, fma(X, 1/z, M) - M
       const float rz = 1.0f / zc;
-      const float uf = std::nearbyint(P[0] * rz);
-      const float vf = std::nearbyint(P[1] * rz);
-      if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
-      const size_t pix = (size_t)vf * W + (size_t)uf;
+      constexpr float M = 0x1.8p23f;
+      const float uf = std::fmaf(P[0], rz, M);
+      const float vf = std::fmaf(P[1], rz, M);
+      if (!(uf >= M && uf <= M + (float)(W - 1) && vf >= M && vf <= M + (float)(H - 1))) continue;
+      const size_t pix = (size_t)(vf - M) * W + (size_t)(uf - M);
       const float t = std::fmaf(-zc, inv_mu, dm[pix]);
       if (!(t > -1.0f)) continue;
       const float dist = std::fmin(1.0f, t);

@@ -33,6 +33,9 @@ __device__ __forceinline__ uint32_t read_lane_u32(uint32_t v, int lane) {
 __device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
 // A value every lane holds identically, moved to a scalar register.
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long v) {
+  return (unsigned long long)uniform((uint32_t)v) | (unsigned long long)uniform((uint32_t)(v >> 32)) << 32;
+}
 
 __device__ __forceinline__ void raise_flag(FrameCtl* ctl, uint32_t f) { atomicOr(&ctl->flags, f); }
 
@@ -208,16 +211,14 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
     const int v = v0 + 16 * r;
     dr[r] = 0.0f;
     if (u < a.W && v < a.H) {
+      const size_t staged = ((size_t)parity * kMaxBatch + jb) * a.W * a.H + (size_t)v * a.W + u;
       if (a.rgba) {  // the pixel's colour, packed for the integration's one-word gathers
-        const size_t pix = (size_t)v * a.W + u;
-        const uint8_t* px = a.rgb + ((size_t)fs * a.W * a.H + pix) * 3;
-        a.rgba[((size_t)parity * kMaxBatch + jb) * a.W * a.H + pix] =
-            (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
+        const uint8_t* px = a.rgb + ((size_t)fs * a.W * a.H + (size_t)v * a.W + u) * 3;
+        a.rgba[staged] = (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
       }
       dr[r] = depth_at<kU16>(base, a.depth_scale, v * a.W + u);
       // the pre-scaled depth the integration of this frame gathers
-      a.dm[((size_t)parity * kMaxBatch + jb) * a.W * a.H + (size_t)v * a.W + u] =
-          scaled_depth(dr[r], a.max_depth, a.inv_mu);
+      a.dm[staged] = scaled_depth(dr[r], a.max_depth, a.inv_mu);
     }
   }
 
@@ -297,10 +298,15 @@ __global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
 
 // ============================================================================ integration
 // tsdf_update (vh_math.h) with RN(1/(w + 1)) by recip_rn (the correctly rounded reciprocal).
-// cap_or_inf: the weight cap, +inf when uncapped (min is then the identity).
-__device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap_or_inf) {
+// cap_or_inf: the weight cap, +inf when uncapped (min is then the identity). kTab: RN(1/w)
+// from the workgroup's table (rcp_table) instead of the reciprocal instructions; the launch
+// takes it when every weight it can meet is below kRcpTab - 1 (the values are equal).
+constexpr uint32_t kRcpTab = 1024;
+template <bool kTab>
+__device__ __forceinline__ void mean_update(float& s, float& w, float dist, float cap_or_inf, const float* rcp_tab) {
   const float w_new = w + 1.0f;
-  s = __builtin_fmaf(dist - s, recip_rn(w_new), s);
+  const float r = kTab ? rcp_tab[(uint32_t)w_new] : recip_rn(w_new);
+  s = __builtin_fmaf(dist - s, r, s);
   w = fminf(w_new, cap_or_inf);
 }
 
@@ -358,20 +364,22 @@ __device__ __forceinline__ bool block_inside(const FrameArgs& a, const float ax[
 // the fetches are in flight together. Branch-free: every lane computes, the observation flag
 // selects. dmf: the frame's pre-scaled depth image (scaled_depth of each pixel).
 // kInside (block_inside holds): the range checks are known to pass and are left out; the
-// values computed are the same.
-template <int kStorage, bool kColor, bool kInside>
+// values computed are the same. (Issuing the next frame's gathers before this frame's
+// updates, with two register sets, was slower: 92 against 86 us per FP32 batch launch.)
+template <int kStorage, bool kColor, bool kInside, bool kTab>
 __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[3], const float cl[3],
                                              const float* dmf, const uint32_t* rgba,
                                              typename VoxelIO<kStorage>::Word old[8], uint32_t cold[8],
-                                             uint32_t& upd) {
+                                             uint32_t& upd, const float* rcp_tab) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
+  const uint32_t w4 = (4u * (uint32_t)a.W) & 0xffffffu, bias = pix_bias(a.W);
   // two halves of 4 voxels: fewer values live at once (occupancy) for 4 fetches in flight
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
     float zc[4];
     int pix[4];
-    uint32_t off[4];  // kInside: byte offset of the pixel's scaled depth (32-bit, off a scalar base)
+    uint32_t off[4];  // kInside: byte offset of the pixel in the depth (and RGB) image, off a scalar base
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float fq = (float)(h + r);
@@ -379,10 +387,9 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[
       zc[r] = __builtin_fmaf(fq, ax[2], cl[2]);
       const float iz = recip_rn(zc[r]);
       if constexpr (kInside) {
-        const float uf = rintf(X * iz), vf = rintf(Y * iz);
-        const uint32_t p = ((uint32_t)vf & 0xffffffu) * ((uint32_t)a.W & 0xffffffu) + (uint32_t)uf;
-        if constexpr (kColor) pix[r] = (int)p;  // the pixel index serves the depth and RGB gathers
-        else off[r] = p << 2;
+        const uint32_t ub = __float_as_uint(__builtin_fmaf(X, iz, kPixMagic));
+        const uint32_t vb = __float_as_uint(__builtin_fmaf(Y, iz, kPixMagic));
+        off[r] = (vb & 0xffffffu) * w4 + ((ub << 2) - bias);
       } else {
         pix[r] = pixel_of(X, Y, zc[r], iz, a.W, a.H);
       }
@@ -390,13 +397,16 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[
     float dv[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      if constexpr (kInside && !kColor) dv[r] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmf) + off[r]);
-      else dv[r] = dmf[kInside ? pix[r] : max(pix[r], 0)];
+      if constexpr (kInside) dv[r] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmf) + off[r]);
+      else dv[r] = dmf[max(pix[r], 0)];
     }
     uint32_t px[4];
     if constexpr (kColor) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) px[r] = rgba[kInside ? pix[r] : max(pix[r], 0)];
+      for (int r = 0; r < 4; ++r) {
+        if constexpr (kInside) px[r] = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(rgba) + off[r]);
+        else px[r] = rgba[max(pix[r], 0)];
+      }
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -410,7 +420,7 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[
         const uint32_t c = color_update(cold[q], wv, px[r]);
         cold[q] = ok ? c : cold[q];
       }
-      mean_update(sv, wv, dist, a.cap);
+      mean_update<kTab>(sv, wv, dist, a.cap, rcp_tab);
       const Word nw = IO::pack(sv, wv);
       old[q] = ok ? nw : old[q];
       if constexpr (kColor) upd |= (uint32_t)ok << q;
@@ -426,11 +436,11 @@ struct FrameLds {
   BlockFrame h;
 };
 
-// s_frm: the batch's k frames.
-template <int kStorage, bool kColor>
+// s_frm: the batch's k frames; rcp_tab: RN(1/w) for w < kRcpTab (read when kTab).
+template <int kStorage, bool kColor, bool kTab>
 __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t f0, uint32_t k,
                                  uint32_t parity, unsigned long long* s_val, unsigned long long* s_key,
-                                 uint32_t* s_mark, const FrameLds* s_frm) {
+                                 uint32_t* s_mark, const FrameLds* s_frm, const float* rcp_tab) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
@@ -519,9 +529,9 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         const size_t fo = ((size_t)parity * kMaxBatch + jf) * frame_px;
         const uint32_t* rgba = kColor ? a.rgba + fo : nullptr;
         if (block_inside(a, hf.ax, cl))
-          frame_update<kStorage, kColor, true>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd);
+          frame_update<kStorage, kColor, true, kTab>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd, rcp_tab);
         else
-          frame_update<kStorage, kColor, false>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd);
+          frame_update<kStorage, kColor, false, kTab>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd, rcp_tab);
       }
       // Depth-only volumes write the block whole (unobserved voxels keep the words read,
       // fresh blocks zero there; the wave owns the block in this launch), which saves the
@@ -624,12 +634,15 @@ __device__ __forceinline__ void stage_poses(const FrameArgs& a, uint32_t cur, ui
   __syncthreads();
 }
 // The poses and the homogeneous steps of a batch's frames into LDS. All 256 lanes call it.
-__device__ __forceinline__ void stage_frames(const FrameArgs& a, uint32_t cur, uint32_t k, FrameLds* s_frm) {
+__device__ __forceinline__ void stage_frames(const FrameArgs& a, uint32_t cur, uint32_t k, FrameLds* s_frm,
+                                             float* rcp_tab, bool tab) {
   const uint32_t t = threadIdx.x;
   if (t < 12u * k) {
     const uint32_t j = t / 12u, q = t - 12u * j;
     s_frm[j].T[q] = a.poses[16 * (size_t)input_slot(cur, j, a.n_seq) + q];
   }
+  if (tab)
+    for (uint32_t i = t; i < kRcpTab; i += blockDim.x) rcp_tab[i] = recip_rn((float)i);
   __syncthreads();
   if (t < k) {
     float T[12];
@@ -640,20 +653,34 @@ __device__ __forceinline__ void stage_frames(const FrameArgs& a, uint32_t cur, u
   __syncthreads();
 }
 
+// True when every weight the batch can meet is below kRcpTab - 1: capped below it, or fewer
+// frames integrated (with this batch) than that (a voxel gains at most 1 per frame; a merge
+// of streamed blocks adds the weights of disjoint frames).
+__device__ __forceinline__ bool rcp_table_covers(const FrameArgs& a, const FrameCtl* ctl, uint32_t k) {
+  return a.cap < (float)(kRcpTab - 1) || uniform64(ctl->frames) + k < (unsigned long long)(kRcpTab - 1);
+}
+
 // Integration of the batch of k frames that the last allocation launch prepared.
 template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate(FrameArgs a, uint32_t k) {
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
   __shared__ uint32_t s_mark[kIntegrateWaves][64];
   __shared__ FrameLds s_frm[kMaxBatch];
+  __shared__ float s_rcp[kRcpTab];
   const int wave = threadIdx.x >> 6;
   const FrameCtl* ctl = a.ctl;
   // control values are uniform: keep them (and what derives from them) in scalar registers
   const uint32_t f0 = uniform(ctl->frame_id), parity = uniform(ctl->batch_id) & 1u,
                  cur = uniform(ctl->cursor[a.cursor_id]);
-  stage_frames(a, cur, k, s_frm);
-  integrate_worker<kStorage, kColor>(a, blockIdx.x * kIntegrateWaves + wave, gridDim.x * kIntegrateWaves, f0, k,
-                                     parity, s_val[wave], s_key[wave], s_mark[wave], s_frm);
+  const bool tab = rcp_table_covers(a, ctl, k);
+  stage_frames(a, cur, k, s_frm, s_rcp, tab);
+  const uint32_t worker = blockIdx.x * kIntegrateWaves + wave, workers = gridDim.x * kIntegrateWaves;
+  if (tab)
+    integrate_worker<kStorage, kColor, true>(a, worker, workers, f0, k, parity, s_val[wave], s_key[wave], s_mark[wave],
+                                             s_frm, s_rcp);
+  else
+    integrate_worker<kStorage, kColor, false>(a, worker, workers, f0, k, parity, s_val[wave], s_key[wave],
+                                              s_mark[wave], s_frm, s_rcp);
   batch_done(a, gridDim.x, f0, k, parity);
 }
 
@@ -670,6 +697,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
   __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
   __shared__ uint32_t s_mark[kIntegrateWaves][64];
   __shared__ FrameLds s_frm[kMaxBatch];
+  __shared__ float s_rcp[kRcpTab];
   const uint32_t b = blockIdx.x;
   const bool alloc = b >= int_groups;
   const uint32_t idx = alloc ? b - int_groups : b;
@@ -681,9 +709,15 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
                                  input_slot(cur, k_int + jb, a.n_seq));
   } else {
     const int wave = threadIdx.x >> 6;
-    stage_frames(a, cur, k_int, s_frm);  // uniform branch: the whole workgroup integrates
-    integrate_worker<kStorage, kColor>(a, idx * kIntegrateWaves + wave, int_groups * kIntegrateWaves, f0, k_int,
-                                       batch & 1u, s_val[wave], s_key[wave], s_mark[wave], s_frm);
+    const bool tab = rcp_table_covers(a, ctl, k_int);
+    stage_frames(a, cur, k_int, s_frm, s_rcp, tab);  // uniform branch: the whole workgroup integrates
+    const uint32_t worker = idx * kIntegrateWaves + wave, workers = int_groups * kIntegrateWaves;
+    if (tab)
+      integrate_worker<kStorage, kColor, true>(a, worker, workers, f0, k_int, batch & 1u, s_val[wave], s_key[wave],
+                                               s_mark[wave], s_frm, s_rcp);
+    else
+      integrate_worker<kStorage, kColor, false>(a, worker, workers, f0, k_int, batch & 1u, s_val[wave], s_key[wave],
+                                                s_mark[wave], s_frm, s_rcp);
   }
   batch_done(a, gridDim.x, f0, k_int, batch & 1u);
 }

@@ -141,24 +141,37 @@ VH_HD void column_h(const BlockFrame& f, const float cb[3], int j, int i, float 
 // rules, src/tsdf.cu:706-738), in two steps so that a caller can issue all depth fetches of a
 // block before using any of them:
 //   pixel: one correctly rounded reciprocal iz = 1/z (passed in by the caller) serves both
-//     image coordinates, rint(X * iz), rint(Y * iz) (nearest pixel, ties to even; the
-//     reference divides per coordinate and rounds ties away from zero, cam2frame,
-//     src/tsdf.cu:75-79); -1 when z is outside 2^-64 <= z <= 2^64 or the pixel outside the
-//     image. The range bound keeps z and 1/z normal, where the kernels' two-instruction
+//     image coordinates; each is the exact product X * iz rounded once to the nearest integer
+//     (ties to even) by adding 1.5 * 2^23 in the same fused multiply-add: fma(X, iz, M) - M
+//     (the reference divides per coordinate and rounds ties away from zero, cam2frame,
+//     src/tsdf.cu:75-79). Within the image the sum lies in [2^23, 2^24), where the float
+//     spacing is 1, so the sum is M plus the rounded coordinate, and its bits give the pixel
+//     index without a conversion. -1 when z is outside 2^-64 <= z <= 2^64 or the pixel outside
+//     the image. The range bound keeps z and 1/z normal, where the kernels' three-instruction
 //     reciprocal equals the IEEE quotient.
 //   truncation: the pixel's pre-scaled depth m = d * (1/mu) for a depth 0 < d <= max_depth
 //     (-inf otherwise) gives t = fma(-z, 1/mu, m) = (d - z) / mu with one rounding; the voxel
 //     is observed iff t > -1 (the reference's d - z > -mu, with d > 0 and d <= max_depth), and
 //     dist = min(1, t).
 constexpr float kMinZ = 0x1p-64f, kMaxZ = 0x1p64f;
+constexpr float kPixMagic = 0x1.8p23f;  // M = 1.5 * 2^23, bits 0x4B400000
+constexpr uint32_t kPixMagicBits = 0x4B400000u;
 VH_HD int pixel_of(float X, float Y, float z, float iz, int W, int H) {
-  const float uf = rintf(X * iz);
-  const float vf = rintf(Y * iz);
-  const bool inside = (z >= kMinZ) & (z <= kMaxZ) & (uf >= 0.0f) & (uf < (float)W) & (vf >= 0.0f) & (vf < (float)H);
+  const float uf = fmaf(X, iz, kPixMagic);
+  const float vf = fmaf(Y, iz, kPixMagic);
+  const bool inside = (z >= kMinZ) & (z <= kMaxZ) & (uf >= kPixMagic) & (uf <= kPixMagic + (float)(W - 1)) &
+                      (vf >= kPixMagic) & (vf <= kPixMagic + (float)(H - 1));
   // both factors are below 2^24 (W, H <= 16384): the masks let the compiler use a 24-bit
   // multiply-add instead of a 64-bit one
-  return inside ? (int)(((uint32_t)vf & 0xffffffu) * ((uint32_t)W & 0xffffffu) + (uint32_t)uf) : -1;
+  return inside ? (int)(((uint32_t)(vf - kPixMagic) & 0xffffffu) * ((uint32_t)W & 0xffffffu) + (uint32_t)(uf - kPixMagic))
+                : -1;
 }
+// Byte offset 4 (v W + u) of pixel (u, v) from the sums uf = M + u, vf = M + v of pixel_of
+// (both inside the image): the low 24 bits of vf are 0x400000 + v and the bits of uf are
+// 0x4B400000 + u, so low24(vf) * 4W + 4 bits(uf) exceeds the offset by
+// pix_bias(W) = 4 (0x400000 W + 0x4B400000) modulo 2^32 (one shift-add and one 24-bit
+// multiply-add).
+VH_HD uint32_t pix_bias(int P) { return 4u * (0x400000u * (uint32_t)P + kPixMagicBits); }
 VH_HD float scaled_depth(float d, float max_depth, float inv_mu) {
   return (d > 0.0f && d <= max_depth) ? d * inv_mu : -INFINITY;
 }
