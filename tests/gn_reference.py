@@ -128,9 +128,18 @@ def motion_prior(T_start, T, sigma_t, sigma_rot):
     st, sr = float(np.float32(sigma_t)), float(np.float32(sigma_rot))
     lt, lr = (1.0 / (st * st) if st > 0 else 0.0), (1.0 / (sr * sr) if sr > 0 else 0.0)
     lam = np.array([lt] * 3 + [lr] * 3)
-    xi = se3_log(np.linalg.inv(np.asarray(T_start, np.float64)) @ np.asarray(T, np.float64))
-    M = adjoint(np.linalg.inv(np.asarray(T, np.float64)))
-    return 0.5 * float(xi @ (lam * xi)), M.T @ (lam[:, None] * M), M.T @ (lam * xi)
+    T = np.asarray(T, np.float64)
+    xi = se3_log(np.linalg.inv(np.asarray(T_start, np.float64)) @ T)
+    # M^T Lam M and M^T Lam xi in closed form (R R^T = I): [[lt I, -lt [t]x], [lt [t]x,
+    # lt (|t|^2 I - t t^T) + lr I]] and (g, t x g + lr R xi_phi), g = lt R xi_rho
+    R, t = T[:3, :3], T[:3, 3]
+    K = skew(t)
+    A = np.zeros((6, 6))
+    A[:3, :3], A[:3, 3:], A[3:, :3] = lt * np.eye(3), -lt * K, lt * K
+    A[3:, 3:] = lt * (float(t @ t) * np.eye(3) - np.outer(t, t)) + lr * np.eye(3)
+    g = lt * (R @ xi[:3])
+    b = np.concatenate([g, np.cross(t, g) + lr * (R @ xi[3:])])
+    return 0.5 * float(xi @ (lam * xi)), A, b
 
 
 def _tracked(ref, pts, T, sc, T_start, prior):

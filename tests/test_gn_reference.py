@@ -104,3 +104,19 @@ def test_motion_prior_weights_translation_and_rotation_by_their_sigmas():
     assert abs(motion_prior(T_start, T_start @ se3_exp(xr), st, sr)[0] - 2.0) < 1e-9
     assert motion_prior(T_start, T_start @ se3_exp(xr), st, 0.0)[0] < 1e-20
     assert motion_prior(T_start, T_start @ se3_exp(xt), 0.0, sr)[0] < 1e-20
+
+
+def test_motion_prior_closed_form_equals_the_adjoint_products():
+    """The closed form of M^T Lam M and M^T Lam xi equals the products with M = Ad(T^-1)."""
+    rng = np.random.default_rng(11)
+    st, sr = 0.003, 0.02
+    lt, lr = 1 / float(np.float32(st)) ** 2, 1 / float(np.float32(sr)) ** 2
+    lam = np.array([lt] * 3 + [lr] * 3)
+    for _ in range(10):
+        T_start = se3_exp(random_xi(rng, 2.0))
+        T = se3_exp(random_xi(rng, 0.3) * 0.01) @ T_start
+        _, A, b = motion_prior(T_start, T, st, sr)
+        M = adjoint(np.linalg.inv(T))
+        xi = se3_log(np.linalg.inv(T_start) @ T)
+        assert np.abs(A - M.T @ (lam[:, None] * M)).max() < 1e-9 * np.abs(A).max()
+        assert np.abs(b - M.T @ (lam * xi)).max() < 1e-9 * np.abs(b).max()

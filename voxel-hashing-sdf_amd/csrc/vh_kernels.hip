@@ -1704,6 +1704,43 @@ __device__ __forceinline__ void wg_reduce_store(double (&acc)[N], double* out) {
   __syncthreads();
 }
 
+// Sums acc[0..N) (N <= 32) over the workgroup (256 lanes) into out[0..N) by recursive halving
+// within each wave: at lane distance m = 32, 16, 8, 4, 2 every lane sends the half of its
+// values its partner (lane ^ m) keeps and adds the partner's half of its own, so the values
+// per lane halve each step (31 double exchanges instead of 6 per value); a last exchange at
+// distance 1 completes the wave's sums, which lane 2 i + 1 ... hold for value i; the four waves
+// are added in order. Fixed tree, deterministic.
+template <int N>
+__device__ __forceinline__ void wg_reduce_store_rs(const double (&acc)[N], double* out) {
+  static_assert(N <= 32, "at most 32 values");
+  __shared__ double s_part[4][32];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double v[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) v[k] = k < N ? acc[k] : 0.0;
+#pragma unroll
+  for (int h = 16; h >= 1; h >>= 1) {  // h values per lane after this step; distance 2h
+    const bool upper = (lane & (2 * h)) != 0;  // this lane keeps the upper half
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double send = upper ? v[i] : v[i + h];
+      const double keep = upper ? v[i + h] : v[i];
+      v[i] = keep + shfl_xor_d(send, 2 * h);
+    }
+  }
+  const double total = v[0] + shfl_xor_d(v[0], 1);
+  // value index of this lane: the halves it kept, from bit 5 (most significant) to bit 1
+  const int idx = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
+                  ((lane >> 1) & 1);
+  if ((lane & 1) == 0) s_part[wave][idx] = total;
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int k = threadIdx.x;
+    out[k] = ((s_part[0][k] + s_part[1][k]) + s_part[2][k]) + s_part[3][k];
+  }
+  __syncthreads();
+}
+
 // Residuals r_i = sdf(R p_i + t) and their derivatives with respect to a left se(3)
 // perturbation (rho, phi): dr/drho = grad, dr/dphi = q x grad.
 template <int kStorage>
@@ -1770,18 +1807,19 @@ __device__ __forceinline__ double huber_weight(double e) {
 // point without a valid sample adds rho(s), the cost of a saturated residual r = 1.
 __device__ __forceinline__ void gn_solve_wg(const double* partials, int grid, const GnState* st_in, GnState* st_out, double* s_cand,
                             double* cand_out, int final_step, double* stats_row);
-__device__ __forceinline__ void gn_step(const double* tot, const GnState* st_in, GnState* st_out, double* s_cand, double* cand_out,
-                        int final_step, double* stats_row);
+__device__ __forceinline__ void gn_step(const double* tot, GnState& S, double* s_cand, double* cand_out, int final_step,
+                                        double* stats_row);
 
 // One evaluation. With a previous evaluation (prev_grid > 0) every workgroup first takes its
 // Levenberg-Marquardt step itself (gn_solve_wg over the previous partials: the same sums in
 // the same order, so every workgroup gets the same candidate; workgroup 0 stores the state,
 // the candidate and the stats row), which saves a launch per evaluation and needs no ordering
 // between workgroups; otherwise the candidate is a.pose.
-// Register budget: the step (one lane, FP64 matrices unrolled) would otherwise take 280 VGPRs
-// and one wave per SIMD; capped at 4 waves per SIMD, only its own values spill.
+// Register budget: 3 waves per SIMD (154 VGPRs, no spills; the 31 FP64 sums and the robust
+// weighting of a point need more than the 128 of 4 waves, which spilled into the point loop).
+// The grid is at most 512 workgroups of 4 waves, so every wave of a launch is resident at once.
 template <int kStorage>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gn_accum(GnArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gn_accum(GnArgs a) {
   __shared__ double s_pose[12];
   if (a.prev_grid > 0) {
     const bool w0 = blockIdx.x == 0;
@@ -1826,7 +1864,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     acc[28] += 1.0;
     acc[30] += huber_rho(e);
   }
-  wg_reduce_store<kGnAcc>(acc, a.partials + kGnAcc * (size_t)blockIdx.x);
+  wg_reduce_store_rs<kGnAcc>(acc, a.partials + kGnAcc * (size_t)blockIdx.x);
 }
 
 // One workgroup, FP64: deterministic sum of the partials at the candidate pose, then a
@@ -1915,71 +1953,75 @@ __device__ __forceinline__ void motion_prior(const GnState& S, double* Ab, doubl
     xi[i] = a;
     xi[3 + i] = ph[i];
   }
-  // M = Ad(T^-1) = [[R^T, [p]x R^T], [0, R^T]], p = -R^T t
-  double Rt[3][3], p[3];
+  // With M = Ad(T^-1) = [[R^T, [p]x R^T], [0, R^T]], p = -R^T t, and R R^T = I:
+  // M^T Lam M = [[lt I, -lt [t]x], [lt [t]x, lt (|t|^2 I - t t^T) + lr I]] and
+  // M^T Lam xi = (g, t x g + lr R xi_phi) with g = lt R xi_rho.
+  const double lt = S.lam_t, lr = S.lam_r;
+  const double tv[3] = {T[3], T[7], T[11]};
+  const double tt = tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2];
+  const double Kt[3][3] = {{0.0, -tv[2], tv[1]}, {tv[2], 0.0, -tv[0]}, {-tv[1], tv[0], 0.0}};
+  cost += 0.5 * (lt * (xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]) +
+                 lr * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) Rt[i][j] = T[4 * j + i];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) p[i] = -(Rt[i][0] * T[3] + Rt[i][1] * T[7] + Rt[i][2] * T[11]);
-  const double Pk[3][3] = {{0.0, -p[2], p[1]}, {p[2], 0.0, -p[0]}, {-p[1], p[0], 0.0}};
-  double M[6][6];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      M[i][j] = Rt[i][j];
-      M[i][3 + j] = Pk[i][0] * Rt[0][j] + Pk[i][1] * Rt[1][j] + Pk[i][2] * Rt[2][j];
-      M[3 + i][j] = 0.0;
-      M[3 + i][3 + j] = Rt[i][j];
-    }
-  const double lam[6] = {S.lam_t, S.lam_t, S.lam_t, S.lam_r, S.lam_r, S.lam_r};
-  double q = 0.0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) q += lam[k] * xi[k] * xi[k];
-  cost += 0.5 * q;
-#pragma unroll
-  for (int a = 0, m = 0; a < 6; ++a) {
+  for (int a = 0, m = 0; a < 6; ++a)
 #pragma unroll
     for (int b = a; b < 6; ++b, ++m) {
-      double acc = 0.0;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) acc += M[k][a] * lam[k] * M[k][b];
-      Ab[m] += acc;
+      double v;
+      if (a < 3 && b < 3) v = a == b ? lt : 0.0;
+      else if (a < 3) v = -lt * Kt[a][b - 3];
+      else v = lt * ((a == b ? tt : 0.0) - tv[a - 3] * tv[b - 3]) + (a == b ? lr : 0.0);
+      Ab[m] += v;
     }
-    double g = 0.0;
+  double g[3], gr[3];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) g += M[k][a] * lam[k] * xi[k];
-    Ab[21 + a] += g;
+  for (int i = 0; i < 3; ++i) {
+    g[i] = lt * (T[4 * i] * xi[0] + T[4 * i + 1] * xi[1] + T[4 * i + 2] * xi[2]);
+    gr[i] = lr * (T[4 * i] * xi[3] + T[4 * i + 1] * xi[4] + T[4 * i + 2] * xi[5]);
   }
+  Ab[21] += g[0];
+  Ab[22] += g[1];
+  Ab[23] += g[2];
+  Ab[24] += (tv[1] * g[2] - tv[2] * g[1]) + gr[0];
+  Ab[25] += (tv[2] * g[0] - tv[0] * g[2]) + gr[1];
+  Ab[26] += (tv[0] * g[1] - tv[1] * g[0]) + gr[2];
 }
 
-// The partials are summed by all 256 lanes at once (each lane a strided subset of the
-// workgroups, then the fixed workgroup tree), so the loads are in flight together. All 256
-// lanes call it; the new candidate is in s_cand on return. st_out, cand_out and stats_row
-// (global, each may be null) receive the new state, the candidate and the evaluation's row.
+// The partials are summed by 248 lanes at once: lane (v, j) (v < kGnAcc, j < 8) sums
+// accumulator v over the workgroups g = j, j + 8, ... in order, then lanes v < kGnAcc add the 8
+// sums in a fixed tree, so every workgroup gets the same sums in the same order. Meanwhile
+// lanes below the state's size stage the solver state in LDS, so the step (one lane) reads it
+// from there. All 256 lanes call it; the new candidate is in s_cand on return. st_out,
+// cand_out and stats_row (global, each may be null) receive the new state, the candidate and
+// the evaluation's row.
 __device__ __forceinline__ void gn_solve_wg(const double* partials, int grid, const GnState* st_in, GnState* st_out, double* s_cand,
                             double* cand_out, int final_step, double* stats_row) {
+  constexpr int kWays = 8, kStateWords = (int)(sizeof(GnState) / sizeof(double));
+  static_assert(kWays * kGnAcc <= 256 && kStateWords <= 256 && sizeof(GnState) % sizeof(double) == 0, "layout");
+  __shared__ double s_sum[kWays][kGnAcc];
   __shared__ double tot[kGnAcc];
-  double acc[kGnAcc];
-#pragma unroll
-  for (int k = 0; k < kGnAcc; ++k) acc[k] = 0.0;
-  for (int g = threadIdx.x; g < grid; g += 256) {
-#pragma unroll
-    for (int k = 0; k < kGnAcc; ++k) acc[k] += partials[kGnAcc * (size_t)g + k];
+  __shared__ double s_st[kStateWords];
+  const int t = threadIdx.x;
+  if (t < kStateWords) s_st[t] = reinterpret_cast<const double*>(st_in)[t];
+  if (t < kWays * kGnAcc) {
+    const int v = t % kGnAcc, j = t / kGnAcc;
+    double acc = 0.0;
+    for (int g = j; g < grid; g += kWays) acc += partials[kGnAcc * (size_t)g + v];
+    s_sum[j][v] = acc;
   }
-  wg_reduce_store<kGnAcc>(acc, tot);
-  if (threadIdx.x == 0) gn_step(tot, st_in, st_out, s_cand, cand_out, final_step, stats_row);
   __syncthreads();
+  if (t < kGnAcc)
+    tot[t] = ((s_sum[0][t] + s_sum[1][t]) + (s_sum[2][t] + s_sum[3][t])) +
+             ((s_sum[4][t] + s_sum[5][t]) + (s_sum[6][t] + s_sum[7][t]));
+  __syncthreads();
+  if (t == 0) gn_step(tot, *reinterpret_cast<GnState*>(s_st), s_cand, cand_out, final_step, stats_row);
+  __syncthreads();
+  if (st_out && t < kStateWords) reinterpret_cast<double*>(st_out)[t] = s_st[t];
 }
 
-// The Levenberg-Marquardt decision and step of one evaluation (one lane): the state is read
-// once into registers and written once (through the pointers each access would be a dependent
-// memory round trip).
-__device__ __forceinline__ void gn_step(const double* tot, const GnState* st_in, GnState* st_out, double* s_cand, double* cand_out,
-                        int final_step, double* stats_row) {
-  GnState S = *st_in;
+// The Levenberg-Marquardt decision and step of one evaluation (one lane), on the solver state
+// staged in LDS (updated in place; the workgroup writes it out afterwards).
+__device__ __forceinline__ void gn_step(const double* tot, GnState& S, double* s_cand, double* cand_out, int final_step,
+                                        double* stats_row) {
   double c[12], row[kGnStats];
   const double cnt = tot[28];
   // tracking: depth term scaled to n_eff points, plus the motion prior
@@ -2062,7 +2104,6 @@ __device__ __forceinline__ void gn_step(const double* tot, const GnState* st_in,
     S.cand[k] = c[k];
     s_cand[k] = c[k];
   }
-  if (st_out) *st_out = S;
   if (cand_out) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) cand_out[k] = c[k];
