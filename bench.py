@@ -257,9 +257,14 @@ def main():
     ap.add_argument("--no-gn", action="store_true", help="skip the pose-refinement block")
     ap.add_argument("--no-tracked", action="store_true", help="skip the tracked floor_large_4mm block")
     ap.add_argument("--tracked-frames", type=int, default=0, help="override the tracked sequence length")
+    ap.add_argument("--tracked-only", action="store_true",
+                    help="run only the tracked floor_large_4mm block (prints its JSON object)")
     ap.add_argument("--counters", action="store_true",
                     help="short run for hardware-counter passes: one FP32 integration pass over the "
                          "sequence plus a 1 GiB device copy that calibrates FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--counters-fused", action="store_true",
+                    help="with --counters: the captured-graph batches (fused integration + allocation launches) "
+                         "instead of the profiling mode's separate launches")
     ap.add_argument("--counters-mc", action="store_true",
                     help="with --counters: one marching-cubes extraction after the integration pass")
     args = ap.parse_args()
@@ -273,6 +278,9 @@ def main():
 
     vh, wl = __graft_entry__.package()
     vh.load()
+    if args.tracked_only:
+        print(json.dumps({"tracked": tracked_block(vh, wl, args.tracked_frames)}), flush=True)
+        return
     w = dict(wl.WORKLOADS[args.workload])
     if args.seq_batch:
         w["seq_batch"] = args.seq_batch
@@ -296,15 +304,17 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         vol = vh.Volume(vh.config_from_workload(w))
-        vol.set_profiling(True)  # allocation and integration as separate launches, one per frame
+        # allocation and integration as separate launches per batch, or the fused graph batches
+        vol.set_profiling(2 if args.counters_fused else True)
         run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=args.counters_mc)
         st = vol.stats()
         vol.close()
-        log(f"bench: counters run: {nf} frames in {st['profiled_launches']} integration launches, "
-            f"{st['blocks_integrated']} blocks integrated, copy calibration 1 GiB read + 1 GiB written")
+        launches = st["fused_launches"] if args.counters_fused else st["profiled_launches"]
+        log(f"bench: counters run: {nf} frames in {launches} {'fused' if args.counters_fused else 'integration'} "
+            f"launches, {st['blocks_integrated']} blocks integrated, copy calibration 1 GiB read + 1 GiB written")
         print(json.dumps({"counters_run": True, "frames": nf, "touched_total": st["touched_total"],
-                          "blocks_integrated": st["blocks_integrated"], "launches": st["profiled_launches"],
-                          "copy_bytes": 1 << 30}), flush=True)
+                          "blocks_integrated": st["blocks_integrated"], "launches": launches,
+                          "fused": bool(args.counters_fused), "copy_bytes": 1 << 30}), flush=True)
         return
 
     results = {}
@@ -385,6 +395,12 @@ def main():
         run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
         pst = vol.stats()
         vol.set_profiling(False)
+        fst = None
+        if name == "f32":  # the fused batch launches of the captured graphs, HIP events around each
+            vol.set_profiling(2)
+            run_pass(vol, vh, (d_depth, d_pose), nf, 0, d_xyz, cap, final_mc=False, **cr)
+            fst = vol.stats()
+            vol.set_profiling(False)
         stream = None
         if name == "f32":  # the whole volume to the host store and back (after the profiled pass)
             t2 = time.perf_counter()
@@ -398,7 +414,8 @@ def main():
             results["gn"] = gn_block(vh, wl, w, vol, nf)
         results[name] = dict(fps=args.steps * nf / dt, ms_per_step=1e3 * dt / args.steps, fps_integrate=nf / dt_int,
                              blocks=st["blocks_allocated"], tris=ntri, mc_ms=1e3 * mc_s,
-                             mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst, indexed=ix, stream=stream,
+                             mc_blocks_per_s=st["blocks_allocated"] / mc_s, prof=pst, fused=fst, indexed=ix,
+                             stream=stream,
                              vox_bytes=(8 if over.get("storage", 0) == 0 else 4) + (4 if col else 0))
         log(f"bench: {name}: {results[name]['fps']:.1f} frames/s (with MC), {results[name]['fps_integrate']:.1f} "
             f"integrate-only, {st['blocks_allocated']} blocks, {ntri} triangles, MC {1e3 * mc_s:.2f} ms")
@@ -426,8 +443,11 @@ def main():
     valu = None
     if traffic is not None and "valu_busy" in pj:
         valu = {"busy": round(pj["valu_busy"], 3),
-                "source": "rocprofv3 SQ_ACTIVE_INST_VALU / GRBM_GUI_ACTIVE of the same launches: the batched "
-                          "kernel is bound by vector-instruction issue, not by HBM"}
+                "ta_busy": round(pj["ta_busy"], 3) if "ta_busy" in pj else None,
+                "ta_stalled_by_l1": round(pj["ta_stalled_by_tc"], 3) if "ta_stalled_by_tc" in pj else None,
+                "source": "rocprofv3 SQ_ACTIVE_INST_VALU, TA_BUSY_avr, TA_ADDR_STALLED_BY_TC_CYCLES / GRBM_GUI_ACTIVE "
+                          "of the same launches: neither the vector ALUs nor HBM are saturated; the depth gathers "
+                          "keep the vector memory path (texture addresser and L1) busiest"}
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src, valu=valu,
                     kernel="k_integrate<F32,u16>", avg_launch_us=round(1e6 * t_int / launches, 2),
@@ -438,6 +458,30 @@ def main():
                     integrate_us_per_frame=round(1e6 * t_int / p["profiled_frames"], 2),
                     alloc_avg_launch_us=round(1e6 * p["alloc_ms"] / 1e3 / launches, 2))
 
+    # the fused launch (integration of batch b + allocation of batch b+1): algorithmic bytes =
+    # the integration's (as above) + the allocation's, one depth image per frame plus 16 B per
+    # touched block (hash probe and list entry); time from HIP events around each graph launch
+    roofline_fused = None
+    fz = r.get("fused")
+    if fz and fz["fused_launches"]:
+        nb = p["blocks_integrated"] / launches
+        fpl = p["profiled_frames"] / launches
+        tb = p["touched_total"] / p["profiled_frames"]
+        fused_bytes = nb * (512 * r["vox_bytes"] * 2 + 16) + fpl * W * H * 2 + fpl * (W * H * 2 + 16 * tb)
+        t_f = fz["fused_ms"] / 1e3 / fz["fused_launches"]
+        a_f = fused_bytes / t_f / 1e9
+        fj_path = os.path.join(ROOT, "profiles", "k_integrate_alloc_room_medium_5mm.json")
+        fj = json.load(open(fj_path)) if os.path.exists(fj_path) else {}
+        f_traffic, f_src = None, None
+        if args.workload == "room_medium_5mm" and "traffic_bytes_per_dispatch" in fj:
+            f_traffic = int(fj["traffic_bytes_per_dispatch"])
+            f_src = (f"rocprofv3 FETCH_SIZE x{fj['fetch_scale']:.3f} + WRITE_SIZE x{fj['write_scale']:.3f} "
+                     f"(1 GiB copy calibration), {fj['workload']}, {fj['dispatches']} fused launches")
+        roofline_fused = dict(bound="hbm", achieved=round(a_f, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(a_f / HBM_PEAK_GBS, 4), traffic=f_traffic, traffic_source=f_src,
+                              kernel="k_integrate_alloc<F32,u16>", avg_launch_us=round(1e6 * t_f, 2),
+                              timing="HIP events around each captured-graph batch launch on the volume's stream",
+                              bytes_per_launch=int(fused_bytes), launches=fz["fused_launches"])
     out = {"metric": METRIC, "value": round(r["fps"], 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(r["ms_per_step"], 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -450,7 +494,7 @@ def main():
                                     "order) and allocation of batch b+1"},
            "integrate_only_fps": round(r["fps_integrate"], 2), "mc_blocks": r["blocks"],
            "mc_triangles": r["tris"], "mc_ms": round(r["mc_ms"], 3), "mc_blocks_per_s": round(r["mc_blocks_per_s"], 1),
-           "roofline": roofline}
+           "roofline": roofline, "roofline_fused": roofline_fused}
     if r.get("indexed"):
         ix = r["indexed"]
         out["mesh_indexed"] = {"ms": round(ix["ms"], 3), "vertices": ix["vertices"], "faces": ix["faces"],

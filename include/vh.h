@@ -133,6 +133,10 @@ typedef struct vh_stats {
   int64_t carved_total;       /* carving volumes: block-frame updates added by carving (a frame
                                  observes a block that its truncation bands did not cross) */
   double carve_ms;            /* profiling mode: summed carve-pass time */
+  double fused_ms;            /* profiling mode 2: summed time of the fused batch launches (one
+                                 captured graph: integration of a batch and allocation of the
+                                 next), HIP events around each on the volume's stream */
+  int64_t fused_launches;     /* profiling mode 2: fused batch launches timed */
 } vh_stats;
 
 typedef enum vh_prior_mode {
@@ -234,8 +238,10 @@ int vh_bind_tracking(vh_volume* vol, const void* d_depth, int32_t depth_format, 
 int vh_integrate_next(vh_volume* vol, int32_t n);
 /* Waits for queued work and reports any pool/hash/range overflow raised by it. */
 int vh_sync(vh_volume* vol);
-/* Profiling mode: frames launch eagerly with HIP events around each kernel; the summed
- * kernel times appear in vh_stats. */
+/* Profiling mode 1: frames launch eagerly with HIP events around each kernel; the summed
+ * kernel times appear in vh_stats. Mode 2: bound sequences run their captured graphs as
+ * without profiling, with HIP events around each fused batch launch (vh_stats.fused_ms).
+ * 0: off. Switching clears the sums. */
 int vh_set_profiling(vh_volume* vol, int32_t on);
 int vh_get_stats(vh_volume* vol, vh_stats* out);
 /* The volume's HIP stream (hipStream_t), for callers that order their own work on it. */

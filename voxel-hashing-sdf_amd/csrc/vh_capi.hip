@@ -151,13 +151,15 @@ struct vh_volume {
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
   int batch = kMaxBatch;  // frames per launch on bound sequences
   float ring_scale = -1.0f;  // depth scale baked into the ring uint16 graph
-  // profiling
-  bool profiling = false;
+  // profiling: 1 = eager launches with events around each kernel, 2 = the captured graphs with
+  // events around each fused batch launch
+  int profiling = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<hipEvent_t> ev_pending;  // groups of 5: start, after tracking, after alloc, after carve, after integrate
   std::vector<int> ev_frames;          // frames of each group
-  double alloc_ms = 0.0, integrate_ms = 0.0, track_ms = 0.0, carve_ms = 0.0;
-  int64_t profiled_frames = 0, profiled_launches = 0;
+  std::vector<hipEvent_t> ev_fused;    // pairs around fused batch launches (profiling 2)
+  double alloc_ms = 0.0, integrate_ms = 0.0, track_ms = 0.0, carve_ms = 0.0, fused_ms = 0.0;
+  int64_t profiled_frames = 0, profiled_launches = 0, fused_launches = 0;
   // marching-cubes output buffer (grown on demand) and scan scratch
   float* mc_tris = nullptr;
   int32_t* mc_cubes = nullptr;
@@ -394,7 +396,7 @@ hipEvent_t take_event(vh_volume* v) {
 // profiling mode a batch of k = max(ki, ka) frames runs eagerly as (tracking,) allocation of k
 // frames and integration of k frames, with events around each.
 int launch_frame(vh_volume* v, int which, float staged_scale, int ki = 1, int ka = 1) {
-  if (v->profiling) {
+  if (v->profiling == 1) {
     hipEvent_t e[5] = {take_event(v), take_event(v), take_event(v), take_event(v), take_event(v)};
     if (!e[0] || !e[1] || !e[2] || !e[3] || !e[4]) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
     FrameArgs a = frame_args(v, which);
@@ -432,6 +434,16 @@ int launch_frame(vh_volume* v, int which, float staged_scale, int ki = 1, int ka
     (void)hipGraphDestroy(graph);
     VH_HIP(e);
   }
+  if (v->profiling == 2 && which == 2) {
+    hipEvent_t e0 = take_event(v), e1 = take_event(v);
+    if (!e0 || !e1) return fail(VH_ERR_DEVICE, "hipEventCreate failed");
+    VH_HIP(hipEventRecord(e0, v->stream));
+    VH_HIP(hipGraphLaunch(g, v->stream));
+    VH_HIP(hipEventRecord(e1, v->stream));
+    v->ev_fused.push_back(e0);
+    v->ev_fused.push_back(e1);
+    return VH_OK;
+  }
   VH_HIP(hipGraphLaunch(g, v->stream));
   return VH_OK;
 }
@@ -464,6 +476,14 @@ int collect_profile(vh_volume* v) {
   for (hipEvent_t e : v->ev_pending) v->ev_pool.push_back(e);
   v->ev_pending.clear();
   v->ev_frames.clear();
+  for (size_t k = 0; k + 1 < v->ev_fused.size(); k += 2) {
+    float t = 0.0f;
+    VH_HIP(hipEventElapsedTime(&t, v->ev_fused[k], v->ev_fused[k + 1]));
+    v->fused_ms += t;
+    v->fused_launches += 1;
+  }
+  for (hipEvent_t e : v->ev_fused) v->ev_pool.push_back(e);
+  v->ev_fused.clear();
   return VH_OK;
 }
 
@@ -1334,7 +1354,7 @@ int vh_integrate_next(vh_volume* v, int32_t n) {
   }
   // batches of up to `batch` frames; the last may be shorter
   const int K = v->batch;
-  if (v->profiling) {
+  if (v->profiling == 1) {
     for (int f = 0; f < n; f += K)
       if ((rc = launch_frame(v, 2, v->seq_scale, std::min(K, n - f), std::min(K, n - f)))) return rc;
     return VH_OK;
@@ -1368,9 +1388,10 @@ int vh_set_profiling(vh_volume* v, int32_t on) {
   VH_HIP(hipStreamSynchronize(v->stream));
   rc = collect_profile(v);
   if (rc) return rc;
-  v->profiling = on != 0;
-  v->alloc_ms = v->integrate_ms = v->track_ms = v->carve_ms = 0.0;
-  v->profiled_frames = v->profiled_launches = 0;
+  if (on < 0 || on > 2) return fail(VH_ERR_INVALID_ARG, "profiling mode must be 0, 1 or 2");
+  v->profiling = on;
+  v->alloc_ms = v->integrate_ms = v->track_ms = v->carve_ms = v->fused_ms = 0.0;
+  v->profiled_frames = v->profiled_launches = v->fused_launches = 0;
   return VH_OK;
 }
 
@@ -1404,6 +1425,8 @@ int vh_get_stats(vh_volume* v, vh_stats* s) {
   s->blocks_integrated = (int64_t)c.integrated_total;
   s->carved_total = (int64_t)c.carved_total;
   s->carve_ms = v->carve_ms;
+  s->fused_ms = v->fused_ms;
+  s->fused_launches = v->fused_launches;
   s->track_ms = v->track_ms;
   return VH_OK;
 }

@@ -73,7 +73,8 @@ class Stats(C.Structure):
                 ("integrate_ms", C.c_double), ("alloc_ms", C.c_double), ("profiled_frames", C.c_int64),
                 ("track_ms", C.c_double), ("profiled_launches", C.c_int64), ("blocks_integrated", C.c_int64),
                 ("host_blocks", C.c_int64), ("streamed_out_total", C.c_int64), ("streamed_in_total", C.c_int64),
-                ("stream_merged_total", C.c_int64), ("carved_total", C.c_int64), ("carve_ms", C.c_double)]
+                ("stream_merged_total", C.c_int64), ("carved_total", C.c_int64), ("carve_ms", C.c_double),
+                ("fused_ms", C.c_double), ("fused_launches", C.c_int64)]
 
 
 class Mesh(C.Structure):
@@ -412,8 +413,10 @@ class Volume:
         finally:
             self._pending = []
 
-    def set_profiling(self, on: bool):
-        _check(_lib.vh_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on):
+        """False/0: off; True/1: eager launches with events around each kernel; 2: the captured
+        graphs with events around each fused batch launch (stats fused_ms, fused_launches)."""
+        _check(_lib.vh_set_profiling(self._h, 2 if on == 2 else (1 if on else 0)))
 
     def stats(self) -> dict:
         s = Stats()
