@@ -337,7 +337,7 @@ def main():
     del pinned
     log(f"bench: host input: {results['pcie']:.1f} frames/s")
 
-    variants = [("f32", {})]
+    variants = [("f32", {}), ("f32v", dict(carve=1))]
     if not args.no_f16:
         variants += [("f16r", dict(storage=2, weight_cap=255)), ("f16", dict(storage=1, weight_cap=255))]
     d_rgb = d_vrgb = None
@@ -537,6 +537,19 @@ def main():
                     "integrate_kernel_gbs": round(ah, 1),
                     "integrate_avg_launch_us": round(1e6 * ph["integrate_ms"] / 1e3 / ph["profiled_launches"], 2)}
 
+    if "f32v" in results:  # free-space carving (vh_config.carve = 1) against band-only integration
+        h = results["f32v"]
+        ph = h["prof"]
+        out["carve"] = {"value": round(h["fps"], 2), "integrate_only_fps": round(h["fps_integrate"], 2),
+                        "vs_band_only": round(h["fps"] / r["fps"], 3),
+                        "integrate_vs_band_only": round(h["fps_integrate"] / r["fps_integrate"], 3),
+                        "carve_us_per_batch": round(1e6 * ph["carve_ms"] / 1e3 / ph["profiled_launches"], 2),
+                        "integrate_avg_launch_us": round(1e6 * ph["integrate_ms"] / 1e3 / ph["profiled_launches"], 2),
+                        "blocks_integrated_per_batch": round(ph["blocks_integrated"] / ph["profiled_launches"], 1),
+                        "carved_block_frames": ph["carved_total"], "mc_ms": round(h["mc_ms"], 3),
+                        "note": "FP32 with carving: after each batch's allocation k_carve lists every resident block "
+                                "a frame may update, so voxels in front of the surface move toward +1"}
+
     colour_labels = {"f32c": ("color", "f32", "fp32 tsdf/weight (8 B) + colour word (4 B) per voxel"),
                      "f16rc": ("color_fp16", "f16r", "F16R tsdf/weight (4 B) + colour word (4 B) per voxel")}
     for name, (key, base, desc) in colour_labels.items():
@@ -683,6 +696,33 @@ def cpu_baseline(vh, w, depth, poses, budget_s, no_f16, wl=None):
             errs[key]["pose_jacobian_vs_fd"] = g
             log(f"bench: fp32 pose Jacobian vs FD: relative error {g['rel_err']:.2e} over {g['points']} points")
         vol.close()
+    # carving: the GPU's FP32 carving volume after the first frames against the oracle's FP64
+    # shadow in carving mode (which tests every allocated block each frame)
+    nc = min(n, 150)
+    t = time.perf_counter()
+    cref = oracle.Oracle(w, carve=1)
+    for k in range(nc):
+        cref.integrate(depth[k], poses[k])
+    vol = vh.Volume(vh.config_from_workload(w, carve=1))
+    for k in range(nc):
+        vol.integrate(depth[k], poses[k])
+    keys, s, wt = oracle.sort_blocks(*vol.export_blocks())
+    rk, rs, rw, r64, _ = cref.export()
+    c = {"err_frames": nc}
+    if np.array_equal(keys, rk):
+        m = wt > 0
+        e = np.abs(s[m].astype(np.float64) - r64[m])
+        c.update(tsdf_err_max=float(e.max()), tsdf_err_rms=float(np.sqrt((e ** 2).mean())),
+                 bit_exact_vs_fp32_twin=bool(np.array_equal(s.view(np.uint32), rs.view(np.uint32))))
+        xyz, cubes = vol.extract_mesh()
+        h = oracle.hausdorff(xyz, cubes, *cref.mc64())
+        c.update({"hausdorff_m": h["hausdorff"], "hausdorff_q0.999_m": h["q0.999"],
+                  "hausdorff_q0.99999_m": h["q0.99999"]})
+    else:
+        c["tsdf_err"] = "block sets differ from the oracle"
+    vol.close()
+    errs["carve"] = c
+    log(f"bench: carving, first {nc} frames: {c} ({time.perf_counter() - t:.1f} s)")
     return base, errs
 
 
