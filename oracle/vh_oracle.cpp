@@ -183,8 +183,10 @@ struct Oracle {
 
 // ------------------------------------------------------------------ band traversal
 // Blocks crossed by the segment of pixel (u,v) between camera depths max(d-mu, min_depth)
-// and d+mu. Camera point: ((u-cx)*t/fx, (v-cy)*t/fy, t) as frame2cam (src/tsdf.cu:67-73);
-// world point R p + t as cam2base (src/tsdf.cu:95-105); block coordinates world / (8 vox).
+// and d+mu. Camera point: (rx*t, ry*t, t) with the ray per unit depth rx = (u-cx)*RN(1/fx),
+// ry = (v-cy)*RN(1/fy) (frame2cam, src/tsdf.cu:67-73, with the divisions by the focal lengths
+// taken as products by their rounded reciprocals); world point R p + t as cam2base
+// (src/tsdf.cu:95-105); block coordinates world * RN(1/(8 vox)).
 // Walk: from floor(start) to floor(end), stepping the axis whose next face is crossed
 // first among the axes that still have to move (lowest axis on ties).
 template <class Emit>
@@ -192,18 +194,19 @@ void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit em
   float t0 = d - o.mu;
   if (t0 < o.min_depth) t0 = o.min_depth;
   const float t1 = d + o.mu;
-  const float bsz = o.vox * 8.0f;
+  const float ibsz = 1.0f / (o.vox * 8.0f), ifx = 1.0f / o.fx, ify = 1.0f / o.fy;
+  const float rx = ((float)u - o.cx) * ifx, ry = ((float)v - o.cy) * ify;
   float cam[2][3];
   const float ts[2] = {t0, t1};
   for (int e = 0; e < 2; ++e) {
-    cam[e][0] = ((float)u - o.cx) * ts[e] / o.fx;
-    cam[e][1] = ((float)v - o.cy) * ts[e] / o.fy;
+    cam[e][0] = rx * ts[e];
+    cam[e][1] = ry * ts[e];
     cam[e][2] = ts[e];
   }
   float p[2][3];
   for (int e = 0; e < 2; ++e)
     for (int r = 0; r < 3; ++r)
-      p[e][r] = (cam[e][0] * T[4 * r] + cam[e][1] * T[4 * r + 1] + cam[e][2] * T[4 * r + 2] + T[4 * r + 3]) / bsz;
+      p[e][r] = (cam[e][0] * T[4 * r] + cam[e][1] * T[4 * r + 1] + cam[e][2] * T[4 * r + 2] + T[4 * r + 3]) * ibsz;
   for (int e = 0; e < 2; ++e)
     for (int r = 0; r < 3; ++r)
       if (!(std::fabs(p[e][r]) < 1048000.0f)) return;

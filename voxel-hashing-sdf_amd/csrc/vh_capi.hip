@@ -246,6 +246,7 @@ FrameArgs frame_args(vh_volume* v, int which) {
   a.min_depth = c.min_depth;
   a.vox = c.vox_size;
   a.bsz = c.vox_size * (float)kB;
+  a.band = band_scale(a.K, a.bsz);
   a.mu = c.trunc;
   a.inv_mu = 1.0f / c.trunc;
   a.cap = c.weight_cap > 0 ? (float)c.weight_cap : INFINITY;

@@ -125,6 +125,7 @@ struct FrameArgs {
   // camera and volume parameters
   int32_t W, H;
   Intr K;
+  BandScale band;  // reciprocals for the band setup
   float max_depth, min_depth, vox, bsz, mu, inv_mu;
   float cap;  // weight cap, +inf when uncapped
   // tables

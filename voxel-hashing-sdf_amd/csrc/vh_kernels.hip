@@ -97,8 +97,9 @@ __host__ __device__ constexpr uint32_t alloc_tiles(int W, int H, int rows) {
   return (uint32_t)(((W + 15) >> 4) * ((H + 16 * rows - 1) / (16 * rows)));
 }
 
+static_assert(kSetSlots == 1 << 10, "set_hash gives 10 bits");
 __device__ __forceinline__ uint32_t set_hash(uint64_t key) {
-  return (uint32_t)((key * 0x9e3779b97f4a7c15ull) >> 54) & (kSetSlots - 1);
+  return (((uint32_t)key ^ (uint32_t)(key >> 32)) * 0x9e3779b1u) >> 22;
 }
 
 // The block list, its stripe lengths and the slot marks are double-buffered by batch
@@ -185,12 +186,22 @@ __device__ void claim_keys(const FrameArgs& a, uint64_t key, bool act, uint32_t 
 
 // Allocation of one tile of frame `frame` (index jb of its batch, input slot fs): 16 pixel
 // columns by 16 * kRows rows, 256 lanes, each lane one pixel of every 16-row band.
+// The allocation's LDS: the key set, the tile's distinct keys and, for carving volumes, the
+// bands' largest depths.
+template <int kRows>
+struct AllocLds {
+  unsigned long long set[kSetSlots];
+  unsigned long long uniq[kUniqCap];
+  uint32_t nuniq;
+  float tmax[4][kRows];
+};
+
 template <bool kU16, int kRows>
 __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, uint32_t jb, uint32_t parity,
-                           uint32_t fs) {
-  __shared__ unsigned long long s_set[kSetSlots];
-  __shared__ unsigned long long s_uniq[kUniqCap];
-  __shared__ uint32_t s_nuniq;
+                           uint32_t fs, AllocLds<kRows>& L) {
+  unsigned long long* s_set = L.set;
+  unsigned long long* s_uniq = L.uniq;
+  uint32_t& s_nuniq = L.nuniq;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tiles_x = (a.W + 15) >> 4;
   const int u = ((int)(tile % tiles_x) << 4) + ((wave & 1) << 3) + (lane & 7);
@@ -223,7 +234,7 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
   }
 
   // carving volumes: the largest valid depth of each 16 x 16 band (a max-depth tile)
-  __shared__ float s_tmax[4][kRows];
+  float (*s_tmax)[kRows] = L.tmax;
   if (a.tilemax) {
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
@@ -249,7 +260,7 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
     BandDDA s;
     bool active = false;
     if (u < a.W && v < a.H && depth_valid(dr[r], a.max_depth)) {
-      active = band_setup(T, a.K, a.bsz, a.mu, a.min_depth, u, v, dr[r], s);
+      active = band_setup(T, a.K, a.band, a.mu, a.min_depth, u, v, dr[r], s);
       if (!active) raise_flag(ctl, kFlagRange);
     }
     // phase 1: distinct keys of the tile into LDS; keys that do not fit are claimed directly
@@ -290,10 +301,11 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
 // Allocation of the batch about to be integrated: `tiles` workgroups per frame.
 template <bool kU16, int kRows>
 __global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
+  __shared__ AllocLds<kRows> L;
   const uint32_t jb = blockIdx.x / tiles;
   const FrameCtl* ctl = a.ctl;
   alloc_tile<kU16, kRows>(a, blockIdx.x - jb * tiles, ctl->frame_id + jb, jb, ctl->batch_id & 1u,
-                   input_slot(ctl->cursor[a.cursor_id], jb, a.n_seq));
+                          input_slot(ctl->cursor[a.cursor_id], jb, a.n_seq), L);
 }
 
 // ============================================================================ integration
@@ -335,51 +347,142 @@ __device__ __forceinline__ uint32_t color_update(uint32_t c, float w_old, uint32
 // Worker w of `workers` (one wave each) takes an equal share of the list, its 64 stripes
 // seen as one index space; workers synchronise only within their own wave.
 // kColor: the colour pool is updated from each frame's packed RGB at the same pixels.
-// True iff every voxel of the block projects inside the image at a depth within the
-// projection's range: then the per-voxel range checks can be skipped (frame_update<..,
-// true>). The lane's column runs from voxel 0 to voxel 7 of cl + q*ax (homogeneous
-// coordinates); the block's lattice box is the convex hull of the columns' end points, and for
-// depths > 0 the perspective image of a convex combination is a convex combination of the
-// images, so if every end point has z in [2^-60, 2^60] and X/z in [1, W-2] (likewise Y/z),
-// every voxel's z lies in [2^-64, 2^64] and its rounded pixel inside the image (the arithmetic
-// of the voxel's own projection differs from the end points' by far less than the 1-pixel
-// margin).
-__device__ __forceinline__ bool block_inside(const FrameArgs& a, const float ax[3], const float cl[3]) {
+// Where a block's voxels fall in a frame, which selects frame_update's variant:
+constexpr int kPlaceEdge = 0;    // some voxel may project outside the image: per-voxel checks
+constexpr int kPlaceInside = 1;  // every voxel inside: no checks, depth gathered from the image
+constexpr int kPlaceWindow = 2;  // every voxel inside, its pixels in a window staged in LDS
+// LDS window per wave (pixels): the pre-scaled depth of the pixel rectangle that a block's
+// voxels project into, rows of a power-of-two pitch <= 64. A 5 mm block at 2 m covers about
+// 12 x 12 pixels; at 0.5 m about 50 x 50, which does not fit and is gathered from the image.
+constexpr uint32_t kWin = 512;
+
+// A block's window in a frame: the image rectangle of columns ulo .. ulo + cols - 1 and rows
+// vlo .. vlo + rows - 1 (both inside the image), in LDS at pitch 2^shift.
+struct DepthWindow {
+  uint32_t first;  // byte offset 4 (vlo W + ulo) of the rectangle's first pixel in the image
+  uint32_t shift;  // log2 of the LDS row pitch in pixels, <= 6
+  uint32_t rows, cols;
+  uint32_t bias;   // LDS byte offset of pixel (u, v) = (bits(M + v) << (shift + 2)) + (bits(M + u) << 2) - bias
+};
+
+// Smallest and largest of x over lanes 0, 7, 56 and 63, the lanes whose columns end at the
+// block's 8 corners: the half-row mirror (DPP) pairs lane 0 with 7 and 56 with 63, then two
+// reads of lanes 0 and 56.
+__device__ __forceinline__ uint32_t half_mirror(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t corner_min(uint32_t x) {
+  x = min(x, half_mirror(x));
+  return min((uint32_t)__builtin_amdgcn_readlane((int)x, 0), (uint32_t)__builtin_amdgcn_readlane((int)x, 56));
+}
+__device__ __forceinline__ uint32_t corner_max(uint32_t x) {
+  x = max(x, half_mirror(x));
+  return max((uint32_t)__builtin_amdgcn_readlane((int)x, 0), (uint32_t)__builtin_amdgcn_readlane((int)x, 56));
+}
+
+// Placement of a block in a frame (wave-uniform). kPlaceEdge unless every voxel of the block
+// projects inside the image at a depth within the projection's range, which frame_update
+// then need not check: the lane's column runs from voxel 0 to voxel 7 of cl + q*ax
+// (homogeneous coordinates); the block's lattice box is the convex hull of the columns' end
+// points, and for depths > 0 the perspective image of a convex combination is a convex
+// combination of the images, so if every end point has z in [2^-60, 2^60] and X/z in
+// [1, W-2] (likewise Y/z), every voxel's z lies in [2^-64, 2^64] and its rounded pixel inside
+// the image (the arithmetic of the voxel's own projection differs from the end points' by far
+// less than the 1-pixel margin).
+// The same hull bounds the voxels' pixels: with umin, umax the least and largest u = X/z of
+// the block's 8 corners (lanes 0, 7, 56, 63, both ends), a voxel's u lies in
+// [umin - e, umax + e], e well below 1/2 for the same reason, and its pixel round(u) in
+// [round(umin - 1/2), round(umax + 1/2)] (for e < 1/2 these roundings cannot cross the
+// voxel's); likewise v. The rectangle is the block's window; kPlaceWindow when it fits kWin
+// at a pitch of at most 64 pixels.
+__device__ __forceinline__ int block_place(const FrameArgs& a, const float ax[3], const float cl[3], DepthWindow& w) {
   bool in = true;
+  float u[2], v[2];
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     const float fq = e ? 7.0f : 0.0f;
     const float z = __builtin_fmaf(fq, ax[2], cl[2]);
     const float iz = recip_rn(z);
-    const float u = __builtin_fmaf(fq, ax[0], cl[0]) * iz;
-    const float v = __builtin_fmaf(fq, ax[1], cl[1]) * iz;
-    in = in & (z >= 0x1p-60f) & (z <= 0x1p60f) & (u >= 1.0f) & (u <= (float)(a.W - 2)) & (v >= 1.0f) &
-         (v <= (float)(a.H - 2));
+    u[e] = __builtin_fmaf(fq, ax[0], cl[0]) * iz;
+    v[e] = __builtin_fmaf(fq, ax[1], cl[1]) * iz;
+    in = in & (z >= 0x1p-60f) & (z <= 0x1p60f) & (u[e] >= 1.0f) & (u[e] <= (float)(a.W - 2)) & (v[e] >= 1.0f) &
+         (v[e] <= (float)(a.H - 2));
   }
-  return __builtin_amdgcn_ballot_w64(!in) == 0;  // wave-uniform
+  // Computed whatever the placement (no value of the window is used unless it is
+  // kPlaceWindow), so that the window stays in scalar registers. Inside the image u - 1/2 >=
+  // 1/2 and u + 1/2 <= W - 3/2 are exact; adding M rounds them to integers, and the bits of
+  // the sums are M's plus the bounds.
+  const uint32_t lu = corner_min(__float_as_uint((fminf(u[0], u[1]) - 0.5f) + kPixMagic));
+  const uint32_t hu = corner_max(__float_as_uint((fmaxf(u[0], u[1]) + 0.5f) + kPixMagic));
+  const uint32_t lv = corner_min(__float_as_uint((fminf(v[0], v[1]) - 0.5f) + kPixMagic));
+  const uint32_t hv = corner_max(__float_as_uint((fmaxf(v[0], v[1]) + 0.5f) + kPixMagic));
+  const uint32_t cols = hu - lu + 1u, rows = hv - lv + 1u;
+  // pitch: the power of two >= cols (2 for one column), at most 64
+  const uint32_t shift = 32u - (uint32_t)__builtin_clz(min(cols - 1u, 63u) | 1u);
+  w.first = 4u * ((lv - kPixMagicBits) * (uint32_t)a.W + (lu - kPixMagicBits));
+  w.shift = shift;
+  w.rows = rows;
+  w.cols = cols;
+  w.bias = (lv << (shift + 2u)) + (lu << 2);
+  const bool inside = __builtin_amdgcn_ballot_w64(!in) == 0;
+  const bool fits = cols <= 64u && rows <= (kWin >> shift);
+  return inside ? (fits ? kPlaceWindow : kPlaceInside) : kPlaceEdge;
 }
+
+// LDS addresses as integers: the byte address of an LDS location, and a read from one.
+typedef __attribute__((address_space(3))) float LdsFloat;
+__device__ __forceinline__ uint32_t lds_address(const float* p) { return (uint32_t)(size_t)(const LdsFloat*)p; }
+__device__ __forceinline__ float lds_read(uint32_t byte_address) { return *(const LdsFloat*)(size_t)byte_address; }
+
+// The window's pixels of pre-scaled depth image dmf into the wave's LDS window at byte address
+// win (row r at win + 4 (r << shift)), by loads that write LDS directly: load t covers rows
+// t * 64 >> shift onwards, lane l column l mod 2^shift of row l >> shift, and writes word
+// 64 t + l. The loads use no registers, so the caller computes its pixels while they are in
+// flight and waits for them (window_ready) before its first read.
+__device__ __forceinline__ void stage_window(const FrameArgs& a, const float* dmf, const DepthWindow& w, uint32_t win) {
+  const uint32_t lane = threadIdx.x & 63u, w4 = 4u * (uint32_t)a.W;
+  const uint32_t row = lane >> w.shift, col = lane & ((1u << w.shift) - 1u), step = 64u >> w.shift;
+  const char* src = reinterpret_cast<const char*>(dmf) + w.first;
+  const uint32_t off = row * w4 + 4u * col;
+  const int left = col < w.cols && row < w.rows ? (int)(w.rows - row) : 0;  // rows the lane loads, at most
+  const uint32_t n_it = (w.rows + step - 1u) >> (6u - w.shift);             // loads of the widest lane
+  for (uint32_t t = 0; t < n_it; ++t) {
+    if ((int)(t * step) < left)
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + t * step * w4 + off),
+                                       (__attribute__((address_space(3))) void*)(size_t)(win + 256u * t), 4, 0, 0);
+  }
+}
+// Wait for the window's loads (vmcnt 0; the compiler does not order a read through an integer
+// LDS address after them by itself).
+__device__ __forceinline__ void window_ready() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // One frame's update of the lane's 8 voxels of a block (homogeneous column cl, step ax): per
 // half, all projections, then all pre-scaled depth (and colour) fetches, then the updates, so
 // the fetches are in flight together. Branch-free: every lane computes, the observation flag
 // selects. dmf: the frame's pre-scaled depth image (scaled_depth of each pixel).
-// kInside (block_inside holds): the range checks are known to pass and are left out; the
-// values computed are the same. (Issuing the next frame's gathers before this frame's
-// updates, with two register sets, was slower: 92 against 86 us per FP32 batch launch.)
-template <int kStorage, bool kColor, bool kInside, bool kTab>
+// kPlace (block_place's result): kPlaceInside and kPlaceWindow leave out the range checks,
+// which are known to pass (the values computed are the same); kPlaceWindow reads the depth
+// from the window that stage_window is loading (pitch 2^wshift; wbase: the LDS byte address of
+// pixel (u, v) is (bits(M + v) << (wshift + 2)) + (bits(M + u) << 2) + wbase modulo 2^32, the
+// window's address minus DepthWindow::bias). (Issuing the next frame's gathers before
+// this frame's updates, with two register sets, was slower: 92 against 86 us per FP32 batch
+// launch.)
+template <int kStorage, bool kColor, int kPlace, bool kTab>
 __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[3], const float cl[3],
-                                             const float* dmf, const uint32_t* rgba,
-                                             typename VoxelIO<kStorage>::Word old[8], uint32_t cold[8],
-                                             uint32_t& upd, const float* rcp_tab) {
+                                             const float* dmf, const uint32_t* rgba, uint32_t wshift,
+                                             uint32_t wbase, typename VoxelIO<kStorage>::Word old[8],
+                                             uint32_t cold[8], uint32_t& upd, const float* rcp_tab) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
+  constexpr bool kInside = kPlace != kPlaceEdge, kWindow = kPlace == kPlaceWindow;
   const uint32_t w4 = (4u * (uint32_t)a.W) & 0xffffffu, bias = pix_bias(a.W);
   // two halves of 4 voxels: fewer values live at once (occupancy) for 4 fetches in flight
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
     float zc[4];
     int pix[4];
-    uint32_t off[4];  // kInside: byte offset of the pixel in the depth (and RGB) image, off a scalar base
+    uint32_t off[4];   // kInside: byte offset of the pixel in the depth (and RGB) image, off a scalar base
+    uint32_t loff[4];  // kWindow: byte offset of the pixel in the LDS window
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float fq = (float)(h + r);
@@ -389,7 +492,12 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[
       if constexpr (kInside) {
         const uint32_t ub = __float_as_uint(__builtin_fmaf(X, iz, kPixMagic));
         const uint32_t vb = __float_as_uint(__builtin_fmaf(Y, iz, kPixMagic));
-        off[r] = (vb & 0xffffffu) * w4 + ((ub << 2) - bias);
+        if constexpr (kWindow) {
+          loff[r] = (((vb << wshift) + ub) << 2) + wbase;
+          if constexpr (kColor) off[r] = (vb & 0xffffffu) * w4 + ((ub << 2) - bias);
+        } else {
+          off[r] = (vb & 0xffffffu) * w4 + ((ub << 2) - bias);
+        }
       } else {
         pix[r] = pixel_of(X, Y, zc[r], iz, a.W, a.H);
       }
@@ -397,7 +505,11 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[
     float dv[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      if constexpr (kInside) dv[r] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmf) + off[r]);
+      if constexpr (kWindow) {
+        if (h == 0 && r == 0) window_ready();  // the pixels are computed: now wait for the window
+        dv[r] = lds_read(loff[r]);
+      }
+      else if constexpr (kInside) dv[r] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmf) + off[r]);
       else dv[r] = dmf[max(pix[r], 0)];
     }
     uint32_t px[4];
@@ -436,11 +548,12 @@ struct FrameLds {
   BlockFrame h;
 };
 
-// s_frm: the batch's k frames; rcp_tab: RN(1/w) for w < kRcpTab (read when kTab).
+// s_frm: the batch's k frames; rcp_tab: RN(1/w) for w < kRcpTab (read when kTab); s_win: the
+// LDS byte address of the wave's depth window (kWin pixels).
 template <int kStorage, bool kColor, bool kTab>
 __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t f0, uint32_t k,
                                  uint32_t parity, unsigned long long* s_val, unsigned long long* s_key,
-                                 uint32_t* s_mark, const FrameLds* s_frm, const float* rcp_tab) {
+                                 uint32_t* s_mark, const FrameLds* s_frm, const float* rcp_tab, uint32_t s_win) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   FrameCtl* ctl = a.ctl;
@@ -528,10 +641,19 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         column_h(hf, hb, lane >> 3, lane & 7, cl);
         const size_t fo = ((size_t)parity * kMaxBatch + jf) * frame_px;
         const uint32_t* rgba = kColor ? a.rgba + fo : nullptr;
-        if (block_inside(a, hf.ax, cl))
-          frame_update<kStorage, kColor, true, kTab>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd, rcp_tab);
-        else
-          frame_update<kStorage, kColor, false, kTab>(a, hf.ax, cl, a.dm + fo, rgba, old, cold, upd, rcp_tab);
+        const float* dmf = a.dm + fo;
+        DepthWindow win;
+        const int place = block_place(a, hf.ax, cl, win);
+        if (place == kPlaceWindow) {
+          stage_window(a, dmf, win, s_win);
+          frame_update<kStorage, kColor, kPlaceWindow, kTab>(a, hf.ax, cl, dmf, rgba, win.shift, s_win - win.bias,
+                                                             old, cold, upd, rcp_tab);
+        } else if (place == kPlaceInside) {
+          frame_update<kStorage, kColor, kPlaceInside, kTab>(a, hf.ax, cl, dmf, rgba, 0u, 0u, old, cold, upd,
+                                                             rcp_tab);
+        } else {
+          frame_update<kStorage, kColor, kPlaceEdge, kTab>(a, hf.ax, cl, dmf, rgba, 0u, 0u, old, cold, upd, rcp_tab);
+        }
       }
       // Depth-only volumes write the block whole (unobserved voxels keep the words read,
       // fresh blocks zero there; the wave owns the block in this launch), which saves the
@@ -660,27 +782,43 @@ __device__ __forceinline__ bool rcp_table_covers(const FrameArgs& a, const Frame
   return a.cap < (float)(kRcpTab - 1) || uniform64(ctl->frames) + k < (unsigned long long)(kRcpTab - 1);
 }
 
+// The integration's LDS: per wave the staged list entries, slot values, keys and marks and
+// the depth window; per workgroup the batch's frames and the reciprocal table.
+struct IntegrateLds {
+  FrameLds frm[kMaxBatch];  // first: a frame's words are then within one read's offset range
+  unsigned long long val[kIntegrateWaves][64], key[kIntegrateWaves][64];
+  uint32_t mark[kIntegrateWaves][64];
+  float rcp[kRcpTab];
+  float win[kIntegrateWaves][kWin];
+};
+
+// The workgroup's share of the batch of k frames (first frame f0, list parity, first input
+// cur). All 256 lanes call it.
+template <int kStorage, bool kColor>
+__device__ __forceinline__ void integrate_group(const FrameArgs& a, uint32_t group, uint32_t groups, uint32_t f0,
+                                                uint32_t k, uint32_t parity, uint32_t cur, IntegrateLds& L) {
+  const int wave = threadIdx.x >> 6;
+  const bool tab = rcp_table_covers(a, a.ctl, k);
+  stage_frames(a, cur, k, L.frm, L.rcp, tab);
+  const uint32_t worker = group * kIntegrateWaves + wave, workers = groups * kIntegrateWaves;
+  const uint32_t win = uniform(lds_address(L.win[wave]));
+  if (tab)
+    integrate_worker<kStorage, kColor, true>(a, worker, workers, f0, k, parity, L.val[wave], L.key[wave],
+                                             L.mark[wave], L.frm, L.rcp, win);
+  else
+    integrate_worker<kStorage, kColor, false>(a, worker, workers, f0, k, parity, L.val[wave], L.key[wave],
+                                              L.mark[wave], L.frm, L.rcp, win);
+}
+
 // Integration of the batch of k frames that the last allocation launch prepared.
 template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate(FrameArgs a, uint32_t k) {
-  __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
-  __shared__ uint32_t s_mark[kIntegrateWaves][64];
-  __shared__ FrameLds s_frm[kMaxBatch];
-  __shared__ float s_rcp[kRcpTab];
-  const int wave = threadIdx.x >> 6;
+  __shared__ IntegrateLds L;
   const FrameCtl* ctl = a.ctl;
   // control values are uniform: keep them (and what derives from them) in scalar registers
   const uint32_t f0 = uniform(ctl->frame_id), parity = uniform(ctl->batch_id) & 1u,
                  cur = uniform(ctl->cursor[a.cursor_id]);
-  const bool tab = rcp_table_covers(a, ctl, k);
-  stage_frames(a, cur, k, s_frm, s_rcp, tab);
-  const uint32_t worker = blockIdx.x * kIntegrateWaves + wave, workers = gridDim.x * kIntegrateWaves;
-  if (tab)
-    integrate_worker<kStorage, kColor, true>(a, worker, workers, f0, k, parity, s_val[wave], s_key[wave], s_mark[wave],
-                                             s_frm, s_rcp);
-  else
-    integrate_worker<kStorage, kColor, false>(a, worker, workers, f0, k, parity, s_val[wave], s_key[wave],
-                                              s_mark[wave], s_frm, s_rcp);
+  integrate_group<kStorage, kColor>(a, blockIdx.x, gridDim.x, f0, k, parity, cur, L);
   batch_done(a, gridDim.x, f0, k, parity);
 }
 
@@ -694,10 +832,11 @@ template <int kStorage, bool kU16, bool kColor>
 __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate_alloc(FrameArgs a, uint32_t int_groups,
                                                                          uint32_t alloc_groups, uint32_t tiles,
                                                                          uint32_t k_int) {
-  __shared__ unsigned long long s_val[kIntegrateWaves][64], s_key[kIntegrateWaves][64];
-  __shared__ uint32_t s_mark[kIntegrateWaves][64];
-  __shared__ FrameLds s_frm[kMaxBatch];
-  __shared__ float s_rcp[kRcpTab];
+  // a workgroup either allocates or integrates: the two share the LDS
+  __shared__ union {
+    AllocLds<kAllocRows> alloc;
+    IntegrateLds integ;
+  } L;
   const uint32_t b = blockIdx.x;
   const bool alloc = b >= int_groups;
   const uint32_t idx = alloc ? b - int_groups : b;
@@ -706,18 +845,9 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
   if (alloc) {
     const uint32_t jb = idx / tiles;
     alloc_tile<kU16, kAllocRows>(a, idx - jb * tiles, f0 + k_int + jb, jb, (batch + 1u) & 1u,
-                                 input_slot(cur, k_int + jb, a.n_seq));
-  } else {
-    const int wave = threadIdx.x >> 6;
-    const bool tab = rcp_table_covers(a, ctl, k_int);
-    stage_frames(a, cur, k_int, s_frm, s_rcp, tab);  // uniform branch: the whole workgroup integrates
-    const uint32_t worker = idx * kIntegrateWaves + wave, workers = int_groups * kIntegrateWaves;
-    if (tab)
-      integrate_worker<kStorage, kColor, true>(a, worker, workers, f0, k_int, batch & 1u, s_val[wave], s_key[wave],
-                                               s_mark[wave], s_frm, s_rcp);
-    else
-      integrate_worker<kStorage, kColor, false>(a, worker, workers, f0, k_int, batch & 1u, s_val[wave], s_key[wave],
-                                                s_mark[wave], s_frm, s_rcp);
+                                 input_slot(cur, k_int + jb, a.n_seq), L.alloc);
+  } else {  // uniform branch: the whole workgroup integrates
+    integrate_group<kStorage, kColor>(a, idx, int_groups, f0, k_int, batch & 1u, cur, L.integ);
   }
   batch_done(a, gridDim.x, f0, k_int, batch & 1u);
 }

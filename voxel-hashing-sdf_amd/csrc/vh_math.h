@@ -295,22 +295,31 @@ struct BandDDA {
   int ex, ey, ez;                // last block
 };
 
-VH_HD bool band_setup(const float* T, const Intr& K, float bsz, float mu, float min_depth, int u,
+// Reciprocals of the intrinsics' focal lengths and of the block size, RN(1/x) of each: the band
+// setup multiplies by them instead of dividing.
+struct BandScale {
+  float ifx, ify, ibsz;
+};
+VH_HD BandScale band_scale(const Intr& K, float bsz) { return BandScale{1.0f / K.fx, 1.0f / K.fy, 1.0f / bsz}; }
+
+// Segment of pixel (u, v) between camera depths t0 = max(d - mu, min_depth) and t1 = d + mu,
+// in block units: the pixel's ray per unit depth r = ((u - cx) * RN(1/fx), (v - cy) * RN(1/fy)),
+// camera points (r t, t), world points R p + t (cam_to_world), block coordinates
+// world * RN(1/bsz).
+VH_HD bool band_setup(const float* T, const Intr& K, const BandScale& sc, float mu, float min_depth, int u,
                       int v, float d, BandDDA& s) {
   float t0 = d - mu;
   if (t0 < min_depth) t0 = min_depth;
   const float t1 = d + mu;
-  const float ru = (float)u - K.cx, rv = (float)v - K.cy;
-  const float cax = ru * t0 / K.fx, cay = rv * t0 / K.fy;
-  const float cbx = ru * t1 / K.fx, cby = rv * t1 / K.fy;
+  const float rx = ((float)u - K.cx) * sc.ifx, ry = ((float)v - K.cy) * sc.ify;
   float wax, way, waz, wbx, wby, wbz;
-  cam_to_world(T, cax, cay, t0, wax, way, waz);
-  cam_to_world(T, cbx, cby, t1, wbx, wby, wbz);
-  s.ax = wax / bsz; s.ay = way / bsz; s.az = waz / bsz;
-  const float bx = wbx / bsz, by = wby / bsz, bz = wbz / bsz;
-  const float lim = 1048000.0f;  // keeps every block inside the 21-bit key range
-  if (!(fabsf(s.ax) < lim && fabsf(s.ay) < lim && fabsf(s.az) < lim && fabsf(bx) < lim &&
-        fabsf(by) < lim && fabsf(bz) < lim))
+  cam_to_world(T, rx * t0, ry * t0, t0, wax, way, waz);
+  cam_to_world(T, rx * t1, ry * t1, t1, wbx, wby, wbz);
+  s.ax = wax * sc.ibsz; s.ay = way * sc.ibsz; s.az = waz * sc.ibsz;
+  const float bx = wbx * sc.ibsz, by = wby * sc.ibsz, bz = wbz * sc.ibsz;
+  const float lim = 1048000.0f;  // keeps every block inside the 21-bit key range (NaN fails)
+  if (!((fabsf(s.ax) < lim) & (fabsf(s.ay) < lim) & (fabsf(s.az) < lim) & (fabsf(bx) < lim) & (fabsf(by) < lim) &
+        (fabsf(bz) < lim)))
     return false;
   s.dx = bx - s.ax; s.dy = by - s.ay; s.dz = bz - s.az;
   s.ix = (int)floorf(s.ax); s.iy = (int)floorf(s.ay); s.iz = (int)floorf(s.az);
