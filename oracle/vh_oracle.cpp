@@ -188,7 +188,8 @@ struct Oracle {
 // taken as products by their rounded reciprocals); world point R p + t as cam2base
 // (src/tsdf.cu:95-105); block coordinates world * RN(1/(8 vox)).
 // Walk: from floor(start) to floor(end), stepping the axis whose next face is crossed
-// first among the axes that still have to move (lowest axis on ties).
+// first among the axes that still have to move (lowest axis on ties); an axis's crossing
+// parameter is (face - start) * RN(1/direction) (the reciprocal taken once per pixel).
 template <class Emit>
 void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit emit) {
   float t0 = d - o.mu;
@@ -211,11 +212,12 @@ void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit em
     for (int r = 0; r < 3; ++r)
       if (!(std::fabs(p[e][r]) < 1048000.0f)) return;
   int cur[3], end[3];
-  float dir[3];
+  float rdir[3];  // 1/direction, 2^64 with its sign outside 2^-64 <= |direction| <= 2^64
   for (int r = 0; r < 3; ++r) {
     cur[r] = (int)std::floor(p[0][r]);
     end[r] = (int)std::floor(p[1][r]);
-    dir[r] = p[1][r] - p[0][r];
+    const float dir = p[1][r] - p[0][r], m = std::fabs(dir);
+    rdir[r] = (m >= 0x1p-64f && m <= 0x1p64f) ? 1.0f / dir : std::copysign(0x1p64f, dir);
   }
   emit(cur[0], cur[1], cur[2]);
   while (cur[0] != end[0] || cur[1] != end[1] || cur[2] != end[2]) {
@@ -224,7 +226,7 @@ void band_blocks(const Oracle& o, const float* T, int u, int v, float d, Emit em
     for (int r = 0; r < 3; ++r) {
       if (cur[r] == end[r]) continue;
       const float face = (float)(end[r] > cur[r] ? cur[r] + 1 : cur[r]);
-      const float t = (face - p[0][r]) / dir[r];
+      const float t = (face - p[0][r]) * rdir[r];
       if (best < 0 || t < bt) {
         best = r;
         bt = t;

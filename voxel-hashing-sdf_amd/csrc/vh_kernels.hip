@@ -260,7 +260,7 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
     BandDDA s;
     bool active = false;
     if (u < a.W && v < a.H && depth_valid(dr[r], a.max_depth)) {
-      active = band_setup(T, a.K, a.band, a.mu, a.min_depth, u, v, dr[r], s);
+      active = band_setup(T, a.K, a.band, a.mu, a.min_depth, u, v, dr[r], s, [](float x) { return recip_rn(x); });
       if (!active) raise_flag(ctl, kFlagRange);
     }
     // phase 1: distinct keys of the tile into LDS; keys that do not fit are claimed directly

@@ -290,10 +290,22 @@ VH_HD float f16r_inv_scale_of(float hf) {
 // exactly at the end block after |e - i|_1 steps. band_setup returns false if the band
 // leaves the key range.
 struct BandDDA {
-  float ax, ay, az, dx, dy, dz;  // start point and direction, block units
-  int ix, iy, iz;                // current block
-  int ex, ey, ez;                // last block
+  float ax, ay, az;  // start point, block units
+  float rx, ry, rz;  // band_recip of the direction (end - start)
+  int ix, iy, iz;    // current block
+  int ex, ey, ez;    // last block
 };
+
+// Reciprocal of a direction component for the crossing parameters: RN(1/d) for 2^-64 <= |d|
+// <= 2^64, else 2^64 with d's sign (an axis that moves has |d| >= 2^-64 unless its segment
+// straddles the face 0 within 2^-64 of it; a zero component's axis does not move). rn: a
+// correctly rounded reciprocal valid on that range (the kernels pass their three-instruction
+// one).
+template <class Rn>
+VH_HD float band_recip(float d, Rn rn) {
+  const float m = fabsf(d);
+  return (m >= 0x1p-64f && m <= 0x1p64f) ? rn(d) : copysignf(0x1p64f, d);
+}
 
 // Reciprocals of the intrinsics' focal lengths and of the block size, RN(1/x) of each: the band
 // setup multiplies by them instead of dividing.
@@ -306,8 +318,9 @@ VH_HD BandScale band_scale(const Intr& K, float bsz) { return BandScale{1.0f / K
 // in block units: the pixel's ray per unit depth r = ((u - cx) * RN(1/fx), (v - cy) * RN(1/fy)),
 // camera points (r t, t), world points R p + t (cam_to_world), block coordinates
 // world * RN(1/bsz).
+template <class Rn>
 VH_HD bool band_setup(const float* T, const Intr& K, const BandScale& sc, float mu, float min_depth, int u,
-                      int v, float d, BandDDA& s) {
+                      int v, float d, BandDDA& s, Rn rn) {
   float t0 = d - mu;
   if (t0 < min_depth) t0 = min_depth;
   const float t1 = d + mu;
@@ -321,23 +334,24 @@ VH_HD bool band_setup(const float* T, const Intr& K, const BandScale& sc, float 
   if (!((fabsf(s.ax) < lim) & (fabsf(s.ay) < lim) & (fabsf(s.az) < lim) & (fabsf(bx) < lim) & (fabsf(by) < lim) &
         (fabsf(bz) < lim)))
     return false;
-  s.dx = bx - s.ax; s.dy = by - s.ay; s.dz = bz - s.az;
+  s.rx = band_recip(bx - s.ax, rn); s.ry = band_recip(by - s.ay, rn); s.rz = band_recip(bz - s.az, rn);
   s.ix = (int)floorf(s.ax); s.iy = (int)floorf(s.ay); s.iz = (int)floorf(s.az);
   s.ex = (int)floorf(bx); s.ey = (int)floorf(by); s.ez = (int)floorf(bz);
   return true;
 }
-// Crossing parameter of the next block face along one axis (only for axes still moving).
-VH_HD float band_t(int i, int e, float a, float dir) {
+// Crossing parameter of the next block face along one axis (only for axes still moving):
+// (face - a) * band_recip(direction).
+VH_HD float band_t(int i, int e, float a, float rdir) {
   const float bound = (float)(e > i ? i + 1 : i);
-  return (bound - a) / dir;
+  return (bound - a) * rdir;
 }
 // Advances to the next block; returns false when the current block was the last one.
 VH_HD bool band_next(BandDDA& s) {
   const bool mx = s.ix != s.ex, my = s.iy != s.ey, mz = s.iz != s.ez;
   if (!(mx || my || mz)) return false;
-  const float tx = mx ? band_t(s.ix, s.ex, s.ax, s.dx) : 0.0f;
-  const float ty = my ? band_t(s.iy, s.ey, s.ay, s.dy) : 0.0f;
-  const float tz = mz ? band_t(s.iz, s.ez, s.az, s.dz) : 0.0f;
+  const float tx = mx ? band_t(s.ix, s.ex, s.ax, s.rx) : 0.0f;
+  const float ty = my ? band_t(s.iy, s.ey, s.ay, s.ry) : 0.0f;
+  const float tz = mz ? band_t(s.iz, s.ez, s.az, s.rz) : 0.0f;
   // smallest crossing parameter first; ties go to the lower axis
   const bool take_x = mx && (!my || !(ty < tx)) && (!mz || !(tz < tx));
   const bool take_y = !take_x && my && (!mz || !(tz < ty));
