@@ -983,6 +983,15 @@ __global__ __launch_bounds__(256) void k_carve(FrameArgs a, uint32_t k) {
 constexpr int kHalo = 9;
 constexpr int kHalo3 = kHalo * kHalo * kHalo;
 constexpr int kItemCap = 1024;  // (cube, triangle) items listed per round
+// Extraction workgroups: kMcWaves independent waves, one block each, each with its own LDS
+// (McShared). A wave only exchanges data with itself through LDS, and executes its LDS
+// instructions in order, so wave_sync (a barrier to the compiler's reordering) suffices
+// between a write and other lanes' reads. One wave per workgroup: four (24 instead of 16
+// resident waves per CU) made the room_medium_5mm extraction slower, 0.87 against 0.69 ms.
+constexpr int kMcWaves = 1;
+__device__ __forceinline__ int mc_lane() { return (int)(threadIdx.x & 63u); }
+__device__ __forceinline__ int mc_wave() { return (int)(threadIdx.x >> 6); }
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
 // LDS of one extraction wave (about 6 KB).
 struct McShared {
@@ -995,7 +1004,7 @@ struct McShared {
 };
 
 __device__ __forceinline__ void mc_load_counts(McShared& S) {
-  for (int c = threadIdx.x; c < 256; c += 64) S.cnt[c] = (uint8_t)(kTriTable[c] >> 60);
+  for (int c = mc_lane(); c < 256; c += 64) S.cnt[c] = (uint8_t)(kTriTable[c] >> 60);
 }
 
 // One wave per pool block. The block's 8^3 voxels and the +x/+y/+z faces, edges and corner
@@ -1009,11 +1018,13 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
                                          int& gz0) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
-  const int lane = threadIdx.x;
+  const int lane = mc_lane();
   int bx, by, bz;
   unpack_key(a.block_keys[b], bx, by, bz);
-  if (lane < 8) S.nb[lane] = a.nbrs[(size_t)b * 8 + lane];  // k_mc_neighbours
-  __syncthreads();
+  // the neighbourhood (k_mc_neighbours) in lanes 0..7, read by the wave from its registers
+  const uint32_t nbv = lane < 8 ? a.nbrs[(size_t)b * 8 + lane] : kNoBlock;
+  if (lane < 8) S.nb[lane] = nbv;
+  auto nb_of = [&](int o) { return (uint32_t)__builtin_amdgcn_readlane((int)nbv, o); };
   const Word* vox = reinterpret_cast<const Word*>(a.voxels);
   const float nan = __builtin_nanf("");
   auto value = [&](Word w) {
@@ -1028,7 +1039,7 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
 #pragma unroll
   for (int k = 0; k < 8; ++k) w[k] = own[64 * k];
   // faces x = 8 (voxel (0, hi, lo) of +x), y = 8 ((hi, 0, lo) of +y), z = 8 ((hi, lo, 0) of +z)
-  const uint32_t nfx = S.nb[4], nfy = S.nb[2], nfz = S.nb[1];
+  const uint32_t nfx = nb_of(4), nfy = nb_of(2), nfz = nb_of(1);
   const Word wfx = nfx != kNoBlock ? vox[(size_t)nfx * kB3 + lane] : IO::zero();
   const Word wfy = nfy != kNoBlock ? vox[(size_t)nfy * kB3 + hi * 64 + lo] : IO::zero();
   const Word wfz = nfz != kNoBlock ? vox[(size_t)nfz * kB3 + hi * 64 + lo * 8] : IO::zero();
@@ -1037,10 +1048,11 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
   int edst = 0;
   uint32_t enb = kNoBlock;
   int esrc = 0;
-  if (lane < 8) { enb = S.nb[6]; esrc = lane; edst = 8 * 81 + 8 * 9 + lane; }
-  else if (lane < 16) { enb = S.nb[5]; esrc = (lane - 8) * 8; edst = 8 * 81 + (lane - 8) * 9 + 8; }
-  else if (lane < 24) { enb = S.nb[3]; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; }
-  else if (lane == 24) { enb = S.nb[7]; esrc = 0; edst = kHalo3 - 1; }
+  const uint32_t ne6 = nb_of(6), ne5 = nb_of(5), ne3 = nb_of(3), ne7 = nb_of(7);
+  if (lane < 8) { enb = ne6; esrc = lane; edst = 8 * 81 + 8 * 9 + lane; }
+  else if (lane < 16) { enb = ne5; esrc = (lane - 8) * 8; edst = 8 * 81 + (lane - 8) * 9 + 8; }
+  else if (lane < 24) { enb = ne3; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; }
+  else if (lane == 24) { enb = ne7; esrc = 0; edst = kHalo3 - 1; }
   const Word we = enb != kNoBlock ? vox[(size_t)enb * kB3 + esrc] : IO::zero();
   if constexpr (kColor) {
     const uint32_t* cp = a.colors;
@@ -1064,7 +1076,7 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
   S.val[hi * 81 + 8 * 9 + lo] = nfy != kNoBlock ? value(wfy) : nan;
   S.val[hi * 81 + lo * 9 + 8] = nfz != kNoBlock ? value(wfz) : nan;
   if (lane < 25) S.val[edst] = enb != kNoBlock ? value(we) : nan;
-  __syncthreads();
+  wave_sync();
   gx0 = bx * kB;
   gy0 = by * kB;
   gz0 = bz * kB;
@@ -1077,7 +1089,7 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
 // set iff cube x of the lane's column has triangles and a corner with |value| < tiny (see
 // mc_count_exact).
 __device__ __forceinline__ uint32_t mc_classify(McShared& S, float tiny, uint32_t& near) {
-  const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
+  const int lane = mc_lane(), cy = lane >> 3, cz = lane & 7;
   float f0[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) f0[k] = S.val[(cy + (k >> 1)) * 9 + cz + (k & 1)];
@@ -1113,7 +1125,7 @@ __device__ __forceinline__ uint32_t mc_classify(McShared& S, float tiny, uint32_
   }
   S.lane_off[lane] = x - ntri;
   if (lane == 63) S.lane_off[64] = x;
-  __syncthreads();
+  wave_sync();
   return S.lane_off[64];
 }
 
@@ -1217,7 +1229,7 @@ __device__ __forceinline__ uint32_t mc_count_near(const McShared& S, int ci, int
 }
 
 __device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t near, int gx0, int gy0, int gz0) {
-  const int lane = threadIdx.x, cy = lane >> 3, cz = lane & 7;
+  const int lane = mc_lane(), cy = lane >> 3, cz = lane & 7;
   uint32_t n = 0;
   for (int cx = 0; cx < 8; ++cx) {
     const int ci = S.cases[cx * 64 + lane];
@@ -1237,7 +1249,7 @@ __device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t n
 template <bool kColor, bool kPos, class Emit>
 __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint32_t total, int gx0, int gy0, int gz0,
                                          Emit emit) {
-  const int lane = threadIdx.x;
+  const int lane = mc_lane();
   const uint32_t my0 = S.lane_off[lane], my1 = S.lane_off[lane + 1];
   for (uint32_t r0 = 0; r0 < total; r0 += kItemCap) {
     const uint32_t r1 = min(total, r0 + (uint32_t)kItemCap);
@@ -1250,7 +1262,7 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
           if (idx >= r0 && idx < r1) S.items[idx - r0] = (uint16_t)((cube << 3) | t);
       }
     }
-    __syncthreads();
+    wave_sync();
     for (uint32_t i0 = r0; i0 < r1; i0 += 64) {
       const uint32_t item = i0 + lane;
       bool valid = false;
@@ -1270,7 +1282,7 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
       }
       emit(valid, gx0 + cx, gy0 + cy, gz0 + cz, ci, t, P, C);
     }
-    __syncthreads();
+    wave_sync();  // the next round overwrites the items
   }
 }
 
@@ -1306,21 +1318,21 @@ __global__ __launch_bounds__(256) void k_mc_neighbours(McArgs a) {
 }
 
 template <int kStorage>
-__global__ __launch_bounds__(64) void k_mc_count(McArgs a) {
-  __shared__ McShared S;
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a) {
+  __shared__ McShared SS[kMcWaves];
+  McShared& S = SS[mc_wave()];
+  const int lane = mc_lane();
+  const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
+  if (b >= a.n_blocks) return;
   mc_load_counts(S);
-  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
-    int gx0, gy0, gz0;
-    mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
-    uint32_t near;
-    mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
-    uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
+  int gx0, gy0, gz0;
+  mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
+  uint32_t near;
+  mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
+  uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, m);
-    if (lane == 0) a.counts[b] = cnt;
-    __syncthreads();
-  }
+  for (int m = 32; m >= 1; m >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, m);
+  if (lane == 0) a.counts[b] = cnt;
 }
 
 // Exclusive scan of counts within chunks of kScanChunk blocks (one workgroup of 256 each,
@@ -1403,13 +1415,17 @@ __device__ __forceinline__ uint32_t block_offset(const uint32_t* counts, const u
 }
 
 template <int kStorage, bool kColor>
-__global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
-  __shared__ McShared S;
-  __shared__ uint32_t s_col[kColor ? kHalo3 : 1];
-  mc_load_counts(S);
-  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+__global__ __launch_bounds__(64 * kMcWaves) void k_mc_write(McArgs a) {
+  __shared__ McShared SS[kMcWaves];
+  __shared__ uint32_t s_cols[kMcWaves][kColor ? kHalo3 : 1];
+  McShared& S = SS[mc_wave()];
+  uint32_t* s_col = s_cols[mc_wave()];
+  const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
+  if (b >= a.n_blocks) return;
+  {
     uint32_t pos = block_offset(a.counts, a.chunk_sums, b);
-    if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos) continue;  // no triangles
+    if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos) return;  // no triangles
+    mc_load_counts(S);
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
     uint32_t near;
@@ -1434,7 +1450,6 @@ __global__ __launch_bounds__(64) void k_mc_write(McArgs a) {
       }
       pos += (uint32_t)__popcll(m);
     });
-    __syncthreads();
   }
 }
 
@@ -1465,12 +1480,16 @@ __device__ __forceinline__ void edge_owner(int e, int cx, int cy, int cz, int& o
 // owns are gathered in LDS and merged with one atomic OR per word; edges owned by a +x/+y/+z
 // neighbour go to its words directly.
 template <int kStorage>
-__global__ __launch_bounds__(64) void k_ix_mark(McArgs a) {
-  __shared__ McShared S;
-  __shared__ uint32_t s_bits[kEdgeWords];
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * kMcWaves) void k_ix_mark(McArgs a) {
+  __shared__ McShared SS[kMcWaves];
+  __shared__ uint32_t s_bitss[kMcWaves][kEdgeWords];
+  McShared& S = SS[mc_wave()];
+  uint32_t* s_bits = s_bitss[mc_wave()];
+  const int lane = mc_lane();
+  const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
+  if (b >= a.n_blocks) return;
   mc_load_counts(S);
-  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+  {
     if (lane < kEdgeWords) s_bits[lane] = 0u;
     int gx0, gy0, gz0;
     mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
@@ -1488,9 +1507,8 @@ __global__ __launch_bounds__(64) void k_ix_mark(McArgs a) {
       }
     }
     if (lane == 0) a.counts[b] = total;
-    __syncthreads();
+    wave_sync();
     if (lane < kEdgeWords && s_bits[lane]) atomicOr(&a.ebits[(size_t)b * kEdgeWords + lane], s_bits[lane]);
-    __syncthreads();
   }
 }
 
@@ -1514,17 +1532,22 @@ __global__ __launch_bounds__(256) void k_ix_count(McArgs a) {
 
 // Faces (vertex ids of each table triangle's corners) and the vertices each block owns.
 template <int kStorage, bool kColor>
-__global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
-  __shared__ McShared S;
-  __shared__ uint32_t s_col[kColor ? kHalo3 : 1];
-  __shared__ uint32_t s_vbase[8];
-  const int lane = threadIdx.x;
-  mc_load_counts(S);
-  for (uint32_t b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+__global__ __launch_bounds__(64 * kMcWaves) void k_ix_write(McArgs a) {
+  __shared__ McShared SS[kMcWaves];
+  __shared__ uint32_t s_cols[kMcWaves][kColor ? kHalo3 : 1];
+  __shared__ uint32_t s_vbases[kMcWaves][8];
+  McShared& S = SS[mc_wave()];
+  uint32_t* s_col = s_cols[mc_wave()];
+  uint32_t* s_vbase = s_vbases[mc_wave()];
+  const int lane = mc_lane();
+  const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
+  if (b >= a.n_blocks) return;
+  {
     uint32_t pos = block_offset(a.counts, a.chunk_sums, b);
     if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos &&
         block_offset(a.vcounts, a.vchunks, b + 1, a.n_blocks, a.vtotal) == block_offset(a.vcounts, a.vchunks, b))
-      continue;  // no faces and no vertices of its own
+      return;  // no faces and no vertices of its own
+    mc_load_counts(S);
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
     if (lane < 8) {
@@ -1585,7 +1608,6 @@ __global__ __launch_bounds__(64) void k_ix_write(McArgs a) {
         ++id;
       }
     }
-    __syncthreads();
   }
 }
 
@@ -2406,13 +2428,13 @@ hipError_t launch_integrate_alloc(const FrameArgs& a, int storage, int depth_u16
   return hipGetLastError();
 }
 
-// Grid of the per-block extraction kernels: one 64-lane workgroup per block, so the
-// dispatcher balances blocks of unequal work (empty, dense, near-corner). Measured on the
-// final room_medium_5mm volume: 0.74 ms per extraction against 0.87 ms with 8,192
-// block-striding workgroups and 0.85 ms with one resident round of them.
+// Grid of the per-block extraction kernels: one wave per block, kMcWaves per workgroup, so
+// the dispatcher balances blocks of unequal work (empty, dense, near-corner). Measured on the
+// final room_medium_5mm volume: 0.74 ms per extraction with one-wave workgroups against
+// 0.87 ms with 8,192 block-striding workgroups and 0.85 ms with one resident round of them.
 template <class Kernel>
 static uint32_t block_grid(Kernel, uint32_t n_blocks) {
-  return n_blocks;
+  return (n_blocks + kMcWaves - 1) / kMcWaves;
 }
 
 hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
@@ -2421,13 +2443,13 @@ hipError_t launch_mc(const McArgs& a, int storage, hipStream_t st) {
   hipLaunchKernelGGL(k_mc_neighbours, dim3((uint32_t)(((size_t)a.n_blocks * 8 + 255) / 256)), dim3(256), 0, st, a);
   with_storage(storage, [&](auto S) {
     auto kc = k_mc_count<decltype(S)::value>;
-    hipLaunchKernelGGL(kc, dim3(block_grid(kc, a.n_blocks)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(kc, dim3(block_grid(kc, a.n_blocks)), dim3(64 * kMcWaves), 0, st, a);
   });
   hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, a, n_chunks);
   with_storage(storage, [&](auto S) {
     auto kw = a.colors ? k_mc_write<decltype(S)::value, true> : k_mc_write<decltype(S)::value, false>;
-    hipLaunchKernelGGL(kw, dim3(block_grid(kw, a.n_blocks)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(kw, dim3(block_grid(kw, a.n_blocks)), dim3(64 * kMcWaves), 0, st, a);
   });
   return hipGetLastError();
 }
@@ -2438,7 +2460,7 @@ hipError_t launch_mesh_index_count(const McArgs& a, int storage, hipStream_t st)
   hipLaunchKernelGGL(k_mc_neighbours, dim3((uint32_t)(((size_t)a.n_blocks * 8 + 255) / 256)), dim3(256), 0, st, a);
   with_storage(storage, [&](auto S) {
     auto km = k_ix_mark<decltype(S)::value>;
-    hipLaunchKernelGGL(km, dim3(block_grid(km, a.n_blocks)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(km, dim3(block_grid(km, a.n_blocks)), dim3(64 * kMcWaves), 0, st, a);
   });
   hipLaunchKernelGGL(k_ix_count, dim3(min((a.n_blocks + 3u) / 4u, 16384u)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_scan_chunks, dim3(n_chunks), dim3(256), 0, st, a);
@@ -2456,7 +2478,7 @@ hipError_t launch_mesh_index_write(const McArgs& a, int storage, hipStream_t st)
   if (a.n_blocks == 0) return hipSuccess;
   with_storage(storage, [&](auto S) {
     auto kw = (a.colors && a.vrgb) ? k_ix_write<decltype(S)::value, true> : k_ix_write<decltype(S)::value, false>;
-    hipLaunchKernelGGL(kw, dim3(block_grid(kw, a.n_blocks)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(kw, dim3(block_grid(kw, a.n_blocks)), dim3(64 * kMcWaves), 0, st, a);
   });
   return hipGetLastError();
 }

@@ -331,9 +331,9 @@ VH_HD bool band_setup(const float* T, const Intr& K, const BandScale& sc, float 
   s.ax = wax * sc.ibsz; s.ay = way * sc.ibsz; s.az = waz * sc.ibsz;
   const float bx = wbx * sc.ibsz, by = wby * sc.ibsz, bz = wbz * sc.ibsz;
   const float lim = 1048000.0f;  // keeps every block inside the 21-bit key range (NaN fails)
-  if (!((fabsf(s.ax) < lim) & (fabsf(s.ay) < lim) & (fabsf(s.az) < lim) & (fabsf(bx) < lim) & (fabsf(by) < lim) &
-        (fabsf(bz) < lim)))
-    return false;
+  const int in_range = (int)(fabsf(s.ax) < lim) & (int)(fabsf(s.ay) < lim) & (int)(fabsf(s.az) < lim) &
+                       (int)(fabsf(bx) < lim) & (int)(fabsf(by) < lim) & (int)(fabsf(bz) < lim);
+  if (!in_range) return false;
   s.rx = band_recip(bx - s.ax, rn); s.ry = band_recip(by - s.ay, rn); s.rz = band_recip(bz - s.az, rn);
   s.ix = (int)floorf(s.ax); s.iy = (int)floorf(s.ay); s.iz = (int)floorf(s.az);
   s.ex = (int)floorf(bx); s.ey = (int)floorf(by); s.ez = (int)floorf(bz);
