@@ -446,8 +446,9 @@ def main():
                 "ta_busy": round(pj["ta_busy"], 3) if "ta_busy" in pj else None,
                 "ta_stalled_by_l1": round(pj["ta_stalled_by_tc"], 3) if "ta_stalled_by_tc" in pj else None,
                 "source": "rocprofv3 SQ_ACTIVE_INST_VALU, TA_BUSY_avr, TA_ADDR_STALLED_BY_TC_CYCLES / GRBM_GUI_ACTIVE "
-                          "of the same launches: neither the vector ALUs nor HBM are saturated; the depth gathers "
-                          "keep the vector memory path (texture addresser and L1) busiest"}
+                          "of the same launches: each block's depth window is loaded into LDS once per frame, so "
+                          "the texture path is no longer stalled; the launch is bound by vector instruction issue "
+                          "(projection, correctly rounded reciprocal and update per voxel and frame), not by HBM"}
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src, valu=valu,
                     kernel="k_integrate<F32,u16>", avg_launch_us=round(1e6 * t_int / launches, 2),
@@ -477,8 +478,13 @@ def main():
             f_traffic = int(fj["traffic_bytes_per_dispatch"])
             f_src = (f"rocprofv3 FETCH_SIZE x{fj['fetch_scale']:.3f} + WRITE_SIZE x{fj['write_scale']:.3f} "
                      f"(1 GiB copy calibration), {fj['workload']}, {fj['dispatches']} fused launches")
+        f_valu = None
+        if f_traffic is not None and "valu_busy" in fj:
+            f_valu = {"busy": round(fj["valu_busy"], 3),
+                      "source": "rocprofv3 SQ_ACTIVE_INST_VALU / GRBM_GUI_ACTIVE of the same launches: the fused "
+                                "launch is bound by vector instruction issue (integration + band walks)"}
         roofline_fused = dict(bound="hbm", achieved=round(a_f, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                              frac=round(a_f / HBM_PEAK_GBS, 4), traffic=f_traffic, traffic_source=f_src,
+                              frac=round(a_f / HBM_PEAK_GBS, 4), traffic=f_traffic, traffic_source=f_src, valu=f_valu,
                               kernel="k_integrate_alloc<F32,u16>", avg_launch_us=round(1e6 * t_f, 2),
                               timing="HIP events around each captured-graph batch launch on the volume's stream",
                               bytes_per_launch=int(fused_bytes), launches=fz["fused_launches"])
