@@ -371,60 +371,55 @@ struct DepthWindow {
 __device__ __forceinline__ uint32_t half_mirror(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xf, 0xf, false);
 }
-__device__ __forceinline__ uint32_t corner_min(uint32_t x) {
-  x = min(x, half_mirror(x));
-  return min((uint32_t)__builtin_amdgcn_readlane((int)x, 0), (uint32_t)__builtin_amdgcn_readlane((int)x, 56));
+__device__ __forceinline__ int32_t corner_min(int32_t x) {
+  x = min(x, (int32_t)half_mirror((uint32_t)x));
+  return min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 56));
 }
-__device__ __forceinline__ uint32_t corner_max(uint32_t x) {
-  x = max(x, half_mirror(x));
-  return max((uint32_t)__builtin_amdgcn_readlane((int)x, 0), (uint32_t)__builtin_amdgcn_readlane((int)x, 56));
+__device__ __forceinline__ int32_t corner_max(int32_t x) {
+  x = max(x, (int32_t)half_mirror((uint32_t)x));
+  return max(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 56));
 }
 
-// Placement of a block in a frame (wave-uniform). kPlaceEdge unless every voxel of the block
-// projects inside the image at a depth within the projection's range, which frame_update
-// then need not check: the lane's column runs from voxel 0 to voxel 7 of cl + q*ax
-// (homogeneous coordinates); the block's lattice box is the convex hull of the columns' end
-// points, and for depths > 0 the perspective image of a convex combination is a convex
-// combination of the images, so if every end point has z in [2^-60, 2^60] and X/z in
-// [1, W-2] (likewise Y/z), every voxel's z lies in [2^-64, 2^64] and its rounded pixel inside
-// the image (the arithmetic of the voxel's own projection differs from the end points' by far
-// less than the 1-pixel margin).
-// The same hull bounds the voxels' pixels: with umin, umax the least and largest u = X/z of
-// the block's 8 corners (lanes 0, 7, 56, 63, both ends), a voxel's u lies in
-// [umin - e, umax + e], e well below 1/2 for the same reason, and its pixel round(u) in
-// [round(umin - 1/2), round(umax + 1/2)] (for e < 1/2 these roundings cannot cross the
-// voxel's); likewise v. The rectangle is the block's window; kPlaceWindow when it fits kWin
-// at a pitch of at most 64 pixels.
+// Placement of a block in a frame (wave-uniform). The lane's column runs from voxel 0 to
+// voxel 7 of cl + q*ax (homogeneous coordinates); the block's lattice box is the convex hull
+// of its 8 corners, the column end points of lanes 0, 7, 56 and 63. For depths > 0 the
+// perspective image of a convex combination is a convex combination of the images, and a
+// voxel's own projection differs from the exact one, as the corners' do, by far less than
+// 1/2 pixel. So if every end point has z in [2^-60, 2^60], every voxel's z lies in
+// [2^-64, 2^64], and with umin, umax the least and largest u = X/z of the corners a voxel's
+// pixel round(u) lies in [round(umin - 1/2), round(umax + 1/2)] (for errors below 1/2 these
+// roundings cannot cross the voxel's); likewise v. That rectangle is the block's window.
+// kPlaceEdge unless the window lies inside the image; then kPlaceWindow when it fits kWin at a
+// pitch of at most 64 pixels, else kPlaceInside. The bounds are the bits of (u -+ 1/2) + M as
+// signed integers: for positive sums below 2^24 they are M's plus the rounded bound, larger
+// sums exceed every image bound and negative ones lie below them.
 __device__ __forceinline__ int block_place(const FrameArgs& a, const float ax[3], const float cl[3], DepthWindow& w) {
-  bool in = true;
+  bool zin = true;
   float u[2], v[2];
 #pragma unroll
-  for (int e = 0; e < 2; ++e) {
-    const float fq = e ? 7.0f : 0.0f;
-    const float z = __builtin_fmaf(fq, ax[2], cl[2]);
+  for (int e = 0; e < 2; ++e) {  // voxel 0 of the column is cl itself
+    const float z = e ? __builtin_fmaf(7.0f, ax[2], cl[2]) : cl[2];
     const float iz = recip_rn(z);
-    u[e] = __builtin_fmaf(fq, ax[0], cl[0]) * iz;
-    v[e] = __builtin_fmaf(fq, ax[1], cl[1]) * iz;
-    in = in & (z >= 0x1p-60f) & (z <= 0x1p60f) & (u[e] >= 1.0f) & (u[e] <= (float)(a.W - 2)) & (v[e] >= 1.0f) &
-         (v[e] <= (float)(a.H - 2));
+    u[e] = (e ? __builtin_fmaf(7.0f, ax[0], cl[0]) : cl[0]) * iz;
+    v[e] = (e ? __builtin_fmaf(7.0f, ax[1], cl[1]) : cl[1]) * iz;
+    zin = zin & (z >= 0x1p-60f) & (z <= 0x1p60f);
   }
-  // Computed whatever the placement (no value of the window is used unless it is
-  // kPlaceWindow), so that the window stays in scalar registers. Inside the image u - 1/2 >=
-  // 1/2 and u + 1/2 <= W - 3/2 are exact; adding M rounds them to integers, and the bits of
-  // the sums are M's plus the bounds.
-  const uint32_t lu = corner_min(__float_as_uint((fminf(u[0], u[1]) - 0.5f) + kPixMagic));
-  const uint32_t hu = corner_max(__float_as_uint((fmaxf(u[0], u[1]) + 0.5f) + kPixMagic));
-  const uint32_t lv = corner_min(__float_as_uint((fminf(v[0], v[1]) - 0.5f) + kPixMagic));
-  const uint32_t hv = corner_max(__float_as_uint((fmaxf(v[0], v[1]) + 0.5f) + kPixMagic));
-  const uint32_t cols = hu - lu + 1u, rows = hv - lv + 1u;
+  // computed whatever the placement, so that the window stays in scalar registers
+  const int32_t lu = corner_min((int32_t)__float_as_uint((fminf(u[0], u[1]) - 0.5f) + kPixMagic));
+  const int32_t hu = corner_max((int32_t)__float_as_uint((fmaxf(u[0], u[1]) + 0.5f) + kPixMagic));
+  const int32_t lv = corner_min((int32_t)__float_as_uint((fminf(v[0], v[1]) - 0.5f) + kPixMagic));
+  const int32_t hv = corner_max((int32_t)__float_as_uint((fmaxf(v[0], v[1]) + 0.5f) + kPixMagic));
+  const uint32_t cols = (uint32_t)(hu - lu) + 1u, rows = (uint32_t)(hv - lv) + 1u;
   // pitch: the power of two >= cols (2 for one column), at most 64
   const uint32_t shift = 32u - (uint32_t)__builtin_clz(min(cols - 1u, 63u) | 1u);
-  w.first = 4u * ((lv - kPixMagicBits) * (uint32_t)a.W + (lu - kPixMagicBits));
+  w.first = 4u * (((uint32_t)lv - kPixMagicBits) * (uint32_t)a.W + ((uint32_t)lu - kPixMagicBits));
   w.shift = shift;
   w.rows = rows;
   w.cols = cols;
-  w.bias = (lv << (shift + 2u)) + (lu << 2);
-  const bool inside = __builtin_amdgcn_ballot_w64(!in) == 0;
+  w.bias = ((uint32_t)lv << (shift + 2u)) + ((uint32_t)lu << 2);
+  const int32_t mb = (int32_t)kPixMagicBits;
+  const bool inside = __builtin_amdgcn_ballot_w64(!zin) == 0 && lu >= mb && hu <= mb + a.W - 1 && lv >= mb &&
+                      hv <= mb + a.H - 1;
   const bool fits = cols <= 64u && rows <= (kWin >> shift);
   return inside ? (fits ? kPlaceWindow : kPlaceInside) : kPlaceEdge;
 }
@@ -447,8 +442,9 @@ __device__ __forceinline__ void stage_window(const FrameArgs& a, const float* dm
   const int left = col < w.cols && row < w.rows ? (int)(w.rows - row) : 0;  // rows the lane loads, at most
   const uint32_t n_it = (w.rows + step - 1u) >> (6u - w.shift);             // loads of the widest lane
   for (uint32_t t = 0; t < n_it; ++t) {
+    const char* rows_t = src + t * step * w4;  // uniform
     if ((int)(t * step) < left)
-      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + t * step * w4 + off),
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rows_t + off),
                                        (__attribute__((address_space(3))) void*)(size_t)(win + 256u * t), 4, 0, 0);
   }
 }
@@ -485,9 +481,10 @@ __device__ __forceinline__ void frame_update(const FrameArgs& a, const float ax[
     uint32_t loff[4];  // kWindow: byte offset of the pixel in the LDS window
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float fq = (float)(h + r);
-      const float X = __builtin_fmaf(fq, ax[0], cl[0]), Y = __builtin_fmaf(fq, ax[1], cl[1]);
-      zc[r] = __builtin_fmaf(fq, ax[2], cl[2]);
+      const float fq = (float)(h + r);  // voxel 0 is the column itself (fma(0, a, c) = c but for -0)
+      const float X = h + r ? __builtin_fmaf(fq, ax[0], cl[0]) : cl[0];
+      const float Y = h + r ? __builtin_fmaf(fq, ax[1], cl[1]) : cl[1];
+      zc[r] = h + r ? __builtin_fmaf(fq, ax[2], cl[2]) : cl[2];
       const float iz = recip_rn(zc[r]);
       if constexpr (kInside) {
         const uint32_t ub = __float_as_uint(__builtin_fmaf(X, iz, kPixMagic));
