@@ -1087,32 +1087,47 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
 // mc_count_exact).
 __device__ __forceinline__ uint32_t mc_classify(McShared& S, float tiny, uint32_t& near) {
   const int lane = mc_lane(), cy = lane >> 3, cz = lane & 7;
-  float f0[4];
+  // One x-plane of the column's cubes: its 4 corners, slot k = (dy << 1) | dz, tested once
+  // and shared by the cubes on either side. lo: the plane's negative corners as case bits of
+  // the cube whose low-x face it is (reference corners 0, 4, 1, 5 for k = 0..3); hi: as case
+  // bits of the cube whose high-x face it is (corners 3, 7, 2, 6); obs: all 4 observed; sml:
+  // one below tiny.
+  auto plane = [&](int x, uint32_t& lo, uint32_t& hi, bool& obs, bool& sml) {
+    constexpr uint32_t kLo[4] = {1u, 16u, 2u, 32u}, kHi[4] = {8u, 128u, 4u, 64u};
+    lo = 0u;
+    hi = 0u;
+    obs = true;
+    sml = false;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) f0[k] = S.val[(cy + (k >> 1)) * 9 + cz + (k & 1)];
+    for (int k = 0; k < 4; ++k) {
+      const float f = S.val[x * 81 + (cy + (k >> 1)) * 9 + cz + (k & 1)];
+      const bool neg = f < 0.0f;
+      lo |= neg ? kLo[k] : 0u;
+      hi |= neg ? kHi[k] : 0u;
+      obs = obs & !__builtin_isnan(f);
+      sml = sml | (__builtin_fabsf(f) < tiny);
+    }
+  };
+  uint32_t lo0, hi0;
+  bool obs0, sml0;
+  plane(0, lo0, hi0, obs0, sml0);
   uint32_t ntri = 0;
   near = 0u;
-#pragma unroll 4  // fully unrolled, the 8 cubes' values and cases need 94 VGPRs (occupancy 5)
+#pragma unroll 4
   for (int cx = 0; cx < 8; ++cx) {
-    float f1[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) f1[k] = S.val[(cx + 1) * 81 + (cy + (k >> 1)) * 9 + cz + (k & 1)];
-    // corner c -> (dx, dy, dz); face slot k = (dy << 1) | dz
-    const float val[8] = {f0[0], f0[2], f1[2], f1[0], f0[1], f0[3], f1[3], f1[1]};
-    bool obs = true, small = false;
-    int ci = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      obs = obs && !__builtin_isnan(val[c]);
-      small = small || __builtin_fabsf(val[c]) < tiny;
-      if (val[c] < 0.0f) ci |= 1 << c;
-    }
-    if (!obs || ci == 255) ci = 0;
-    if (ci != 0 && small) near |= 1u << cx;
+    uint32_t lo1, hi1;
+    bool obs1, sml1;
+    plane(cx + 1, lo1, hi1, obs1, sml1);
+    // bit c set iff corner c < 0 (reference corner order); triangles iff all 8 observed and
+    // the case is not 0 or 255
+    uint32_t ci = lo0 | hi1;
+    if (!(obs0 & obs1) || ci == 255u) ci = 0u;
+    if (ci != 0u && (sml0 | sml1)) near |= 1u << cx;
     S.cases[cx * 64 + lane] = (uint8_t)ci;
     ntri += S.cnt[ci];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) f0[k] = f1[k];
+    lo0 = lo1;
+    obs0 = obs1;
+    sml0 = sml1;
   }
   uint32_t x = ntri;
 #pragma unroll
