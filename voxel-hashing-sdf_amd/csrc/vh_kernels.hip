@@ -1010,8 +1010,11 @@ __device__ __forceinline__ void mc_load_counts(McShared& S) {
 // corner, so no lane divides an index. A corner is observed iff its block exists and its
 // weight is > 0; unobserved corners hold NaN. kColor: the colour pool words of the same
 // voxels go to col (9^3, same order) with the same accesses.
+// Returns true iff the staged values (observed ones) include both a negative value and a
+// non-negative one: otherwise every cube's case is 0 or 255 (or it is unobserved), and the
+// block has no triangles.
 template <int kStorage, bool kColor>
-__device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& S, uint32_t* col, int& gx0, int& gy0,
+__device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, McShared& S, uint32_t* col, int& gx0, int& gy0,
                                          int& gz0) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
@@ -1067,16 +1070,23 @@ __device__ __forceinline__ void mc_stage(const McArgs& a, uint32_t b, McShared& 
     col[hi * 81 + lo * 9 + 8] = cfz;
     if (lane < 25) col[edst] = ce;
   }
+  bool neg = false, nonneg = false;  // NaN (unobserved) is neither
+  auto put = [&](int i, float v) {
+    S.val[i] = v;
+    neg = neg | (v < 0.0f);
+    nonneg = nonneg | (v >= 0.0f);
+  };
 #pragma unroll
-  for (int k = 0; k < 8; ++k) S.val[k * 81 + hi * 9 + lo] = value(w[k]);
-  S.val[8 * 81 + hi * 9 + lo] = nfx != kNoBlock ? value(wfx) : nan;
-  S.val[hi * 81 + 8 * 9 + lo] = nfy != kNoBlock ? value(wfy) : nan;
-  S.val[hi * 81 + lo * 9 + 8] = nfz != kNoBlock ? value(wfz) : nan;
-  if (lane < 25) S.val[edst] = enb != kNoBlock ? value(we) : nan;
+  for (int k = 0; k < 8; ++k) put(k * 81 + hi * 9 + lo, value(w[k]));
+  put(8 * 81 + hi * 9 + lo, nfx != kNoBlock ? value(wfx) : nan);
+  put(hi * 81 + 8 * 9 + lo, nfy != kNoBlock ? value(wfy) : nan);
+  put(hi * 81 + lo * 9 + 8, nfz != kNoBlock ? value(wfz) : nan);
+  if (lane < 25) put(edst, enb != kNoBlock ? value(we) : nan);
   wave_sync();
   gx0 = bx * kB;
   gy0 = by * kB;
   gz0 = bz * kB;
+  return __builtin_amdgcn_ballot_w64(neg) != 0 && __builtin_amdgcn_ballot_w64(nonneg) != 0;
 }
 
 // Lane l classifies the 8 cubes of its (y, z) column (x = 0..7), sharing each x-face's
@@ -1338,7 +1348,10 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a) {
   if (b >= a.n_blocks) return;
   mc_load_counts(S);
   int gx0, gy0, gz0;
-  mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
+  if (!mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0)) {  // no sign change: no triangles
+    if (lane == 0) a.counts[b] = 0u;
+    return;
+  }
   uint32_t near;
   mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
   uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
@@ -1504,7 +1517,10 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_mark(McArgs a) {
   {
     if (lane < kEdgeWords) s_bits[lane] = 0u;
     int gx0, gy0, gz0;
-    mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0);
+    if (!mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0)) {  // no sign change: no faces, no edges
+      if (lane == 0) a.counts[b] = 0u;
+      return;
+    }
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);
     const int cy = lane >> 3, cz = lane & 7;
