@@ -1000,8 +1000,22 @@ struct McShared {
   uint16_t items[kItemCap];  // (cube << 3) | t of the current round
 };
 
-__device__ __forceinline__ void mc_load_counts(McShared& S) {
-  for (int c = mc_lane(); c < 256; c += 64) S.cnt[c] = (uint8_t)(kTriTable[c] >> 60);
+// The table's triangle counts into S.cnt in two steps, so that the loads can be issued
+// first and the LDS writes made after the block's own loads are out (one lane: 4 cases).
+struct McCounts {
+  uint64_t t[4];
+};
+__device__ __forceinline__ McCounts mc_fetch_counts() {
+  const int lane = mc_lane();
+  McCounts c;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) c.t[j] = kTriTable[lane + 64 * j];
+  return c;
+}
+__device__ __forceinline__ void mc_put_counts(McShared& S, const McCounts& c) {
+  const int lane = mc_lane();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) S.cnt[lane + 64 * j] = (uint8_t)(c.t[j] >> 60);
 }
 
 // One wave per pool block. The block's 8^3 voxels and the +x/+y/+z faces, edges and corner
@@ -1021,9 +1035,10 @@ __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, McShared& 
   const int lane = mc_lane();
   int bx, by, bz;
   unpack_key(a.block_keys[b], bx, by, bz);
-  // the neighbourhood (k_mc_neighbours) in lanes 0..7, read by the wave from its registers
-  const uint32_t nbv = lane < 8 ? a.nbrs[(size_t)b * 8 + lane] : kNoBlock;
-  if (lane < 8) S.nb[lane] = nbv;
+  // the neighbourhood (k_mc_neighbours) in lanes 0..7, read by the wave from its registers;
+  // loaded first, so that waiting for it does not wait for the block's own voxels
+  const uint32_t nbr = a.nbrs[(size_t)b * 8 + (lane & 7)];
+  const uint32_t nbv = lane < 8 ? nbr : kNoBlock;
   auto nb_of = [&](int o) { return (uint32_t)__builtin_amdgcn_readlane((int)nbv, o); };
   const Word* vox = reinterpret_cast<const Word*>(a.voxels);
   const float nan = __builtin_nanf("");
@@ -1038,11 +1053,15 @@ __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, McShared& 
   Word w[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) w[k] = own[64 * k];
-  // faces x = 8 (voxel (0, hi, lo) of +x), y = 8 ((hi, 0, lo) of +y), z = 8 ((hi, lo, 0) of +z)
+  if (lane < 8) S.nb[lane] = nbv;
+  // faces x = 8 (voxel (0, hi, lo) of +x), y = 8 ((hi, 0, lo) of +y), z = 8 ((hi, lo, 0) of +z);
+  // an absent neighbour's loads read this block instead (their values are not used), so that
+  // every load is issued at once, with no wait between them
   const uint32_t nfx = nb_of(4), nfy = nb_of(2), nfz = nb_of(1);
-  const Word wfx = nfx != kNoBlock ? vox[(size_t)nfx * kB3 + lane] : IO::zero();
-  const Word wfy = nfy != kNoBlock ? vox[(size_t)nfy * kB3 + hi * 64 + lo] : IO::zero();
-  const Word wfz = nfz != kNoBlock ? vox[(size_t)nfz * kB3 + hi * 64 + lo * 8] : IO::zero();
+  const auto safe = [&](uint32_t n) { return (size_t)(n != kNoBlock ? n : b) * kB3; };
+  const Word wfx = vox[safe(nfx) + lane];
+  const Word wfy = vox[safe(nfy) + hi * 64 + lo];
+  const Word wfz = vox[safe(nfz) + hi * 64 + lo * 8];
   // edges (8, 8, z) of +x+y, (8, y, 8) of +x+z, (x, 8, 8) of +y+z (8 lanes each), corner
   // (8, 8, 8) of +x+y+z (lane 24)
   int edst = 0;
@@ -1053,16 +1072,16 @@ __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, McShared& 
   else if (lane < 16) { enb = ne5; esrc = (lane - 8) * 8; edst = 8 * 81 + (lane - 8) * 9 + 8; }
   else if (lane < 24) { enb = ne3; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; }
   else if (lane == 24) { enb = ne7; esrc = 0; edst = kHalo3 - 1; }
-  const Word we = enb != kNoBlock ? vox[(size_t)enb * kB3 + esrc] : IO::zero();
+  const Word we = vox[safe(enb) + esrc];
   if constexpr (kColor) {
     const uint32_t* cp = a.colors;
     uint32_t c[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) c[k] = cp[(size_t)b * kB3 + lane + 64 * k];
-    const uint32_t cfx = nfx != kNoBlock ? cp[(size_t)nfx * kB3 + lane] : 0u;
-    const uint32_t cfy = nfy != kNoBlock ? cp[(size_t)nfy * kB3 + hi * 64 + lo] : 0u;
-    const uint32_t cfz = nfz != kNoBlock ? cp[(size_t)nfz * kB3 + hi * 64 + lo * 8] : 0u;
-    const uint32_t ce = enb != kNoBlock ? cp[(size_t)enb * kB3 + esrc] : 0u;
+    const uint32_t rfx = cp[safe(nfx) + lane], rfy = cp[safe(nfy) + hi * 64 + lo], rfz = cp[safe(nfz) + hi * 64 + lo * 8],
+                   re = cp[safe(enb) + esrc];
+    const uint32_t cfx = nfx != kNoBlock ? rfx : 0u, cfy = nfy != kNoBlock ? rfy : 0u, cfz = nfz != kNoBlock ? rfz : 0u,
+                   ce = enb != kNoBlock ? re : 0u;
 #pragma unroll
     for (int k = 0; k < 8; ++k) col[k * 81 + hi * 9 + lo] = c[k];
     col[8 * 81 + hi * 9 + lo] = cfx;
@@ -1346,12 +1365,14 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a) {
   const int lane = mc_lane();
   const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
   if (b >= a.n_blocks) return;
-  mc_load_counts(S);
+  const McCounts counts = mc_fetch_counts();
   int gx0, gy0, gz0;
   if (!mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0)) {  // no sign change: no triangles
     if (lane == 0) a.counts[b] = 0u;
     return;
   }
+  mc_put_counts(S, counts);
+  wave_sync();
   uint32_t near;
   mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
   uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
@@ -1450,9 +1471,11 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_write(McArgs a) {
   {
     uint32_t pos = block_offset(a.counts, a.chunk_sums, b);
     if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos) return;  // no triangles
-    mc_load_counts(S);
+    const McCounts counts = mc_fetch_counts();
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
+    mc_put_counts(S, counts);
+    wave_sync();
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);
     mc_items<kColor, true>(S, s_col, total, gx0, gy0, gz0,
@@ -1513,7 +1536,7 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_mark(McArgs a) {
   const int lane = mc_lane();
   const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
   if (b >= a.n_blocks) return;
-  mc_load_counts(S);
+  const McCounts counts = mc_fetch_counts();
   {
     if (lane < kEdgeWords) s_bits[lane] = 0u;
     int gx0, gy0, gz0;
@@ -1521,6 +1544,8 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_mark(McArgs a) {
       if (lane == 0) a.counts[b] = 0u;
       return;
     }
+    mc_put_counts(S, counts);
+    wave_sync();
     uint32_t near;
     const uint32_t total = mc_classify(S, 0.0f, near);
     const int cy = lane >> 3, cz = lane & 7;
@@ -1575,9 +1600,11 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_write(McArgs a) {
     if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos &&
         block_offset(a.vcounts, a.vchunks, b + 1, a.n_blocks, a.vtotal) == block_offset(a.vcounts, a.vchunks, b))
       return;  // no faces and no vertices of its own
-    mc_load_counts(S);
+    const McCounts counts = mc_fetch_counts();
     int gx0, gy0, gz0;
     mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
+    mc_put_counts(S, counts);
+    wave_sync();
     if (lane < 8) {
       const uint32_t nb = S.nb[lane];
       s_vbase[lane] = nb != kNoBlock ? a.vchunks[nb / kScanChunk] + a.vcounts[nb] : 0u;
