@@ -1827,9 +1827,11 @@ __device__ __forceinline__ bool sample_sdf(const QueryArgs& q, double px, double
   const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
   const float fx = (float)(gxd - flx), fy = (float)(gyd - fly), fz = (float)(gzd - flz);
   const Word* voxw = reinterpret_cast<const Word*>(q.voxels);
+  // the corners' blocks first (one lookup per distinct block), then all 8 voxel loads at
+  // once, then the weight tests: a test after each load would make every load wait for the
+  // one before it
   uint64_t last_key = kEmptyKey;
   uint32_t blk = kNoBlock;
-  float s[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int cx = ix + (k >> 2), cy = iy + ((k >> 1) & 1), cz = iz + (k & 1);
@@ -1837,15 +1839,22 @@ __device__ __forceinline__ bool sample_sdf(const QueryArgs& q, double px, double
     if (key != last_key) {
       last_key = key;
       blk = find_block(q.hash, key);
+      if (blk >= q.pool_cap) return false;
     }
-    if (blk >= q.pool_cap) return false;
-    const uint32_t id = blk * kB3 + (uint32_t)((((cx & 7) * 8) + (cy & 7)) * 8 + (cz & 7));
-    float sv, wv;
-    IO::unpack(voxw[id], sv, wv);
-    if (!(wv > 0.0f)) return false;
-    s[k] = sv;
-    vid[k] = id;
+    vid[k] = blk * kB3 + (uint32_t)((((cx & 7) * 8) + (cy & 7)) * 8 + (cz & 7));
   }
+  Word wd[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) wd[k] = voxw[vid[k]];
+  float s[8];
+  bool seen = true;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float wv;
+    IO::unpack(wd[k], s[k], wv);
+    seen = seen & (wv > 0.0f);
+  }
+  if (!seen) return false;
   trilinear_weights(fx, fy, fz, w);
   float acc = 0.0f;
 #pragma unroll
@@ -2211,8 +2220,17 @@ __device__ __forceinline__ void gn_solve_wg(const double* partials, int grid, co
   if (t < kStateWords) s_st[t] = reinterpret_cast<const double*>(st_in)[t];
   if (t < kWays * kGnAcc) {
     const int v = t % kGnAcc, j = t / kGnAcc;
+    // in order g = j, j + 8, ...; the loads of 8 terms in flight together, then their sum
     double acc = 0.0;
-    for (int g = j; g < grid; g += kWays) acc += partials[kGnAcc * (size_t)g + v];
+    int g = j;
+    for (; g + 7 * kWays < grid; g += 8 * kWays) {
+      double p[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) p[r] = partials[kGnAcc * (size_t)(g + r * kWays) + v];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc += p[r];
+    }
+    for (; g < grid; g += kWays) acc += partials[kGnAcc * (size_t)g + v];
     s_sum[j][v] = acc;
   }
   __syncthreads();
