@@ -196,7 +196,9 @@ struct AllocLds {
   float tmax[4][kRows];
 };
 
-template <bool kU16, int kRows>
+// kColor: the volume keeps colour (a.rgba is set), known at compile time so that no branch
+// separates the tile's loads.
+template <bool kU16, int kRows, bool kColor>
 __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, uint32_t jb, uint32_t parity,
                            uint32_t fs, AllocLds<kRows>& L) {
   unsigned long long* s_set = L.set;
@@ -215,21 +217,33 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
   const size_t elem = kU16 ? 2 : 4;
   const void* base = reinterpret_cast<const char*>(a.depth) + (size_t)fs * a.W * a.H * elem;
 
-  // the lane's depths of all bands, fetched together
+  // the lane's depths (and colours) of all bands: every load is issued before the first use
+  // (a pixel outside the image loads a pixel inside it, whose value is not used), so the loads
+  // are in flight together
   float dr[kRows];
+  uint32_t px3[kRows][3];
+  const int uc = min(u, a.W - 1);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int vc = min(v0 + 16 * r, a.H - 1);
+    dr[r] = depth_at<kU16>(base, a.depth_scale, vc * a.W + uc);
+    if constexpr (kColor) {
+      const uint8_t* px = a.rgb + ((size_t)fs * a.W * a.H + (size_t)vc * a.W + uc) * 3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) px3[r][c] = px[c];
+    }
+  }
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const int v = v0 + 16 * r;
-    dr[r] = 0.0f;
     if (u < a.W && v < a.H) {
       const size_t staged = ((size_t)parity * kMaxBatch + jb) * a.W * a.H + (size_t)v * a.W + u;
-      if (a.rgba) {  // the pixel's colour, packed for the integration's one-word gathers
-        const uint8_t* px = a.rgb + ((size_t)fs * a.W * a.H + (size_t)v * a.W + u) * 3;
-        a.rgba[staged] = (uint32_t)px[0] | (uint32_t)px[1] << 8 | (uint32_t)px[2] << 16;
-      }
-      dr[r] = depth_at<kU16>(base, a.depth_scale, v * a.W + u);
+      // the pixel's colour, packed for the integration's one-word gathers
+      if constexpr (kColor) a.rgba[staged] = px3[r][0] | px3[r][1] << 8 | px3[r][2] << 16;
       // the pre-scaled depth the integration of this frame gathers
       a.dm[staged] = scaled_depth(dr[r], a.max_depth, a.inv_mu);
+    } else {
+      dr[r] = 0.0f;
     }
   }
 
@@ -299,12 +313,12 @@ __device__ void alloc_tile(const FrameArgs& a, uint32_t tile, uint32_t frame, ui
 }
 
 // Allocation of the batch about to be integrated: `tiles` workgroups per frame.
-template <bool kU16, int kRows>
+template <bool kU16, int kRows, bool kColor>
 __global__ __launch_bounds__(256) void k_alloc(FrameArgs a, uint32_t tiles) {
   __shared__ AllocLds<kRows> L;
   const uint32_t jb = blockIdx.x / tiles;
   const FrameCtl* ctl = a.ctl;
-  alloc_tile<kU16, kRows>(a, blockIdx.x - jb * tiles, ctl->frame_id + jb, jb, ctl->batch_id & 1u,
+  alloc_tile<kU16, kRows, kColor>(a, blockIdx.x - jb * tiles, ctl->frame_id + jb, jb, ctl->batch_id & 1u,
                           input_slot(ctl->cursor[a.cursor_id], jb, a.n_seq), L);
 }
 
@@ -841,7 +855,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
   const uint32_t f0 = uniform(ctl->frame_id), batch = uniform(ctl->batch_id), cur = uniform(ctl->cursor[a.cursor_id]);
   if (alloc) {
     const uint32_t jb = idx / tiles;
-    alloc_tile<kU16, kAllocRows>(a, idx - jb * tiles, f0 + k_int + jb, jb, (batch + 1u) & 1u,
+    alloc_tile<kU16, kAllocRows, kColor>(a, idx - jb * tiles, f0 + k_int + jb, jb, (batch + 1u) & 1u,
                                  input_slot(cur, k_int + jb, a.n_seq), L.alloc);
   } else {  // uniform branch: the whole workgroup integrates
     integrate_group<kStorage, kColor>(a, idx, int_groups, f0, k_int, batch & 1u, cur, L.integ);
@@ -2436,14 +2450,28 @@ __global__ __launch_bounds__(64) void k_track_end(TrackArgs t) {
 }  // namespace
 
 // ============================================================================ launchers
+template <class F>
+static void with_input(bool u16, bool color, F f) {
+  using T = std::true_type;
+  using N = std::false_type;
+  if (u16) {
+    if (color) f(T(), T());
+    else f(T(), N());
+  } else {
+    if (color) f(N(), T());
+    else f(N(), N());
+  }
+}
+
 hipError_t launch_alloc(const FrameArgs& a, int depth_u16, int k, hipStream_t st) {
   const int rows = k > 1 ? kAllocRows : 1;
   const uint32_t tiles = alloc_tiles(a.W, a.H, rows);
   const dim3 g(tiles * k), b(256);
-  if (depth_u16 && rows > 1) hipLaunchKernelGGL((k_alloc<true, kAllocRows>), g, b, 0, st, a, tiles);
-  else if (depth_u16) hipLaunchKernelGGL((k_alloc<true, 1>), g, b, 0, st, a, tiles);
-  else if (rows > 1) hipLaunchKernelGGL((k_alloc<false, kAllocRows>), g, b, 0, st, a, tiles);
-  else hipLaunchKernelGGL((k_alloc<false, 1>), g, b, 0, st, a, tiles);
+  with_input(depth_u16 != 0, a.rgba != nullptr, [&](auto U, auto C) {
+    constexpr bool u16 = decltype(U)::value, col = decltype(C)::value;
+    if (rows > 1) hipLaunchKernelGGL((k_alloc<u16, kAllocRows, col>), g, b, 0, st, a, tiles);
+    else hipLaunchKernelGGL((k_alloc<u16, 1, col>), g, b, 0, st, a, tiles);
+  });
   return hipGetLastError();
 }
 
@@ -2458,18 +2486,6 @@ static void with_storage(int storage, F f) {
 }
 
 // Calls f(std::integral_constant<bool, u16>(), std::integral_constant<bool, colour>()).
-template <class F>
-static void with_input(bool u16, bool color, F f) {
-  using T = std::true_type;
-  using N = std::false_type;
-  if (u16) {
-    if (color) f(T(), T());
-    else f(T(), N());
-  } else {
-    if (color) f(N(), T());
-    else f(N(), N());
-  }
-}
 
 hipError_t launch_integrate(const FrameArgs& a, int storage, int grid, int k, hipStream_t st) {
   with_storage(storage, [&](auto S) {
