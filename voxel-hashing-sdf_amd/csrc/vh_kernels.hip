@@ -1041,17 +1041,21 @@ __device__ __forceinline__ void mc_put_counts(McShared& S, const McCounts& c) {
 // Returns true iff the staged values (observed ones) include both a negative value and a
 // non-negative one: otherwise every cube's case is 0 or 255 (or it is unobserved), and the
 // block has no triangles.
+// The neighbourhood word of lane l of block b (k_mc_neighbours): lanes 0..7 hold the 8 entries.
+__device__ __forceinline__ uint32_t mc_nbr(const McArgs& a, uint32_t b) {
+  return a.nbrs[(size_t)b * 8 + (mc_lane() & 7)];
+}
+// nbr: mc_nbr(a, b), which the caller may have loaded early.
 template <int kStorage, bool kColor>
-__device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, McShared& S, uint32_t* col, int& gx0, int& gy0,
-                                         int& gz0) {
+__device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, uint32_t nbr, McShared& S, uint32_t* col,
+                                         int& gx0, int& gy0, int& gz0) {
   using IO = VoxelIO<kStorage>;
   using Word = typename IO::Word;
   const int lane = mc_lane();
   int bx, by, bz;
   unpack_key(a.block_keys[b], bx, by, bz);
-  // the neighbourhood (k_mc_neighbours) in lanes 0..7, read by the wave from its registers;
-  // loaded first, so that waiting for it does not wait for the block's own voxels
-  const uint32_t nbr = a.nbrs[(size_t)b * 8 + (lane & 7)];
+  // the neighbourhood in lanes 0..7, read by the wave from its registers; loaded before the
+  // block's own voxels, so that waiting for it does not wait for them
   const uint32_t nbv = lane < 8 ? nbr : kNoBlock;
   auto nb_of = [&](int o) { return (uint32_t)__builtin_amdgcn_readlane((int)nbv, o); };
   const Word* vox = reinterpret_cast<const Word*>(a.voxels);
@@ -1381,7 +1385,7 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a) {
   if (b >= a.n_blocks) return;
   const McCounts counts = mc_fetch_counts();
   int gx0, gy0, gz0;
-  if (!mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0)) {  // no sign change: no triangles
+  if (!mc_stage<kStorage, false>(a, b, mc_nbr(a, b), S, nullptr, gx0, gy0, gz0)) {  // no sign change: no triangles
     if (lane == 0) a.counts[b] = 0u;
     return;
   }
@@ -1483,11 +1487,12 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_write(McArgs a) {
   const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
   if (b >= a.n_blocks) return;
   {
+    const uint32_t nbr = mc_nbr(a, b);  // with the offsets: the staging then needs one round trip
+    const McCounts counts = mc_fetch_counts();
     uint32_t pos = block_offset(a.counts, a.chunk_sums, b);
     if (block_offset(a.counts, a.chunk_sums, b + 1, a.n_blocks, a.total) == pos) return;  // no triangles
-    const McCounts counts = mc_fetch_counts();
     int gx0, gy0, gz0;
-    mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
+    mc_stage<kStorage, kColor>(a, b, nbr, S, s_col, gx0, gy0, gz0);
     mc_put_counts(S, counts);
     wave_sync();
     uint32_t near;
@@ -1554,7 +1559,7 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_mark(McArgs a) {
   {
     if (lane < kEdgeWords) s_bits[lane] = 0u;
     int gx0, gy0, gz0;
-    if (!mc_stage<kStorage, false>(a, b, S, nullptr, gx0, gy0, gz0)) {  // no sign change: no faces, no edges
+    if (!mc_stage<kStorage, false>(a, b, mc_nbr(a, b), S, nullptr, gx0, gy0, gz0)) {  // no sign change: no faces, no edges
       if (lane == 0) a.counts[b] = 0u;
       return;
     }
@@ -1616,7 +1621,7 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_write(McArgs a) {
       return;  // no faces and no vertices of its own
     const McCounts counts = mc_fetch_counts();
     int gx0, gy0, gz0;
-    mc_stage<kStorage, kColor>(a, b, S, s_col, gx0, gy0, gz0);
+    mc_stage<kStorage, kColor>(a, b, mc_nbr(a, b), S, s_col, gx0, gy0, gz0);
     mc_put_counts(S, counts);
     wave_sync();
     if (lane < 8) {
