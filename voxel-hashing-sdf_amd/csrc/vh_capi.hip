@@ -58,9 +58,12 @@ constexpr int kIntegrateGrid = 5120;
 // more of the GPU. room_medium_5mm, integration-only frames/s by worker count (3,072 /
 // 4,096 / 8,192): FP32 63.8k / 62.6k / 59.6k, FP16 63.9k / 63.8k / 62.2k, F16R 51.1k /
 // 52.2k / 51.8k, FP32 + colour 40.6k / 39.6k / 38.9k, F16R + colour 35.4k / 36.5k / 36.0k;
-// FP32 with 2,048 workers 56.4k. F16R, whose voxel update costs more, takes 4,096.
+// FP32 with 2,048 workers 56.4k. Average fused launch after the depth-window and load-order
+// work (2,048 / 3,072 / 4,096 / 5,120 workers): FP32 115.1 / 100.8 / 101.2 / 102.6 us, FP16
+// 124.8 / 106.9 / 105.1 / 106.2, F16R 142.4 / 126.1 / 121.7 / 123.3, FP32 + colour 195.9 /
+// 167.6 / 164.1 / 171.5. Plain FP32 takes 3,072; every costlier voxel update takes 4,096.
 constexpr int kFusedGrid = 3072;
-constexpr int kFusedGridF16R = 4096;
+constexpr int kFusedGridHeavy = 4096;
 constexpr int kGnGrid = 512;  // most workgroups of a solver or backward reduction
 
 // Solver grid for n points: one point per lane, up to kGnGrid workgroups (a partial per
@@ -85,10 +88,10 @@ int integrate_grid() {
   static const int g = grid_setting("VH_INTEGRATE_GRID", kIntegrateGrid);
   return g;
 }
-int fused_grid(int storage) {
+int fused_grid(int storage, bool color) {
   static const int g = grid_setting("VH_FUSED_GRID", kFusedGrid);
-  static const int g16r = grid_setting("VH_FUSED_GRID", kFusedGridF16R);
-  return storage == VH_STORAGE_F16R ? g16r : g;
+  static const int heavy = grid_setting("VH_FUSED_GRID", kFusedGridHeavy);
+  return storage == VH_STORAGE_F32 && !color ? g : heavy;
 }
 
 }  // namespace
@@ -365,7 +368,7 @@ int enqueue_frame_kernels(vh_volume* v, int which, float staged_scale, int ki, i
   if (fa < 2 || fa >= 6) a.depth_scale = u16 ? staged_scale : 1.0f;
   const bool carve = v->cfg.carve != 0;
   if (which == 2) {
-    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, fused_grid(v->cfg.storage), ki, ka, v->stream));
+    VH_HIP(launch_integrate_alloc(a, v->cfg.storage, u16, fused_grid(v->cfg.storage, v->cfg.color != 0), ki, ka, v->stream));
     if (carve) VH_HIP(launch_carve(a, ka, v->stream));  // the batch just allocated
     return VH_OK;
   }

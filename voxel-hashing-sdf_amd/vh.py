@@ -618,6 +618,7 @@ class Scene:
         if rc:
             raise VhError(rc, f"vh_scene_create({preset}) failed")
         self._h = h
+        self._lib = lib  # held for __del__: module globals may already be cleared at exit
         info = SceneInfo()
         lib.vh_scene_info_get(h, C.byref(info))
         self.width, self.height, self.n_frames = info.width, info.height, info.n_frames
@@ -627,7 +628,7 @@ class Scene:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            _lib.vh_scene_destroy(self._h)
+            self._lib.vh_scene_destroy(self._h)
             self._h = None
 
     def pose(self, f: int) -> np.ndarray:
