@@ -993,7 +993,7 @@ __global__ __launch_bounds__(256) void k_carve(FrameArgs a, uint32_t k) {
 // ============================================================================ marching cubes
 constexpr int kHalo = 9;
 constexpr int kHalo3 = kHalo * kHalo * kHalo;
-constexpr int kItemCap = 1024;  // (cube, triangle) items listed per round
+constexpr int kItemCap = 512;  // (cube, triangle) items listed per round (a block averages ~80)
 // Extraction workgroups: kMcWaves independent waves, one block each, each with its own LDS
 // (McShared). A wave only exchanges data with itself through LDS, and executes its LDS
 // instructions in order, so wave_sync (a barrier to the compiler's reordering) suffices
@@ -1004,32 +1004,97 @@ __device__ __forceinline__ int mc_lane() { return (int)(threadIdx.x & 63u); }
 __device__ __forceinline__ int mc_wave() { return (int)(threadIdx.x >> 6); }
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
-// LDS of one extraction wave (about 6 KB).
+// Cases in corner-plane order: bit 4 * dx + 2 * dy + dz of a permuted case is the sign bit of
+// the cube corner at offset (dx, dy, dz), so that the low nibble holds the x = cx face and the
+// high nibble the x = cx + 1 face, each in slot order (dy << 1) | dz. kPermOfCase maps a case
+// (bit c: reference corner c) to its permuted form, four cases per word.
+struct McPerm {
+  uint32_t w[64];
+};
+constexpr McPerm mc_perm_table() {
+  McPerm t{};
+  for (int ci = 0; ci < 256; ++ci) {
+    uint32_t p = 0;
+    for (int c = 0; c < 8; ++c)
+      p |= (uint32_t)((ci >> c) & 1) << (4 * corner_dx(c) + 2 * corner_dy(c) + corner_dz(c));
+    t.w[ci >> 2] |= p << (8 * (ci & 3));
+  }
+  return t;
+}
+constexpr McPerm kPermHost = mc_perm_table();
+constexpr bool mc_perm_ok() {  // a bijection fixing 0 and 255, and corner 0 of the x = cx face at bit 0
+  bool seen[256] = {};
+  for (int ci = 0; ci < 256; ++ci) {
+    const uint32_t p = (kPermHost.w[ci >> 2] >> (8 * (ci & 3))) & 255u;
+    if (seen[p]) return false;
+    seen[p] = true;
+  }
+  return (kPermHost.w[0] & 255u) == 0u && (kPermHost.w[63] >> 24) == 255u && ((kPermHost.w[0] >> 8) & 255u) == 1u;
+}
+static_assert(mc_perm_ok(), "permuted case table is not a bijection");
+__device__ __constant__ const McPerm kPermOfCase = mc_perm_table();
+
+// LDS of one extraction wave (about 4.9 KB).
 struct McShared {
-  float val[kHalo3];         // 9^3 tile (x*81 + y*9 + z): tsdf, NaN where unobserved
-  uint32_t nb[8];            // pool blocks of the neighbourhood, index (dx << 2) | (dy << 1) | dz
-  uint8_t cnt[256];          // triangles of each case (table count)
-  uint8_t cases[512];        // case per cube (x*64 + y*8 + z); 0 = no triangles
+  float val[kHalo3];   // 9^3 tile (x*81 + y*9 + z): tsdf, NaN where unobserved
+  uint64_t col[81];    // column words (mc_col_bits) of the 9 x 9 (y, z) columns, index y*9 + z
+  uint32_t nb[8];      // pool blocks of the neighbourhood, index (dx << 2) | (dy << 1) | dz
+  uint16_t pc[256];    // by permuted case: case | triangles << 8 (case 255 stored as 0)
+  uint8_t cnt[256];    // triangles of each case (table count)
+  uint8_t cases[512];  // case per cube (x*64 + y*8 + z); 0 = no triangles
+};
+// Item lists of the write passes (k_mc_write, k_ix_write).
+struct McItems {
   uint32_t lane_off[65];     // exclusive prefix of table triangles per lane
   uint16_t items[kItemCap];  // (cube << 3) | t of the current round
 };
 
-// The table's triangle counts into S.cnt in two steps, so that the loads can be issued
-// first and the LDS writes made after the block's own loads are out (one lane: 4 cases).
+// The table's triangle counts into S.cnt and S.pc in two steps, so that the loads can be
+// issued first and the LDS writes made after the block's own loads are out (one lane: cases
+// 4l .. 4l + 3).
 struct McCounts {
   uint64_t t[4];
+  uint32_t p;
 };
 __device__ __forceinline__ McCounts mc_fetch_counts() {
   const int lane = mc_lane();
   McCounts c;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) c.t[j] = kTriTable[lane + 64 * j];
+  for (int j = 0; j < 4; ++j) c.t[j] = kTriTable[4 * lane + j];
+  c.p = kPermOfCase.w[lane];
   return c;
 }
 __device__ __forceinline__ void mc_put_counts(McShared& S, const McCounts& c) {
   const int lane = mc_lane();
+  uint32_t packed = 0u;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) S.cnt[lane + 64 * j] = (uint8_t)(c.t[j] >> 60);
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t ci = 4u * (uint32_t)lane + (uint32_t)j, n = (uint32_t)(c.t[j] >> 60);
+    packed |= n << (8 * j);
+    S.pc[(c.p >> (8 * j)) & 255u] = (uint16_t)((ci == 255u ? 0u : ci) | n << 8);
+  }
+  *reinterpret_cast<uint32_t*>(&S.cnt[4 * lane]) = packed;
+}
+
+// Column word of one (y, z) column of the tile, built from its 9 values (x = 0..8) by
+// mc_col_bits: bit 4x (x < 8) or bit 32 (x = 8) set iff the value is negative, bit 33 + x
+// iff it is unobserved (NaN), bit 42 + x iff |value| < tiny. The 4x spacing interleaves the
+// four columns of a cube's corners into one nibble per x-plane.
+__device__ __forceinline__ void mc_col_bits(float v, int x, float tiny, uint32_t& q, uint32_t& m) {
+  const uint32_t neg = v < 0.0f ? 1u : 0u;
+  if (x < 8) q |= neg << (4 * x);
+  else m |= neg;
+  m |= (__builtin_isnan(v) ? 2u : 0u) << x;
+  m |= (__builtin_fabsf(v) < tiny ? 1024u : 0u) << x;
+}
+__device__ __forceinline__ uint64_t mc_col_word(uint32_t q, uint32_t m) { return (uint64_t)q | (uint64_t)m << 32; }
+
+// Threshold of mc_count_exact for the block at lattice origin (gx0, gy0, gz0): above the
+// snap bound 1e-5 and above ulp(c) for every lattice coordinate c of the block and its
+// +1 layer (ulp(c) <= |c| * 2^-23).
+__device__ __forceinline__ float mc_tiny(int gx0, int gy0, int gz0) {
+  const int m = max(max(max(abs(gx0), abs(gx0 + 8)), max(abs(gy0), abs(gy0 + 8))), max(abs(gz0), abs(gz0 + 8)));
+  return 2e-5f + (float)(m + 1) * 0x1p-22f;
 }
 
 // One wave per pool block. The block's 8^3 voxels and the +x/+y/+z faces, edges and corner
@@ -1040,13 +1105,14 @@ __device__ __forceinline__ void mc_put_counts(McShared& S, const McCounts& c) {
 // voxels go to col (9^3, same order) with the same accesses.
 // Returns true iff the staged values (observed ones) include both a negative value and a
 // non-negative one: otherwise every cube's case is 0 or 255 (or it is unobserved), and the
-// block has no triangles.
+// block has no triangles. The column words of the 81 columns go to S.col, with kTiny the
+// |value| < mc_tiny bits (otherwise none).
 // The neighbourhood word of lane l of block b (k_mc_neighbours): lanes 0..7 hold the 8 entries.
 __device__ __forceinline__ uint32_t mc_nbr(const McArgs& a, uint32_t b) {
   return a.nbrs[(size_t)b * 8 + (mc_lane() & 7)];
 }
 // nbr: mc_nbr(a, b), which the caller may have loaded early.
-template <int kStorage, bool kColor>
+template <int kStorage, bool kColor, bool kTiny = false>
 __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, uint32_t nbr, McShared& S, uint32_t* col,
                                          int& gx0, int& gy0, int& gz0) {
   using IO = VoxelIO<kStorage>;
@@ -1082,14 +1148,14 @@ __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, uint32_t n
   const Word wfz = vox[safe(nfz) + hi * 64 + lo * 8];
   // edges (8, 8, z) of +x+y, (8, y, 8) of +x+z, (x, 8, 8) of +y+z (8 lanes each), corner
   // (8, 8, 8) of +x+y+z (lane 24)
-  int edst = 0;
+  int edst = 0, ecol = 0, ex = 8;  // tile index, column and x of the lane's edge value
   uint32_t enb = kNoBlock;
   int esrc = 0;
   const uint32_t ne6 = nb_of(6), ne5 = nb_of(5), ne3 = nb_of(3), ne7 = nb_of(7);
-  if (lane < 8) { enb = ne6; esrc = lane; edst = 8 * 81 + 8 * 9 + lane; }
-  else if (lane < 16) { enb = ne5; esrc = (lane - 8) * 8; edst = 8 * 81 + (lane - 8) * 9 + 8; }
-  else if (lane < 24) { enb = ne3; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; }
-  else if (lane == 24) { enb = ne7; esrc = 0; edst = kHalo3 - 1; }
+  if (lane < 8) { enb = ne6; esrc = lane; edst = 8 * 81 + 8 * 9 + lane; ecol = 72 + lane; }
+  else if (lane < 16) { enb = ne5; esrc = (lane - 8) * 8; edst = 8 * 81 + (lane - 8) * 9 + 8; ecol = (lane - 8) * 9 + 8; }
+  else if (lane < 24) { enb = ne3; esrc = (lane - 16) * 64; edst = (lane - 16) * 81 + 8 * 9 + 8; ecol = 80; ex = lane - 16; }
+  else if (lane == 24) { enb = ne7; esrc = 0; edst = kHalo3 - 1; ecol = 80; }
   const Word we = vox[safe(enb) + esrc];
   if constexpr (kColor) {
     const uint32_t* cp = a.colors;
@@ -1107,125 +1173,104 @@ __device__ __forceinline__ bool mc_stage(const McArgs& a, uint32_t b, uint32_t n
     col[hi * 81 + lo * 9 + 8] = cfz;
     if (lane < 25) col[edst] = ce;
   }
-  bool neg = false, nonneg = false;  // NaN (unobserved) is neither
+  // lanes holding a negative / a non-negative value (NaN, unobserved, is neither), gathered as
+  // ballots so that the OR is one scalar instruction per value
+  uint64_t neg = 0u, nonneg = 0u;
   auto put = [&](int i, float v) {
     S.val[i] = v;
-    neg = neg | (v < 0.0f);
-    nonneg = nonneg | (v >= 0.0f);
+    neg |= __builtin_amdgcn_ballot_w64(v < 0.0f);
+    nonneg |= __builtin_amdgcn_ballot_w64(v >= 0.0f);
   };
-#pragma unroll
-  for (int k = 0; k < 8; ++k) put(k * 81 + hi * 9 + lo, value(w[k]));
-  put(8 * 81 + hi * 9 + lo, nfx != kNoBlock ? value(wfx) : nan);
-  put(hi * 81 + 8 * 9 + lo, nfy != kNoBlock ? value(wfy) : nan);
-  put(hi * 81 + lo * 9 + 8, nfz != kNoBlock ? value(wfz) : nan);
-  if (lane < 25) put(edst, enb != kNoBlock ? value(we) : nan);
-  wave_sync();
   gx0 = bx * kB;
   gy0 = by * kB;
   gz0 = bz * kB;
-  return __builtin_amdgcn_ballot_w64(neg) != 0 && __builtin_amdgcn_ballot_w64(nonneg) != 0;
+  const float tiny = kTiny ? mc_tiny(gx0, gy0, gz0) : 0.0f;
+  // the border columns (y, 8), (8, z) and (8, 8) collect their values from other lanes
+  if (lane < 17) S.col[lane < 8 ? lane * 9 + 8 : 64 + lane] = 0u;
+  // own column (hi, lo): x = 0..7 from the block, x = 8 from the +x face
+  uint32_t cq = 0u, cm = 0u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float v = value(w[k]);
+    put(k * 81 + hi * 9 + lo, v);
+    mc_col_bits(v, k, tiny, cq, cm);
+  }
+  const float vfx = nfx != kNoBlock ? value(wfx) : nan, vfy = nfy != kNoBlock ? value(wfy) : nan,
+              vfz = nfz != kNoBlock ? value(wfz) : nan, ve = enb != kNoBlock ? value(we) : nan;
+  put(8 * 81 + hi * 9 + lo, vfx);
+  mc_col_bits(vfx, 8, tiny, cq, cm);
+  S.col[hi * 9 + lo] = mc_col_word(cq, cm);
+  put(hi * 81 + 8 * 9 + lo, vfy);
+  put(hi * 81 + lo * 9 + 8, vfz);
+  // the edge values: lanes 25.. hold NaN, which adds no bit, so the ballots are taken by the
+  // whole wave (a ballot inside the branch would leave neg and nonneg differing across lanes)
+  if (lane < 25) S.val[edst] = ve;
+  neg |= __builtin_amdgcn_ballot_w64(ve < 0.0f);
+  nonneg |= __builtin_amdgcn_ballot_w64(ve >= 0.0f);
+  wave_sync();  // the border words are clear
+  uint32_t yq = 0u, ym = 0u, zq = 0u, zm = 0u, eq = 0u, em = 0u;
+  mc_col_bits(vfy, hi, tiny, yq, ym);  // (hi, 8, lo): column (8, lo) at x = hi
+  mc_col_bits(vfz, hi, tiny, zq, zm);  // (hi, lo, 8): column (lo, 8) at x = hi
+  mc_col_bits(ve, ex, tiny, eq, em);
+  atomicOr(reinterpret_cast<unsigned long long*>(&S.col[72 + lo]), (unsigned long long)mc_col_word(yq, ym));
+  atomicOr(reinterpret_cast<unsigned long long*>(&S.col[lo * 9 + 8]), (unsigned long long)mc_col_word(zq, zm));
+  if (lane < 25) atomicOr(reinterpret_cast<unsigned long long*>(&S.col[ecol]), (unsigned long long)mc_col_word(eq, em));
+  wave_sync();
+  return neg != 0u && nonneg != 0u;
 }
 
-// Lane l classifies the 8 cubes of its (y, z) column (x = 0..7), sharing each x-face's
-// corners with the next cube. A cube has triangles iff its 8 corners are observed and its
-// case is not 0 or 255 (bit c set iff corner c < 0, reference corner order). Returns the
-// block's table triangle count; lane_off holds the per-lane exclusive prefix. near: bit x
-// set iff cube x of the lane's column has triangles and a corner with |value| < tiny (see
-// mc_count_exact).
-__device__ __forceinline__ uint32_t mc_classify(McShared& S, float tiny, uint32_t& near) {
-  const int lane = mc_lane(), cy = lane >> 3, cz = lane & 7;
-  // One x-plane of the column's cubes: its 4 corners, slot k = (dy << 1) | dz, tested once
-  // and shared by the cubes on either side. lo: the plane's negative corners as case bits of
-  // the cube whose low-x face it is (reference corners 0, 4, 1, 5 for k = 0..3); hi: as case
-  // bits of the cube whose high-x face it is (corners 3, 7, 2, 6); obs: all 4 observed; sml:
-  // one below tiny.
-  auto plane = [&](int x, uint32_t& lo, uint32_t& hi, bool& obs, bool& sml) {
-    constexpr uint32_t kLo[4] = {1u, 16u, 2u, 32u}, kHi[4] = {8u, 128u, 4u, 64u};
-    lo = 0u;
-    hi = 0u;
-    obs = true;
-    sml = false;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float f = S.val[x * 81 + (cy + (k >> 1)) * 9 + cz + (k & 1)];
-      const bool neg = f < 0.0f;
-      lo |= neg ? kLo[k] : 0u;
-      hi |= neg ? kHi[k] : 0u;
-      obs = obs & !__builtin_isnan(f);
-      sml = sml | (__builtin_fabsf(f) < tiny);
-    }
-  };
-  uint32_t lo0, hi0;
-  bool obs0, sml0;
-  plane(0, lo0, hi0, obs0, sml0);
+// Lane l classifies the 8 cubes of its (y, z) column (x = 0..7) from the column words of
+// its 4 corner columns (slot k = (dy << 1) | dz: column (y + dy, z + dz)): interleaved, each
+// x-plane is one nibble, and a cube's permuted case is the byte of its two faces. A cube has
+// triangles iff its 8 corners are observed and its case is not 0 or 255 (bit c set iff
+// corner c < 0, reference corner order); its case goes to S.cases. Returns the lane's table
+// triangle count. near: bit x set iff cube x of the column has triangles and a corner with
+// |value| < tiny (the kTiny bits of mc_stage; see mc_count_exact).
+__device__ __forceinline__ uint32_t mc_classify(McShared& S, uint32_t& near) {
+  const int lane = mc_lane(), i = (lane >> 3) * 9 + (lane & 7);
+  const uint64_t w0 = S.col[i], w1 = S.col[i + 1], w2 = S.col[i + 9], w3 = S.col[i + 10];
+  const uint32_t q = (uint32_t)w0 | (uint32_t)w1 << 1 | (uint32_t)w2 << 2 | (uint32_t)w3 << 3;
+  const uint32_t m0 = (uint32_t)(w0 >> 32), m1 = (uint32_t)(w1 >> 32), m2 = (uint32_t)(w2 >> 32),
+                 m3 = (uint32_t)(w3 >> 32);
+  const uint32_t q8 = (m0 & 1u) | (m1 & 1u) << 1 | (m2 & 1u) << 2 | (m3 & 1u) << 3;  // x = 8 nibble
+  const uint32_t um = m0 | m1 | m2 | m3;  // unobserved / small bits of any corner column
   uint32_t ntri = 0;
   near = 0u;
-#pragma unroll 4
+#pragma unroll
   for (int cx = 0; cx < 8; ++cx) {
-    uint32_t lo1, hi1;
-    bool obs1, sml1;
-    plane(cx + 1, lo1, hi1, obs1, sml1);
-    // bit c set iff corner c < 0 (reference corner order); triangles iff all 8 observed and
-    // the case is not 0 or 255
-    uint32_t ci = lo0 | hi1;
-    if (!(obs0 & obs1) || ci == 255u) ci = 0u;
-    if (ci != 0u && (sml0 | sml1)) near |= 1u << cx;
+    const uint32_t p = cx < 7 ? (q >> (4 * cx)) & 255u : (q >> 28) | (q8 << 4);
+    const uint32_t e = S.pc[p];
+    const bool obs = ((um >> (1 + cx)) & 3u) == 0u;
+    const uint32_t ci = obs ? e & 255u : 0u, n = obs ? e >> 8 : 0u;
+    if (n != 0u && ((um >> (10 + cx)) & 3u) != 0u) near |= 1u << cx;
     S.cases[cx * 64 + lane] = (uint8_t)ci;
-    ntri += S.cnt[ci];
-    lo0 = lo1;
-    obs0 = obs1;
-    sml0 = sml1;
+    ntri += n;
   }
-  uint32_t x = ntri;
+  return ntri;
+}
+
+// Exclusive prefix of the lanes' counts n into I.lane_off, I.lane_off[64] the total, which it
+// returns.
+__device__ __forceinline__ uint32_t mc_lane_prefix(McItems& I, uint32_t n) {
+  const int lane = mc_lane();
+  uint32_t x = n;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const uint32_t y = (uint32_t)__shfl_up((int)x, off);
     if (lane >= off) x += y;
   }
-  S.lane_off[lane] = x - ntri;
-  if (lane == 63) S.lane_off[64] = x;
+  I.lane_off[lane] = x - n;
+  if (lane == 63) I.lane_off[64] = x;
   wave_sync();
-  return S.lane_off[64];
+  return I.lane_off[64];
 }
 
-// Vertices (voxel units) of table triangle t of cube (cx, cy, cz) with case ci, and with
-// kColor their colours; false for a degenerate triangle (two equal vertices; the
-// reference's test at src/tsdf.cu:1055-1058, with its typo corrected).
-template <bool kColor>
-__device__ __forceinline__ bool mc_triangle(const McShared& S, const uint32_t* col, int ci, int t, int cx, int cy,
-                                            int cz, int gx0, int gy0, int gz0, float* P, uint32_t* C) {
-  const uint64_t pk = kTriTable[ci];
+__device__ __forceinline__ uint32_t mc_wave_sum(uint32_t n) {
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
-    const int ca = edge_a(e), cb = edge_b(e);
-    const int ia = (cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca));
-    const int ib = (cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb));
-    const float va = S.val[ia], vb = S.val[ib];
-    const float pax = (float)(gx0 + cx + corner_dx(ca)), pay = (float)(gy0 + cy + corner_dy(ca)),
-                paz = (float)(gz0 + cz + corner_dz(ca)), pbx = (float)(gx0 + cx + corner_dx(cb)),
-                pby = (float)(gy0 + cy + corner_dy(cb)), pbz = (float)(gz0 + cz + corner_dz(cb));
-    if constexpr (kColor)
-      vertex_interp_rgb(pax, pay, paz, pbx, pby, pbz, va, vb, col[ia], col[ib], P[3 * j], P[3 * j + 1], P[3 * j + 2],
-                        C[j]);
-    else
-      vertex_interp(pax, pay, paz, pbx, pby, pbz, va, vb, P[3 * j], P[3 * j + 1], P[3 * j + 2]);
-  }
-  const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
-  const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
-  const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
-  return !(d01 || d12 || d20);
+  for (int m = 32; m >= 1; m >>= 1) n += (uint32_t)__shfl_xor((int)n, m);
+  return n;
 }
 
-// Triangles that k_mc_write will emit for the lane's column, without computing vertices
-// where none can coincide. Vertices of one triangle lie on three distinct cube edges, in
-// the closed edge segments (the other two coordinates are exact lattice values, the
-// interpolation parameter lies in [0, 1]); two such points can only be equal at a corner
-// the two edges share, and a vertex lands exactly on corner a of its edge only by a snap
-// (|v_a| <= 1e-5, or |v_a - v_b| <= 1e-5 with opposite signs) or by rounding a + mu*(b - a)
-// to a, which needs |mu| <= ulp(a)/2 and so |v_a| <= ulp(a) (|v_b - v_a| <= 2); likewise for
-// b. A cube whose corners all have |value| >= tiny (tiny above both bounds) therefore emits
-// exactly its table count; the others ("near" bits) are counted triangle by triangle.
 // Per cube edge e, 5 bits at 5e: the offset of its lower end point in the cube (dx | dy << 1
 // | dz << 2) and its axis << 3 (0 x, 1 y, 2 z).
 constexpr uint64_t edge_lo_table() {
@@ -1241,6 +1286,82 @@ constexpr uint64_t edge_lo_table() {
   return t;
 }
 constexpr uint64_t kEdgeLo = edge_lo_table();
+// kEdgeUp bit e: the reference's first end point of cube edge e (edge_a) is its upper end.
+constexpr uint32_t edge_up_table() {
+  uint32_t m = 0;
+  for (int e = 0; e < 12; ++e) {
+    const int a = edge_a(e), b = edge_b(e);
+    if (corner_dx(a) + corner_dy(a) + corner_dz(a) > corner_dx(b) + corner_dy(b) + corner_dz(b)) m |= 1u << e;
+  }
+  return m;
+}
+constexpr uint32_t kEdgeUp = edge_up_table();
+
+// Vertex (voxel units) of cube edge e of cube (cx, cy, cz): vertex_interp from corner
+// edge_a(e) to edge_b(e) (the reference's VertexInterp, src/tsdf.cu:1640-1660), with kColor
+// the vertex_interp_rgb colour. The end points differ along one axis only, so the other two
+// coordinates are the lower end point's exactly (mu * 0 adds zero), and p2 - p1 along that
+// axis is exactly +-1; the edge is decoded from the packed tables, and the quotient is formed
+// on every lane and dropped where a snap applies, so that the wave takes no branch.
+template <bool kColor>
+__device__ __forceinline__ void mc_vertex(const McShared& S, const uint32_t* col, int e, int cx, int cy, int cz,
+                                          int gx0, int gy0, int gz0, float& x, float& y, float& z, uint32_t& c) {
+  const uint32_t info = (uint32_t)(kEdgeLo >> (5 * e)) & 31u;
+  const int lx = cx + (int)(info & 1u), ly = cy + (int)((info >> 1) & 1u), lz = cz + (int)((info >> 2) & 1u);
+  const uint32_t axis = info >> 3;
+  const int il = lx * 81 + ly * 9 + lz, iu = il + (axis == 0u ? 81 : (axis == 1u ? 9 : 1));
+  const bool up = ((kEdgeUp >> e) & 1u) != 0u;
+  const int ia = up ? iu : il, ib = up ? il : iu;
+  const float va = S.val[ia], vb = S.val[ib];
+  const float fx = (float)(gx0 + lx), fy = (float)(gy0 + ly), fz = (float)(gz0 + lz);
+  const int g = axis == 0u ? gx0 + lx : (axis == 1u ? gy0 + ly : gz0 + lz);
+  const float pl = (float)g, pu = (float)(g + 1);
+  const float pa = up ? pu : pl, pb = up ? pl : pu;
+  const bool sa = near_iso(va), sb = !sa && near_iso(vb), sd = !sa && !sb && near_iso(va - vb);
+  const float mu = (0.0f - va) / (vb - va);
+  const float t = sa || sd ? pa : (sb ? pb : pa + mu * (pb - pa));
+  x = axis == 0u ? t : fx;
+  y = axis == 1u ? t : fy;
+  z = axis == 2u ? t : fz;
+  if constexpr (kColor) {
+    const uint32_t c1 = col[ia], c2 = col[ib];
+    uint32_t cl = 0u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int a = (int)((c1 >> (8 * k)) & 255u), b = (int)((c2 >> (8 * k)) & 255u);
+      const float v = (float)a + mu * (float)(b - a);
+      cl |= (uint32_t)(uint8_t)v << (8 * k);
+    }
+    c = sa || sd ? c1 : (sb ? c2 : cl);
+  }
+}
+
+// Vertices (voxel units) of table triangle t of cube (cx, cy, cz) with case ci, and with
+// kColor their colours; false for a degenerate triangle (two equal vertices; the
+// reference's test at src/tsdf.cu:1055-1058, with its typo corrected).
+template <bool kColor>
+__device__ __forceinline__ bool mc_triangle(const McShared& S, const uint32_t* col, int ci, int t, int cx, int cy,
+                                            int cz, int gx0, int gy0, int gz0, float* P, uint32_t* C) {
+  const uint64_t pk = kTriTable[ci];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    mc_vertex<kColor>(S, col, (int)((pk >> (4 * (3 * t + j))) & 15u), cx, cy, cz, gx0, gy0, gz0, P[3 * j],
+                      P[3 * j + 1], P[3 * j + 2], C[j]);
+  const bool d01 = P[0] == P[3] && P[1] == P[4] && P[2] == P[5];
+  const bool d12 = P[3] == P[6] && P[4] == P[7] && P[5] == P[8];
+  const bool d20 = P[6] == P[0] && P[7] == P[1] && P[8] == P[2];
+  return !(d01 || d12 || d20);
+}
+
+// Triangles that k_mc_write will emit for the lane's column, without computing vertices
+// where none can coincide. Vertices of one triangle lie on three distinct cube edges, in
+// the closed edge segments (the other two coordinates are exact lattice values, the
+// interpolation parameter lies in [0, 1]); two such points can only be equal at a corner
+// the two edges share, and a vertex lands exactly on corner a of its edge only by a snap
+// (|v_a| <= 1e-5, or |v_a - v_b| <= 1e-5 with opposite signs) or by rounding a + mu*(b - a)
+// to a, which needs |mu| <= ulp(a)/2 and so |v_a| <= ulp(a) (|v_b - v_a| <= 2); likewise for
+// b. A cube whose corners all have |value| >= tiny (tiny above both bounds) therefore emits
+// exactly its table count; the others ("near" bits) are counted triangle by triangle.
 
 // Edges of case ci whose two corners lie on opposite sides (bit e): corners 0-3 and 4-7 form
 // the z = 0 and z = 1 rings (edges 0-3 and 4-7 join ring neighbours i, i + 1 mod 4), edges
@@ -1261,39 +1382,43 @@ static_assert(crossed_edges_ok(), "crossed_edges disagrees with the edge table")
 // Non-degenerate table triangles of a near cube (case ci at (cx, cy, cz)): vertex_end tells
 // on which corner, if any, each vertex lands, so no positions are formed; a triangle is
 // degenerate when two of its vertices land on the same corner (the only way two vertices
-// on distinct edges coincide), which is what mc_triangle's position test finds.
+// on distinct edges coincide), which is what mc_triangle's position test finds. Corners are
+// named by their tile index; edges are decoded as in mc_vertex.
 __device__ __forceinline__ uint32_t mc_count_near(const McShared& S, int ci, int cx, int cy, int cz, int gx0,
                                                   int gy0, int gz0) {
   const uint64_t pk = kTriTable[ci];
   const int nt = S.cnt[ci];
-  const int g0[3] = {gx0 + cx, gy0 + cy, gz0 + cz};
   uint32_t n = 0;
   for (int t = 0; t < nt; ++t) {
     int cid[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int e = (int)((pk >> (4 * (3 * t + j))) & 15u);
-      const int ca = edge_a(e), cb = edge_b(e);
-      const float va = S.val[(cx + corner_dx(ca)) * 81 + (cy + corner_dy(ca)) * 9 + (cz + corner_dz(ca))];
-      const float vb = S.val[(cx + corner_dx(cb)) * 81 + (cy + corner_dy(cb)) * 9 + (cz + corner_dz(cb))];
-      const int axis = (int)((kEdgeLo >> (5 * e + 3)) & 3u);
-      const int da = axis == 0 ? corner_dx(ca) : (axis == 1 ? corner_dy(ca) : corner_dz(ca));
-      const int db = axis == 0 ? corner_dx(cb) : (axis == 1 ? corner_dy(cb) : corner_dz(cb));
-      const int end = vertex_end((float)(g0[axis] + da), (float)(g0[axis] + db), va, vb);
-      cid[j] = end == 0 ? ca : (end == 1 ? cb : -1 - j);
+      const uint32_t info = (uint32_t)(kEdgeLo >> (5 * e)) & 31u;
+      const int lx = cx + (int)(info & 1u), ly = cy + (int)((info >> 1) & 1u), lz = cz + (int)((info >> 2) & 1u);
+      const uint32_t axis = info >> 3;
+      const int il = lx * 81 + ly * 9 + lz, iu = il + (axis == 0u ? 81 : (axis == 1u ? 9 : 1));
+      const bool up = ((kEdgeUp >> e) & 1u) != 0u;
+      const int ia = up ? iu : il, ib = up ? il : iu;
+      const int g = axis == 0u ? gx0 + lx : (axis == 1u ? gy0 + ly : gz0 + lz);
+      const float pl = (float)g, pu = (float)(g + 1);
+      const int end = vertex_end(up ? pu : pl, up ? pl : pu, S.val[ia], S.val[ib]);
+      cid[j] = end == 0 ? ia : (end == 1 ? ib : -1 - j);
     }
     n += (cid[0] != cid[1] && cid[1] != cid[2] && cid[2] != cid[0]) ? 1u : 0u;
   }
   return n;
 }
 
-__device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t near, int gx0, int gy0, int gz0) {
+// The lane's emitted triangles from its table count n: near cubes are counted triangle by
+// triangle.
+__device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t near, uint32_t n, int gx0, int gy0,
+                                                   int gz0) {
   const int lane = mc_lane(), cy = lane >> 3, cz = lane & 7;
-  uint32_t n = 0;
-  for (int cx = 0; cx < 8; ++cx) {
+  for (uint32_t m = near; m; m &= m - 1u) {
+    const int cx = __builtin_ctz(m);
     const int ci = S.cases[cx * 64 + lane];
-    if (!((near >> cx) & 1u)) n += (uint32_t)S.cnt[ci];
-    else n += mc_count_near(S, ci, cx, cy, cz, gx0, gy0, gz0);
+    n += mc_count_near(S, ci, cx, cy, cz, gx0, gy0, gz0) - (uint32_t)S.cnt[ci];
   }
   return n;
 }
@@ -1306,10 +1431,10 @@ __device__ __forceinline__ uint32_t mc_count_exact(const McShared& S, uint32_t n
 // kColor makes C hold the vertex colours interpolated from the colour tile col (otherwise
 // zeros). Without kPos, P and C are not computed and every item is valid.
 template <bool kColor, bool kPos, class Emit>
-__device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint32_t total, int gx0, int gy0, int gz0,
-                                         Emit emit) {
+__device__ __forceinline__ void mc_items(McShared& S, McItems& I, const uint32_t* col, uint32_t total, int gx0, int gy0,
+                                         int gz0, Emit emit) {
   const int lane = mc_lane();
-  const uint32_t my0 = S.lane_off[lane], my1 = S.lane_off[lane + 1];
+  const uint32_t my0 = I.lane_off[lane], my1 = I.lane_off[lane + 1];
   for (uint32_t r0 = 0; r0 < total; r0 += kItemCap) {
     const uint32_t r1 = min(total, r0 + (uint32_t)kItemCap);
     if (my0 < r1 && my1 > r0) {
@@ -1318,7 +1443,7 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
         const int cube = cx * 64 + lane;
         const int n = S.cnt[S.cases[cube]];
         for (int t = 0; t < n; ++t, ++idx)
-          if (idx >= r0 && idx < r1) S.items[idx - r0] = (uint16_t)((cube << 3) | t);
+          if (idx >= r0 && idx < r1) I.items[idx - r0] = (uint16_t)((cube << 3) | t);
       }
     }
     wave_sync();
@@ -1329,7 +1454,7 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
       float P[9];
       uint32_t C[3] = {0u, 0u, 0u};
       if (item < r1) {
-        const int it = S.items[item - r0];
+        const int it = I.items[item - r0];
         const int cube = it >> 3;
         t = it & 7;
         cx = cube >> 6;
@@ -1345,13 +1470,6 @@ __device__ __forceinline__ void mc_items(McShared& S, const uint32_t* col, uint3
   }
 }
 
-// Threshold of mc_count_exact for the block at lattice origin (gx0, gy0, gz0): above the
-// snap bound 1e-5 and above ulp(c) for every lattice coordinate c of the block and its
-// +1 layer (ulp(c) <= |c| * 2^-23).
-__device__ __forceinline__ float mc_tiny(int gx0, int gy0, int gz0) {
-  const int m = max(max(max(abs(gx0), abs(gx0 + 8)), max(abs(gy0), abs(gy0 + 8))), max(abs(gz0), abs(gz0 + 8)));
-  return 2e-5f + (float)(m + 1) * 0x1p-22f;
-}
 
 // The pool indices of each block's 8 neighbourhood blocks (+x/+y/+z and their combinations,
 // index (dx << 2) | (dy << 1) | dz; itself at 0; kNoBlock where absent), one lane per entry,
@@ -1385,17 +1503,16 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a) {
   if (b >= a.n_blocks) return;
   const McCounts counts = mc_fetch_counts();
   int gx0, gy0, gz0;
-  if (!mc_stage<kStorage, false>(a, b, mc_nbr(a, b), S, nullptr, gx0, gy0, gz0)) {  // no sign change: no triangles
+  if (!mc_stage<kStorage, false, true>(a, b, mc_nbr(a, b), S, nullptr, gx0, gy0, gz0)) {  // no sign change: no triangles
     if (lane == 0) a.counts[b] = 0u;
     return;
   }
   mc_put_counts(S, counts);
   wave_sync();
   uint32_t near;
-  mc_classify(S, mc_tiny(gx0, gy0, gz0), near);
-  uint32_t cnt = mc_count_exact(S, near, gx0, gy0, gz0);
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, m);
+  uint32_t cnt = mc_classify(S, near);
+  if (__ballot(near != 0u)) cnt = mc_count_exact(S, near, cnt, gx0, gy0, gz0);
+  cnt = mc_wave_sum(cnt);
   if (lane == 0) a.counts[b] = cnt;
 }
 
@@ -1481,8 +1598,10 @@ __device__ __forceinline__ uint32_t block_offset(const uint32_t* counts, const u
 template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64 * kMcWaves) void k_mc_write(McArgs a) {
   __shared__ McShared SS[kMcWaves];
+  __shared__ McItems IS[kMcWaves];
   __shared__ uint32_t s_cols[kMcWaves][kColor ? kHalo3 : 1];
   McShared& S = SS[mc_wave()];
+  McItems& I = IS[mc_wave()];
   uint32_t* s_col = s_cols[mc_wave()];
   const uint32_t b = blockIdx.x * kMcWaves + (uint32_t)mc_wave();
   if (b >= a.n_blocks) return;
@@ -1496,8 +1615,8 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_write(McArgs a) {
     mc_put_counts(S, counts);
     wave_sync();
     uint32_t near;
-    const uint32_t total = mc_classify(S, 0.0f, near);
-    mc_items<kColor, true>(S, s_col, total, gx0, gy0, gz0,
+    const uint32_t total = mc_lane_prefix(I, mc_classify(S, near));
+    mc_items<kColor, true>(S, I, s_col, total, gx0, gy0, gz0,
                            [&](bool valid, int x, int y, int z, int ci, int t, const float* P, const uint32_t* C) {
       const uint64_t m = __ballot(valid);
       const uint32_t p = pos + lanes_below(m);
@@ -1566,7 +1685,7 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_mark(McArgs a) {
     mc_put_counts(S, counts);
     wave_sync();
     uint32_t near;
-    const uint32_t total = mc_classify(S, 0.0f, near);
+    const uint32_t total = mc_wave_sum(mc_classify(S, near));
     const int cy = lane >> 3, cz = lane & 7;
     for (int cx = 0; cx < 8; ++cx) {
       const int ci = S.cases[cx * 64 + lane];
@@ -1606,9 +1725,11 @@ __global__ __launch_bounds__(256) void k_ix_count(McArgs a) {
 template <int kStorage, bool kColor>
 __global__ __launch_bounds__(64 * kMcWaves) void k_ix_write(McArgs a) {
   __shared__ McShared SS[kMcWaves];
+  __shared__ McItems IS[kMcWaves];
   __shared__ uint32_t s_cols[kMcWaves][kColor ? kHalo3 : 1];
   __shared__ uint32_t s_vbases[kMcWaves][8];
   McShared& S = SS[mc_wave()];
+  McItems& I = IS[mc_wave()];
   uint32_t* s_col = s_cols[mc_wave()];
   uint32_t* s_vbase = s_vbases[mc_wave()];
   const int lane = mc_lane();
@@ -1629,8 +1750,8 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_write(McArgs a) {
       s_vbase[lane] = nb != kNoBlock ? a.vchunks[nb / kScanChunk] + a.vcounts[nb] : 0u;
     }
     uint32_t near;
-    const uint32_t total = mc_classify(S, 0.0f, near);  // ends with a barrier
-    mc_items<false, false>(S, nullptr, total, gx0, gy0, gz0,
+    const uint32_t total = mc_lane_prefix(I, mc_classify(S, near));  // ends with a barrier
+    mc_items<false, false>(S, I, nullptr, total, gx0, gy0, gz0,
                            [&](bool valid, int x, int y, int z, int ci, int t, const float*, const uint32_t*) {
       const uint64_t m = __ballot(valid);
       const uint32_t p = pos + lanes_below(m);
