@@ -1297,24 +1297,18 @@ constexpr uint32_t edge_up_table() {
 }
 constexpr uint32_t kEdgeUp = edge_up_table();
 
-// Vertex (voxel units) of cube edge e of cube (cx, cy, cz): vertex_interp from corner
-// edge_a(e) to edge_b(e) (the reference's VertexInterp, src/tsdf.cu:1640-1660), with kColor
-// the vertex_interp_rgb colour. The end points differ along one axis only, so the other two
-// coordinates are the lower end point's exactly (mu * 0 adds zero), and p2 - p1 along that
-// axis is exactly +-1; the edge is decoded from the packed tables, and the quotient is formed
-// on every lane and dropped where a snap applies, so that the wave takes no branch.
+// vertex_interp (the reference's VertexInterp, src/tsdf.cu:1640-1660) from tile corner ia to
+// tile corner ib of one lattice edge along axis, with kColor vertex_interp_rgb's colour. The
+// end points differ along that axis only, so the other two coordinates are the lower end
+// point's (fx, fy, fz) exactly (mu * 0 adds zero), and p2 - p1 along the axis is exactly +-1
+// (g: the lower end point's lattice coordinate on the axis; up: ia is the upper end). The
+// quotient is formed on every lane and dropped where a snap applies, so that the wave takes no
+// branch.
 template <bool kColor>
-__device__ __forceinline__ void mc_vertex(const McShared& S, const uint32_t* col, int e, int cx, int cy, int cz,
-                                          int gx0, int gy0, int gz0, float& x, float& y, float& z, uint32_t& c) {
-  const uint32_t info = (uint32_t)(kEdgeLo >> (5 * e)) & 31u;
-  const int lx = cx + (int)(info & 1u), ly = cy + (int)((info >> 1) & 1u), lz = cz + (int)((info >> 2) & 1u);
-  const uint32_t axis = info >> 3;
-  const int il = lx * 81 + ly * 9 + lz, iu = il + (axis == 0u ? 81 : (axis == 1u ? 9 : 1));
-  const bool up = ((kEdgeUp >> e) & 1u) != 0u;
-  const int ia = up ? iu : il, ib = up ? il : iu;
+__device__ __forceinline__ void mc_interp(const McShared& S, const uint32_t* col, int ia, int ib, uint32_t axis, int g,
+                                          bool up, float fx, float fy, float fz, float& x, float& y, float& z,
+                                          uint32_t& c) {
   const float va = S.val[ia], vb = S.val[ib];
-  const float fx = (float)(gx0 + lx), fy = (float)(gy0 + ly), fz = (float)(gz0 + lz);
-  const int g = axis == 0u ? gx0 + lx : (axis == 1u ? gy0 + ly : gz0 + lz);
   const float pl = (float)g, pu = (float)(g + 1);
   const float pa = up ? pu : pl, pb = up ? pl : pu;
   const bool sa = near_iso(va), sb = !sa && near_iso(vb), sd = !sa && !sb && near_iso(va - vb);
@@ -1334,6 +1328,21 @@ __device__ __forceinline__ void mc_vertex(const McShared& S, const uint32_t* col
     }
     c = sa || sd ? c1 : (sb ? c2 : cl);
   }
+}
+
+// Vertex (voxel units) of cube edge e of cube (cx, cy, cz): mc_interp from corner edge_a(e) to
+// edge_b(e), the edge decoded from the packed tables.
+template <bool kColor>
+__device__ __forceinline__ void mc_vertex(const McShared& S, const uint32_t* col, int e, int cx, int cy, int cz,
+                                          int gx0, int gy0, int gz0, float& x, float& y, float& z, uint32_t& c) {
+  const uint32_t info = (uint32_t)(kEdgeLo >> (5 * e)) & 31u;
+  const int lx = cx + (int)(info & 1u), ly = cy + (int)((info >> 1) & 1u), lz = cz + (int)((info >> 2) & 1u);
+  const uint32_t axis = info >> 3;
+  const int il = lx * 81 + ly * 9 + lz, iu = il + (axis == 0u ? 81 : (axis == 1u ? 9 : 1));
+  const bool up = ((kEdgeUp >> e) & 1u) != 0u;
+  const int g = axis == 0u ? gx0 + lx : (axis == 1u ? gy0 + ly : gz0 + lz);
+  mc_interp<kColor>(S, col, up ? iu : il, up ? il : iu, axis, g, up, (float)(gx0 + lx), (float)(gy0 + ly),
+                    (float)(gz0 + lz), x, y, z, c);
 }
 
 // Vertices (voxel units) of table triangle t of cube (cx, cy, cz) with case ci, and with
@@ -1781,13 +1790,9 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_ix_write(McArgs a) {
         const int i0 = lx * 81 + ly * 9 + lz, i1 = ux * 81 + uy * 9 + uz;
         float px, py, pz;
         uint32_t c = 0u;
-        if constexpr (kColor)
-          vertex_interp_rgb((float)(gx0 + lx), (float)(gy0 + ly), (float)(gz0 + lz), (float)(gx0 + ux),
-                            (float)(gy0 + uy), (float)(gz0 + uz), S.val[i0], S.val[i1], s_col[i0], s_col[i1], px,
-                            py, pz, c);
-        else
-          vertex_interp((float)(gx0 + lx), (float)(gy0 + ly), (float)(gz0 + lz), (float)(gx0 + ux),
-                        (float)(gy0 + uy), (float)(gz0 + uz), S.val[i0], S.val[i1], px, py, pz);
+        const int g = axis == 0 ? gx0 + lx : (axis == 1 ? gy0 + ly : gz0 + lz);
+        mc_interp<kColor>(S, s_col, i0, i1, (uint32_t)axis, g, false, (float)(gx0 + lx), (float)(gy0 + ly),
+                          (float)(gz0 + lz), px, py, pz, c);
         if (id < a.vertex_cap) {
           float* q = a.vertices + (size_t)id * 3;
           q[0] = px * a.vox;
