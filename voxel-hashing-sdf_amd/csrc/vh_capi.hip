@@ -196,6 +196,7 @@ struct vh_volume {
   int64_t mc_cap = 0;
   uint32_t* mc_counts = nullptr;   // pool_blocks
   uint32_t* mc_nbrs = nullptr;     // pool_blocks x 8
+  uint32_t* pin = nullptr;         // page-locked words for small device-to-host reads (kPinWords)
   uint32_t* mc_chunks = nullptr;   // pool_blocks / kScanChunk + 1
   uint32_t* mc_total = nullptr;
   // scratch for queries / Gauss-Newton
@@ -491,6 +492,16 @@ int collect_profile(vh_volume* v) {
   return VH_OK;
 }
 
+// Device words read through the volume's page-locked words (a pageable destination makes
+// the copy go through a staging buffer): n <= kPinWords words from dev into out.
+constexpr int kPinWords = 16;
+int read_words(vh_volume* v, const uint32_t* dev, uint32_t* out, int n) {
+  VH_HIP(hipMemcpyAsync(v->pin, dev, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+  VH_HIP(hipStreamSynchronize(v->stream));
+  for (int i = 0; i < n; ++i) out[i] = v->pin[i];
+  return VH_OK;
+}
+
 int read_ctl(vh_volume* v, FrameCtl* out) {
   VH_HIP(hipMemcpyAsync(out, v->ctl, sizeof(FrameCtl), hipMemcpyDeviceToHost, v->stream));
   VH_HIP(hipStreamSynchronize(v->stream));
@@ -585,6 +596,7 @@ void free_volume(vh_volume* v) {
     if (v->ring_done[k]) (void)hipEventDestroy(v->ring_done[k]);
   }
   if (v->ring_pose_host) (void)hipHostFree(v->ring_pose_host);
+  if (v->pin) (void)hipHostFree(v->pin);
   if (v->copy_stream) (void)hipStreamDestroy(v->copy_stream);
   for (hipEvent_t e : v->ev_pending) (void)hipEventDestroy(e);
   free_index_buffers(v);
@@ -613,15 +625,15 @@ int ensure_mc_scratch(vh_volume* v) {
 
 // Marching-cubes arguments over the volume's current blocks (reads the frame control block).
 int mc_args(vh_volume* v, McArgs& a) {
-  FrameCtl c{};
+  uint32_t pool_count = 0;
   int rc = ensure_mc_scratch(v);
-  if (rc || (rc = read_ctl(v, &c))) return rc;
+  if (rc || (rc = read_words(v, &v->ctl->pool_count, &pool_count, 1))) return rc;
   a = McArgs{};
   a.hash = HashView{v->keys, v->vals, v->cfg.log2_buckets, ((uint64_t)1 << v->cfg.log2_buckets) - 1};
   a.block_keys = v->block_keys;
   a.voxels = v->voxels;
   a.pool_cap = (uint32_t)v->cfg.pool_blocks;
-  a.n_blocks = std::min<uint32_t>(c.pool_count, (uint32_t)v->cfg.pool_blocks);
+  a.n_blocks = std::min<uint32_t>(pool_count, (uint32_t)v->cfg.pool_blocks);
   a.vox = v->cfg.vox_size;
   a.counts = v->mc_counts;
   a.nbrs = v->mc_nbrs;
@@ -642,9 +654,10 @@ int run_mc(vh_volume* v, float* d_xyz, int32_t* d_cubes, uint8_t* d_rgb, int64_t
   uint32_t cnt = 0;
   if (a.n_blocks) {
     VH_HIP(launch_mc(a, v->cfg.storage, v->stream));
-    VH_HIP(hipMemcpyAsync(&cnt, v->mc_total, sizeof cnt, hipMemcpyDeviceToHost, v->stream));
+    if ((rc = read_words(v, v->mc_total, &cnt, 1))) return rc;
+  } else {
+    VH_HIP(hipStreamSynchronize(v->stream));
   }
-  VH_HIP(hipStreamSynchronize(v->stream));
   *n_tris = cnt;
   if ((int64_t)cnt > capacity) return fail(VH_ERR_TRUNCATED, "mesh larger than the output buffer");
   return VH_OK;
@@ -702,10 +715,14 @@ int index_count(vh_volume* v, McArgs& a, int64_t* n_faces, int64_t* n_vertices) 
   uint32_t tot[2] = {0u, 0u};
   if (a.n_blocks) {
     VH_HIP(launch_mesh_index_count(a, v->cfg.storage, v->stream));
-    VH_HIP(hipMemcpyAsync(&tot[0], v->mc_total, 4, hipMemcpyDeviceToHost, v->stream));
-    VH_HIP(hipMemcpyAsync(&tot[1], v->ix_vtotal, 4, hipMemcpyDeviceToHost, v->stream));
+    VH_HIP(hipMemcpyAsync(&v->pin[0], v->mc_total, 4, hipMemcpyDeviceToHost, v->stream));
+    VH_HIP(hipMemcpyAsync(&v->pin[1], v->ix_vtotal, 4, hipMemcpyDeviceToHost, v->stream));
   }
   VH_HIP(hipStreamSynchronize(v->stream));
+  if (a.n_blocks) {
+    tot[0] = v->pin[0];
+    tot[1] = v->pin[1];
+  }
   *n_faces = tot[0];
   *n_vertices = tot[1];
   if (tot[1] > 0x7fffffffu) return fail(VH_ERR_INVALID_ARG, "more than 2^31 vertices");
@@ -1089,6 +1106,11 @@ int vh_create(const vh_config* cfg, vh_volume** out) {
   if (hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess) {
     delete v;
     return fail(VH_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  if (hipHostMalloc((void**)&v->pin, kPinWords * 4, hipHostMallocDefault) != hipSuccess) {
+    v->pin = nullptr;
+    free_volume(v);
+    return fail(VH_ERR_OOM, "hipHostMalloc failed");
   }
   v->n_slots = (uint64_t)kSlotsPerBucket << cfg->log2_buckets;
   // blocks listed per batch, per stripe (one stripe per image region, so shares are uneven):
