@@ -510,11 +510,9 @@ int read_ctl(vh_volume* v, FrameCtl* out) {
 
 // Waits for the stream, folds profiling events, reports and clears overflow flags.
 int sync_and_check(vh_volume* v) {
-  VH_HIP(hipStreamSynchronize(v->stream));
-  int rc = collect_profile(v);
-  if (rc) return rc;
   uint32_t flags = 0;
-  VH_HIP(hipMemcpy(&flags, &v->ctl->flags, sizeof flags, hipMemcpyDeviceToHost));
+  int rc = read_words(v, &v->ctl->flags, &flags, 1);  // after the stream's work
+  if (rc || (rc = collect_profile(v))) return rc;
   if (!flags) return VH_OK;
   const uint32_t zero = 0;
   VH_HIP(hipMemcpy(&v->ctl->flags, &zero, sizeof zero, hipMemcpyHostToDevice));
