@@ -639,16 +639,24 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         for (int q = 0; q < 8; ++q) cold[q] = fresh ? 0u : cp[64 * q];
       }
       uint32_t upd = 0u;  // colour volumes: bit q set when voxel q was observed in the batch
+      // the block origin in homogeneous pixel coordinates per frame of the batch: lane j < k
+      // computes frame j's and the frame loop takes it with readlane, so these wave-uniform
+      // values are computed once per block rather than by every lane in every frame
+      float hbl[3] = {0.0f, 0.0f, 0.0f};
+      if ((uint32_t)lane < k) {
+        float T[12], cb[3];
+#pragma unroll
+        for (int q = 0; q < 12; ++q) T[q] = s_frm[lane].T[q];
+        block_origin_cam(T, a.vox, bx, by, bz, cb);
+        to_pixel_h(a.K, cb, hbl);
+      }
       for (uint32_t m = mark; m != 0u; m &= m - 1u) {
         const uint32_t jf = (uint32_t)__builtin_ctz(m);
         const FrameLds& F = s_frm[jf];
-        float T[12];
-#pragma unroll
-        for (int q = 0; q < 12; ++q) T[q] = F.T[q];
         const BlockFrame hf = F.h;
-        float cb[3], hb[3], cl[3];
-        block_origin_cam(T, a.vox, bx, by, bz, cb);
-        to_pixel_h(a.K, cb, hb);
+        float hb[3], cl[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) hb[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hbl[r]), (int)jf));
         column_h(hf, hb, lane >> 3, lane & 7, cl);
         const size_t fo = ((size_t)parity * kMaxBatch + jf) * frame_px;
         const uint32_t* rgba = kColor ? a.rgba + fo : nullptr;
