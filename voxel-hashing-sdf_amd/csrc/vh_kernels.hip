@@ -1029,7 +1029,7 @@ constexpr McPerm mc_perm_table() {
   }
   return t;
 }
-constexpr McPerm kPermHost = mc_perm_table();
+constexpr McPerm kPermHost = mc_perm_table();  // the compile-time copy that mc_perm_ok checks
 constexpr bool mc_perm_ok() {  // a bijection fixing 0 and 255, and corner 0 of the x = cx face at bit 0
   bool seen[256] = {};
   for (int ci = 0; ci < 256; ++ci) {
@@ -1051,6 +1051,7 @@ struct McShared {
   uint8_t cnt[256];    // triangles of each case (table count)
   uint8_t cases[512];  // case per cube (x*64 + y*8 + z); 0 = no triangles
 };
+static_assert(offsetof(McShared, cnt) % 4 == 0, "mc_put_counts writes S.cnt four entries at a time");
 // Item lists of the write passes (k_mc_write, k_ix_write).
 struct McItems {
   uint32_t lane_off[65];     // exclusive prefix of table triangles per lane
