@@ -560,8 +560,9 @@ struct FrameLds {
 };
 
 // s_frm: the batch's k frames; rcp_tab: RN(1/w) for w < kRcpTab (read when kTab); s_win: the
-// LDS byte address of the wave's depth window (kWin pixels).
-template <int kStorage, bool kColor, bool kTab>
+// LDS byte address of the wave's depth window (kWin pixels). kShared: the block origin of
+// each frame is computed once per block (launches of several frames), else in the frame loop.
+template <int kStorage, bool kColor, bool kTab, bool kShared>
 __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t workers, uint32_t f0, uint32_t k,
                                  uint32_t parity, unsigned long long* s_val, unsigned long long* s_key,
                                  uint32_t* s_mark, const FrameLds* s_frm, const float* rcp_tab, uint32_t s_win) {
@@ -639,11 +640,12 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         for (int q = 0; q < 8; ++q) cold[q] = fresh ? 0u : cp[64 * q];
       }
       uint32_t upd = 0u;  // colour volumes: bit q set when voxel q was observed in the batch
-      // the block origin in homogeneous pixel coordinates per frame of the batch: lane j < k
-      // computes frame j's and the frame loop takes it with readlane, so these wave-uniform
-      // values are computed once per block rather than by every lane in every frame
+      // the block origin in homogeneous pixel coordinates per frame of the batch: with kShared
+      // lane j < k computes frame j's and the frame loop takes it with readlane, so these
+      // wave-uniform values are computed once per block rather than by every lane in every
+      // frame (a single-frame launch would only add the readlanes, and computes it inline)
       float hbl[3] = {0.0f, 0.0f, 0.0f};
-      if ((uint32_t)lane < k) {
+      if (kShared && (uint32_t)lane < k) {
         float T[12], cb[3];
 #pragma unroll
         for (int q = 0; q < 12; ++q) T[q] = s_frm[lane].T[q];
@@ -655,8 +657,17 @@ __device__ void integrate_worker(const FrameArgs& a, uint32_t worker, uint32_t w
         const FrameLds& F = s_frm[jf];
         const BlockFrame hf = F.h;
         float hb[3], cl[3];
+        if constexpr (kShared) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r) hb[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hbl[r]), (int)jf));
+          for (int r = 0; r < 3; ++r)
+            hb[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hbl[r]), (int)jf));
+        } else {
+          float T[12], cb[3];
+#pragma unroll
+          for (int q = 0; q < 12; ++q) T[q] = F.T[q];
+          block_origin_cam(T, a.vox, bx, by, bz, cb);
+          to_pixel_h(a.K, cb, hb);
+        }
         column_h(hf, hb, lane >> 3, lane & 7, cl);
         const size_t fo = ((size_t)parity * kMaxBatch + jf) * frame_px;
         const uint32_t* rgba = kColor ? a.rgba + fo : nullptr;
@@ -813,7 +824,7 @@ struct IntegrateLds {
 
 // The workgroup's share of the batch of k frames (first frame f0, list parity, first input
 // cur). All 256 lanes call it.
-template <int kStorage, bool kColor>
+template <int kStorage, bool kColor, bool kShared>
 __device__ __forceinline__ void integrate_group(const FrameArgs& a, uint32_t group, uint32_t groups, uint32_t f0,
                                                 uint32_t k, uint32_t parity, uint32_t cur, IntegrateLds& L) {
   const int wave = threadIdx.x >> 6;
@@ -822,22 +833,22 @@ __device__ __forceinline__ void integrate_group(const FrameArgs& a, uint32_t gro
   const uint32_t worker = group * kIntegrateWaves + wave, workers = groups * kIntegrateWaves;
   const uint32_t win = uniform(lds_address(L.win[wave]));
   if (tab)
-    integrate_worker<kStorage, kColor, true>(a, worker, workers, f0, k, parity, L.val[wave], L.key[wave],
-                                             L.mark[wave], L.frm, L.rcp, win);
+    integrate_worker<kStorage, kColor, true, kShared>(a, worker, workers, f0, k, parity, L.val[wave], L.key[wave],
+                                                      L.mark[wave], L.frm, L.rcp, win);
   else
-    integrate_worker<kStorage, kColor, false>(a, worker, workers, f0, k, parity, L.val[wave], L.key[wave],
-                                              L.mark[wave], L.frm, L.rcp, win);
+    integrate_worker<kStorage, kColor, false, kShared>(a, worker, workers, f0, k, parity, L.val[wave], L.key[wave],
+                                                       L.mark[wave], L.frm, L.rcp, win);
 }
 
-// Integration of the batch of k frames that the last allocation launch prepared.
-template <int kStorage, bool kColor>
+// Integration of the batch of k frames that the last allocation launch prepared (kShared: k > 1).
+template <int kStorage, bool kColor, bool kShared>
 __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k_integrate(FrameArgs a, uint32_t k) {
   __shared__ IntegrateLds L;
   const FrameCtl* ctl = a.ctl;
   // control values are uniform: keep them (and what derives from them) in scalar registers
   const uint32_t f0 = uniform(ctl->frame_id), parity = uniform(ctl->batch_id) & 1u,
                  cur = uniform(ctl->cursor[a.cursor_id]);
-  integrate_group<kStorage, kColor>(a, blockIdx.x, gridDim.x, f0, k, parity, cur, L);
+  integrate_group<kStorage, kColor, kShared>(a, blockIdx.x, gridDim.x, f0, k, parity, cur, L);
   batch_done(a, gridDim.x, f0, k, parity);
 }
 
@@ -866,7 +877,7 @@ __global__ __launch_bounds__(64 * kIntegrateWaves) VH_INTEGRATE_OCCUPANCY void k
     alloc_tile<kU16, kAllocRows, kColor>(a, idx - jb * tiles, f0 + k_int + jb, jb, (batch + 1u) & 1u,
                                  input_slot(cur, k_int + jb, a.n_seq), L.alloc);
   } else {  // uniform branch: the whole workgroup integrates
-    integrate_group<kStorage, kColor>(a, idx, int_groups, f0, k_int, batch & 1u, cur, L.integ);
+    integrate_group<kStorage, kColor, true>(a, idx, int_groups, f0, k_int, batch & 1u, cur, L.integ);
   }
   batch_done(a, gridDim.x, f0, k_int, batch & 1u);
 }
@@ -2630,8 +2641,10 @@ static void with_storage(int storage, F f) {
 hipError_t launch_integrate(const FrameArgs& a, int storage, int grid, int k, hipStream_t st) {
   with_storage(storage, [&](auto S) {
     const dim3 g(grid / kIntegrateWaves), b(64 * kIntegrateWaves);
-    if (a.colors) hipLaunchKernelGGL((k_integrate<decltype(S)::value, true>), g, b, 0, st, a, (uint32_t)k);
-    else hipLaunchKernelGGL((k_integrate<decltype(S)::value, false>), g, b, 0, st, a, (uint32_t)k);
+    constexpr int kS = decltype(S)::value;
+    const auto kern = a.colors ? (k > 1 ? k_integrate<kS, true, true> : k_integrate<kS, true, false>)
+                               : (k > 1 ? k_integrate<kS, false, true> : k_integrate<kS, false, false>);
+    hipLaunchKernelGGL(kern, g, b, 0, st, a, (uint32_t)k);
   });
   return hipGetLastError();
 }
